@@ -1,0 +1,307 @@
+/*
+ * lambda_gpu.h — C ABI of the MI355X wavefront spectral path tracer.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes, no
+ * exceptions across the boundary. Every scene buffer passed in is copied;
+ * the caller keeps ownership of its memory (the reference's
+ * lambdaTriangleMeshSetBuffer took ownership, Lambda.cpp:211-244; this ABI
+ * does not).
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the Zoophish/Lambda source tree).
+ *
+ * Threading: one render in flight per (device, stream); calls on one device
+ * are serialised by the caller.
+ */
+#ifndef LAMBDA_GPU_H
+#define LAMBDA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGPU_NONE 0xFFFFFFFFu
+#define LGPU_ABI_VERSION 1u
+
+typedef enum lgpu_status {
+  LGPU_OK = 0,
+  LGPU_ERR_INVALID_ARGUMENT = 1,
+  LGPU_ERR_HIP = 2,
+  LGPU_ERR_STATE = 3,
+  LGPU_ERR_OUT_OF_MEMORY = 4,
+  LGPU_ERR_UNSUPPORTED = 5,
+  LGPU_ERR_GRAPH = 6,
+  LGPU_ERR_NO_DEVICE = 7
+} lgpu_status;
+
+typedef struct lgpu_device_s *lgpu_device;
+typedef struct lgpu_scene_s *lgpu_scene;
+typedef struct lgpu_film_s *lgpu_film;
+
+/* ------------------------------------------------------------------------
+ * Shader-node networks (reference: src/shading/graph/ShaderGraph.h:72-327,
+ * include/lambda/Material.h:42-75). A material is a small DAG of nodes; each
+ * node has one typed output and up to four inputs that name other nodes of
+ * the same material. The graph is compiled to bytecode at
+ * lgpu_scene_add_material().
+ * --------------------------------------------------------------------- */
+typedef enum lgpu_node_type {
+  /* inputs (GraphInputs.cpp) */
+  LGPU_NODE_SCALAR = 1,         /* out scalar = p[0] */
+  LGPU_NODE_VEC2 = 2,           /* out vec2 = p[0..1] */
+  LGPU_NODE_VEC3 = 3,           /* out vec3 = p[0..2] */
+  LGPU_NODE_RGB = 4,            /* out colour = p[0..2] (RGBInput) */
+  LGPU_NODE_BLACKBODY = 5,      /* out spectrum; in[0] temperature (scalar) */
+  LGPU_NODE_IMAGE_COLOUR = 6,   /* out colour; p[0] = texture id */
+  LGPU_NODE_IMAGE_SCALAR = 7,   /* out scalar; p[0] = texture id */
+  LGPU_NODE_UV = 8,             /* out vec2 = surface uv (SurfaceInput) */
+  LGPU_NODE_FRESNEL = 9,        /* out scalar = FrDielectric(woL.y, 1, p[0]) */
+  /* maths / converters (GraphMaths.cpp, GraphConverters.cpp) */
+  LGPU_NODE_SCALAR_MATHS = 16,  /* out scalar = in[0] op in[1]; p[0] = op */
+  LGPU_NODE_VECTOR_MATHS = 17,  /* out vec3 = in[0] op in[1]; p[0] = op */
+  LGPU_NODE_DOT = 18,           /* out scalar */
+  LGPU_NODE_LENGTH = 19,        /* out scalar */
+  LGPU_NODE_MERGE_RGB = 20,     /* out colour from three scalars */
+  LGPU_NODE_SEPARATE_R = 21,    /* out scalar: channel p[0] of a colour */
+  /* textures (GraphTexture.cpp) */
+  LGPU_NODE_CHECKER = 32,       /* out scalar; p[0] = scale */
+  /* BxDF nodes (GraphBxDF.cpp) */
+  LGPU_NODE_LAMBERTIAN = 64,    /* in[0] albedo */
+  LGPU_NODE_OREN_NAYAR_BRDF = 65, /* in[0] albedo, in[1] sigma (radians) */
+  LGPU_NODE_OREN_NAYAR_BTDF = 66, /* in[0] albedo, in[1] sigma */
+  LGPU_NODE_MICROFACET_BRDF = 67, /* in[0] albedo, in[1] roughness;
+                                     p[0] distribution (0 Beckmann, 1 GGX);
+                                     p[1] fresnel (0 dielectric, 1 conductor);
+                                     p[2] dielectric etaT;
+                                     p[3..5] conductor eta RGB, p[6..8] k RGB;
+                                     p[9] != 0: roughness socket is alpha */
+  LGPU_NODE_FRESNEL_BSDF = 68,  /* in[0] albedo, in[1] ior */
+  LGPU_NODE_SPECULAR_BRDF = 69, /* in[0] albedo */
+  LGPU_NODE_SPECULAR_BTDF = 70, /* in[0] albedo; p[0] etaT */
+  LGPU_NODE_GHOST_BTDF = 71,    /* in[0] alpha */
+  LGPU_NODE_MIX_BSDF = 72,      /* in[0] bxdf a, in[1] bxdf b, in[2] ratio */
+  LGPU_NODE_ROUGH_DIELECTRIC = 73 /* project extension (no reference BTDF,
+                                     Microfacet.h:28): in[0] albedo,
+                                     in[1] alpha; p[0] ior (GGX, Walter 2007) */
+} lgpu_node_type;
+
+typedef enum lgpu_maths_op {
+  LGPU_OP_ADD = 0, LGPU_OP_SUBTRACT = 1, LGPU_OP_MULTIPLY = 2, LGPU_OP_DIVIDE = 3
+} lgpu_maths_op;
+
+typedef struct lgpu_node {
+  uint32_t type;      /* lgpu_node_type */
+  uint32_t in[4];     /* node indices within the material, LGPU_NONE = unset */
+  float p[12];        /* constant parameters */
+} lgpu_node;
+
+/* reference: Material {mediaBoundary, bxdf, light} (src/shading/Material.h:18-32)
+ * plus MeshLight {emission, intensity} (src/lighting/MeshLight.h:13-16). */
+typedef struct lgpu_material_desc {
+  uint32_t num_nodes;
+  const lgpu_node *nodes;
+  uint32_t bxdf_root;        /* node index of the BxDF, LGPU_NONE = pass-through */
+  uint32_t emission_root;    /* node index of the emission socket, LGPU_NONE = not a light */
+  float emission_intensity;  /* MeshLight::intensity */
+  uint32_t medium_interior;  /* medium id or LGPU_NONE (MediaBoundary) */
+  uint32_t medium_exterior;
+} lgpu_material_desc;
+
+/* reference: TriangleMesh (src/core/TriangleMesh.h:14-41) */
+typedef struct lgpu_mesh_desc {
+  uint32_t num_vertices;
+  uint32_t num_triangles;
+  const float *positions;    /* 3 * num_vertices */
+  const uint32_t *indices;   /* 3 * num_triangles */
+  const float *normals;      /* 3 * num_vertices or NULL (flat shading) */
+  const float *tangents;     /* 4 * num_vertices (w = bitangent sign) or NULL */
+  const float *uvs;          /* 2 * num_vertices or NULL */
+  uint32_t material;         /* material id */
+} lgpu_mesh_desc;
+
+typedef enum lgpu_light_type {
+  LGPU_LIGHT_POINT = 1,       /* PointLight.cpp */
+  LGPU_LIGHT_SPOT = 2,        /* Spotlight.cpp */
+  LGPU_LIGHT_ENVIRONMENT = 3  /* EnvironmentLight.cpp */
+} lgpu_light_type;
+
+typedef struct lgpu_light_desc {
+  uint32_t type;
+  float position[3];
+  float axis[3];             /* spot axis */
+  float cone_angle;          /* spot: radians */
+  float falloff_start;       /* spot: radians */
+  float intensity;
+  float rgb[3];              /* emission colour (used when temperature == 0) */
+  float temperature;         /* > 0: blackbody emission */
+  /* environment: equirectangular FP32 RGB, row-major, width x height */
+  uint32_t env_width, env_height;
+  const float *env_rgb;
+  float env_offset[2];       /* EnvironmentLight::offset (phi, theta) */
+} lgpu_light_desc;
+
+typedef enum lgpu_medium_type {
+  LGPU_MEDIUM_HOMOGENEOUS = 1, /* HomogeneousMedium.cpp */
+  LGPU_MEDIUM_GRID = 2         /* project extension: FP32 density grid */
+} lgpu_medium_type;
+
+typedef struct lgpu_medium_desc {
+  uint32_t type;
+  float sigma_a[3];
+  float sigma_s[3];
+  float g;                   /* Henyey-Greenstein asymmetry */
+  /* grid: density in [0,1] scales sigma_a/sigma_s; z-major nx*ny*nz */
+  uint32_t nx, ny, nz;
+  const float *density;
+  float bounds_min[3], bounds_max[3];
+} lgpu_medium_desc;
+
+typedef struct lgpu_texture_desc {
+  uint32_t width, height, channels; /* channels: 1 or 3 */
+  const float *data;                /* row-major */
+} lgpu_texture_desc;
+
+typedef enum lgpu_light_strategy {
+  LGPU_LIGHTS_POWER = 0,   /* PowerLightSampler (LightSampler.cpp:14-35) */
+  LGPU_LIGHTS_TREE = 1     /* ManyLightSampler (ManyLightSampler.cpp) */
+} lgpu_light_strategy;
+
+/* reference: PinholeCamera/ThinLensCamera/SphericalCamera (Camera.cpp:7-71),
+ * CircularAperture/BladeAperture (Aperture.cpp:5-27). Axes are given
+ * explicitly; lgpu_camera_set_rotation() fills them as Camera::SetRotation. */
+typedef enum lgpu_camera_type {
+  LGPU_CAMERA_PINHOLE = 0, LGPU_CAMERA_THIN_LENS = 1, LGPU_CAMERA_SPHERICAL = 2
+} lgpu_camera_type;
+typedef enum lgpu_aperture_type {
+  LGPU_APERTURE_NONE = 0, LGPU_APERTURE_CIRCULAR = 1, LGPU_APERTURE_BLADE = 2
+} lgpu_aperture_type;
+
+typedef struct lgpu_camera {
+  uint32_t type;
+  float origin[3];
+  float x_hat[3], y_hat[3], z_hat[3];
+  float tan_fov;            /* tan(fov / 2) */
+  float aspect;             /* height / width */
+  float focal_length;       /* thin lens */
+  uint32_t aperture;        /* lgpu_aperture_type */
+  float aperture_size;
+  uint32_t blades;
+  float offset_phi, offset_theta; /* spherical */
+} lgpu_camera;
+
+typedef enum lgpu_integrator {
+  LGPU_INTEGRATOR_PATH = 0,          /* PathIntegrator.cpp:15-88 */
+  LGPU_INTEGRATOR_DIRECT = 1,        /* DirectLightingIntegrator.cpp:13-29 */
+  LGPU_INTEGRATOR_VOLPATH = 2,       /* VolumetricPathIntegrator.cpp:20-162 */
+  LGPU_INTEGRATOR_MIS_VOLPATH = 3    /* MISVolumetricPathIntegrator.cpp:71-215 */
+} lgpu_integrator;
+
+typedef enum lgpu_spectral_mode {
+  LGPU_SPECTRAL_RGB = 0,   /* the reference's RGBSpectrum (Spectrum.h:24) */
+  LGPU_SPECTRAL_HERO4 = 1  /* 4 hero wavelengths per path over [360, 830) nm */
+} lgpu_spectral_mode;
+
+typedef enum lgpu_estimator_mode {
+  LGPU_ESTIMATOR_FAITHFUL = 0,  /* reference semantics, defects included */
+  LGPU_ESTIMATOR_CORRECTED = 1  /* defects fixed (unbiased estimator) */
+} lgpu_estimator_mode;
+
+typedef struct lgpu_render_desc {
+  lgpu_scene scene;
+  lgpu_film film;
+  lgpu_camera camera;
+  uint32_t integrator;      /* lgpu_integrator */
+  uint32_t spp_begin;       /* sample indices [spp_begin, spp_end) per pixel */
+  uint32_t spp_end;
+  uint32_t max_bounces;     /* PathIntegrator::maxBounces */
+  uint32_t min_bounces;     /* PathIntegrator::minBounces (RR starts after) */
+  uint32_t spectral_mode;
+  uint32_t estimator_mode;
+  uint32_t seed;            /* Philox key word 0 */
+  uint32_t path_capacity;   /* paths in flight; 0 = choose */
+  uint32_t flags;           /* LGPU_RENDER_COUNT_TRAVERSAL: count BVH nodes / triangles visited */
+} lgpu_render_desc;
+
+#define LGPU_RENDER_COUNT_TRAVERSAL 1u
+
+typedef struct lgpu_render_stats {
+  double ms_total;          /* host wall time of the call */
+  double ms_camera, ms_extend, ms_shade, ms_shadow; /* summed kernel times */
+  uint64_t samples;
+  uint64_t extension_rays;
+  uint64_t shadow_rays;
+  uint64_t nodes_visited;   /* with LGPU_RENDER_COUNT_TRAVERSAL: inner BVH2 nodes fetched (extension rays) */
+  uint64_t tris_tested;     /* with LGPU_RENDER_COUNT_TRAVERSAL: triangles tested (extension rays) */
+  uint32_t waves;           /* path-pool refills */
+  uint32_t launches_extend, launches_shade, launches_shadow;
+} lgpu_render_stats;
+
+/* ---------------- device (Lambda.h:37-40 lambdaCreateDevice/Release) ---- */
+lgpu_status lgpu_device_create(int hip_device, lgpu_device *out);
+lgpu_status lgpu_device_destroy(lgpu_device dev);
+/* Message of the last failing call on this thread. */
+const char *lgpu_last_error(void);
+uint32_t lgpu_abi_version(void);
+
+/* ---------------- scene (Scene.cpp:18-26,109-121; Lambda.cpp:168-244) --- */
+lgpu_status lgpu_scene_create(lgpu_device dev, lgpu_scene *out);
+lgpu_status lgpu_scene_destroy(lgpu_scene scene);
+lgpu_status lgpu_scene_add_texture(lgpu_scene scene, const lgpu_texture_desc *desc, uint32_t *id);
+lgpu_status lgpu_scene_add_medium(lgpu_scene scene, const lgpu_medium_desc *desc, uint32_t *id);
+/* Material.cpp:94-195 lambdaCreateShaderNode + Material: graph -> bytecode */
+lgpu_status lgpu_scene_add_material(lgpu_scene scene, const lgpu_material_desc *desc, uint32_t *id);
+/* Scene::AddObject with TriangleMesh; an emissive material makes a MeshLight */
+lgpu_status lgpu_scene_add_mesh(lgpu_scene scene, const lgpu_mesh_desc *desc, uint32_t *object_id);
+/* Scene::AddLight for point / spot / environment lights */
+lgpu_status lgpu_scene_add_light(lgpu_scene scene, const lgpu_light_desc *desc, uint32_t *id);
+/* Scene::Commit + LightSampler::Commit: builds BVH and light sampler, uploads. */
+lgpu_status lgpu_scene_commit(lgpu_scene scene, uint32_t light_strategy);
+/* Introspection for tests: counts after commit. */
+lgpu_status lgpu_scene_info(lgpu_scene scene, uint32_t *num_triangles, uint32_t *num_bvh_nodes,
+                            uint32_t *num_lights, uint32_t *num_tree_nodes);
+/* Light-tree export for tests: per node {center xyz, diag, axis xyz, thetaO,
+ * thetaE, power, child0, child1 (as float bits), first, count, parent} = 15 floats. */
+lgpu_status lgpu_scene_light_tree(lgpu_scene scene, float *nodes, uint32_t max_nodes);
+
+/* ---------------- camera helper (Camera.cpp:14-35) ---------------------- */
+void lgpu_camera_init(lgpu_camera *cam, uint32_t type, const float origin[3], float width, float height);
+void lgpu_camera_set_rotation(lgpu_camera *cam, float phi, float theta);
+void lgpu_camera_set_forwards(lgpu_camera *cam, const float forwards[3]);
+void lgpu_camera_set_fov(lgpu_camera *cam, float fov);
+
+/* ---------------- film (Film.h:19-45; Lambda.h:121-126) ----------------- */
+lgpu_status lgpu_film_create(lgpu_device dev, uint32_t width, uint32_t height, lgpu_film *out);
+lgpu_status lgpu_film_destroy(lgpu_film film);
+lgpu_status lgpu_film_clear(lgpu_film film);
+/* FP64 accumulation: rgb_sum[3*px] and sample counts[px]. Either may be NULL. */
+lgpu_status lgpu_film_read(lgpu_film film, double *rgb_sum, uint32_t *counts);
+/* Film::ToRGBTexture: RGBA32F mean per pixel (alpha = 1). */
+lgpu_status lgpu_film_read_rgba(lgpu_film film, float *rgba);
+
+/* ---------------- render (MosaicRenderer::Render / TileRenderer /
+ * lambdaStartProgressiveRenderer: Render.cpp:43-86, MosaicRenderer.cpp:22-80) */
+lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
+
+/* Per-path radiance for an explicit list of (pixel, sample) pairs. out holds
+ * 5 floats per path: radiance lanes 0..3 (RGB in lanes 0..2, or the four hero
+ * wavelengths) and lambda0 (HERO4) or 0. desc->film may be NULL. */
+lgpu_status lgpu_trace_paths(const lgpu_render_desc *desc, uint32_t num_paths,
+                             const uint32_t *pixels, const uint32_t *samples, float *out);
+
+/* Closest-hit queries (Scene::Intersect, Scene.cpp:28-43): rays are 8 floats
+ * {ox,oy,oz,tmin, dx,dy,dz,tmax}; hits are 4 words {t, u, v, prim} with
+ * prim = LGPU_NONE on a miss (t, u, v as float bits). */
+lgpu_status lgpu_intersect(lgpu_scene scene, uint32_t num_rays, const float *rays, uint32_t *hits);
+/* Any-hit queries (Scene::MutualVisibility, Scene.cpp:83-99): out[i] = 1 if occluded. */
+lgpu_status lgpu_occluded(lgpu_scene scene, uint32_t num_rays, const float *rays, uint32_t *occluded);
+
+/* Philox4x32-10 on the device: counters (4 words each) and one key (2 words) -> 4 words each. */
+lgpu_status lgpu_philox(lgpu_device dev, uint32_t n, const uint32_t *counters, const uint32_t key[2], uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAMBDA_GPU_H */

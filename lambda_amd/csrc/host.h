@@ -1,0 +1,105 @@
+// Host-side objects behind the opaque C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lambda_gpu.h"
+#include "lgpu_internal.h"
+
+namespace lgpu {
+
+void SetError(const std::string &msg);
+lgpu_status Fail(lgpu_status s, const std::string &msg);
+#define LGPU_HIP(call)                                                                             \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) return ::lgpu::Fail(LGPU_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// A device allocation owned by the host object.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  hipError_t upload(const void *src, size_t n) {
+    release();
+    if (!n) return hipSuccess;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) return e;
+    bytes = n;
+    return hipMemcpy(ptr, src, n, hipMemcpyHostToDevice);
+  }
+  hipError_t alloc(size_t n) {
+    if (n <= bytes && ptr) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct HostMaterial {
+  std::vector<lgpu_node> nodes;
+  lgpu_material_desc desc;
+};
+struct HostMesh {
+  std::vector<float> pos, nrm, tan, uv;
+  std::vector<uint32_t> idx;
+  uint32_t material;
+};
+struct HostLight {
+  lgpu_light_desc desc;
+  std::vector<float> env;
+  int mesh = -1;  // mesh lights
+};
+
+}  // namespace lgpu
+
+struct lgpu_device_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  void *wave = nullptr;               // path-state pool of the render loop
+  void (*wave_free)(void *) = nullptr;
+};
+
+struct lgpu_film_s {
+  lgpu_device dev = nullptr;
+  uint32_t width = 0, height = 0;
+  lgpu::DevBuf sum;    // double x3 per pixel
+  lgpu::DevBuf count;  // uint32 per pixel
+};
+
+struct lgpu_scene_s {
+  lgpu_device dev = nullptr;
+  bool committed = false;
+  std::vector<lgpu::HostMaterial> materials;
+  std::vector<lgpu::HostMesh> meshes;
+  std::vector<lgpu::HostLight> lights;  // scene light order (mesh lights interleaved)
+  std::vector<std::vector<float>> textures;
+  std::vector<uint32_t> tex_dims;        // w, h, ch per texture
+  std::vector<lgpu_medium_desc> media;
+  std::vector<std::vector<float>> media_density;
+
+  // host copies kept for introspection
+  std::vector<lgpu::TreeNode> tree;
+  uint32_t num_bvh_nodes = 0;
+  uint32_t bvh_depth = 0;
+  uint32_t num_tris = 0;
+  uint32_t num_tlights = 0;
+
+  lgpu::DevScene dsc{};
+  std::vector<lgpu::DevBuf> bufs;  // owns every array referenced by dsc
+};
+
+namespace lgpu {
+lgpu_status CommitScene(lgpu_scene s, uint32_t strategy);
+}

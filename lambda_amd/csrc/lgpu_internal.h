@@ -1,0 +1,253 @@
+// Internal data layouts shared by the host scene builder (C++) and the HIP
+// kernels. Everything here lives in HBM as flat arrays; the kernels read them
+// through the DevScene pointer bundle passed by value.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lgpu {
+
+constexpr float kPi = 3.14159265358979323846264338f;
+constexpr float kSurfaceEpsilon = 1e-5f;  // ScatterEvent.h:12
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kLdsStack = 32;             // per-thread BVH stack entries held in LDS
+constexpr int kMaxBvhDepth = 64;          // builder guarantees depth <= this
+
+// ---------------------------------------------------------------- BVH2
+// Two child boxes per node (64 B, one node = 4 x 16-B loads):
+//   a = {c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y}
+//   b = {c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y}
+//   c = {c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z}
+//   d = {child0, child1, -, -}; a child >= 0 is an inner node index, a child
+//       with the top bit set is a leaf: bits 0..25 first triangle (BVH order),
+//       bits 26..30 triangle count.
+struct BvhNode {
+  float a[4], b[4], c[4];
+  uint32_t child[2], pad[2];
+};
+constexpr uint32_t kLeafBit = 0x80000000u;
+inline uint32_t MakeLeaf(uint32_t first, uint32_t count) { return kLeafBit | (count << 26) | first; }
+
+// Triangles in BVH order: {v0.xyz, prim}, {e1.xyz, 0}, {e2.xyz, 0} with
+// e1 = v1 - v0, e2 = v2 - v0 (48 B).
+struct BvhTri {
+  float v0[3];
+  uint32_t prim;
+  float e1[4];
+  float e2[4];
+};
+
+// ---------------------------------------------------------------- geometry for shading
+struct MeshRec {
+  uint32_t material;
+  uint32_t first_tri;   // global triangle index of local triangle 0
+  uint32_t num_tris;
+  uint32_t flags;       // bit0: smooth normals+tangents, bit1: uvs
+  uint32_t mesh_light;  // scene light index or kNone
+  uint32_t first_vtx;
+  uint32_t pad[2];
+};
+constexpr uint32_t kMeshSmooth = 1u, kMeshUV = 2u;
+
+// ---------------------------------------------------------------- lights
+enum LightKind : uint32_t { kLightMesh = 0, kLightTri = 1, kLightPoint = 2, kLightSpot = 3, kLightEnv = 4 };
+struct LightRec {
+  uint32_t kind;
+  uint32_t mesh;        // mesh index (mesh / triangle lights)
+  uint32_t tri;         // triangle light: local triangle index
+  uint32_t parent;      // triangle light: scene index of its MeshLight
+  uint32_t cdf_off;     // mesh light: offset of its triangle-area Piecewise1D in dist pool
+  uint32_t count;       // mesh light: number of triangles
+  float intensity;
+  float temperature;
+  float rgb[4];         // point/spot colour (RGB-mode blackbody precomputed)
+  float pos[4];
+  float axis[4];
+  float cos_cone, cos_falloff, power, pad;
+};
+
+// Piecewise1D (Piecewise.cpp:9-44) in a flat pool: [n, integral, pdf[n], cdf[n+1]]
+// stored as floats starting at an offset; n stored as float bits.
+
+// Light tree (ManyLightSampler.h:48-76), flattened depth-first, left child first.
+struct TreeNode {
+  float center[3];
+  float diag;
+  float axis[3];
+  float thetaO;
+  float thetaE;
+  float power;
+  uint32_t child[2];    // kNone for leaves
+  uint32_t first;       // leaf: first position in the tree light order
+  uint32_t count;
+  uint32_t parent;      // kNone at the root
+  uint32_t leaf_dist;   // leaf: offset of its Piecewise1D in the dist pool
+};
+
+struct EnvRec {
+  uint32_t w, h;
+  uint32_t tex_off;     // float offset of RGB data in the texel pool
+  uint32_t marg_off;    // Piecewise1D offset of the marginal
+  uint32_t cond_off;    // offset of conditional 0; conditional v at cond_off + v * cond_stride
+  uint32_t cond_stride;
+  float offset[2];
+  float intensity;
+  float radius;
+  uint32_t light;       // scene light index
+  uint32_t pad;
+};
+
+// ---------------------------------------------------------------- materials / bytecode
+// Socket value types (ShaderGraph SocketType)
+enum ValType : uint32_t { kScalar = 0, kVec2 = 1, kVec3 = 2, kColour = 3, kSpectrum = 4, kBxdf = 5 };
+
+// VM instructions: 16 B. Registers are float4; programs are straight-line.
+enum Op : uint32_t {
+  kOpConst = 1,      // r[dst] = consts[imm]
+  kOpUV = 2,         // r[dst] = (u, v, 0, 0)
+  kOpTex = 3,        // r[dst] = bilinear(texture imm, fract(uv))
+  kOpChecker = 4,    // r[dst].x = checker(uv * consts[imm].x)
+  kOpFresnel = 5,    // r[dst].x = FrDielectric(woL.y, 1, consts[imm].x)
+  kOpSMaths = 6,     // r[dst].x = r[a].x op r[b].x ; op in c
+  kOpVMaths = 7,     // r[dst].xyz = r[a] op r[b] ; op in c
+  kOpDot = 8,
+  kOpLength = 9,
+  kOpMerge = 10,     // r[dst] = (r[a].x, r[b].x, r[c].x)
+  kOpSeparate = 11,  // r[dst].x = r[a][c]
+  kOpBlackbody = 12, // r[dst] = normalised Planck at the hero wavelengths of T = r[a].x
+  kOpToReal = 13,    // r[dst].x = AsReal(r[a], type c)
+  kOpToColour = 14,  // r[dst] = AsColour(r[a], type c)
+  kOpToVec3 = 15,
+  kOpToSpectrum = 16 // r[dst] = AsSpectrum(r[a], type c)
+};
+struct Instr {
+  uint32_t op, dst, a, b, c, imm, pad[2];
+};
+constexpr int kMaxRegs = 16;
+
+// A socket read by a lobe or an emitter, already converted to the type the
+// reader requests (GetAs<Real>, GetAs<Vec2> or GetAsSpectrum).
+enum SocketKind : uint32_t {
+  kSockScalar = 0,      // value.x (as spectrum: broadcast)
+  kSockColour = 1,      // value.rgb; as spectrum: RGB in RGB mode, uplift at the hero wavelengths
+  kSockBlackbody = 2,   // value.rgb = RGB-mode blackbody, value.w = temperature
+  kSockVec2 = 3,        // value.xy (anisotropic roughness)
+  kSockProgram = 4      // run code[prog_off .. +prog_len); result in r[res]
+};
+struct Socket {
+  float value[4];
+  uint32_t kind;
+  uint32_t prog_off, prog_len, res;
+  uint32_t res_type;    // ValType of r[res] for program sockets
+  uint32_t pad[3];
+};
+
+enum LobeKind : uint32_t {
+  kLambertian = 64, kOrenNayarR = 65, kOrenNayarT = 66, kMicrofacet = 67, kFresnelBsdf = 68,
+  kSpecularR = 69, kSpecularT = 70, kGhost = 71, kMix = 72, kRoughDielectric = 73
+};
+struct Lobe {
+  uint32_t kind;
+  uint32_t sock[3];     // socket indices (albedo, roughness/sigma/ior/alpha/ratio ...)
+  uint32_t child[2];    // mix children (lobe indices)
+  float p[10];          // constant parameters (microfacet p[0..9], ior, etaT)
+};
+
+struct MaterialRec {
+  uint32_t lobe;        // root lobe index, kNone = pass-through
+  uint32_t emission;    // socket index, kNone = not emissive
+  float intensity;
+  uint32_t med_in, med_out;
+  uint32_t pad[3];
+};
+
+// ---------------------------------------------------------------- path state (SoA)
+enum PathFlags : uint32_t {
+  kFlagCamera = 1u,        // ray came from the camera (or pass-through at depth 0)
+  kFlagEmitOnly = 2u,      // terminated (RR / max depth): gather emission at the next hit only
+  kFlagSpecular = 4u,      // last BSDF sample was a delta lobe
+  kFlagNoEmit = 8u,        // pass-through at depth > 0: no emission at the next hit
+  kFlagRR = 16u,           // RR survived: divide beta by (1 - q) at the next vertex
+  kFlagDead = 32u
+};
+
+struct PathState {
+  float4 *ray_o;     // xyz
+  float4 *ray_d;     // xyz
+  float4 *beta;
+  float4 *L;
+  float4 *hit;       // {t, u, v, prim bits}
+  uint4 *ids;        // {pixel, sample, dim, (depth << 8) | flags}
+  float4 *mis;       // {scatter pdf, light dist pdf, q, lambda0}
+  uint32_t *nee_light;  // {kind:1 | index} of the light picked for NEE at the previous vertex
+  float4 *prev_p;    // shading point of the previous vertex
+  float4 *prev_n;    // shading normal of the previous vertex
+  uint32_t *pass;    // pass-through count
+};
+
+struct ShadowRay {
+  float o[3];
+  float tmax;
+  float d[3];
+  uint32_t path;
+  float contrib[4];
+};
+
+struct DevScene {
+  const BvhNode *nodes;
+  const BvhTri *tris;
+  const uint4 *tri_vtx;      // {i0, i1, i2, mesh} global vertex indices
+  const float4 *vtx_pos;
+  const float4 *vtx_nrm;
+  const float4 *vtx_tan;
+  const float2 *vtx_uv;
+  const MeshRec *meshes;
+  const MaterialRec *materials;
+  const Lobe *lobes;
+  const Socket *sockets;
+  const Instr *code;
+  const float4 *consts;
+  const float *texels;
+  const uint4 *textures;     // {w, h, channels, offset}
+  const LightRec *lights;    // scene lights (add order)
+  const LightRec *tlights;   // tree lights
+  const uint32_t *tree_order;// leaf position -> tree light index
+  const TreeNode *tree;
+  const uint32_t *tri_tlight;// global triangle -> tree light (kNone)
+  const uint32_t *tlight_leaf; // tree light -> (leaf node index)
+  const uint32_t *tlight_slot; // tree light -> position inside its leaf
+  const float *dist;         // Piecewise1D pool
+  EnvRec env;
+  uint32_t has_env;
+  uint32_t env_light;        // scene light index of the environment light
+  uint32_t strategy;         // 0 power, 1 tree
+  uint32_t num_lights;
+  uint32_t power_dist;       // Piecewise1D offset (power sampler)
+  float inv_total_power;
+  float tree_power, inf_power;
+  uint32_t num_tree_nodes;
+  uint32_t num_tris;
+};
+
+struct CameraRec {
+  uint32_t type;
+  float origin[3], x_hat[3], y_hat[3], z_hat[3];
+  float tan_fov, aspect, focal_length;
+  uint32_t aperture;
+  float aperture_size;
+  uint32_t blades;
+  float offset_phi, offset_theta;
+};
+
+struct RenderParams {
+  CameraRec cam;
+  uint32_t width, height;
+  uint32_t spp_begin;
+  uint32_t max_bounces, min_bounces;
+  uint32_t mode;        // 0 RGB, 1 HERO4
+  uint32_t corrected;   // 0 faithful, 1 corrected
+  uint32_t seed;
+  uint32_t integrator;
+};
+
+}  // namespace lgpu

@@ -1,0 +1,463 @@
+// Device-side shading: hit processing, BxDF lobes, light sampling and the
+// light tree. Each function restates the reference function it cites with the
+// same floating-point operation order.
+#pragma once
+#include "dmath.h"
+#include "lgpu_internal.h"
+#include "vm.h"
+
+namespace lgpu {
+
+struct Hit {
+  V3 point, normalG, normalS, tangent, bitangent;
+  float u, v;
+  float tFar;
+  uint32_t prim, local, mesh;
+};
+struct Ev {  // ScatterEvent.h:18-40
+  V3 wo, wi, woL, wiL;
+  int sidedness;
+  float eta;
+};
+__device__ inline V3 ToLocal(const Hit &h, V3 v) { return WorldToLocal(v, h.tangent, h.normalS, h.bitangent); }
+__device__ inline V3 ToWorld(const Hit &h, V3 v) { return LocalToWorld(v, h.tangent, h.normalS, h.bitangent); }
+
+// Object::Hit + TriangleMesh::ProcessHit (Object.h:27-35, TriangleMesh.cpp:80-114)
+__device__ inline void ProcessHit(const DevScene &sc, V3 o, V3 d, float t, float bu, float bv, uint32_t prim, bool corrected,
+                                  Hit &h) {
+  const uint4 tv = sc.tri_vtx[prim];
+  const MeshRec m = sc.meshes[tv.w];
+  const V3 p0 = ToV3(sc.vtx_pos[tv.x]), p1 = ToV3(sc.vtx_pos[tv.y]), p2 = ToV3(sc.vtx_pos[tv.z]);
+  h.tFar = t;
+  h.point = v3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+  h.normalG = Norm(Cross(p1 - p0, p2 - p0));
+  h.prim = prim;
+  h.local = prim - m.first_tri;
+  h.mesh = tv.w;
+  if (m.flags & kMeshUV) {
+    const float2 a = sc.vtx_uv[tv.x], b = sc.vtx_uv[tv.y], c = sc.vtx_uv[tv.z];
+    h.u = a.x + (b.x - a.x) * bu + (c.x - a.x) * bv;
+    h.v = a.y + (b.y - a.y) * bu + (c.y - a.y) * bv;
+  } else {
+    h.u = 0.f;
+    h.v = 0.f;
+  }
+  if (m.flags & kMeshSmooth) {
+    const V3 n0 = ToV3(sc.vtx_nrm[tv.x]), n1 = ToV3(sc.vtx_nrm[tv.y]), n2 = ToV3(sc.vtx_nrm[tv.z]);
+    const float4 t0 = sc.vtx_tan[tv.x], t1 = sc.vtx_tan[tv.y], t2 = sc.vtx_tan[tv.z];
+    const V3 T0 = ToV3(t0), T1 = ToV3(t1), T2 = ToV3(t2);
+    h.normalS = Norm(n0 + (n1 - n0) * bu + (n2 - n0) * bv);
+    h.tangent = Norm(T0 + (T1 - T0) * bu + (T2 - T0) * bv);
+    const V3 b0 = Cross(n0, T0) * t0.w, b1 = Cross(n1, T1) * t1.w, b2 = Cross(n2, T2) * t2.w;
+    h.bitangent = Norm(b0 + (b1 - b0) * bu + (b2 - b0) * bv);
+  } else {
+    h.normalS = h.normalG;
+    const V3 c1 = Cross(h.normalG, v3(0, 0, 1));
+    const V3 c2 = Cross(h.normalG, v3(0, 1, 0));
+    h.tangent = (Dot(c1, c1) > Dot(c2, c2) ? c1 : c2);
+    if (corrected) h.tangent = Norm(h.tangent);  // unnormalised in the reference (TriangleMesh.cpp:108-111)
+    h.bitangent = Cross(h.tangent, h.normalG);
+  }
+}
+
+// ---------------------------------------------------------------- BxDF helpers (ScatterEvent.h:44-98, Sampling.h)
+__device__ inline bool SameHemisphere(V3 a, V3 b) { return a.y * b.y > 0; }
+__device__ inline float CosineHemispherePdf(V3 wo, V3 wi) { return SameHemisphere(wo, wi) ? fabsf(wi.y) * kInvPi : 0.f; }
+__device__ inline float Cos2Theta(V3 w) { return w.y * w.y; }
+__device__ inline float Sin2Theta(V3 w) { return fmaxf(0.f, 1.f - Cos2Theta(w)); }
+__device__ inline float SinTheta(V3 w) { return sqrtf(Sin2Theta(w)); }
+__device__ inline float TanTheta(V3 w) { return SinTheta(w) / w.y; }
+__device__ inline float Tan2Theta(V3 w) { return Sin2Theta(w) / Cos2Theta(w); }
+__device__ inline float CosPhi(V3 w) { const float s = SinTheta(w); return (s < 0.000001f) ? 0.f : Clampf(w.x / s, -1.f, 1.f); }
+__device__ inline float SinPhi(V3 w) { const float s = SinTheta(w); return (s < 0.000001f) ? 0.f : Clampf(w.z / s, -1.f, 1.f); }
+__device__ inline float Cos2Phi(V3 w) { return CosPhi(w) * CosPhi(w); }
+__device__ inline float Sin2Phi(V3 w) { return SinPhi(w) * SinPhi(w); }
+__device__ inline V3 Reflect(V3 w, V3 n) { return w - n * 2.f * Dot(n, w); }
+__device__ inline bool Refract(V3 w, V3 n, float eta, V3 *wr) {
+  const float cosI = Dot(w, n);
+  const float sin2I = fmaxf(0.f, 1.f - cosI * cosI);
+  const float sin2T = eta * eta * sin2I;
+  if (sin2T >= 1) { *wr = v3(-1, 1, -1) * w; return false; }
+  const float cosT = sqrtf(1 - sin2T);
+  *wr = w * -eta + n * (eta * cosI - cosT);
+  return true;
+}
+__device__ inline void SampleUnitDisk(float ux, float uy, float *x, float *y) {
+  const float r = sqrtf(ux);
+  const float theta = uy * kPi2;
+  *x = r * cosf(theta);
+  *y = r * sinf(theta);
+}
+__device__ inline V3 SampleCosineHemisphere(float ux, float uy) {
+  float dx, dy;
+  SampleUnitDisk(ux, uy, &dx, &dy);
+  const float up = sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy));
+  return v3(dx, up, dy);
+}
+
+__device__ inline Spec FrConductor(float cosI, Spec etaI, Spec etaT, Spec k) {  // Fresnel.cpp:23-45
+  cosI = Clampf(cosI, -1.f, 1.f);
+  const Spec eta = etaT / etaI;
+  const Spec etak = k / etaI;
+  const float c2 = cosI * cosI;
+  const float s2 = 1.f - c2;
+  const Spec eta2 = eta * eta;
+  const Spec etak2 = etak * etak;
+  const Spec t0 = eta2 - etak2 - s2;
+  const Spec a2pb2 = SqrtS(t0 * t0 + eta2 * 4.f * etak2);
+  const Spec t1 = a2pb2 + c2;
+  const Spec a = SqrtS((a2pb2 + t0) * .5f);
+  const Spec t2 = a * cosI * 2.f;
+  const Spec rS = (t1 - t2) / (t1 + t2);
+  const Spec t3 = a2pb2 * c2 + s2 * s2;
+  const Spec t4 = t2 * s2;
+  const Spec rP = rS * (t3 - t4) / (t3 + t4);
+  return (rP + rS) * .5f;
+}
+
+// MicrofacetDistribution.cpp:11-71 (ggx = TrowbridgeReitz)
+__device__ inline float MfD(bool ggx, V3 wh, float ax, float ay) {
+  const float tan2 = Tan2Theta(wh);
+  if (isinf(tan2)) return 0.f;
+  const float cos4 = Cos2Theta(wh) * Cos2Theta(wh);
+  if (!ggx) return expf(-tan2 * (Cos2Phi(wh) / (ax * ax) + Sin2Phi(wh) / (ay * ay))) / (kPi * ax * ay * cos4);
+  const float e = (Cos2Phi(wh) / (ax * ax) + Sin2Phi(wh) / (ay * ay)) * tan2;
+  return 1.f / (kPi * ax * ay * cos4 * (1 + e) * (1 + e));
+}
+__device__ inline float MfLambda(bool ggx, V3 w, float ax, float ay) {
+  const float absTan = fabsf(TanTheta(w));
+  if (isinf(absTan)) return 0.f;
+  const float alpha = sqrtf(Cos2Phi(w) * ax * ax + Sin2Phi(w) * ay * ay);
+  if (!ggx) {
+    const float a = 1 / (alpha * absTan);
+    if (a >= 1.6f) return 0.f;
+    return (1 - 1.259f * a + 0.396f * a * a) / (3.535f * a + 2.181f * a * a);
+  }
+  const float a2t2 = (alpha * absTan) * (alpha * absTan);
+  return (-1 + sqrtf(1.f + a2t2)) / 2;
+}
+__device__ inline float MfG(bool ggx, V3 wo, V3 wi, float ax, float ay) {
+  return 1.f / (1.f + MfLambda(ggx, wo, ax, ay) + MfLambda(ggx, wi, ax, ay));
+}
+__device__ inline float MfPdf(bool ggx, V3 wh, float ax, float ay) { return MfD(ggx, wh, ax, ay) * fabsf(wh.y); }
+__device__ inline V3 MfSampleWh(bool ggx, float ux, float uy, V3 wo, float ax) {
+  if (!ggx) {
+    const float lg = ux == 1 ? 0.f : logf(1 - ux);
+    const float phi = kPi2 * uy;
+    const float tan2 = -ax * ax * lg;
+    const float cosT = 1 / sqrtf(1 + tan2);
+    const float sinT = sqrtf(fmaxf(0.f, 1 - cosT * cosT));
+    const V3 h = SphericalDirection(sinT, cosT, phi);
+    return !SameHemisphere(wo, h) ? -h : h;
+  }
+  const float phi = kPi2 * uy;
+  const float tan2 = ax * ax * ux / (1 - ux);
+  const float cosT = 1 / sqrtf(1 + tan2);
+  const float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT));
+  V3 wh = SphericalDirection(sinT, cosT, phi);
+  if (!SameHemisphere(wo, wh)) wh = -wh;
+  return wh;
+}
+__device__ inline float ToAlpha(float r) {  // Microfacet.cpp:49-55
+  r = fmaxf(r, 1e-3f);
+  const float x = logf(r);
+  return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+__device__ inline float OrenNayarTerm(V3 wiL, V3 woL, float sigma) {  // OrenNayar.cpp:10-21
+  const float thetaI = acosf(Clampf(wiL.y, -1.f, 1.f));
+  const float thetaO = acosf(Clampf(woL.y, -1.f, 1.f));
+  const float alpha = fmaxf(thetaI, thetaO);
+  const float beta = fminf(thetaI, thetaO);
+  const float phiI = acosf(CosPhi(wiL));
+  const float phiO = acosf(CosPhi(woL));
+  const float s2 = sigma * sigma;
+  const float A = 1.f - s2 / (2.f * (s2 + .33f));
+  const float B = (.45f * s2) / (s2 + .09f);
+  return A + B * fmaxf(0.f, cosf(phiI - phiO) * Clampf(sinf(alpha), -1.f, 1.f) * Clampf(tanf(beta), -1.f, 1.f));
+}
+// Rough dielectric (project extension: Walter et al. 2007, isotropic GGX)
+__device__ inline void RoughDielectricEval(V3 wo, V3 wi, float alpha, float ior, float *fOut, float *pdfOut) {
+  *fOut = 0.f;
+  *pdfOut = 0.f;
+  if (wo.y == 0 || wi.y == 0) return;
+  const bool reflect = wo.y * wi.y > 0;
+  const float etaI = wo.y > 0 ? 1.f : ior, etaT = wo.y > 0 ? ior : 1.f;
+  V3 wh = reflect ? wo + wi : wo * etaI + wi * etaT;
+  const float l = Mag(wh);
+  if (l == 0) return;
+  wh = wh / l;
+  if (wh.y < 0) wh = -wh;
+  const float dwo = Dot(wo, wh), dwi = Dot(wi, wh);
+  if (!reflect && dwo * dwi >= 0) return;
+  const float F = FrDielectric(dwo, 1.f, ior);
+  const float D = MfD(true, wh, alpha, alpha);
+  const float G = MfG(true, wo, wi, alpha, alpha);
+  const float pwh = D * fabsf(wh.y);
+  if (reflect) {
+    *fOut = D * G * F / (4.f * fabsf(wo.y) * fabsf(wi.y));
+    *pdfOut = pwh * F / (4.f * fabsf(dwo));
+  } else {
+    const float sq = etaI * dwo + etaT * dwi;
+    const float factor = (etaI * etaI) / (etaT * etaT);
+    *fOut = (1.f - F) * D * G * etaT * etaT * fabsf(dwi) * fabsf(dwo) / (fabsf(wo.y) * fabsf(wi.y) * sq * sq) * factor;
+    *pdfOut = pwh * (1.f - F) * etaT * etaT * fabsf(dwi) / (sq * sq);
+  }
+}
+
+// ---------------------------------------------------------------- lobes
+struct Shader {
+  const DevScene *sc;
+  VmCtx vm;
+  bool corrected;
+  __device__ Spec SockS(uint32_t i) const { return i == kNone ? S(0.f) : SocketSpectrum(sc->sockets[i], vm); }
+  __device__ float SockR(uint32_t i) const { return i == kNone ? 0.f : SocketReal(sc->sockets[i], vm); }
+  __device__ void Alpha(const Lobe &L, float *ax, float *ay) const {
+    if (L.sock[1] == kNone) { *ax = *ay = 0.f; }
+    else SocketRoughness(sc->sockets[L.sock[1]], vm, ax, ay);
+    if (L.p[9] == 0.f) { *ax = ToAlpha(*ax); *ay = ToAlpha(*ay); }
+  }
+  __device__ Spec Conductor(const float *rgb) const {
+    if (vm.mode) return S4(UpliftV(rgb[0], rgb[1], rgb[2], vm));
+    Spec s; s.c[0] = rgb[0]; s.c[1] = rgb[1]; s.c[2] = rgb[2]; s.c[3] = 0.f; return s;
+  }
+
+  // BxDF::f for a non-mix lobe
+  __device__ Spec LeafF(const Lobe &L, const Ev &ev) const {
+    switch (L.kind) {
+      case kLambertian: return SockS(L.sock[0]) * kInvPi;  // BxDF.cpp:29-31
+      case kOrenNayarR:
+      case kOrenNayarT: {
+        const float oN = OrenNayarTerm(ev.wiL, ev.woL, SockR(L.sock[1]));
+        return SockS(L.sock[0]) * kInvPi * oN;
+      }
+      case kMicrofacet: {  // Microfacet.cpp:12-25
+        const float cosO = fabsf(ev.woL.y), cosI = fabsf(ev.wiL.y);
+        V3 wh = ev.wiL + ev.woL;
+        if (cosI == 0 || cosO == 0) return S(0.f);
+        if (wh.x == 0 && wh.y == 0 && wh.z == 0) return S(0.f);
+        wh = Norm(wh);
+        const V3 wiL = v3(ev.wiL.x, Clampf(ev.wiL.y, 0.f, 1.f), ev.wiL.z);
+        const bool ggx = L.p[0] != 0.f;
+        const float cosH = Dot(wiL, wh);
+        Spec F;
+        if (L.p[1] != 0.f) {
+          const Spec eta = Conductor(&L.p[3]), k = Conductor(&L.p[6]);
+          F = corrected ? FrConductor(cosH, S(1.f), eta, k) : FrConductor(cosH, eta, S(0.00001f), k);
+        } else {
+          F = corrected ? S(FrDielectric(cosH, 1.f, L.p[2])) : S(FrDielectric(cosH, L.p[2], 0.00001f));
+        }
+        float ax, ay;
+        Alpha(L, &ax, &ay);
+        const float D = MfD(ggx, wh, ax, ay);
+        const float G = MfG(ggx, ev.woL, wiL, ax, ay);
+        return SockS(L.sock[0]) * D * G * F / (4 * cosI * cosO);
+      }
+      case kRoughDielectric: {
+        float f, pdf;
+        RoughDielectricEval(ev.woL, ev.wiL, SockR(L.sock[1]), L.p[0], &f, &pdf);
+        return SockS(L.sock[0]) * f;
+      }
+    }
+    return S(0.f);
+  }
+  __device__ float LeafPdf(const Lobe &L, V3 wo, V3 wi) const {
+    switch (L.kind) {
+      case kLambertian: case kOrenNayarR: case kOrenNayarT: return CosineHemispherePdf(wo, wi);
+      case kMicrofacet: {
+        if (!SameHemisphere(wo, wi)) return 0.f;
+        const V3 wh = Norm(wo + wi);
+        float ax, ay;
+        Alpha(L, &ax, &ay);
+        return MfPdf(L.p[0] != 0.f, wh, ax, ay) / (4 * Dot(wo, wh));
+      }
+      case kRoughDielectric: {
+        float f, pdf;
+        RoughDielectricEval(wo, wi, SockR(L.sock[1]), L.p[0], &f, &pdf);
+        return pdf;
+      }
+    }
+    return 0.f;
+  }
+  __device__ Spec F(uint32_t li, const Ev &ev) const {
+    const Lobe &L = sc->lobes[li];
+    if (L.kind != kMix) return LeafF(L, ev);
+    const float r = SockR(L.sock[2]);  // BxDF.cpp:45-48
+    return LeafF(sc->lobes[L.child[0]], ev) * (1.f - r) + LeafF(sc->lobes[L.child[1]], ev) * r;
+  }
+  __device__ float Pdf(uint32_t li, V3 wo, V3 wi) const {
+    const Lobe &L = sc->lobes[li];
+    if (L.kind != kMix) return LeafPdf(L, wo, wi);
+    const float r = SockR(L.sock[2]);  // BxDF.cpp:62-65
+    return LeafPdf(sc->lobes[L.child[0]], wo, wi) * (1.f - r) + LeafPdf(sc->lobes[L.child[1]], wo, wi) * r;
+  }
+
+  // BxDF::Sample_f for a non-mix lobe; mutates hit point/normal as the reference does.
+  __device__ Spec LeafSample(const Lobe &L, Ev &ev, Hit &h, Rng &rng, float &pdf, bool &spec) const {
+    spec = false;
+    switch (L.kind) {
+      case kLambertian:
+      case kOrenNayarR: {  // BxDF.cpp:7-14
+        const float u0 = rng.Get1D(), u1 = rng.Get1D();
+        ev.wiL = SampleCosineHemisphere(u0, u1);
+        if (ev.woL.y < 0) ev.wiL.y *= -1;
+        pdf = CosineHemispherePdf(ev.woL, ev.wiL);
+        ev.wi = ToWorld(h, ev.wiL);
+        h.point = h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
+        return LeafF(L, ev);
+      }
+      case kOrenNayarT: {  // OrenNayar.cpp:52-59
+        const float u0 = rng.Get1D(), u1 = rng.Get1D();
+        ev.wiL = SampleCosineHemisphere(u0, u1);
+        pdf = CosineHemispherePdf(ev.woL, ev.wiL);
+        ev.wiL.y *= (float)-ev.sidedness;
+        ev.wi = ToWorld(h, ev.wiL);
+        h.point = h.point + h.normalG * (float)-ev.sidedness * kSurfaceEpsilon;
+        return LeafF(L, ev);
+      }
+      case kMicrofacet: {  // Microfacet.cpp:27-38
+        if (ev.woL.y == 0) { pdf = 0.f; return S(0.f); }
+        float ax, ay;
+        Alpha(L, &ax, &ay);
+        const bool ggx = L.p[0] != 0.f;
+        const float u0 = rng.Get1D(), u1 = rng.Get1D();
+        const V3 wh = Norm(MfSampleWh(ggx, u0, u1, ev.woL, ax));
+        ev.wiL = Reflect(-ev.woL, wh);
+        if (!SameHemisphere(ev.woL, ev.wiL)) { pdf = 0.f; return S(0.f); }
+        pdf = MfPdf(ggx, wh, ax, ay) / (4 * Dot(ev.woL, wh));
+        ev.wi = ToWorld(h, ev.wiL);
+        h.point = h.point + h.normalG * kSurfaceEpsilon;
+        return LeafF(L, ev);
+      }
+      case kRoughDielectric: {
+        const float alpha = SockR(L.sock[1]), ior = L.p[0];
+        const float ux = rng.Get1D(), uy = rng.Get1D();
+        const float uc = rng.Get1D();
+        const float phi = kPi2 * uy;
+        const float tan2 = alpha * alpha * ux / (1 - ux);
+        const float cosT = 1 / sqrtf(1 + tan2);
+        const float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT));
+        V3 wh = SphericalDirection(sinT, cosT, phi);
+        if (ev.woL.y < 0) wh = -wh;
+        const float Fr = FrDielectric(Dot(ev.woL, wh), 1.f, ior);
+        if (uc < Fr) {
+          ev.wiL = Reflect(-ev.woL, wh);
+          if (!SameHemisphere(ev.woL, ev.wiL)) { pdf = 0.f; return S(0.f); }
+        } else {
+          const float etaI = ev.woL.y > 0 ? 1.f : ior, etaT = ev.woL.y > 0 ? ior : 1.f;
+          if (!Refract(ev.woL, wh, etaI / etaT, &ev.wiL)) { pdf = 0.f; return S(0.f); }
+          if (SameHemisphere(ev.woL, ev.wiL) || ev.wiL.y == 0) { pdf = 0.f; return S(0.f); }
+        }
+        float f;
+        RoughDielectricEval(ev.woL, ev.wiL, alpha, ior, &f, &pdf);
+        ev.wi = ToWorld(h, ev.wiL);
+        const float side = ev.wiL.y > 0 ? 1.f : -1.f;
+        h.point = h.point + h.normalG * (kSurfaceEpsilon * side * (float)ev.sidedness);
+        return SockS(L.sock[0]) * f;
+      }
+      case kFresnelBsdf: {  // Specular.cpp:18-45
+        spec = true;
+        const bool entering = ev.woL.y > 0;
+        const float ior = SockR(L.sock[1]);
+        const float fr = FrDielectric(ev.woL.y, 1, ior);
+        if (rng.Get1D() < fr) {
+          ev.wiL = v3(-ev.woL.x, ev.woL.y, -ev.woL.z);
+          ev.wi = ToWorld(h, ev.wiL);
+          const float cosT = fabsf(ev.wiL.y);
+          h.point = h.point + h.normalG * (entering ? kSurfaceEpsilon : -kSurfaceEpsilon);
+          pdf = fr;
+          return SockS(L.sock[0]) * fr / cosT;
+        }
+        const float etaI = ev.eta = entering ? 1.f : ior;
+        const float etaT = entering ? ior : 1.f;
+        const V3 nrm = Dot(v3(0, 1, 0), ev.woL) < 0 ? v3(0, -1, 0) : v3(0, 1, 0);
+        if (!Refract(ev.woL, nrm, etaI / etaT, &ev.wiL)) { pdf = 0.f; return S(0.f); }
+        Spec ft = SockS(L.sock[0]) * (1 - fr);
+        ft = ft * ((etaI * etaI) / (etaT * etaT));
+        const float cosT = fabsf(ev.wiL.y);
+        ev.wi = ToWorld(h, ev.wiL);
+        h.point = h.point + h.normalG * (entering ? -kSurfaceEpsilon * 2 : kSurfaceEpsilon * 2);
+        pdf = 1.f - fr;
+        return ft / cosT;
+      }
+      case kSpecularR: {  // Specular.cpp:62-68
+        spec = true;
+        pdf = 1.f;
+        ev.wiL = v3(-ev.woL.x, ev.woL.y, -ev.woL.z);
+        ev.wi = ToWorld(h, ev.wiL);
+        h.point = h.point + h.normalG * kSurfaceEpsilon;
+        return SockS(L.sock[0]) / fabsf(ev.wiL.y);
+      }
+      case kSpecularT: {  // Specular.cpp:76-95
+        spec = true;
+        pdf = 1.f;
+        const V3 woL = ToLocal(h, ev.wo);
+        const bool outside = woL.y > 0;
+        h.normalS = h.normalS * (outside ? 1.f : -1.f);
+        const float etaT0 = L.p[0];
+        const float etaI = outside ? ev.eta : etaT0;
+        const float etaT = outside ? etaT0 : ev.eta;
+        Refract(ev.wo, v3(0, 1, 0) * (outside ? 1.f : -1.f), etaI / etaT, &ev.wiL);
+        const float Fd = FrDielectric(fabsf(ev.wi.y), etaT0, etaT);
+        Spec ft = SockS(L.sock[0]) * (S(1.f) - S(Fd));
+        ft = ft * ((etaI * etaI) / (etaT * etaT));
+        const float inv = 1.f / fabsf(ev.wiL.y);
+        ev.wi = ToWorld(h, ev.wiL);
+        h.point = h.point + h.normalS * .0004f;
+        return ft * inv;
+      }
+      case kGhost: {  // Ghost.cpp:13-28
+        spec = true;
+        pdf = 1.f;
+        const float alpha = SockR(L.sock[0]);
+        const bool inside = ev.woL.y < 0;
+        if (alpha > 0) {
+          ev.wi = -ev.wo;
+          ev.wiL = ToLocal(h, ev.wi);
+          h.point = h.point + h.normalG * kSurfaceEpsilon * 2.f * (inside ? 1.f : -1.f);
+          return S(alpha / fabsf(ev.wiL.y));
+        }
+        pdf = 0.f;
+        return S(0.f);
+      }
+    }
+    pdf = 0.f;
+    return S(0.f);
+  }
+  __device__ Spec Sample(uint32_t li, Ev &ev, Hit &h, Rng &rng, float &pdf, bool &spec) const {
+    const Lobe &L = sc->lobes[li];
+    if (L.kind != kMix) return LeafSample(L, ev, h, rng, pdf, spec);
+    const float r = SockR(L.sock[2]);  // BxDF.cpp:50-60
+    const Lobe &A = sc->lobes[L.child[0]], &B = sc->lobes[L.child[1]];
+    if (rng.Get1D() > r) {
+      const Spec af = LeafSample(A, ev, h, rng, pdf, spec);
+      if (corrected && !spec) pdf = Pdf(li, ev.woL, ev.wiL);
+      else if (corrected) pdf *= (1.f - r);
+      return af * (1.f - r) + LeafF(B, ev) * r;
+    }
+    const Spec bf = LeafSample(B, ev, h, rng, pdf, spec);
+    if (corrected && !spec) pdf = Pdf(li, ev.woL, ev.wiL);
+    else if (corrected) pdf *= r;
+    return bf * r + LeafF(A, ev) * (1.f - r);
+  }
+};
+
+// ---------------------------------------------------------------- light tree importance (ManyLightSampler.cpp:305-326)
+__device__ inline float TreeImportance(const TreeNode &nd, V3 point, V3 normalS) {
+  V3 delta = ToV3(nd.center) - point;
+  const float diam = nd.diag;
+  const float d2 = Dot(delta, delta);
+  const float d = sqrtf(d2);
+  const float invD = 1.f / d;
+  delta = delta * invD;
+  const float theta = acosf(Dot(delta, ToV3(nd.axis)));
+  const float thetaU = atanf((diam * .5f) * invD);
+  const float thetaDash = fmaxf(theta - nd.thetaO - thetaU, 0.f);
+  if (thetaDash < nd.thetaE) {
+    const float E = nd.power;
+    const float thetaI = acosf(Dot(delta, normalS));
+    return (cosf(fmaxf(thetaI - thetaU, 0.f)) * E) / d2 * cosf(thetaDash);
+  }
+  return 0.f;
+}
+
+}  // namespace lgpu

@@ -1,0 +1,225 @@
+"""ctypes binding of liblambda_gpu.so (the C ABI in include/lambda_gpu.h).
+
+The classes mirror the reference's render surface: a Device (lambdaCreateDevice),
+a Scene that takes materials/meshes/lights and is committed with a light
+strategy (Scene::Commit + LightSampler::Commit), a Film (Film.h) and render()
+/ trace_paths() for the render call (MosaicRenderer::Render / TileRenderer).
+There is no CPU fallback: loading fails loudly when the HIP library is absent.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblambda_gpu.so")
+_LIB = None
+
+EXPORTS = [
+    "lgpu_device_create", "lgpu_device_destroy", "lgpu_last_error", "lgpu_abi_version", "lgpu_scene_create",
+    "lgpu_scene_destroy", "lgpu_scene_add_texture", "lgpu_scene_add_medium", "lgpu_scene_add_material",
+    "lgpu_scene_add_mesh", "lgpu_scene_add_light", "lgpu_scene_commit", "lgpu_scene_info", "lgpu_scene_light_tree",
+    "lgpu_camera_init", "lgpu_camera_set_rotation", "lgpu_camera_set_forwards", "lgpu_camera_set_fov",
+    "lgpu_film_create", "lgpu_film_destroy", "lgpu_film_clear", "lgpu_film_read", "lgpu_film_read_rgba",
+    "lgpu_render", "lgpu_trace_paths", "lgpu_intersect", "lgpu_occluded", "lgpu_philox",
+]
+
+
+class LambdaError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("lgpu status %d: %s" % (status, msg))
+        self.status = status
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("liblambda_gpu.so is not built (run __graft_entry__.build() or lambda_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, f32, i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int
+        sigs = {
+            "lgpu_device_create": [i32, vp], "lgpu_device_destroy": [vp], "lgpu_scene_create": [vp, vp],
+            "lgpu_scene_destroy": [vp], "lgpu_scene_add_texture": [vp, vp, vp], "lgpu_scene_add_medium": [vp, vp, vp],
+            "lgpu_scene_add_material": [vp, vp, vp], "lgpu_scene_add_mesh": [vp, vp, vp],
+            "lgpu_scene_add_light": [vp, vp, vp], "lgpu_scene_commit": [vp, u32],
+            "lgpu_scene_info": [vp, vp, vp, vp, vp], "lgpu_scene_light_tree": [vp, vp, u32],
+            "lgpu_film_create": [vp, u32, u32, vp], "lgpu_film_destroy": [vp], "lgpu_film_clear": [vp],
+            "lgpu_film_read": [vp, vp, vp], "lgpu_film_read_rgba": [vp, vp], "lgpu_render": [vp, vp],
+            "lgpu_trace_paths": [vp, u32, vp, vp, vp], "lgpu_intersect": [vp, u32, vp, vp],
+            "lgpu_occluded": [vp, u32, vp, vp], "lgpu_philox": [vp, u32, vp, vp, vp],
+        }
+        for name, args in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.lgpu_last_error.restype = C.c_char_p
+        L.lgpu_abi_version.restype = u32
+        L.lgpu_camera_init.argtypes = [vp, u32, vp, f32, f32]
+        L.lgpu_camera_set_rotation.argtypes = [vp, f32, f32]
+        L.lgpu_camera_set_forwards.argtypes = [vp, vp]
+        L.lgpu_camera_set_fov.argtypes = [vp, f32]
+        _LIB = L
+    return _LIB
+
+
+def _check(status):
+    if status != abi.OK:
+        raise LambdaError(status, lib().lgpu_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Device:
+    """lambdaCreateDevice (Lambda.cpp:148-152): one HIP device and its stream."""
+
+    def __init__(self, index=0):
+        h = C.c_void_p()
+        _check(lib().lgpu_device_create(index, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgpu_device_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def philox(self, counters, key):
+        counters = np.ascontiguousarray(counters, dtype=np.uint32).reshape(-1, 4)
+        key = np.ascontiguousarray(key, dtype=np.uint32)
+        out = np.zeros_like(counters)
+        _check(lib().lgpu_philox(self.h, len(counters), _ptr(counters), _ptr(key), _ptr(out)))
+        return out
+
+
+class Scene:
+    """Scene (Scene.h:25-108) flattened at commit into BVH, light sampler and bytecode."""
+
+    def __init__(self, device):
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().lgpu_scene_create(device.h, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgpu_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _add(self, fn, d):
+        out = C.c_uint32()
+        _check(fn(self.h, C.byref(d), C.byref(out)))
+        return out.value
+
+    def add_texture(self, d):
+        return self._add(lib().lgpu_scene_add_texture, d)
+
+    def add_medium(self, d):
+        return self._add(lib().lgpu_scene_add_medium, d)
+
+    def add_material(self, d):
+        return self._add(lib().lgpu_scene_add_material, d)
+
+    def add_mesh(self, d):
+        return self._add(lib().lgpu_scene_add_mesh, d)
+
+    def add_light(self, d):
+        return self._add(lib().lgpu_scene_add_light, d)
+
+    def commit(self, strategy=abi.LIGHTS_TREE):
+        _check(lib().lgpu_scene_commit(self.h, strategy))
+
+    def info(self):
+        v = [C.c_uint32() for _ in range(4)]
+        _check(lib().lgpu_scene_info(self.h, *[C.byref(x) for x in v]))
+        return dict(triangles=v[0].value, bvh_nodes=v[1].value, lights=v[2].value, tree_nodes=v[3].value)
+
+    def light_tree(self):
+        n = self.info()["tree_nodes"]
+        out = np.zeros((max(n, 1), 15), dtype=np.float32)
+        _check(lib().lgpu_scene_light_tree(self.h, _ptr(out), n))
+        return out[:n]
+
+    def intersect(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        hits = np.zeros((len(rays), 4), dtype=np.uint32)
+        _check(lib().lgpu_intersect(self.h, len(rays), _ptr(rays), _ptr(hits)))
+        return hits
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        occ = np.zeros(len(rays), dtype=np.uint32)
+        _check(lib().lgpu_occluded(self.h, len(rays), _ptr(rays), _ptr(occ)))
+        return occ
+
+
+class Film:
+    """Film (Film.h:19-45) with FP64 per-pixel sums."""
+
+    def __init__(self, device, width, height):
+        self.device, self.width, self.height = device, width, height
+        h = C.c_void_p()
+        _check(lib().lgpu_film_create(device.h, width, height, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgpu_film_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def clear(self):
+        _check(lib().lgpu_film_clear(self.h))
+
+    def read(self):
+        rgb = np.zeros((self.height, self.width, 3), dtype=np.float64)
+        cnt = np.zeros((self.height, self.width), dtype=np.uint32)
+        _check(lib().lgpu_film_read(self.h, _ptr(rgb), _ptr(cnt)))
+        return rgb, cnt
+
+    def read_rgba(self):
+        out = np.zeros((self.height, self.width, 4), dtype=np.float32)
+        _check(lib().lgpu_film_read_rgba(self.h, _ptr(out)))
+        return out
+
+
+def render_desc(scene, film, camera, spp_begin=0, spp_end=1, max_bounces=8, min_bounces=3,
+                spectral=abi.SPECTRAL_RGB, estimator=abi.ESTIMATOR_CORRECTED, seed=0, capacity=0, flags=0,
+                integrator=abi.INTEGRATOR_PATH):
+    d = abi.RenderDesc()
+    d.scene = scene.h.value if scene is not None and hasattr(scene, "h") else None
+    d.film = film.h.value if film is not None else None
+    d.camera = camera
+    d.integrator = integrator
+    d.spp_begin, d.spp_end = spp_begin, spp_end
+    d.max_bounces, d.min_bounces = max_bounces, min_bounces
+    d.spectral_mode, d.estimator_mode = spectral, estimator
+    d.seed, d.path_capacity, d.flags = seed, capacity, flags
+    return d
+
+
+def render(desc):
+    """The render call (MosaicRenderer::Render): accumulates spp [begin, end) into the film."""
+    st = abi.RenderStats()
+    _check(lib().lgpu_render(C.byref(desc), C.byref(st)))
+    return st
+
+
+def trace_paths(desc, pixels, samples):
+    """Per-path radiance (5 floats per path) for explicit (pixel, sample) pairs."""
+    pixels = np.ascontiguousarray(pixels, dtype=np.uint32)
+    samples = np.ascontiguousarray(samples, dtype=np.uint32)
+    out = np.zeros((len(pixels), 5), dtype=np.float32)
+    _check(lib().lgpu_trace_paths(C.byref(desc), len(pixels), _ptr(pixels), _ptr(samples), _ptr(out)))
+    return out

@@ -1,0 +1,2142 @@
+/*
+ * oracle.cpp — CPU restatement of the Zoophish/Lambda render path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker that the HIP product is
+ * compared against (tests/, __graft_entry__.smoke(), bench.py cpu_baseline).
+ * The product library never includes or links it.
+ *
+ * Every function cites the reference file:line it restates. Floating-point
+ * operation order follows the reference, including its SSE conventions:
+ * Vec3 / scalar multiplies by the reciprocal (vec3.h:146-148), Spectrum /
+ * scalar multiplies by the reciprocal (Spectrum.h:128-133), Spectrum / Spectrum
+ * and the RR division divide lane-wise (Spectrum.h:62-66, 104-108).
+ *
+ * Two switches shared with the product:
+ *   spectral mode  RGB   — the reference's RGBSpectrum (Spectrum.h:24)
+ *                  HERO4 — 4 hero wavelengths (project-defined, see below)
+ *   estimator mode FAITHFUL  — the reference's deterministic defects kept
+ *                  CORRECTED — the defects listed at each `corrected` branch fixed
+ *
+ * Deviations that replace undefined or platform-defined behaviour:
+ *   - Halton + blue-noise shifter (HaltonSampler.cpp, SampleShifter.cpp) are
+ *     replaced by Philox4x32-10 keyed on (seed, stream) with counter
+ *     (pixel, sample, dim/4, 0); one output word per dimension.
+ *   - ManyLightSampler keeps triangle lights in an unordered_map
+ *     (ManyLightSampler.cpp:273-281) and splits with std::partition
+ *     (:215); both orders are implementation-defined. Here triangle lights are
+ *     ordered by (mesh, triangle) and the split uses a stable partition.
+ *   - float -> unsigned bucket conversion of NaN offsets (flat axes,
+ *     ManyLightSampler.cpp:144,210) is UB; it is defined as bucket 0.
+ *   - CIE 1931 colour matching uses the analytic multi-lobe fit of Wyman,
+ *     Sloan and Shirley (JCGT 2013) instead of the 471-entry table
+ *     (SPD_Data.cpp), sampled at the same 471 wavelengths.
+ *
+ * parity unpinned: the Embree hit convention (Ng orientation, barycentrics) is
+ * not pinned by any reference test; this oracle defines Ng = (v1-v0)x(v2-v0)
+ * and (u, v) weighting v1 and v2 (maths.h:57-60).
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+static const float PI = 3.14159265358979323846264338f;
+static const float PI2 = 2.f * PI;
+static const float INV_PI = 1.f / PI;
+static const float INV_PI2 = 1.f / (2.f * PI);
+static const float INV_PI4 = 1.f / (4.f * PI);
+static const float SURFACE_EPSILON = 1e-5f;  // ScatterEvent.h:12
+
+// ---------------------------------------------------------------- maths (vec3.h, maths.h)
+struct V2 { float x = 0, y = 0; V2() {} V2(float a, float b) : x(a), y(b) {} };
+struct V3 {
+  float x = 0, y = 0, z = 0;
+  V3() {}
+  V3(float a, float b, float c) : x(a), y(b), z(c) {}
+  explicit V3(const float *p) : x(p[0]), y(p[1]), z(p[2]) {}
+  V3 operator+(const V3 &o) const { return V3(x + o.x, y + o.y, z + o.z); }
+  V3 operator-(const V3 &o) const { return V3(x - o.x, y - o.y, z - o.z); }
+  V3 operator*(const V3 &o) const { return V3(x * o.x, y * o.y, z * o.z); }
+  V3 operator/(const V3 &o) const { return V3(x / o.x, y / o.y, z / o.z); }
+  V3 operator*(float s) const { return V3(x * s, y * s, z * s); }
+  V3 operator/(float s) const { const float inv = 1.f / s; return V3(x * inv, y * inv, z * inv); }  // vec3.h:146
+  V3 operator-() const { return *this * -1.f; }
+  V3 &operator+=(const V3 &o) { x += o.x; y += o.y; z += o.z; return *this; }
+  float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+static inline float Dot(const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 Cross(const V3 &a, const V3 &b) {
+  return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float Mag(const V3 &v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+static inline V3 Norm(const V3 &v) { return v / Mag(v); }
+static inline float DistSq(const V3 &a, const V3 &b) { const V3 d = b - a; return d.x * d.x + d.y * d.y + d.z * d.z; }
+static inline float Clampf(float v, float lo, float hi) { return std::max(std::min(v, hi), lo); }
+static inline float Fract(float r) { return r - std::floor(r); }
+// vec3.h:200-212, y is the normal axis
+static inline V3 WorldToLocal(const V3 &v, const V3 &t, const V3 &n, const V3 &bt) { return V3(Dot(t, v), Dot(n, v), Dot(bt, v)); }
+static inline V3 LocalToWorld(const V3 &v, const V3 &t, const V3 &n, const V3 &bt) { return t * v.x + n * v.y + bt * v.z; }
+static inline V3 SphericalDirection(float sinT, float cosT, float phi) { return V3(sinT * std::cos(phi), cosT, sinT * std::sin(phi)); }
+static inline float SphericalTheta(const V3 &v) { return std::acos(std::max(std::min(v.y, 1.f), -1.f)); }
+static inline float SphericalPhi(const V3 &v) {
+  const float p = std::atan2(v.z, v.x);
+  return (p < 0) ? (float)(p + 3.14159265359 * 2.) : p;  // vec3.h:231-234 (double constant)
+}
+// vec3.h:237-241 (Rodrigues with sqrt(1-cos^2) and an unnormalised axis, as written)
+static inline V3 Rotate(const V3 &v, const V3 &axis, float theta) {
+  const float c = std::cos(theta);
+  const float s = std::sqrt(1.f - c * c);
+  return v * c + Cross(axis, v) * s + axis * Dot(axis, v) * (1.f - c);
+}
+
+struct Bounds {
+  V3 mn, mx;  // default (0,0,0)-(0,0,0), bounds.h:61-64
+  Bounds() {}
+  explicit Bounds(const V3 &p) : mn(p), mx(p) {}
+  Bounds(const V3 &a, const V3 &b) : mn(a), mx(b) {}
+  float SurfaceArea() const { const V3 s = mx - mn; return (s.x * s.y + s.y * s.z + s.x * s.z) * 2.f; }
+  float MaxLength() const { const V3 b = mx - mn; return std::max(std::max(b.x, b.y), b.z); }
+  V3 Center() const { return mn + (mx - mn) * .5f; }
+  float Diagonal() const { return Mag(mx - mn); }
+  V3 Offset(const V3 &p) const { return (p - mn) / (mx - mn); }
+};
+static inline Bounds Union(const Bounds &a, const Bounds &b) {  // bounds.h:108-113
+  return Bounds(V3(std::min(a.mn.x, b.mn.x), std::min(a.mn.y, b.mn.y), std::min(a.mn.z, b.mn.z)),
+                V3(std::max(a.mx.x, b.mx.x), std::max(a.mx.y, b.mx.y), std::max(a.mx.z, b.mx.z)));
+}
+static inline Bounds Union(const Bounds &b, const V3 &p) {  // bounds.h:116-121
+  return Bounds(V3(std::min(p.x, b.mn.x), std::min(p.y, b.mn.y), std::min(p.z, b.mn.z)),
+                V3(std::max(p.x, b.mx.x), std::max(p.y, b.mx.y), std::max(p.z, b.mx.z)));
+}
+
+// maths.h:77-90
+template <class P>
+static inline unsigned FindInterval(int size, const P &pred) {
+  int first = 0, len = size;
+  while (len > 0) {
+    int half = len >> 1, middle = first + half;
+    if (pred(middle)) { first = middle + 1; len -= half + 1; }
+    else len = half;
+  }
+  return (unsigned)std::max(std::min(first - 1, size - 2), 0);
+}
+
+// ---------------------------------------------------------------- spectrum (Spectrum.h:27-182, 306-374)
+enum { RGB = 0, HERO4 = 1 };
+struct Spec {
+  float c[4];
+  Spec(float v = 0.f) { c[0] = c[1] = c[2] = c[3] = v; }
+  Spec operator+(const Spec &o) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] + o.c[i]; return r; }
+  Spec operator-(const Spec &o) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] - o.c[i]; return r; }
+  Spec operator*(const Spec &o) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] * o.c[i]; return r; }
+  Spec operator/(const Spec &o) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] / o.c[i]; return r; }
+  Spec operator*(float s) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] * s; return r; }
+  Spec operator/(float s) const { const float inv = 1.f / s; Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] * inv; return r; }
+  Spec operator+(float s) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] + s; return r; }
+  Spec operator-(float s) const { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = c[i] - s; return r; }
+  Spec &operator+=(const Spec &o) { for (int i = 0; i < 4; ++i) c[i] += o.c[i]; return *this; }
+  Spec &operator*=(const Spec &o) { for (int i = 0; i < 4; ++i) c[i] *= o.c[i]; return *this; }
+};
+static inline Spec SqrtS(const Spec &s) { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = std::sqrt(s.c[i]); return r; }
+
+struct Lanes { int mode; float lambda[4]; };
+static inline int NL(const Lanes &l) { return l.mode == HERO4 ? 4 : 3; }
+static inline bool IsBlack(const Spec &s, const Lanes &l) {
+  for (int i = 0; i < NL(l); ++i) if (s.c[i] != 0) return false;
+  return true;
+}
+// RGBSpectrum::y (Spectrum.h:328-331); HERO4: mean over the hero wavelengths
+static inline float Luminance(const Spec &s, const Lanes &l) {
+  if (l.mode == HERO4) return (s.c[0] + s.c[1] + s.c[2] + s.c[3]) * .25f;
+  return 0.212671f * s.c[0] + 0.715160f * s.c[1] + 0.072169f * s.c[2];
+}
+
+// Analytic CIE 1931 fit (Wyman, Sloan, Shirley 2013, multi-lobe piecewise Gaussians)
+static inline float Lobe(float x, float mu, float s1, float s2) {
+  const float t = (x - mu) / (x < mu ? s1 : s2);
+  return std::exp(-.5f * t * t);
+}
+static inline void CMF(float l, float xyz[3]) {
+  xyz[0] = 1.056f * Lobe(l, 599.8f, 37.9f, 31.0f) + 0.362f * Lobe(l, 442.0f, 16.0f, 26.7f) - 0.065f * Lobe(l, 501.1f, 20.4f, 26.2f);
+  xyz[1] = 0.821f * Lobe(l, 568.8f, 46.9f, 40.5f) + 0.286f * Lobe(l, 530.9f, 16.3f, 31.1f);
+  xyz[2] = 1.217f * Lobe(l, 437.0f, 11.8f, 36.0f) + 0.681f * Lobe(l, 459.0f, 26.0f, 13.8f);
+}
+static const double CIE_Y_INTEGRAL = 106.856895;  // SPD_Data.h:22
+
+// Project RGB -> spectrum uplift: smoothstep partition of unity (blue/green
+// edge 470-510 nm, green/red edge 570-610 nm); white (1,1,1) -> 1 everywhere.
+static inline float Smooth01(float a, float b, float x) {
+  float t = (x - a) / (b - a);
+  t = Clampf(t, 0.f, 1.f);
+  return t * t * (3.f - 2.f * t);
+}
+static inline float Uplift(const float rgb[3], float l) {
+  const float t1 = Smooth01(470.f, 510.f, l);
+  const float t2 = Smooth01(570.f, 610.f, l);
+  return rgb[0] * t2 + rgb[1] * (t1 - t2) + rgb[2] * (1.f - t1);
+}
+
+// GraphInputs.cpp:87-104
+static float Planck(float lambdaNm, float T) {
+  const float c = 299792458.f, h = 6.62606957e-34f, kb = 1.3806488e-23f;
+  const float l = lambdaNm * 1e-9f;
+  const float l5 = l * l * l * l * l;
+  return (2.f * h * c * c) / (l5 * (std::exp((h * c) / (l * kb * T)) - 1.f));
+}
+static float BlackbodyNormalised(float lambdaNm, float T) {
+  const float lambdaMax = 2.8977721e-3f / T * 1e9f;
+  return Planck(lambdaNm, T) / Planck(lambdaMax, T);
+}
+// BlackbodyInput::MakeBlackbodySpectrum with Spectrum::nSamples = 3 (GraphInputs.cpp:106-116)
+// then RGBSpectrum::FromSampled (Spectrum.h:353-373) over 471 CIE wavelengths.
+static void BlackbodyRGB(float T, float out[3]) {
+  const unsigned n = 3;
+  const float invSamples = 1.f / (float)n;
+  float lam[3], v[3];
+  for (unsigned i = 0; i < n; ++i) lam[i] = 400.f + 300.f * invSamples * (float)i;
+  for (unsigned i = 0; i < n; ++i) v[i] = BlackbodyNormalised(lam[i], T);
+  float xyz[3] = {0, 0, 0};
+  for (unsigned i = 0; i < 471; ++i) {
+    const float li = 360.f + (float)i;
+    float val;
+    if (li <= lam[0]) val = v[0];
+    else if (li >= lam[n - 1]) val = v[n - 1];
+    else {
+      const int off = (int)FindInterval((int)n, [&](int k) { return lam[k] <= li; });
+      const float t = (li - lam[off]) / (lam[off + 1] - lam[off]);
+      val = v[off] + (v[off + 1] - v[off]) * t;
+    }
+    float cmf[3];
+    CMF(li, cmf);
+    xyz[0] += val * cmf[0];
+    xyz[1] += val * cmf[1];
+    xyz[2] += val * cmf[2];
+  }
+  const float scale = (float)(830 - 360) / (float)(CIE_Y_INTEGRAL * 471);
+  for (int k = 0; k < 3; ++k) xyz[k] *= scale;
+  // XYZToRGB, Spectrum.h:194-198 (double coefficients)
+  out[0] = (float)(3.240479 * xyz[0] - 1.537150 * xyz[1] - 0.498535 * xyz[2]);
+  out[1] = (float)(-0.969256 * xyz[0] + 1.875991 * xyz[1] + 0.041556 * xyz[2]);
+  out[2] = (float)(0.055648 * xyz[0] - 0.204043 * xyz[1] + 1.057311 * xyz[2]);
+}
+
+// ---------------------------------------------------------------- Philox4x32-10 sampler
+static inline void Philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+static inline float U32ToFloat(uint32_t u) { return (float)(u >> 8) * (1.f / 16777216.f); }
+
+// Sampler interface (Sampler.h:9-25): sequential per-path dimension counter.
+struct Sampler {
+  uint32_t pixel = 0, sample = 0, dim = 0, seed = 0;
+  uint32_t cachedBlock = 0xFFFFFFFFu, cache[4];
+  float Get1D() {
+    const uint32_t blk = dim >> 2;
+    if (blk != cachedBlock) {
+      cache[0] = pixel; cache[1] = sample; cache[2] = blk; cache[3] = 0;
+      Philox(cache, seed, 0u);
+      cachedBlock = blk;
+    }
+    return U32ToFloat(cache[dim++ & 3]);
+  }
+  V2 Get2D() { const float a = Get1D(); const float b = Get1D(); return V2(a, b); }
+};
+// Wavelength stream: key word 1 = 1, dimension block 0x3FFFFFFF.
+static inline float WavelengthU(uint32_t pixel, uint32_t sample, uint32_t seed) {
+  uint32_t c[4] = {pixel, sample, 0x3FFFFFFFu, 0};
+  Philox(c, seed, 1u);
+  return U32ToFloat(c[0]);
+}
+static inline void HeroWavelengths(float u, float lam[4]) {
+  const float l0 = 360.f + 470.f * u;
+  for (int i = 0; i < 4; ++i) {
+    float l = l0 + 117.5f * (float)i;
+    if (l >= 830.f) l -= 470.f;
+    lam[i] = l;
+  }
+}
+
+// ---------------------------------------------------------------- Piecewise (Piecewise.cpp)
+struct Piecewise1D {
+  std::vector<float> pdf, cdf;
+  float integral = 0;
+  Piecewise1D() {}
+  Piecewise1D(const float *d, unsigned n) : pdf(d, d + n), cdf(n + 1) {  // Piecewise.cpp:9-24
+    cdf[0] = 0;
+    for (unsigned i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + pdf[i - 1] / (float)n;
+    integral = cdf[n];
+    if (integral == 0) { for (unsigned i = 1; i < n + 1; ++i) cdf[i] = (float)i / (float)n; }
+    else { for (unsigned i = 1; i < n + 1; ++i) cdf[i] /= integral; }
+  }
+  unsigned SampleDiscrete(float u, float *p) const {  // :38-44
+    const unsigned off = FindInterval((int)cdf.size(), [&](int i) { return cdf[i] <= u; });
+    if (p) *p = pdf[off] / (integral * (float)pdf.size());
+    return off;
+  }
+  float SampleContinuous(float u, float *p, int *offOut) const {  // :26-35
+    const unsigned off = FindInterval((int)cdf.size(), [&](int i) { return cdf[i] <= u; });
+    if (offOut) *offOut = (int)off;
+    float du = u - cdf[off];
+    if ((cdf[off + 1] - cdf[off]) > 0) du /= (cdf[off + 1] - cdf[off]);
+    if (p) *p = pdf[off] / integral;
+    return ((float)off + du) / (float)pdf.size();
+  }
+  float PDF(unsigned i) const { return pdf[i]; }  // Piecewise.h:26-28 (raw weight, A28)
+  float PDFNormalised(unsigned i) const { return pdf[i] / (integral * (float)pdf.size()); }
+};
+struct Piecewise2D {  // Piecewise.cpp:50-76
+  std::vector<Piecewise1D> cond;
+  Piecewise1D marg;
+  Piecewise2D() {}
+  Piecewise2D(const float *f, unsigned nu, unsigned nv) {
+    for (unsigned v = 0; v < nv; ++v) cond.emplace_back(&f[v * nu], nu);
+    std::vector<float> m;
+    for (unsigned v = 0; v < nv; ++v) m.push_back(cond[v].integral);
+    marg = Piecewise1D(m.data(), nv);
+  }
+  V2 SampleContinuous(const V2 &u, float *pdf) const {
+    float p0, p1;
+    int v;
+    const float d1 = marg.SampleContinuous(u.y, &p1, &v);
+    const float d0 = cond[v].SampleContinuous(u.x, &p0, nullptr);
+    *pdf = p0 * p1;
+    return V2(d0, d1);
+  }
+  float PDF(const V2 &uv) const {
+    const int nu = (int)cond[0].pdf.size(), nv = (int)marg.pdf.size();
+    const int iu = std::max(std::min((int)(uv.x * (float)nu), nu - 1), 0);
+    const int iv = std::max(std::min((int)(uv.y * (float)nv), nv - 1), 0);
+    return cond[iv].pdf[iu] / marg.integral;
+  }
+};
+
+// ---------------------------------------------------------------- scene data
+struct Texture {
+  unsigned w = 0, h = 0, ch = 0;
+  std::vector<float> data;
+  // texture_t::GetPixelUVBilinear (Texture.h:71-81), RGB or scalar
+  void Bilinear(float u, float v, float out[3]) const {
+    const float fx = std::min((float)(w - 1), Clampf(u, 0.f, 1.f) * (float)w);
+    const float fy = std::min((float)(h - 1), Clampf(v, 0.f, 1.f) * (float)h);
+    const unsigned x = (unsigned)fx, y = (unsigned)fy;
+    const unsigned x1 = std::min(x + 1, w - 1), y1 = std::min(y + 1, h - 1);
+    const float tx = fx - (float)x, ty = fy - (float)y;
+    for (unsigned c = 0; c < 3; ++c) {
+      const unsigned cc = ch == 1 ? 0 : c;
+      const float p00 = data[(y * w + x) * ch + cc], p10 = data[(y * w + x1) * ch + cc];
+      const float p01 = data[(y1 * w + x) * ch + cc], p11 = data[(y1 * w + x1) * ch + cc];
+      const float ya = p00 + (p10 - p00) * tx;
+      const float yb = p01 + (p11 - p01) * tx;
+      out[c] = ya + (yb - ya) * ty;
+    }
+  }
+};
+
+struct Material {
+  std::vector<lgpu_node> nodes;
+  uint32_t bxdf = LGPU_NONE, emission = LGPU_NONE;
+  float intensity = 1.f;
+  uint32_t medIn = LGPU_NONE, medOut = LGPU_NONE;
+  std::vector<float> bbRGB;  // per node: blackbody RGB for constant temperatures (RGB mode)
+  std::vector<char> bbConst;
+};
+
+struct Mesh {
+  std::vector<V3> p, n;
+  std::vector<float> tan;  // 4 per vertex
+  std::vector<V2> uv;
+  std::vector<uint32_t> idx;
+  uint32_t material = 0;
+  uint32_t firstTri = 0;
+  int meshLight = -1;  // index into lights
+};
+
+enum LightKind { L_MESH = 0, L_TRI = 1, L_POINT = 2, L_SPOT = 3, L_ENV = 4 };
+struct Light {
+  int kind = L_MESH;
+  int mesh = -1;          // L_MESH / L_TRI
+  uint32_t tri = 0;       // L_TRI: local triangle
+  int parentMeshLight = -1;
+  Piecewise1D triDist;    // L_MESH (MeshLight::InitDistribution, MeshLight.cpp:118-125)
+  V3 position, axis;
+  float cosCone = 0, cosFalloff = 0;
+  float intensity = 1;
+  float rgb[3] = {1, 1, 1};
+  float temperature = 0;
+  float bbRGB[3] = {0, 0, 0};
+  // env
+  Texture env;
+  Piecewise2D envDist;
+  V2 offset;
+  float radius = 0;
+  Bounds envBounds;
+};
+
+struct Hit {
+  V3 point, normalG, normalS, tangent, bitangent;
+  V2 uv;
+  float tFar = INFINITY;
+  uint32_t prim = 0;   // global triangle index
+  uint32_t local = 0;  // RayHit::primId (mesh-local)
+  int mesh = -1;
+};
+
+struct Event {  // ScatterEvent.h:18-40
+  V3 wo, wi, woL, wiL;
+  Hit *hit = nullptr;
+  int sidedness = 1;
+  float eta = 1.001f;
+  Lanes lanes;
+  V3 ToLocal(const V3 &v) const { return WorldToLocal(v, hit->tangent, hit->normalS, hit->bitangent); }
+  V3 ToWorld(const V3 &v) const { return LocalToWorld(v, hit->tangent, hit->normalS, hit->bitangent); }
+  void SurfaceLocalise() { woL = ToLocal(wo); wiL = ToLocal(wi); sidedness = (woL.y > 0) ? 1 : -1; }
+};
+
+// Light-tree node (ManyLightSampler.h:48-76)
+struct Cone { V3 axis = V3(0, 1, 0); float thetaO = 0, thetaE = 0; };
+struct LNode {
+  LNode *child[2] = {nullptr, nullptr};
+  LNode *parent = nullptr;
+  unsigned first = 0, count = 0;
+  Bounds bounds;
+  Cone cone;
+  float power = 0;
+  int leafIndex = -1;
+  bool IsLeaf() const { return !child[0] && !child[1]; }
+};
+
+struct BVHNode { Bounds b; uint32_t left = 0, right = 0, first = 0, count = 0; };
+
+}  // namespace orc
+
+using namespace orc;
+
+struct oracle_scene_s {
+  std::vector<Texture> textures;
+  std::vector<lgpu_medium_desc> media;
+  std::vector<Material> materials;
+  std::vector<Mesh> meshes;
+  std::vector<uint32_t> triMesh;  // global tri -> mesh
+  std::vector<Light> lights;      // scene lights in add order (Scene::lights)
+  int envLight = -1;
+  uint32_t strategy = LGPU_LIGHTS_TREE;
+  bool committed = false;
+  // power sampler (LightSampler.cpp:14-35)
+  Piecewise1D powerDist;
+  float invTotalPower = 0;
+  // tree (ManyLightSampler)
+  std::vector<Light> tlights;           // tree light records
+  std::vector<int> tOrder;              // permutation of tlights
+  std::vector<std::unique_ptr<LNode>> tnodes;
+  LNode *root = nullptr;
+  std::vector<Piecewise1D> leafDist;
+  std::vector<std::pair<LNode *, unsigned>> lightLeaf;  // per tlight index
+  std::vector<int> triToTLight;         // global tri -> tlight, -1
+  std::vector<int> sceneToTLight;       // scene light -> tlight (point/spot), -1
+  float treePower = 0, infPower = 0;
+  // BVH
+  std::vector<BVHNode> bvh;
+  std::vector<uint32_t> bvhTris;
+  Bounds sceneBounds;
+};
+
+namespace orc {
+typedef oracle_scene_s Scene;
+
+// ---------------------------------------------------------------- geometry
+static inline void TriVerts(const Scene &s, uint32_t tri, V3 &v0, V3 &v1, V3 &v2) {
+  const Mesh &m = s.meshes[s.triMesh[tri]];
+  const uint32_t l = tri - m.firstTri;
+  v0 = m.p[m.idx[3 * l]]; v1 = m.p[m.idx[3 * l + 1]]; v2 = m.p[m.idx[3 * l + 2]];
+}
+// TriangleMesh::GetTriangleAreaAndNormal (TriangleMesh.h:60-64)
+static inline void TriAreaNormal(const V3 &v0, const V3 &v1, const V3 &v2, float *area, V3 *normal) {
+  const V3 c = Cross(v1 - v0, v2 - v0);
+  *area = Mag(c) * .5f;
+  if (normal) *normal = c / (*area * 2.f);
+}
+// Moller-Trumbore; the same arithmetic order is used by the product's kernel.
+static inline bool RayTri(const V3 &o, const V3 &d, const V3 &v0, const V3 &v1, const V3 &v2, float tmin, float tmax,
+                          float *t, float *u, float *v) {
+  const V3 e1 = v1 - v0, e2 = v2 - v0;
+  const V3 pv = Cross(d, e2);
+  const float det = Dot(e1, pv);
+  if (det == 0.f) return false;
+  const float inv = 1.f / det;
+  const V3 tv = o - v0;
+  const float uu = Dot(tv, pv) * inv;
+  if (uu < 0.f || uu > 1.f) return false;
+  const V3 qv = Cross(tv, e1);
+  const float vv = Dot(d, qv) * inv;
+  if (vv < 0.f || uu + vv > 1.f) return false;
+  const float tt = Dot(e2, qv) * inv;
+  if (!(tt > tmin && tt < tmax)) return false;
+  *t = tt; *u = uu; *v = vv;
+  return true;
+}
+
+static void BuildBVH(Scene &s) {  // median split on the longest centroid axis
+  const uint32_t nt = (uint32_t)s.triMesh.size();
+  s.bvh.clear();
+  s.bvhTris.resize(nt);
+  for (uint32_t i = 0; i < nt; ++i) s.bvhTris[i] = i;
+  std::vector<V3> cen(nt);
+  std::vector<Bounds> tb(nt);
+  for (uint32_t i = 0; i < nt; ++i) {
+    V3 a, b, c;
+    TriVerts(s, i, a, b, c);
+    tb[i] = Union(Union(Bounds(a), b), c);
+    cen[i] = (a + b + c) * (1.f / 3.f);
+  }
+  if (nt == 0) return;
+  struct Item { uint32_t node, first, count; };
+  s.bvh.push_back(BVHNode());
+  std::vector<Item> stack{{0, 0, nt}};
+  while (!stack.empty()) {
+    Item it = stack.back();
+    stack.pop_back();
+    Bounds b = tb[s.bvhTris[it.first]], cb(cen[s.bvhTris[it.first]]);
+    for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+      b = Union(b, tb[s.bvhTris[i]]);
+      cb = Union(cb, cen[s.bvhTris[i]]);
+    }
+    s.bvh[it.node].b = b;
+    if (it.count <= 4) {
+      s.bvh[it.node].first = it.first;
+      s.bvh[it.node].count = it.count;
+      continue;
+    }
+    const V3 ext = cb.mx - cb.mn;
+    const int ax = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
+    const uint32_t mid = it.first + it.count / 2;
+    std::nth_element(s.bvhTris.begin() + it.first, s.bvhTris.begin() + mid, s.bvhTris.begin() + it.first + it.count,
+                     [&](uint32_t a, uint32_t c) { return cen[a][ax] < cen[c][ax] || (cen[a][ax] == cen[c][ax] && a < c); });
+    const uint32_t l = (uint32_t)s.bvh.size();
+    s.bvh.push_back(BVHNode());
+    s.bvh.push_back(BVHNode());
+    s.bvh[it.node].left = l;
+    s.bvh[it.node].right = l + 1;
+    s.bvh[it.node].count = 0;
+    stack.push_back({l, it.first, mid - it.first});
+    stack.push_back({l + 1, mid, it.first + it.count - mid});
+  }
+  s.sceneBounds = s.bvh[0].b;
+}
+
+static inline bool BoxHit(const Bounds &b, const V3 &o, const V3 &invd, float tmin, float tmax) {
+  float t0 = tmin, t1 = tmax;
+  for (int a = 0; a < 3; ++a) {
+    float tn = (b.mn[a] - o[a]) * invd[a], tf = (b.mx[a] - o[a]) * invd[a];
+    if (tn > tf) std::swap(tn, tf);
+    if (std::isnan(tn)) tn = -INFINITY;
+    if (std::isnan(tf)) tf = INFINITY;
+    t0 = std::max(t0, tn);
+    t1 = std::min(t1, tf * 1.0000004f);
+    if (t0 > t1) return false;
+  }
+  return true;
+}
+
+// Closest hit; ties in t resolve to the lower triangle index.
+static bool ClosestHit(const Scene &s, const V3 &o, const V3 &d, float tmin, float tmax, float *t, float *u, float *v,
+                       uint32_t *prim) {
+  if (s.bvh.empty()) return false;
+  const V3 invd(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  bool found = false;
+  float bt = tmax;
+  uint32_t stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BVHNode &n = s.bvh[stack[--sp]];
+    if (!BoxHit(n.b, o, invd, tmin, bt)) continue;
+    if (n.count) {
+      for (uint32_t i = n.first; i < n.first + n.count; ++i) {
+        const uint32_t tri = s.bvhTris[i];
+        V3 a, b, c;
+        TriVerts(s, tri, a, b, c);
+        float tt, uu, vv;
+        if (RayTri(o, d, a, b, c, tmin, tmax, &tt, &uu, &vv)) {
+          if (!found || tt < bt || (tt == bt && tri < *prim)) {
+            found = true; bt = tt; *t = tt; *u = uu; *v = vv; *prim = tri;
+          }
+        }
+      }
+    } else {
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+  }
+  return found;
+}
+static bool AnyHit(const Scene &s, const V3 &o, const V3 &d, float tmin, float tmax) {
+  if (s.bvh.empty()) return false;
+  const V3 invd(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  uint32_t stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BVHNode &n = s.bvh[stack[--sp]];
+    if (!BoxHit(n.b, o, invd, tmin, tmax)) continue;
+    if (n.count) {
+      for (uint32_t i = n.first; i < n.first + n.count; ++i) {
+        V3 a, b, c;
+        TriVerts(s, s.bvhTris[i], a, b, c);
+        float tt, uu, vv;
+        if (RayTri(o, d, a, b, c, tmin, tmax, &tt, &uu, &vv)) return true;
+      }
+    } else {
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+  }
+  return false;
+}
+
+// Scene::Intersect (Scene.cpp:28-43) + Object::Hit (Object.h:27-35) + TriangleMesh::ProcessHit (TriangleMesh.cpp:80-114)
+static bool Intersect(const Scene &s, const V3 &o, const V3 &d, Hit &h, bool corrected) {
+  float t, u, v;
+  uint32_t prim;
+  if (!ClosestHit(s, o, d, 0.f, INFINITY, &t, &u, &v, &prim)) return false;
+  if (!(t > 0 && t < INFINITY)) return false;
+  const Mesh &m = s.meshes[s.triMesh[prim]];
+  const uint32_t l = prim - m.firstTri;
+  const uint32_t i0 = m.idx[3 * l], i1 = m.idx[3 * l + 1], i2 = m.idx[3 * l + 2];
+  h.tFar = t;
+  h.point = V3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+  h.normalG = Norm(Cross(m.p[i1] - m.p[i0], m.p[i2] - m.p[i0]));
+  h.prim = prim;
+  h.local = l;
+  h.mesh = (int)s.triMesh[prim];
+  if (!m.uv.empty()) {
+    const V2 &a = m.uv[i0], &b = m.uv[i1], &c = m.uv[i2];
+    h.uv = V2(a.x + (b.x - a.x) * u + (c.x - a.x) * v, a.y + (b.y - a.y) * u + (c.y - a.y) * v);
+  } else h.uv = V2(0, 0);
+  if (!m.n.empty() && !m.tan.empty()) {
+    const V3 &n0 = m.n[i0], &n1 = m.n[i1], &n2 = m.n[i2];
+    const V3 t0(&m.tan[4 * i0]), t1(&m.tan[4 * i1]), t2(&m.tan[4 * i2]);
+    h.normalS = Norm(n0 + (n1 - n0) * u + (n2 - n0) * v);
+    h.tangent = Norm(t0 + (t1 - t0) * u + (t2 - t0) * v);
+    const V3 b0 = Cross(n0, t0) * m.tan[4 * i0 + 3], b1 = Cross(n1, t1) * m.tan[4 * i1 + 3], b2 = Cross(n2, t2) * m.tan[4 * i2 + 3];
+    h.bitangent = Norm(b0 + (b1 - b0) * u + (b2 - b0) * v);
+  } else {
+    h.normalS = h.normalG;
+    const V3 c1 = Cross(h.normalG, V3(0, 0, 1));
+    const V3 c2 = Cross(h.normalG, V3(0, 1, 0));
+    h.tangent = (Dot(c1, c1) > Dot(c2, c2) ? c1 : c2);
+    if (corrected) h.tangent = Norm(h.tangent);  // A39: unnormalised tangent (TriangleMesh.cpp:108-111)
+    h.bitangent = Cross(h.tangent, h.normalG);
+  }
+  return true;
+}
+// Scene::MutualVisibility (Scene.cpp:83-95): occlusion ray with tfar = |p2-p1| - 1e-5
+static bool MutualVisibility(const Scene &s, const V3 &p1, const V3 &p2, V3 *w) {
+  const V3 diff = p2 - p1;
+  const float mag = Mag(diff);
+  const V3 dir = diff / mag;
+  const float tfar = (float)((double)mag - .00001);
+  if (w) *w = dir;
+  return !AnyHit(s, p1, dir, 0.f, tfar);
+}
+static bool RayEscapes(const Scene &s, const V3 &o, const V3 &d) { return !AnyHit(s, o, d, 0.f, INFINITY); }
+
+// ---------------------------------------------------------------- shader graph (ShaderGraph.h:82-251)
+enum VT { VT_SCALAR, VT_VEC2, VT_VEC3, VT_COLOUR, VT_SPECTRUM, VT_BXDF };
+struct Val { int t = VT_SCALAR; float v[4] = {0, 0, 0, 0}; Spec s; };
+
+static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event &ev);
+static float AsReal(const Val &x, const Lanes &l) {
+  switch (x.t) {
+    case VT_SCALAR: return x.v[0];
+    case VT_COLOUR: return x.v[0] * .3f + x.v[1] * .59f + x.v[2] * .11f;
+    case VT_VEC2: return (x.v[0] + x.v[1]) * .5f;
+    case VT_VEC3: return (x.v[0] + x.v[1] + x.v[2]) * (float).333333333333;
+    case VT_SPECTRUM: return Luminance(x.s, l);
+  }
+  return 0;
+}
+static void AsColour(const Val &x, float c[3]) {
+  switch (x.t) {
+    case VT_COLOUR: case VT_VEC3: c[0] = x.v[0]; c[1] = x.v[1]; c[2] = x.v[2]; return;
+    case VT_SCALAR: c[0] = c[1] = c[2] = x.v[0]; return;
+    case VT_VEC2: c[0] = c[1] = c[2] = (x.v[0] + x.v[1]) * .5f; return;
+    case VT_SPECTRUM: c[0] = x.s.c[0]; c[1] = x.s.c[1]; c[2] = x.s.c[2]; return;
+  }
+  c[0] = 1; c[1] = 0; c[2] = 1;
+}
+static V3 AsVec3(const Val &x) {
+  switch (x.t) {
+    case VT_VEC3: return V3(x.v[0], x.v[1], x.v[2]);
+    case VT_VEC2: return V3(x.v[0], x.v[1], 0);
+    case VT_SCALAR: return V3(x.v[0], x.v[0], x.v[0]);
+  }
+  return V3(0, 0, 0);
+}
+static Spec FromRGB(const float c[3], const Lanes &l) {
+  Spec s;
+  if (l.mode == HERO4) { for (int i = 0; i < 4; ++i) s.c[i] = Uplift(c, l.lambda[i]); }
+  else { s.c[0] = c[0]; s.c[1] = c[1]; s.c[2] = c[2]; s.c[3] = 0; }
+  return s;
+}
+static Spec AsSpectrum(const Val &x, const Lanes &l) {  // ShaderGraph.h:232-251
+  switch (x.t) {
+    case VT_COLOUR: { float c[3]; AsColour(x, c); return FromRGB(c, l); }
+    case VT_SCALAR: return Spec(AsReal(x, l));
+    case VT_SPECTRUM: return x.s;
+  }
+  return Spec(0);
+}
+static Val Input(const Scene &sc, const Material &m, uint32_t node, int k, const Event &ev) {
+  const uint32_t j = m.nodes[node].in[k];
+  if (j == LGPU_NONE || j >= m.nodes.size()) return Val();
+  return EvalNode(sc, m, j, ev);
+}
+static float FrDielectric(float cosI, float etaI, float etaT);
+static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event &ev) {
+  const lgpu_node &n = m.nodes[i];
+  Val r;
+  switch (n.type) {
+    case LGPU_NODE_SCALAR: r.t = VT_SCALAR; r.v[0] = n.p[0]; break;
+    case LGPU_NODE_VEC2: r.t = VT_VEC2; r.v[0] = n.p[0]; r.v[1] = n.p[1]; break;
+    case LGPU_NODE_VEC3: r.t = VT_VEC3; r.v[0] = n.p[0]; r.v[1] = n.p[1]; r.v[2] = n.p[2]; break;
+    case LGPU_NODE_RGB: r.t = VT_COLOUR; r.v[0] = n.p[0]; r.v[1] = n.p[1]; r.v[2] = n.p[2]; r.v[3] = 1; break;
+    case LGPU_NODE_BLACKBODY: {  // GraphInputs.cpp:82-85
+      r.t = VT_SPECTRUM;
+      if (ev.lanes.mode != HERO4 && i < m.bbConst.size() && m.bbConst[i]) {
+        r.s.c[0] = m.bbRGB[3 * i]; r.s.c[1] = m.bbRGB[3 * i + 1]; r.s.c[2] = m.bbRGB[3 * i + 2]; r.s.c[3] = 0;
+        break;
+      }
+      const float T = AsReal(Input(sc, m, i, 0, ev), ev.lanes);
+      if (ev.lanes.mode == HERO4) {
+        for (int k = 0; k < 4; ++k) r.s.c[k] = BlackbodyNormalised(ev.lanes.lambda[k], T);
+      } else {
+        float rgb[3];
+        BlackbodyRGB(T, rgb);
+        r.s.c[0] = rgb[0]; r.s.c[1] = rgb[1]; r.s.c[2] = rgb[2]; r.s.c[3] = 0;
+      }
+      break;
+    }
+    case LGPU_NODE_IMAGE_COLOUR:
+    case LGPU_NODE_IMAGE_SCALAR: {  // GraphInputs.cpp:128-146: Fract(uv) then bilinear
+      const Texture &t = sc.textures[(uint32_t)n.p[0]];
+      float c[3];
+      t.Bilinear(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c);
+      if (n.type == LGPU_NODE_IMAGE_COLOUR) { r.t = VT_COLOUR; r.v[0] = c[0]; r.v[1] = c[1]; r.v[2] = c[2]; }
+      else { r.t = VT_SCALAR; r.v[0] = c[0]; }
+      break;
+    }
+    case LGPU_NODE_UV: r.t = VT_VEC2; r.v[0] = ev.hit->uv.x; r.v[1] = ev.hit->uv.y; break;
+    case LGPU_NODE_FRESNEL: r.t = VT_SCALAR; r.v[0] = FrDielectric(ev.woL.y, 1.f, n.p[0]); break;
+    case LGPU_NODE_SCALAR_MATHS: {  // GraphMaths.cpp:17-30
+      const float a = AsReal(Input(sc, m, i, 0, ev), ev.lanes), b = AsReal(Input(sc, m, i, 1, ev), ev.lanes);
+      r.t = VT_SCALAR;
+      switch ((int)n.p[0]) {
+        case LGPU_OP_ADD: r.v[0] = a + b; break;
+        case LGPU_OP_SUBTRACT: r.v[0] = a - b; break;
+        case LGPU_OP_MULTIPLY: r.v[0] = a * b; break;
+        default: r.v[0] = a / b; break;
+      }
+      break;
+    }
+    case LGPU_NODE_VECTOR_MATHS: {
+      const V3 a = AsVec3(Input(sc, m, i, 0, ev)), b = AsVec3(Input(sc, m, i, 1, ev));
+      V3 o;
+      switch ((int)n.p[0]) {
+        case LGPU_OP_ADD: o = a + b; break;
+        case LGPU_OP_SUBTRACT: o = a - b; break;
+        case LGPU_OP_MULTIPLY: o = a * b; break;
+        default: o = a / b; break;
+      }
+      r.t = VT_VEC3; r.v[0] = o.x; r.v[1] = o.y; r.v[2] = o.z;
+      break;
+    }
+    case LGPU_NODE_DOT: r.t = VT_SCALAR; r.v[0] = Dot(AsVec3(Input(sc, m, i, 0, ev)), AsVec3(Input(sc, m, i, 1, ev))); break;
+    case LGPU_NODE_LENGTH: r.t = VT_SCALAR; r.v[0] = Mag(AsVec3(Input(sc, m, i, 0, ev))); break;
+    case LGPU_NODE_MERGE_RGB:
+      r.t = VT_COLOUR;
+      for (int k = 0; k < 3; ++k) r.v[k] = AsReal(Input(sc, m, i, k, ev), ev.lanes);
+      r.v[3] = 1;
+      break;
+    case LGPU_NODE_SEPARATE_R: {
+      float c[3];
+      AsColour(Input(sc, m, i, 0, ev), c);
+      r.t = VT_SCALAR; r.v[0] = c[std::min(std::max((int)n.p[0], 0), 2)];
+      break;
+    }
+    case LGPU_NODE_CHECKER: {  // GraphTexture.cpp:100-108 (surface branch)
+      const float u = Fract(ev.hit->uv.x * n.p[0]), v = Fract(ev.hit->uv.y * n.p[0]);
+      r.t = VT_SCALAR; r.v[0] = (u > .5f) == (v > .5f) ? 0.f : 1.f;
+      break;
+    }
+    default: r.t = VT_BXDF; break;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------- BxDFs
+static inline bool SameHemisphere(const V3 &a, const V3 &b) { return a.y * b.y > 0; }
+static inline float CosineHemispherePdf(const V3 &wo, const V3 &wi) { return SameHemisphere(wo, wi) ? std::abs(wi.y) * INV_PI : 0; }
+static inline float Cos2Theta(const V3 &w) { return w.y * w.y; }
+static inline float Sin2Theta(const V3 &w) { return std::max(0.f, 1.f - Cos2Theta(w)); }
+static inline float SinTheta(const V3 &w) { return std::sqrt(Sin2Theta(w)); }
+static inline float TanTheta(const V3 &w) { return SinTheta(w) / w.y; }
+static inline float Tan2Theta(const V3 &w) { return Sin2Theta(w) / Cos2Theta(w); }
+static inline float CosPhi(const V3 &w) { const float s = SinTheta(w); return (s < 0.000001f) ? 0 : Clampf(w.x / s, -1, 1); }
+static inline float SinPhi(const V3 &w) { const float s = SinTheta(w); return (s < 0.000001f) ? 0 : Clampf(w.z / s, -1, 1); }
+static inline float Cos2Phi(const V3 &w) { return CosPhi(w) * CosPhi(w); }
+static inline float Sin2Phi(const V3 &w) { return SinPhi(w) * SinPhi(w); }
+static inline V3 Reflect(const V3 &w, const V3 &n) { return w - n * 2.f * Dot(n, w); }
+static inline bool Refract(const V3 &w, const V3 &n, float eta, V3 *wr) {  // ScatterEvent.h:52-63
+  const float cosI = Dot(w, n);
+  const float sin2I = std::max(0.f, 1.f - cosI * cosI);
+  const float sin2T = eta * eta * sin2I;
+  if (sin2T >= 1) { *wr = V3(-1, 1, -1) * w; return false; }
+  const float cosT = std::sqrt(1 - sin2T);
+  *wr = w * -eta + n * (eta * cosI - cosT);
+  return true;
+}
+static inline V2 SampleUnitDisk(const V2 &u) {  // Sampling.h:9-13
+  const float r = std::sqrt(u.x);
+  const float theta = u.y * PI2;
+  return V2(r * std::cos(theta), r * std::sin(theta));
+}
+static inline V3 SampleCosineHemisphere(const V2 &u) {  // Sampling.h:15-19
+  const V2 d = SampleUnitDisk(u);
+  const float up = std::sqrt(std::max(0.f, 1.f - d.x * d.x - d.y * d.y));
+  return V3(d.x, up, d.y);
+}
+
+static float FrDielectric(float cosI, float etaI, float etaT) {  // Fresnel.cpp:5-20
+  cosI = Clampf(cosI, -1, 1);
+  if (cosI < 0) { std::swap(etaI, etaT); cosI = std::abs(cosI); }
+  const float sinI = std::sqrt(std::max(0.f, 1 - cosI * cosI));
+  const float sinT = etaI / etaT * sinI;
+  if (sinT >= 1) return 1;
+  const float cosT = std::sqrt(std::max(0.f, 1 - sinT * sinT));
+  const float rPara = (etaT * cosI - etaI * cosT) / (etaT * cosI + etaI * cosT);
+  const float rPerp = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
+  return (rPara * rPara + rPerp * rPerp) * .5f;
+}
+static Spec FrConductor(float cosI, const Spec &etaI, const Spec &etaT, const Spec &k) {  // Fresnel.cpp:23-45
+  cosI = Clampf(cosI, -1, 1);
+  const Spec eta = etaT / etaI;
+  const Spec etak = k / etaI;
+  const float c2 = cosI * cosI;
+  const float s2 = 1.f - c2;
+  const Spec eta2 = eta * eta;
+  const Spec etak2 = etak * etak;
+  const Spec t0 = eta2 - etak2 - s2;
+  const Spec a2pb2 = SqrtS(t0 * t0 + eta2 * 4.f * etak2);
+  const Spec t1 = a2pb2 + c2;
+  const Spec a = SqrtS((a2pb2 + t0) * .5f);
+  const Spec t2 = a * cosI * 2.f;
+  const Spec rS = (t1 - t2) / (t1 + t2);
+  const Spec t3 = a2pb2 * c2 + s2 * s2;
+  const Spec t4 = t2 * s2;
+  const Spec rP = rS * (t3 - t4) / (t3 + t4);
+  return (rP + rS) * .5f;
+}
+
+// MicrofacetDistribution.cpp:11-71
+static float MfD(int ggx, const V3 &wh, float ax, float ay) {
+  const float tan2 = Tan2Theta(wh);
+  if (std::isinf(tan2)) return 0.;
+  const float cos4 = Cos2Theta(wh) * Cos2Theta(wh);
+  if (!ggx) return std::exp(-tan2 * (Cos2Phi(wh) / (ax * ax) + Sin2Phi(wh) / (ay * ay))) / (PI * ax * ay * cos4);
+  const float e = (Cos2Phi(wh) / (ax * ax) + Sin2Phi(wh) / (ay * ay)) * tan2;
+  return 1.f / (PI * ax * ay * cos4 * (1 + e) * (1 + e));
+}
+static float MfLambda(int ggx, const V3 &w, float ax, float ay) {
+  const float absTan = std::abs(TanTheta(w));
+  if (std::isinf(absTan)) return 0.;
+  const float alpha = std::sqrt(Cos2Phi(w) * ax * ax + Sin2Phi(w) * ay * ay);
+  if (!ggx) {
+    const float a = 1 / (alpha * absTan);
+    if (a >= 1.6f) return 0;
+    return (1 - 1.259f * a + 0.396f * a * a) / (3.535f * a + 2.181f * a * a);
+  }
+  const float a2t2 = (alpha * absTan) * (alpha * absTan);
+  return (-1 + std::sqrt(1.f + a2t2)) / 2;
+}
+static float MfG(int ggx, const V3 &wo, const V3 &wi, float ax, float ay) {
+  return 1.f / (1.f + MfLambda(ggx, wo, ax, ay) + MfLambda(ggx, wi, ax, ay));
+}
+static float MfPdf(int ggx, const V3 &wh, float ax, float ay) { return MfD(ggx, wh, ax, ay) * std::abs(wh.y); }
+static V3 MfSampleWh(int ggx, Sampler &smp, const V3 &wo, float ax) {
+  const V2 u = smp.Get2D();
+  if (!ggx) {
+    const float lg = u.x == 1 ? 0 : std::log(1 - u.x);
+    const float phi = PI2 * u.y;
+    const float tan2 = -ax * ax * lg;
+    const float cosT = 1 / std::sqrt(1 + tan2);
+    const float sinT = std::sqrt(std::max(0.f, 1 - cosT * cosT));
+    V3 h = SphericalDirection(sinT, cosT, phi);
+    return !SameHemisphere(wo, h) ? -h : h;
+  }
+  const float phi = PI2 * u.y;
+  const float tan2 = ax * ax * u.x / (1 - u.x);
+  const float cosT = 1 / std::sqrt(1 + tan2);
+  const float sinT = std::sqrt(std::max(0.f, 1.f - cosT * cosT));
+  V3 wh = SphericalDirection(sinT, cosT, phi);
+  if (!SameHemisphere(wo, wh)) wh = -wh;
+  return wh;
+}
+static float ToAlpha(float r) {  // Microfacet.cpp:49-55
+  r = std::max(r, 1e-3f);
+  const float x = std::log(r);
+  return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+
+struct Ctx {
+  const Scene *sc;
+  bool corrected;
+  int mode;
+};
+
+static Spec BxDF_f(const Ctx &c, const Material &m, uint32_t node, Event &ev);
+static float BxDF_Pdf(const Ctx &c, const Material &m, uint32_t node, const V3 &wo, const V3 &wi, Event &ev);
+
+static Spec SockSpec(const Ctx &c, const Material &m, uint32_t node, int k, const Event &ev) {
+  return AsSpectrum(Input(*c.sc, m, node, k, ev), ev.lanes);
+}
+static float SockReal(const Ctx &c, const Material &m, uint32_t node, int k, const Event &ev) {
+  return AsReal(Input(*c.sc, m, node, k, ev), ev.lanes);
+}
+static void MfAlpha(const Ctx &c, const Material &m, uint32_t node, const Event &ev, float *ax, float *ay) {
+  const lgpu_node &n = m.nodes[node];
+  const Val r = Input(*c.sc, m, node, 1, ev);
+  float a, b;
+  if (r.t == VT_VEC2) { a = r.v[0]; b = r.v[1]; }
+  else { a = b = AsReal(r, ev.lanes); }
+  if (n.p[9] == 0.f) { a = ToAlpha(a); b = ToAlpha(b); }
+  *ax = a; *ay = b;
+}
+static Spec ConductorSpec(const float rgb[3], const Lanes &l) { return FromRGB(rgb, l); }
+
+static float OrenNayarTerm(const Event &ev, float sigma) {  // OrenNayar.cpp:10-21
+  const float thetaI = std::acos(Clampf(ev.wiL.y, -1, 1));
+  const float thetaO = std::acos(Clampf(ev.woL.y, -1, 1));
+  const float alpha = std::max(thetaI, thetaO);
+  const float beta = std::min(thetaI, thetaO);
+  const float phiI = std::acos(CosPhi(ev.wiL));
+  const float phiO = std::acos(CosPhi(ev.woL));
+  const float s2 = sigma * sigma;
+  const float A = 1.f - s2 / (2.f * (s2 + .33f));
+  const float B = (.45f * s2) / (s2 + .09f);
+  return A + B * std::max(0.f, std::cos(phiI - phiO) * Clampf(std::sin(alpha), -1, 1) * Clampf(std::tan(beta), -1, 1));
+}
+
+// Rough dielectric (project extension; Walter et al. 2007 GGX, isotropic).
+static void RoughDielectricEval(const V3 &wo, const V3 &wi, float alpha, float ior, float *fOut, float *pdfOut) {
+  *fOut = 0; *pdfOut = 0;
+  if (wo.y == 0 || wi.y == 0) return;
+  const bool reflect = wo.y * wi.y > 0;
+  const float etaI = wo.y > 0 ? 1.f : ior, etaT = wo.y > 0 ? ior : 1.f;
+  V3 wh;
+  if (reflect) wh = wo + wi;
+  else wh = wo * etaI + wi * etaT;
+  const float l = Mag(wh);
+  if (l == 0) return;
+  wh = wh / l;
+  if (wh.y < 0) wh = -wh;
+  const float dwo = Dot(wo, wh), dwi = Dot(wi, wh);
+  if (!reflect && dwo * dwi >= 0) return;
+  const float F = FrDielectric(dwo, 1.f, ior);
+  const float D = MfD(1, wh, alpha, alpha);
+  const float G = MfG(1, wo, wi, alpha, alpha);
+  const float pwh = D * std::abs(wh.y);
+  if (reflect) {
+    *fOut = D * G * F / (4.f * std::abs(wo.y) * std::abs(wi.y));
+    *pdfOut = pwh * F / (4.f * std::abs(dwo));
+  } else {
+    const float sq = etaI * dwo + etaT * dwi;
+    const float factor = (etaI * etaI) / (etaT * etaT);
+    *fOut = (1.f - F) * D * G * etaT * etaT * std::abs(dwi) * std::abs(dwo) / (std::abs(wo.y) * std::abs(wi.y) * sq * sq) * factor;
+    *pdfOut = pwh * (1.f - F) * etaT * etaT * std::abs(dwi) / (sq * sq);
+  }
+}
+
+static Spec BxDF_f(const Ctx &c, const Material &m, uint32_t node, Event &ev) {
+  const lgpu_node &n = m.nodes[node];
+  switch (n.type) {
+    case LGPU_NODE_LAMBERTIAN: return SockSpec(c, m, node, 0, ev) * INV_PI;  // BxDF.cpp:29-31
+    case LGPU_NODE_OREN_NAYAR_BRDF:
+    case LGPU_NODE_OREN_NAYAR_BTDF: {  // OrenNayar.cpp:23-27,44-48
+      const float oN = OrenNayarTerm(ev, SockReal(c, m, node, 1, ev));
+      const Spec alb = SockSpec(c, m, node, 0, ev);
+      return alb * INV_PI * oN;
+    }
+    case LGPU_NODE_MICROFACET_BRDF: {  // Microfacet.cpp:12-25
+      const float cosO = std::abs(ev.woL.y), cosI = std::abs(ev.wiL.y);
+      V3 wh = ev.wiL + ev.woL;
+      if (cosI == 0 || cosO == 0) return Spec(0);
+      if (wh.x == 0 && wh.y == 0 && wh.z == 0) return Spec(0);
+      wh = Norm(wh);
+      const V3 wiL(ev.wiL.x, Clampf(ev.wiL.y, 0, 1), ev.wiL.z);
+      const int ggx = n.p[0] != 0.f;
+      Spec F;
+      const float cosH = Dot(wiL, wh);
+      if (n.p[1] != 0.f) {  // FresnelConductor::Evaluate(cos, etaT = 1e-5)  (A18)
+        const Spec eta = ConductorSpec(&n.p[3], ev.lanes), k = ConductorSpec(&n.p[6], ev.lanes);
+        if (c.corrected) F = FrConductor(cosH, Spec(1.f), eta, k);
+        else F = FrConductor(cosH, eta, Spec(0.00001f), k);
+      } else {  // FresnelDielectric::Evaluate(cos, etaI = 1e-5) -> FrDielectric(cos, etaT, 1e-5)
+        if (c.corrected) F = Spec(FrDielectric(cosH, 1.f, n.p[2]));
+        else F = Spec(FrDielectric(cosH, n.p[2], 0.00001f));
+      }
+      float ax, ay;
+      MfAlpha(c, m, node, ev, &ax, &ay);
+      const float D = MfD(ggx, wh, ax, ay);
+      const float G = MfG(ggx, ev.woL, wiL, ax, ay);
+      const Spec alb = SockSpec(c, m, node, 0, ev);
+      return alb * D * G * F / (4 * cosI * cosO);
+    }
+    case LGPU_NODE_ROUGH_DIELECTRIC: {
+      float f, pdf;
+      RoughDielectricEval(ev.woL, ev.wiL, SockReal(c, m, node, 1, ev), n.p[0], &f, &pdf);
+      return SockSpec(c, m, node, 0, ev) * f;
+    }
+    case LGPU_NODE_MIX_BSDF: {  // BxDF.cpp:45-48
+      const float ratio = SockReal(c, m, node, 2, ev);
+      return BxDF_f(c, m, n.in[0], ev) * (1.f - ratio) + BxDF_f(c, m, n.in[1], ev) * ratio;
+    }
+    default: return Spec(0);  // specular / ghost: f = 0
+  }
+}
+
+static float BxDF_Pdf(const Ctx &c, const Material &m, uint32_t node, const V3 &wo, const V3 &wi, Event &ev) {
+  const lgpu_node &n = m.nodes[node];
+  switch (n.type) {
+    case LGPU_NODE_LAMBERTIAN:
+    case LGPU_NODE_OREN_NAYAR_BRDF:
+    case LGPU_NODE_OREN_NAYAR_BTDF: return CosineHemispherePdf(wo, wi);  // BxDF.cpp:19-21
+    case LGPU_NODE_MICROFACET_BRDF: {  // Microfacet.cpp:40-45
+      if (!SameHemisphere(wo, wi)) return 0;
+      const V3 wh = Norm(wo + wi);
+      float ax, ay;
+      MfAlpha(c, m, node, ev, &ax, &ay);
+      return MfPdf(n.p[0] != 0.f, wh, ax, ay) / (4 * Dot(wo, wh));
+    }
+    case LGPU_NODE_ROUGH_DIELECTRIC: {
+      float f, pdf;
+      RoughDielectricEval(wo, wi, SockReal(c, m, node, 1, ev), n.p[0], &f, &pdf);
+      return pdf;
+    }
+    case LGPU_NODE_MIX_BSDF: {  // BxDF.cpp:62-65
+      const float ratio = SockReal(c, m, node, 2, ev);
+      return BxDF_Pdf(c, m, n.in[0], wo, wi, ev) * (1.f - ratio) + BxDF_Pdf(c, m, n.in[1], wo, wi, ev) * ratio;
+    }
+    default: return 0;
+  }
+}
+
+static bool IsSpecular(const Material &m, uint32_t node) {
+  const uint32_t t = m.nodes[node].type;
+  return t == LGPU_NODE_FRESNEL_BSDF || t == LGPU_NODE_SPECULAR_BRDF || t == LGPU_NODE_SPECULAR_BTDF || t == LGPU_NODE_GHOST_BTDF;
+}
+
+// Sample_f; *specOut reports a delta lobe for the corrected MIS weights.
+static Spec BxDF_Sample(const Ctx &c, const Material &m, uint32_t node, Event &ev, Sampler &smp, float &pdf, bool *specOut) {
+  const lgpu_node &n = m.nodes[node];
+  Hit &h = *ev.hit;
+  *specOut = false;
+  switch (n.type) {
+    case LGPU_NODE_LAMBERTIAN:
+    case LGPU_NODE_OREN_NAYAR_BRDF: {  // BxDF.cpp:7-14
+      ev.wiL = SampleCosineHemisphere(smp.Get2D());
+      if (ev.woL.y < 0) ev.wiL.y *= -1;
+      pdf = CosineHemispherePdf(ev.woL, ev.wiL);
+      ev.wi = ev.ToWorld(ev.wiL);
+      h.point += h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+      return BxDF_f(c, m, node, ev);
+    }
+    case LGPU_NODE_OREN_NAYAR_BTDF: {  // OrenNayar.cpp:52-59
+      ev.wiL = SampleCosineHemisphere(smp.Get2D());
+      pdf = CosineHemispherePdf(ev.woL, ev.wiL);
+      ev.wiL.y *= (float)-ev.sidedness;
+      ev.wi = ev.ToWorld(ev.wiL);
+      h.point += h.normalG * (float)-ev.sidedness * SURFACE_EPSILON;
+      return BxDF_f(c, m, node, ev);
+    }
+    case LGPU_NODE_MICROFACET_BRDF: {  // Microfacet.cpp:27-38
+      if (ev.woL.y == 0) { pdf = 0; return Spec(0); }
+      float ax, ay;
+      MfAlpha(c, m, node, ev, &ax, &ay);
+      const int ggx = n.p[0] != 0.f;
+      const V3 wh = Norm(MfSampleWh(ggx, smp, ev.woL, ax));
+      ev.wiL = Reflect(-ev.woL, wh);
+      if (!SameHemisphere(ev.woL, ev.wiL)) { pdf = 0; return Spec(0); }
+      pdf = MfPdf(ggx, wh, ax, ay) / (4 * Dot(ev.woL, wh));
+      ev.wi = ev.ToWorld(ev.wiL);
+      h.point += h.normalG * SURFACE_EPSILON;
+      return BxDF_f(c, m, node, ev);
+    }
+    case LGPU_NODE_ROUGH_DIELECTRIC: {
+      const float alpha = SockReal(c, m, node, 1, ev), ior = n.p[0];
+      const V2 u = smp.Get2D();
+      const float uc = smp.Get1D();
+      // GGX normal sampling in the hemisphere of wo (same warp as TrowbridgeReitz::Sample_wh)
+      const float phi = PI2 * u.y;
+      const float tan2 = alpha * alpha * u.x / (1 - u.x);
+      const float cosT = 1 / std::sqrt(1 + tan2);
+      const float sinT = std::sqrt(std::max(0.f, 1.f - cosT * cosT));
+      V3 wh = SphericalDirection(sinT, cosT, phi);
+      if (ev.woL.y < 0) wh = -wh;
+      const float F = FrDielectric(Dot(ev.woL, wh), 1.f, ior);
+      if (uc < F) {
+        ev.wiL = Reflect(-ev.woL, wh);
+        if (!SameHemisphere(ev.woL, ev.wiL)) { pdf = 0; return Spec(0); }
+      } else {
+        const float etaI = ev.woL.y > 0 ? 1.f : ior, etaT = ev.woL.y > 0 ? ior : 1.f;
+        if (!Refract(ev.woL, wh, etaI / etaT, &ev.wiL)) { pdf = 0; return Spec(0); }
+        if (SameHemisphere(ev.woL, ev.wiL) || ev.wiL.y == 0) { pdf = 0; return Spec(0); }
+      }
+      float f;
+      RoughDielectricEval(ev.woL, ev.wiL, alpha, ior, &f, &pdf);
+      ev.wi = ev.ToWorld(ev.wiL);
+      const float side = ev.wiL.y > 0 ? 1.f : -1.f;
+      h.point += h.normalG * (SURFACE_EPSILON * side * (float)ev.sidedness);
+      return SockSpec(c, m, node, 0, ev) * f;
+    }
+    case LGPU_NODE_FRESNEL_BSDF: {  // Specular.cpp:18-45
+      *specOut = true;
+      const bool entering = ev.woL.y > 0;
+      const float ior = SockReal(c, m, node, 1, ev);
+      const float fr = FrDielectric(ev.woL.y, 1, ior);
+      if (smp.Get1D() < fr) {
+        ev.wiL = V3(-ev.woL.x, ev.woL.y, -ev.woL.z);
+        ev.wi = ev.ToWorld(ev.wiL);
+        const float cosT = std::abs(ev.wiL.y);
+        h.point += h.normalG * (entering ? SURFACE_EPSILON : -SURFACE_EPSILON);
+        pdf = fr;
+        return SockSpec(c, m, node, 0, ev) * fr / cosT;
+      }
+      const float etaI = ev.eta = entering ? 1 : ior;
+      const float etaT = entering ? ior : 1;
+      const V3 nrm = Dot(V3(0, 1, 0), ev.woL) < 0 ? V3(0, -1, 0) : V3(0, 1, 0);
+      if (!Refract(ev.woL, nrm, etaI / etaT, &ev.wiL)) {  // A17: TIR returns 0 with the pdf unset
+        pdf = 0;
+        return Spec(0);
+      }
+      Spec ft = SockSpec(c, m, node, 0, ev) * (1 - fr);
+      ft = ft * ((etaI * etaI) / (etaT * etaT));
+      const float cosT = std::abs(ev.wiL.y);
+      ev.wi = ev.ToWorld(ev.wiL);
+      h.point += h.normalG * (entering ? -SURFACE_EPSILON * 2 : SURFACE_EPSILON * 2);
+      pdf = 1.f - fr;
+      return ft / cosT;
+    }
+    case LGPU_NODE_SPECULAR_BRDF: {  // Specular.cpp:62-68
+      *specOut = true;
+      pdf = 1;
+      ev.wiL = V3(-ev.woL.x, ev.woL.y, -ev.woL.z);
+      ev.wi = ev.ToWorld(ev.wiL);
+      h.point += h.normalG * SURFACE_EPSILON;
+      return SockSpec(c, m, node, 0, ev) / std::abs(ev.wiL.y);
+    }
+    case LGPU_NODE_SPECULAR_BTDF: {  // Specular.cpp:76-95 (normalS flip and world wi.y kept as written)
+      *specOut = true;
+      pdf = 1;
+      const V3 woL = ev.ToLocal(ev.wo);
+      const bool outside = woL.y > 0;
+      h.normalS = h.normalS * (outside ? 1.f : -1.f);
+      const float etaT0 = n.p[0];
+      const float etaI = outside ? ev.eta : etaT0;
+      const float etaT = outside ? etaT0 : ev.eta;
+      Refract(ev.wo, V3(0, 1, 0) * (outside ? 1.f : -1.f), etaI / etaT, &ev.wiL);
+      // FresnelDielectric::Evaluate(cos, etaI) -> FrDielectric(cos, this->etaT, etaI)
+      const float F = FrDielectric(std::abs(ev.wi.y), etaT0, etaT);
+      Spec ft = SockSpec(c, m, node, 0, ev) * (Spec(1) - Spec(F));
+      ft = ft * ((etaI * etaI) / (etaT * etaT));
+      const float inv = 1.f / std::abs(ev.wiL.y);
+      ev.wi = ev.ToWorld(ev.wiL);
+      h.point += h.normalS * .0004f;
+      return ft * inv;
+    }
+    case LGPU_NODE_GHOST_BTDF: {  // Ghost.cpp:13-28
+      *specOut = true;
+      pdf = 1;
+      const float alpha = SockReal(c, m, node, 0, ev);
+      const bool inside = ev.woL.y < 0;
+      if (alpha > 0) {
+        ev.wi = -ev.wo;
+        ev.wiL = ev.ToLocal(ev.wi);
+        h.point += h.normalG * SURFACE_EPSILON * 2.f * (inside ? 1.f : -1.f);
+        return Spec(alpha / std::abs(ev.wiL.y));
+      }
+      pdf = 0;
+      return Spec(0);
+    }
+    case LGPU_NODE_MIX_BSDF: {  // BxDF.cpp:50-60
+      const float ratio = SockReal(c, m, node, 2, ev);
+      bool sp;
+      if (smp.Get1D() > ratio) {
+        const Spec af = BxDF_Sample(c, m, n.in[0], ev, smp, pdf, &sp);
+        if (c.corrected && !sp) pdf = BxDF_Pdf(c, m, node, ev.woL, ev.wiL, ev);  // A20
+        else if (c.corrected) pdf *= (1.f - ratio);
+        *specOut = sp;
+        return af * (1.f - ratio) + BxDF_f(c, m, n.in[1], ev) * ratio;
+      }
+      const Spec bf = BxDF_Sample(c, m, n.in[1], ev, smp, pdf, &sp);
+      if (c.corrected && !sp) pdf = BxDF_Pdf(c, m, node, ev.woL, ev.wiL, ev);
+      else if (c.corrected) pdf *= ratio;
+      *specOut = sp;
+      return bf * ratio + BxDF_f(c, m, n.in[0], ev) * (1.f - ratio);
+    }
+  }
+  pdf = 0;
+  return Spec(0);
+}
+
+// ---------------------------------------------------------------- lights
+static Spec Emission(const Ctx &c, const Material &m, const Event &ev) {
+  if (m.emission == LGPU_NONE) return Spec(0);
+  return AsSpectrum(EvalNode(*c.sc, m, m.emission, ev), ev.lanes);
+}
+static Spec LightColour(const Light &L, const Lanes &l) {
+  if (L.temperature > 0) {
+    Spec s;
+    if (l.mode == HERO4) { for (int i = 0; i < 4; ++i) s.c[i] = BlackbodyNormalised(l.lambda[i], L.temperature); }
+    else { s.c[0] = L.bbRGB[0]; s.c[1] = L.bbRGB[1]; s.c[2] = L.bbRGB[2]; s.c[3] = 0; }
+    return s;
+  }
+  return FromRGB(L.rgb, l);
+}
+static Spec EnvLookup(const Light &L, const V2 &uv, const Lanes &l) {  // TextureAdapter::GetUV
+  float c[3];
+  L.env.Bilinear(Fract(uv.x), Fract(uv.y), c);
+  return FromRGB(c, l);
+}
+static Spec EnvLe(const Light &L, const V3 &w, const Lanes &l) {  // EnvironmentLight.h:44-47
+  const V2 uv(Fract((SphericalPhi(w) - L.offset.x) * INV_PI2), Fract((SphericalTheta(w) - L.offset.y) * INV_PI));
+  return EnvLookup(L, uv, l) * L.intensity;
+}
+static float EnvPdfLi(const Light &L, const V3 &wi) {  // EnvironmentLight.cpp:60-67
+  const float theta = SphericalTheta(wi) - L.offset.y;
+  const float phi = SphericalPhi(wi) - L.offset.x;
+  const float sinT = std::sin(theta);
+  if (sinT == 0) return 0;
+  return L.envDist.PDF(V2(Fract(phi * INV_PI2), Fract(theta * INV_PI))) / (2.f * PI * PI * sinT);
+}
+static inline V3 SamplePointInTriangle(const V3 &v0, const V3 &v1, const V3 &v2, V2 u, bool corrected) {
+  if (corrected && u.x + u.y > 1.f) { u = V2(1.f - u.x, 1.f - u.y); }  // A36: parallelogram -> fold
+  return v0 + (v1 - v0) * u.x + (v2 - v0) * u.y;  // TriangleMesh.h:73-79
+}
+
+// Light::Sample_Li for each light kind. Returns Li, sets pdf and ev.wi/wiL.
+static Spec SampleLi(const Ctx &c, const Light &L, Event &ev, Sampler &smp, float &pdf) {
+  const Scene &sc = *c.sc;
+  Hit &h = *ev.hit;
+  switch (L.kind) {
+    case L_MESH:
+    case L_TRI: {  // MeshLight.cpp:22-41, 134-151
+      const Mesh &m = sc.meshes[L.mesh];
+      const Light &ML = L.kind == L_TRI ? sc.lights[L.parentMeshLight] : L;
+      uint32_t tri;
+      if (L.kind == L_MESH) tri = L.triDist.SampleDiscrete(smp.Get1D(), &pdf);
+      else tri = L.tri;
+      const V2 u = smp.Get2D();
+      const V3 v0 = m.p[m.idx[3 * tri]], v1 = m.p[m.idx[3 * tri + 1]], v2 = m.p[m.idx[3 * tri + 2]];
+      const V3 pL = SamplePointInTriangle(v0, v1, v2, u, c.corrected);
+      const V3 pS = h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+      if (MutualVisibility(sc, pS, pL, &ev.wi)) {
+        float area;
+        V3 normal;
+        TriAreaNormal(v0, v1, v2, &area, &normal);
+        const float dn = Dot(normal, -ev.wi);
+        const float cosT = std::abs(dn);
+        // A29 corrected: one-sided emission, consistent with PDF_Li
+        if (cosT > 0 && !(c.corrected && dn < 0)) {
+          ev.wiL = ev.ToLocal(ev.wi);
+          if (L.kind == L_MESH) pdf *= DistSq(h.point, pL) / (cosT * area);
+          else if (c.corrected) pdf = DistSq(h.point, pL) / (cosT * area);
+          else pdf = DistSq(h.point, pL) * (cosT * area);  // A6
+          const Material &mat = sc.materials[m.material];
+          return Emission(c, mat, ev) * Spec(1.f) * ML.intensity * INV_PI;
+        }
+      }
+      pdf = 0;
+      return Spec(0);
+    }
+    case L_POINT:
+    case L_SPOT: {  // PointLight.cpp:18-30, Spotlight.cpp:29-42
+      const V3 p = h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+      if (MutualVisibility(sc, p, L.position, &ev.wi)) {
+        pdf = 1;
+        const float d2 = DistSq(L.position, h.point);
+        ev.wiL = ev.ToLocal(ev.wi);
+        Spec e = LightColour(L, ev.lanes) * Spec(1.f);
+        if (L.kind == L_SPOT) {
+          const V3 w = -ev.wi;
+          const float cosT = Dot(w, L.axis);  // Spotlight.cpp:21-27
+          float fall;
+          if (cosT < L.cosCone) fall = 0;
+          else if (cosT > L.cosFalloff) fall = 1;
+          else { const float d = (cosT - L.cosCone) / (L.cosFalloff - L.cosCone); fall = (d * d) * (d * d); }
+          e = e * fall;
+        }
+        return e * L.intensity / d2;
+      }
+      pdf = 0;
+      return Spec(0);
+    }
+    case L_ENV: {  // EnvironmentLight.cpp:24-45
+      const V2 uv = L.envDist.SampleContinuous(smp.Get2D(), &pdf);
+      if (pdf == 0) return Spec(0);
+      const float theta = uv.y * PI + L.offset.y;
+      const float phi = uv.x * PI2 + L.offset.x;
+      const float cosT = std::cos(theta), sinT = std::sin(theta);
+      if (sinT == 0) { pdf = 0; return Spec(0); }
+      ev.wi = SphericalDirection(sinT, cosT, phi);
+      if (RayEscapes(sc, h.point + h.normalG * SURFACE_EPSILON, ev.wi)) {
+        ev.wiL = ev.ToLocal(ev.wi);
+        pdf /= 2 * PI * PI * sinT;
+        return EnvLookup(L, uv, ev.lanes) * Spec(1.f) * L.intensity;
+      }
+      pdf = 0;
+      return Spec(0);
+    }
+  }
+  pdf = 0;
+  return Spec(0);
+}
+
+// Light::PDF_Li(event) for a hit on an emitter (MeshLight.cpp:56-64, 157-164)
+static float PdfLiHit(const Ctx &c, const Light &L, const Event &ev, const Hit &hit) {
+  if (Dot(ev.wi, hit.normalG) > 0) return 0;  // one-sided
+  const float distSq = hit.tFar * hit.tFar;
+  V3 v0, v1, v2;
+  TriVerts(*c.sc, hit.prim, v0, v1, v2);
+  float area;
+  TriAreaNormal(v0, v1, v2, &area, nullptr);
+  const float cosT = std::abs(Dot(hit.normalG, -ev.wi));
+  if (L.kind == L_MESH) {
+    const float triPdf = c.corrected ? L.triDist.PDFNormalised(hit.local) : L.triDist.PDF(hit.local);  // A28
+    return triPdf * distSq / (cosT * area);
+  }
+  return distSq / (cosT * area);
+}
+
+static float LightPower(const Scene &sc, const Light &L);
+
+// ---------------------------------------------------------------- light tree build (ManyLightSampler.cpp:37-303)
+static Bounds TLBounds(const Scene &sc, const Light &L) {
+  if (L.kind == L_TRI) {
+    const Mesh &m = sc.meshes[L.mesh];
+    const V3 p0 = m.p[m.idx[3 * L.tri]], p1 = m.p[m.idx[3 * L.tri + 1]], p2 = m.p[m.idx[3 * L.tri + 2]];
+    return Union(Union(Bounds(p0), p1), p2);
+  }
+  return Bounds(L.position);
+}
+static V3 TLDir(const Scene &sc, const Light &L) {
+  if (L.kind == L_TRI) {
+    const Mesh &m = sc.meshes[L.mesh];
+    float area;
+    V3 n;
+    TriAreaNormal(m.p[m.idx[3 * L.tri]], m.p[m.idx[3 * L.tri + 1]], m.p[m.idx[3 * L.tri + 2]], &area, &n);
+    return n;
+  }
+  if (L.kind == L_SPOT) return L.axis;
+  return V3(0, 1, 0);
+}
+static Cone MakeCone(const V3 &axis) { Cone c; c.axis = axis; c.thetaO = 0; c.thetaE = PI / 2.f; return c; }
+// OrientationCone::Union swaps its (const&) arguments in place when b.thetaO > a.thetaO (:51).
+static Cone ConeUnion(Cone &a, Cone &b) {
+  if (b.thetaO > a.thetaO) std::swap(a, b);
+  const float thetaE = std::max(a.thetaE, b.thetaE);
+  const float thetaD = std::acos(Clampf(Dot(a.axis, b.axis), 0.f, 1.f));
+  if (std::min(thetaD + b.thetaO, PI) <= a.thetaO) { Cone r; r.axis = a.axis; r.thetaO = a.thetaO; r.thetaE = thetaE; return r; }
+  const float thetaO = (a.thetaO + thetaD + b.thetaO) * .5f;
+  if (thetaO >= PI) { Cone r; r.axis = a.axis; r.thetaO = PI; r.thetaE = thetaE; return r; }
+  const float thetaR = thetaO - a.thetaO;
+  Cone r;
+  r.axis = Rotate(a.axis, Cross(a.axis, b.axis), thetaR);
+  r.thetaO = thetaO;
+  r.thetaE = thetaE;
+  return r;
+}
+static float MOmega(const Cone &c) {  // :71-78
+  const float cosO = std::cos(c.thetaO);
+  const float sinO = std::sqrt(1 - cosO * cosO);
+  const float a = PI2 * (1.f - cosO);
+  const float thetaW2 = std::min(c.thetaO + c.thetaE, PI) * 2.f;
+  const float b = thetaW2 * sinO - std::cos(c.thetaO - thetaW2) - 2.f * c.thetaO * sinO + cosO;
+  return a + PI * .5f * b;
+}
+struct Bucket { float power = 0; Bounds b; Cone cone; };
+static unsigned BucketOf(float x) {  // float -> unsigned (NaN / negative -> 0)
+  if (!(x >= 0.f)) return 0;
+  unsigned b = (unsigned)x;
+  return b;
+}
+
+struct TreeBuilder {
+  Scene &sc;
+  std::vector<int> &order;  // tlight indices
+  std::vector<float> powers;
+  explicit TreeBuilder(Scene &s) : sc(s), order(s.tOrder) {}
+  float P(unsigned i) { return powers[order[i]]; }
+  Bounds B(unsigned i) { return TLBounds(sc, sc.tlights[order[i]]); }
+  V3 D(unsigned i) { return TLDir(sc, sc.tlights[order[i]]); }
+
+  float SAOH(LNode &P0, const Bucket &R, const Bucket &L, unsigned axis) {  // :83-91
+    const V3 box = P0.bounds.mx - P0.bounds.mn;
+    const float lenMax = std::max(std::max(box.x, box.y), box.z);
+    const float Kr = lenMax / box[axis];
+    const float m = P0.bounds.SurfaceArea() * MOmega(P0.cone);
+    const float mL = L.power * L.b.SurfaceArea() * MOmega(L.cone);
+    const float mR = R.power * R.b.SurfaceArea() * MOmega(R.cone);
+    return Kr * (mL + mR) / m;
+  }
+  bool SplitAxis(LNode *n, unsigned axis, float *minCost, unsigned *bIdx, Bucket *lb, Bucket *rb) {  // :136-185
+    const unsigned NB = 12;
+    Bucket buckets[12];
+    bool init[12] = {false};
+    for (unsigned i = n->first; i < n->first + n->count; ++i) {
+      unsigned b = BucketOf((float)NB * n->bounds.Offset(B(i).Center())[axis]);
+      if (b == NB) b = NB - 1;
+      if (b > NB) b = 0;
+      if (!init[b]) {
+        buckets[b].power = P(i);
+        buckets[b].b = B(i);
+        buckets[b].cone = MakeCone(D(i));
+        init[b] = true;
+      } else {
+        buckets[b].power += P(i);
+        buckets[b].b = Union(buckets[b].b, B(i));
+        Cone tmp = MakeCone(D(i));
+        buckets[b].cone = ConeUnion(buckets[b].cone, tmp);
+      }
+    }
+    bool cheaper = false;
+    for (unsigned i = 0; i < NB - 1; ++i) {
+      Bucket bL, bR;
+      bL.power = 0; bL.b = buckets[0].b; bL.cone = buckets[0].cone;
+      bR.power = 0; bR.b = buckets[i + 1].b; bR.cone = buckets[i + 1].cone;
+      for (unsigned j = 0; j <= i; ++j) {
+        bL.b = Union(bL.b, buckets[j].b);
+        bL.cone = ConeUnion(bL.cone, buckets[j].cone);
+        bL.power += buckets[j].power;
+      }
+      for (unsigned j = i + 1; j < NB; ++j) {
+        bR.b = Union(bR.b, buckets[j].b);
+        bR.cone = ConeUnion(bR.cone, buckets[j].cone);
+        bR.power += buckets[j].power;
+      }
+      const float cost = SAOH(*n, bR, bL, axis);
+      if (cost < *minCost) { *minCost = cost; *lb = bL; *rb = bR; *bIdx = i; cheaper = true; }
+    }
+    return cheaper;
+  }
+  void InitLeaf(LNode *n) {  // :125-134
+    std::vector<float> d(n->count);
+    for (unsigned i = 0; i < n->count; ++i) {
+      d[i] = P(n->first + i);
+      sc.lightLeaf[order[n->first + i]] = {n, i};
+      sc.treePower += d[i];
+    }
+    n->leafIndex = (int)sc.leafDist.size();
+    sc.leafDist.emplace_back(d.data(), n->count);
+    n->child[0] = n->child[1] = nullptr;
+  }
+  void Build(LNode *n) {  // RecursiveBuild :187-248
+    n->bounds = B(n->first);
+    n->cone = MakeCone(D(n->first));
+    for (unsigned i = 0; i < n->count; ++i) {
+      n->bounds = Union(n->bounds, B(i + n->first));
+      Cone tmp = MakeCone(D(i + n->first));
+      n->cone = ConeUnion(n->cone, tmp);
+    }
+    Bucket lb, rb;
+    float minCost = INFINITY;
+    unsigned bIdx = 0;
+    int axis = -1;
+    for (int a = 0; a < 3; ++a)
+      if (SplitAxis(n, (unsigned)a, &minCost, &bIdx, &lb, &rb)) axis = a;
+    if (n->count > 1 && minCost < n->power && axis >= 0) {
+      const unsigned NB = 12;
+      auto pred = [&](int li) {
+        unsigned b = BucketOf(n->bounds.Offset(TLBounds(sc, sc.tlights[li]).Center())[axis] * (float)NB);
+        if (b == NB) b--;
+        if (b > NB) b = 0;
+        return b <= bIdx;
+      };
+      auto it = std::stable_partition(order.begin() + n->first, order.begin() + n->first + n->count, pred);
+      const unsigned pivot = (unsigned)(it - order.begin());
+      const unsigned ln = pivot - n->first, rn = n->count - ln;
+      if (ln != 0 && rn != 0) {
+        LNode *l = new LNode(), *r = new LNode();
+        sc.tnodes.emplace_back(l);
+        sc.tnodes.emplace_back(r);
+        l->parent = n; l->first = n->first; l->count = ln; l->bounds = lb.b; l->cone = lb.cone; l->power = lb.power;
+        r->parent = n; r->first = pivot; r->count = rn; r->bounds = rb.b; r->cone = rb.cone; r->power = rb.power;
+        n->child[0] = l;
+        n->child[1] = r;
+        Build(l);
+        Build(r);
+      } else InitLeaf(n);
+    } else InitLeaf(n);
+  }
+};
+
+static float TriLightIrradiance(const Ctx &c, const Light &TL) {  // TriangleLight::Irradiance, MeshLight.cpp:203-221
+  const Scene &sc = *c.sc;
+  const Light &ML = sc.lights[TL.parentMeshLight];
+  const Mesh &m = sc.meshes[TL.mesh];
+  const Material &mat = sc.materials[m.material];
+  const unsigned S = 3;
+  const float inv = 1.f / (float)S;
+  Hit fh;
+  Event fe;
+  fe.hit = &fh;
+  fe.lanes.mode = RGB;
+  float irr = 0;
+  for (unsigned a = 0; a < S; ++a)
+    for (unsigned b = 0; b < S; ++b) {
+      if (!m.uv.empty()) {
+        const V2 &u0 = m.uv[m.idx[3 * TL.tri]], &u1 = m.uv[m.idx[3 * TL.tri + 1]], &u2 = m.uv[m.idx[3 * TL.tri + 2]];
+        const float ua = (float)a * inv, ub = (float)b * inv;
+        fh.uv = V2(u0.x + (u1.x - u0.x) * ua + (u2.x - u0.x) * ub, u0.y + (u1.y - u0.y) * ua + (u2.y - u0.y) * ub);
+      }
+      irr += Luminance(Emission(c, mat, fe), fe.lanes);
+    }
+  return irr * inv * inv * ML.intensity;
+}
+static float LightPower(const Scene &sc, const Light &L) {  // Light::Power = Irradiance * Area
+  switch (L.kind) {
+    case L_MESH: {
+      const Mesh &m = sc.meshes[L.mesh];
+      float a = 0;
+      for (size_t i = 0; i < m.idx.size() / 3; ++i) a += Mag(Cross(m.p[m.idx[3 * i + 1]] - m.p[m.idx[3 * i]], m.p[m.idx[3 * i + 2]] - m.p[m.idx[3 * i]]));
+      return L.intensity * (a * .5f);
+    }
+    case L_POINT: return L.intensity * (4 * PI);
+    case L_SPOT: return L.intensity * (PI2 - (1 - .5f * (L.cosFalloff + L.cosCone)));
+    case L_ENV: return L.intensity * (PI * L.radius * L.radius);
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- light samplers
+static float TreeImportance(const LNode *nd, const V3 &point, const V3 &normalS) {
+  V3 delta = nd->bounds.Center() - point;
+  const float diam = nd->bounds.Diagonal();
+  const float d2 = Dot(delta, delta);
+  const float d = std::sqrt(d2);
+  const float invD = 1.f / d;
+  delta = delta * invD;
+  const float theta = std::acos(Dot(delta, nd->cone.axis));
+  const float thetaU = std::atan((diam * .5f) * invD);
+  const float thetaDash = std::max(theta - nd->cone.thetaO - thetaU, 0.f);
+  if (thetaDash < nd->cone.thetaE) {
+    const float E = nd->power;
+    const float thetaI = std::acos(Dot(delta, normalS));
+    return (std::cos(std::max(thetaI - thetaU, 0.f)) * E) / d2 * std::cos(thetaDash);
+  }
+  return 0.f;
+}
+
+struct LightPick { int kind; int index; };  // kind 0: scene light index, 1: tree light index, -1 none
+
+static LightPick SampleLight(const Ctx &c, const Event &ev, Sampler &smp, float *pdf) {
+  const Scene &sc = *c.sc;
+  if (sc.strategy == LGPU_LIGHTS_POWER) {  // LightSampler.cpp:14-16
+    if (sc.lights.empty()) { *pdf = 0; return {-1, -1}; }
+    const unsigned i = sc.powerDist.SampleDiscrete(smp.Get1D(), pdf);
+    return {0, (int)i};
+  }
+  // ManyLightSampler::Sample (ManyLightSampler.cpp:17-27)
+  float eps = smp.Get1D();
+  const float pTree = sc.treePower / (sc.infPower + sc.treePower);
+  if (eps < pTree) {
+    *pdf *= pTree;
+    float e = smp.Get1D();  // A5: the rescaled epsilon is discarded
+    LNode *n = sc.root;
+    while (!n->IsLeaf()) {  // PickLight :355-378
+      const float IL = TreeImportance(n->child[0], ev.hit->point, ev.hit->normalS);
+      const float IR = TreeImportance(n->child[1], ev.hit->point, ev.hit->normalS);
+      const float PL = (IL + IR == 0.f) ? .5f : IL / (IL + IR);  // A10
+      const float PR = 1 - PL;
+      if (e < PL) { e /= PL; *pdf *= PL; n = n->child[0]; }
+      else { e = (e - PL) / PR; *pdf *= PR; n = n->child[1]; }
+    }
+    float lp = 1;
+    const unsigned i = sc.leafDist[n->leafIndex].SampleDiscrete(e, &lp);
+    *pdf *= lp;
+    return {1, sc.tOrder[n->first + i]};
+  }
+  if (sc.envLight < 0) { *pdf = 0; return {-1, -1}; }
+  *pdf *= 1 - pTree;
+  return {0, sc.envLight};
+}
+
+// LightSampler::Pdf for the light hit by a BSDF sample.
+// ManyLightSampler::Pdf / RecursivePDF (ManyLightSampler.cpp:29-35, 380-393);
+// PowerLightSampler::Pdf (LightSampler.cpp:18-20).
+static float LightPdf(const Ctx &c, const LightPick &lp, const V3 &point, const V3 &normalS) {
+  const Scene &sc = *c.sc;
+  if (sc.strategy == LGPU_LIGHTS_POWER) return LightPower(sc, sc.lights[lp.index]) * sc.invTotalPower;
+  if (lp.kind == 0 && lp.index == sc.envLight) return sc.infPower / (sc.infPower + sc.treePower);
+  const std::pair<LNode *, unsigned> &ref = sc.lightLeaf[lp.index];
+  const Piecewise1D &ld = sc.leafDist[ref.first->leafIndex];
+  float p = c.corrected ? ld.PDFNormalised(ref.second) : ld.PDF(ref.second);  // A28
+  const LNode *child = ref.first;
+  for (const LNode *n = ref.first->parent; n; child = n, n = n->parent) {
+    const float IL = TreeImportance(n->child[0], point, normalS);
+    const float IR = TreeImportance(n->child[1], point, normalS);
+    const float p0 = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
+    const float p1 = c.corrected ? 1.f - p0 : 1.f - IR;  // A7
+    p *= (child == n->child[0]) ? p0 : p1;
+  }
+  if (c.corrected) p *= sc.treePower / (sc.infPower + sc.treePower);
+  return p;
+}
+
+static inline float PowerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }  // Integrator.h:22-29
+
+// ---------------------------------------------------------------- integrators
+struct RenderParams {
+  uint32_t integrator, maxBounces, minBounces, mode, seed;
+  bool corrected;
+};
+
+static const Light &LightOf(const Scene &sc, const LightPick &lp) { return lp.kind == 1 ? sc.tlights[lp.index] : sc.lights[lp.index]; }
+static bool SameLight(const LightPick &a, const LightPick &b) { return a.kind == b.kind && a.index == b.index; }
+
+// PathIntegrator::Li (PathIntegrator.cpp:15-88)
+static Spec PathLi(const Ctx &c, const RenderParams &rp, V3 ro, V3 rd, Sampler &smp, const Lanes &lanes) {
+  const Scene &sc = *c.sc;
+  Spec L(0), beta(1);
+  Hit hit;
+  Event ev;
+  ev.hit = &hit;
+  ev.lanes = lanes;
+  ev.wo = -rd;
+  bool scatterIntersect = false;
+  int passThrough = 0;
+  for (int bounces = 0; bounces < (int)rp.maxBounces; ++bounces) {
+    if (bounces == 0 ? Intersect(sc, ro, rd, hit, c.corrected) : scatterIntersect) {
+      const Material &mat = sc.materials[sc.meshes[hit.mesh].material];
+      const bool isLight = mat.emission != LGPU_NONE;
+      if (bounces == 0 && isLight) {
+        const Mesh &m = sc.meshes[hit.mesh];
+        if (!c.corrected || Dot(rd, hit.normalG) < 0)  // A29 corrected: front face only
+          L += Emission(c, mat, ev) * sc.lights[m.meshLight].intensity * INV_PI;
+      }
+      if (mat.bxdf != LGPU_NONE) {
+        ev.wo = -rd;
+        ev.SurfaceLocalise();
+        float lightDistPdf = 1;
+        const V3 shadePoint = hit.point, shadeNormalS = hit.normalS;
+        const LightPick l = SampleLight(c, ev, smp, &lightDistPdf);
+        Spec Ld(0);
+        float scatteringPDF = 0, lightPDF = 0;
+        Spec f, Li(0);
+        bool deltaLight = false;
+        if (l.kind >= 0) {
+          const Light &LL = LightOf(sc, l);
+          deltaLight = LL.kind == L_POINT || LL.kind == L_SPOT;
+          Li = SampleLi(c, LL, ev, smp, lightPDF);
+        }
+        lightPDF *= lightDistPdf;
+        if (lightPDF > 0 && !IsBlack(Li, lanes)) {
+          f = BxDF_f(c, mat, mat.bxdf, ev) * std::abs(ev.wiL.y);
+          scatteringPDF = BxDF_Pdf(c, mat, mat.bxdf, ev.woL, ev.wiL, ev);
+          if (!IsBlack(f, lanes) && scatteringPDF > 0) {
+            const float w = (c.corrected && deltaLight) ? 1.f : PowerHeuristic(lightPDF, scatteringPDF);  // A38
+            Ld += Li * f * w / lightPDF;
+          }
+        }
+        bool specular;
+        f = BxDF_Sample(c, mat, mat.bxdf, ev, smp, scatteringPDF, &specular);
+        f = f * std::abs(ev.wiL.y);
+        ro = hit.point;
+        rd = ev.wi;
+        scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+        if (scatteringPDF > 0 && !IsBlack(f, lanes)) {
+          LightPick nl{-1, -1};
+          if (scatterIntersect) {
+            const Mesh &hm = sc.meshes[hit.mesh];
+            if (hm.meshLight >= 0) {
+              if (sc.strategy == LGPU_LIGHTS_TREE) nl = {1, sc.triToTLight[hit.prim]};
+              else nl = {0, hm.meshLight};
+            }
+          } else if (sc.envLight >= 0) nl = {0, sc.envLight};
+          if (nl.kind >= 0) {
+            // nl != l: the reference compares Light pointers; for the tree a
+            // MeshLight never equals the sampled TriangleLight (:55).
+            bool same = SameLight(nl, l);
+            if (!c.corrected && sc.strategy == LGPU_LIGHTS_TREE && nl.kind == 1) same = false;
+            if (!same) {
+              // A8: faithful evaluates the tree at the hit on the light
+              if (c.corrected) lightDistPdf = LightPdf(c, nl, shadePoint, shadeNormalS);
+              else lightDistPdf = LightPdf(c, nl, hit.point, hit.normalS);
+            }
+            const Light &NL = LightOf(sc, nl);
+            float pl;
+            Spec Le;
+            if (scatterIntersect) {
+              // faithful: MeshLight::PDF_Li (mesh triangle distribution, A28) even under the tree
+              if (!c.corrected && sc.strategy == LGPU_LIGHTS_TREE) pl = PdfLiHit(c, sc.lights[NL.parentMeshLight], ev, hit);
+              else pl = PdfLiHit(c, NL, ev, hit);
+              const Material &hmat = sc.materials[sc.meshes[hit.mesh].material];
+              const int mlIdx = sc.meshes[hit.mesh].meshLight;
+              Le = Emission(c, hmat, ev) * sc.lights[mlIdx].intensity * INV_PI;
+              if (c.corrected && Dot(rd, hit.normalG) > 0) Le = Spec(0);
+            } else {
+              pl = EnvPdfLi(NL, ev.wi);
+              Le = EnvLe(NL, rd, lanes);
+            }
+            lightPDF = lightDistPdf * pl;
+            const float w = (c.corrected && specular) ? 1.f : PowerHeuristic(scatteringPDF, lightPDF);  // A40
+            if (!IsBlack(Le, lanes) && (lightPDF > 0 || c.corrected)) Ld += Le * f * w / scatteringPDF;
+          }
+        } else {
+          if (c.corrected) L += beta * Ld;  // A37: NEE kept when the BSDF sample fails
+          break;
+        }
+        L += beta * Ld;
+        beta = beta * (f / scatteringPDF);
+      } else {
+        // pass-through surface (PathIntegrator.cpp:69-74)
+        ro = hit.point + hit.normalG * (Dot(rd, hit.normalG) < 0 ? -SURFACE_EPSILON : SURFACE_EPSILON);
+        if (bounces > 0) scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+        bounces--;
+        if (++passThrough > 1024) break;  // A27 cap
+        continue;
+      }
+      if (bounces > (int)rp.minBounces) {  // :76-80
+        const float q = std::max(.05f, 1 - Luminance(beta, lanes));
+        if (smp.Get1D() < q) break;
+        beta = beta / Spec(1.f - q);
+      }
+    } else {
+      if (bounces == 0 && sc.envLight >= 0) L += EnvLe(sc.lights[sc.envLight], rd, lanes);
+      break;
+    }
+  }
+  return L;
+}
+
+// Camera::GenerateRay (Camera.cpp:41-71) with the aperture draws of Aperture.cpp:10-27
+static void GenerateRay(const lgpu_camera &cam, float u, float v, Sampler &smp, V3 *o, V3 *d, bool corrected) {
+  const V3 origin(cam.origin), xh(cam.x_hat), yh(cam.y_hat), zh(cam.z_hat);
+  const float tanFov = cam.tan_fov, tanFov2 = 2.f * cam.tan_fov, aspect = cam.aspect;
+  if (cam.type == LGPU_CAMERA_SPHERICAL) {
+    const float phi = PI2 * u + cam.offset_phi;
+    const float theta = PI * v + cam.offset_theta;
+    const V3 dd = Norm(SphericalDirection(std::sin(theta), std::cos(theta), phi));
+    *o = origin;
+    *d = xh * dd.x + yh * dd.y + zh * dd.z;
+    return;
+  }
+  if (cam.type == LGPU_CAMERA_PINHOLE) {
+    const V3 p = origin + xh * (u * -tanFov2 + tanFov) + yh * (v * -tanFov2 * aspect + tanFov * aspect) + zh;
+    *o = origin;
+    *d = Norm(p - origin);
+    return;
+  }
+  const V3 fp = origin + (xh * (u * -tanFov2 + tanFov) + yh * (v * -tanFov2 * aspect + tanFov * aspect) + zh) * cam.focal_length;
+  V2 ap(0, 0);
+  if (cam.aperture == LGPU_APERTURE_CIRCULAR) {
+    const V2 dk = SampleUnitDisk(smp.Get2D());
+    const float s = cam.aperture_size;
+    ap = V2(dk.x * s - s * .5f, dk.y * s - s * .5f);  // A24: offset disk (as written)
+    if (corrected) ap = V2(dk.x * s, dk.y * s);
+  } else if (cam.aperture == LGPU_APERTURE_BLADE) {
+    const float dth = PI2 / (float)cam.blades;
+    const float rnd = PI2 * smp.Get1D();
+    const float tmin = std::floor(rnd / dth) * dth;
+    const float tmax = tmin + dth;
+    const V2 p1(std::cos(tmin), std::sin(tmin)), p2(std::cos(tmax), std::sin(tmax));
+    const V2 uu = smp.Get2D();
+    ap = V2((p1.x * uu.x + p2.x * uu.y) * cam.aperture_size, (p1.y * uu.x + p2.y * uu.y) * cam.aperture_size);
+  }
+  const V3 oo = origin + xh * ap.x + yh * ap.y;
+  *o = oo;
+  *d = Norm(fp - oo);
+}
+
+// TileRenderers::UniformSpp per-sample body (Render.cpp:53-60): returns the path's spectrum.
+static Spec RenderSample(const Ctx &c, const RenderParams &rp, const lgpu_camera &cam, uint32_t W, uint32_t H,
+                         uint32_t pixel, uint32_t sample, float *lambda0) {
+  Sampler smp;
+  smp.pixel = pixel; smp.sample = sample; smp.seed = rp.seed; smp.dim = 0;
+  Lanes lanes;
+  lanes.mode = (int)rp.mode;
+  float l0 = 0;
+  if (rp.mode == HERO4) { HeroWavelengths(WavelengthU(pixel, sample, rp.seed), lanes.lambda); l0 = lanes.lambda[0]; }
+  else { lanes.lambda[0] = lanes.lambda[1] = lanes.lambda[2] = lanes.lambda[3] = 0; }
+  const uint32_t x = pixel % W, y = pixel / W;
+  const float xi = 1.f / (float)W, yi = 1.f / (float)H;
+  const float u = xi * ((float)x + smp.Get1D() - .5f);
+  const float v = yi * ((float)y + smp.Get1D() - .5f);
+  V3 o, d;
+  GenerateRay(cam, u, v, smp, &o, &d, c.corrected);
+  if (lambda0) *lambda0 = l0;
+  return PathLi(c, rp, o, d, smp, lanes);
+}
+
+// Spectral -> linear sRGB for the film (project Hero4 estimator, XYZToRGB Spectrum.h:194-198)
+static void ToRGB(const Spec &L, int mode, const float lambda[4], double rgb[3]) {
+  if (mode != HERO4) { rgb[0] = L.c[0]; rgb[1] = L.c[1]; rgb[2] = L.c[2]; return; }
+  double xyz[3] = {0, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    float cm[3];
+    CMF(lambda[i], cm);
+    for (int k = 0; k < 3; ++k) xyz[k] += (double)L.c[i] * (double)cm[k];
+  }
+  const double s = 470.0 / 4.0 / CIE_Y_INTEGRAL;
+  for (int k = 0; k < 3; ++k) xyz[k] *= s;
+  rgb[0] = 3.240479 * xyz[0] - 1.537150 * xyz[1] - 0.498535 * xyz[2];
+  rgb[1] = -0.969256 * xyz[0] + 1.875991 * xyz[1] + 0.041556 * xyz[2];
+  rgb[2] = 0.055648 * xyz[0] - 0.204043 * xyz[1] + 1.057311 * xyz[2];
+}
+
+static void ParallelFor(uint32_t n, int threads, const std::function<void(uint32_t)> &fn) {
+  if (threads <= 1 || n < 2) { for (uint32_t i = 0; i < n; ++i) fn(i); return; }
+  std::atomic<uint32_t> next(0);
+  std::vector<std::thread> ts;
+  for (int t = 0; t < threads; ++t)
+    ts.emplace_back([&]() { for (uint32_t i; (i = next.fetch_add(1)) < n;) fn(i); });
+  for (auto &t : ts) t.join();
+}
+
+}  // namespace orc
+
+// ======================================================================== C API
+extern "C" {
+
+oracle_scene oracle_scene_create(void) { return new oracle_scene_s(); }
+void oracle_scene_destroy(oracle_scene s) { delete s; }
+
+int oracle_scene_add_texture(oracle_scene s, const lgpu_texture_desc *d) {
+  if (!s || !d || !d->data || !d->width || !d->height || (d->channels != 1 && d->channels != 3)) return -1;
+  Texture t;
+  t.w = d->width; t.h = d->height; t.ch = d->channels;
+  t.data.assign(d->data, d->data + (size_t)d->width * d->height * d->channels);
+  s->textures.push_back(std::move(t));
+  return (int)s->textures.size() - 1;
+}
+int oracle_scene_add_medium(oracle_scene s, const lgpu_medium_desc *d) {
+  if (!s || !d) return -1;
+  s->media.push_back(*d);
+  return (int)s->media.size() - 1;
+}
+int oracle_scene_add_material(oracle_scene s, const lgpu_material_desc *d) {
+  if (!s || !d || (d->num_nodes && !d->nodes)) return -1;
+  Material m;
+  m.nodes.assign(d->nodes, d->nodes + d->num_nodes);
+  m.bxdf = d->bxdf_root; m.emission = d->emission_root; m.intensity = d->emission_intensity;
+  m.medIn = d->medium_interior; m.medOut = d->medium_exterior;
+  m.bbRGB.assign(3 * m.nodes.size(), 0.f);
+  m.bbConst.assign(m.nodes.size(), 0);
+  for (size_t i = 0; i < m.nodes.size(); ++i) {  // constant-temperature blackbody: evaluate once
+    const lgpu_node &n = m.nodes[i];
+    if (n.type == LGPU_NODE_BLACKBODY && n.in[0] < m.nodes.size() && m.nodes[n.in[0]].type == LGPU_NODE_SCALAR) {
+      BlackbodyRGB(m.nodes[n.in[0]].p[0], &m.bbRGB[3 * i]);
+      m.bbConst[i] = 1;
+    }
+  }
+  s->materials.push_back(std::move(m));
+  return (int)s->materials.size() - 1;
+}
+int oracle_scene_add_mesh(oracle_scene s, const lgpu_mesh_desc *d) {
+  if (!s || !d || !d->positions || !d->indices || d->material >= s->materials.size()) return -1;
+  Mesh m;
+  for (uint32_t i = 0; i < d->num_vertices; ++i) m.p.push_back(V3(&d->positions[3 * i]));
+  if (d->normals && d->tangents) {
+    for (uint32_t i = 0; i < d->num_vertices; ++i) m.n.push_back(V3(&d->normals[3 * i]));
+    m.tan.assign(d->tangents, d->tangents + 4 * (size_t)d->num_vertices);
+  }
+  if (d->uvs) for (uint32_t i = 0; i < d->num_vertices; ++i) m.uv.push_back(V2(d->uvs[2 * i], d->uvs[2 * i + 1]));
+  m.idx.assign(d->indices, d->indices + 3 * (size_t)d->num_triangles);
+  m.material = d->material;
+  m.firstTri = (uint32_t)s->triMesh.size();
+  const int mi = (int)s->meshes.size();
+  for (uint32_t t = 0; t < d->num_triangles; ++t) s->triMesh.push_back((uint32_t)mi);
+  const Material &mat = s->materials[d->material];
+  if (mat.emission != LGPU_NONE) {  // MeshLight(mesh) + Scene::AddObject(_addLight) (Scene.cpp:114-121)
+    Light L;
+    L.kind = L_MESH;
+    L.mesh = mi;
+    L.intensity = mat.intensity;
+    std::vector<float> areas(d->num_triangles);
+    for (uint32_t t = 0; t < d->num_triangles; ++t) {
+      float a;
+      TriAreaNormal(m.p[m.idx[3 * t]], m.p[m.idx[3 * t + 1]], m.p[m.idx[3 * t + 2]], &a, nullptr);
+      areas[t] = a;
+    }
+    L.triDist = Piecewise1D(areas.data(), d->num_triangles);
+    m.meshLight = (int)s->lights.size();
+    s->lights.push_back(std::move(L));
+  }
+  s->meshes.push_back(std::move(m));
+  return mi;
+}
+int oracle_scene_add_light(oracle_scene s, const lgpu_light_desc *d) {
+  if (!s || !d) return -1;
+  Light L;
+  L.intensity = d->intensity;
+  L.rgb[0] = d->rgb[0]; L.rgb[1] = d->rgb[1]; L.rgb[2] = d->rgb[2];
+  L.temperature = d->temperature;
+  if (L.temperature > 0) BlackbodyRGB(L.temperature, L.bbRGB);
+  L.position = V3(d->position);
+  if (d->type == LGPU_LIGHT_POINT) L.kind = L_POINT;
+  else if (d->type == LGPU_LIGHT_SPOT) {  // Spotlight.cpp:14-19
+    L.kind = L_SPOT;
+    L.axis = V3(d->axis);
+    L.cosCone = std::cos(d->cone_angle);
+    L.cosFalloff = std::cos(d->falloff_start);
+  } else if (d->type == LGPU_LIGHT_ENVIRONMENT) {  // EnvironmentLight.cpp:7-22
+    if (!d->env_rgb || !d->env_width || !d->env_height || s->envLight >= 0) return -1;
+    L.kind = L_ENV;
+    L.env.w = d->env_width; L.env.h = d->env_height; L.env.ch = 3;
+    L.env.data.assign(d->env_rgb, d->env_rgb + 3 * (size_t)d->env_width * d->env_height);
+    L.offset = V2(d->env_offset[0], d->env_offset[1]);
+    const unsigned w = d->env_width, h = d->env_height;
+    std::vector<float> img((size_t)w * h);
+    for (unsigned y = 0; y < h; ++y) {
+      const float vp = (float)y / (float)h;
+      const float sinT = std::sin(PI * (float)(y + .5) / (float)h);
+      for (unsigned x = 0; x < w; ++x) {
+        const float up = (float)x / (float)w;
+        float c[3];
+        L.env.Bilinear(Fract(up), Fract(vp), c);
+        img[x + y * w] = std::abs(0.212671f * c[0] + 0.715160f * c[1] + 0.072169f * c[2]);
+        img[x + y * w] *= sinT;
+      }
+    }
+    L.envDist = Piecewise2D(img.data(), w, h);
+    s->envLight = (int)s->lights.size();
+  } else return -1;
+  s->lights.push_back(std::move(L));
+  return (int)s->lights.size() - 1;
+}
+
+int oracle_scene_commit(oracle_scene s, uint32_t strategy) {
+  if (!s || s->committed) return -1;
+  s->strategy = strategy;
+  BuildBVH(*s);
+  Ctx c{s, false, RGB};
+  if (s->envLight >= 0) {  // Scene::Commit (Scene.cpp:18-26)
+    Light &E = s->lights[s->envLight];
+    E.envBounds = s->sceneBounds;
+    E.radius = s->sceneBounds.MaxLength();
+  }
+  if (strategy == LGPU_LIGHTS_POWER) {  // PowerLightSampler::Commit (LightSampler.cpp:22-35)
+    const unsigned n = (unsigned)s->lights.size();
+    if (n) {
+      std::vector<float> imp(n);
+      float total = 0;
+      for (unsigned i = 0; i < n; ++i) { imp[i] = LightPower(*s, s->lights[i]); total += imp[i]; }
+      s->powerDist = Piecewise1D(imp.data(), n);
+      s->invTotalPower = 1.f / total;
+    }
+  } else {  // ManyLightSampler::Commit / InitLights (ManyLightSampler.cpp:37-44, 250-303)
+    s->triToTLight.assign(s->triMesh.size(), -1);
+    s->sceneToTLight.assign(s->lights.size(), -1);
+    for (size_t li = 0; li < s->lights.size(); ++li) {
+      const Light &L = s->lights[li];
+      if (L.kind != L_MESH) continue;
+      const Mesh &m = s->meshes[L.mesh];
+      for (uint32_t t = 0; t < m.idx.size() / 3; ++t) {
+        Light T;
+        T.kind = L_TRI; T.mesh = L.mesh; T.tri = t; T.parentMeshLight = (int)li; T.intensity = L.intensity;
+        s->triToTLight[m.firstTri + t] = (int)s->tlights.size();
+        s->tlights.push_back(std::move(T));
+      }
+    }
+    for (size_t li = 0; li < s->lights.size(); ++li) {
+      const Light &L = s->lights[li];
+      if (L.kind == L_POINT || L.kind == L_SPOT) {
+        s->sceneToTLight[li] = (int)s->tlights.size();
+        s->tlights.push_back(L);
+      }
+    }
+    TreeBuilder tb(*s);
+    tb.powers.resize(s->tlights.size());
+    for (size_t i = 0; i < s->tlights.size(); ++i) {
+      const Light &T = s->tlights[i];
+      tb.powers[i] = T.kind == L_TRI ? TriLightIrradiance(c, T) * [&]() { float a; V3 v0, v1, v2; TriVerts(*s, s->meshes[T.mesh].firstTri + T.tri, v0, v1, v2); TriAreaNormal(v0, v1, v2, &a, nullptr); return a; }()
+                                     : LightPower(*s, T);
+    }
+    s->tOrder.resize(s->tlights.size());
+    for (size_t i = 0; i < s->tlights.size(); ++i) s->tOrder[i] = (int)i;
+    s->lightLeaf.assign(s->tlights.size(), {nullptr, 0});
+    s->treePower = 0;
+    s->infPower = 0;
+    if (s->envLight >= 0) s->infPower = LightPower(*s, s->lights[s->envLight]) * .1f;
+    if (!s->tlights.empty()) {
+      LNode *r = new LNode();
+      s->tnodes.emplace_back(r);
+      r->count = (unsigned)s->tlights.size();
+      r->first = 0;
+      r->bounds = tb.B(0);
+      r->power = 0;
+      r->cone = MakeCone(tb.D(0));
+      for (unsigned i = 0; i < r->count; ++i) {
+        r->bounds = Union(r->bounds, tb.B(i));
+        Cone tmp = MakeCone(tb.D(i));
+        r->cone = ConeUnion(r->cone, tmp);
+        r->power += tb.P(i);
+      }
+      s->root = r;
+      tb.Build(r);
+    }
+  }
+  s->committed = true;
+  return 0;
+}
+
+int oracle_light_tree(oracle_scene s, float *out, uint32_t max_nodes) {
+  if (!s || !s->root) return 0;
+  // Depth-first pre-order, left child first: the product's flattening order.
+  std::vector<const LNode *> order;
+  std::vector<const LNode *> st{s->root};
+  while (!st.empty()) {
+    const LNode *n = st.back();
+    st.pop_back();
+    order.push_back(n);
+    if (!n->IsLeaf()) { st.push_back(n->child[1]); st.push_back(n->child[0]); }
+  }
+  auto idxOf = [&](const LNode *n) -> float { for (size_t i = 0; i < order.size(); ++i) if (order[i] == n) return (float)i; return -1.f; };
+  for (size_t i = 0; i < order.size() && i < max_nodes; ++i) {
+    const LNode *n = order[i];
+    float *o = out + 15 * i;
+    const V3 c = n->bounds.Center();
+    o[0] = c.x; o[1] = c.y; o[2] = c.z; o[3] = n->bounds.Diagonal();
+    o[4] = n->cone.axis.x; o[5] = n->cone.axis.y; o[6] = n->cone.axis.z;
+    o[7] = n->cone.thetaO; o[8] = n->cone.thetaE; o[9] = n->power;
+    o[10] = n->IsLeaf() ? -1.f : idxOf(n->child[0]);
+    o[11] = n->IsLeaf() ? -1.f : idxOf(n->child[1]);
+    o[12] = (float)n->first; o[13] = (float)n->count;
+    o[14] = n->parent ? idxOf(n->parent) : -1.f;
+  }
+  return (int)order.size();
+}
+
+static RenderParams Params(const lgpu_render_desc *d) {
+  RenderParams rp;
+  rp.integrator = d->integrator; rp.maxBounces = d->max_bounces; rp.minBounces = d->min_bounces;
+  rp.mode = d->spectral_mode; rp.seed = d->seed; rp.corrected = d->estimator_mode == LGPU_ESTIMATOR_CORRECTED;
+  return rp;
+}
+
+int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width, uint32_t height, uint32_t n,
+                       const uint32_t *pixels, const uint32_t *samples, float *out, int threads) {
+  if (!s || !s->committed || !d || !width || !height) return -1;
+  if (d->integrator != LGPU_INTEGRATOR_PATH) return -3;
+  const RenderParams rp = Params(d);
+  const Ctx c{s, rp.corrected, (int)rp.mode};
+  ParallelFor(n, threads, [&](uint32_t i) {
+    float l0;
+    const Spec L = RenderSample(c, rp, d->camera, width, height, pixels[i], samples[i], &l0);
+    for (int k = 0; k < 4; ++k) out[5 * i + k] = L.c[k];
+    if (rp.mode != HERO4) out[5 * i + 3] = 0;
+    out[5 * i + 4] = l0;
+  });
+  return 0;
+}
+
+int oracle_render(oracle_scene s, const lgpu_render_desc *d, uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
+                  uint32_t w, uint32_t h, int threads, double *rgb, uint32_t *counts) {
+  if (!s || !s->committed || !d || x0 + w > W || y0 + h > H) return -1;
+  if (d->integrator != LGPU_INTEGRATOR_PATH) return -3;
+  const RenderParams rp = Params(d);
+  const Ctx c{s, rp.corrected, (int)rp.mode};
+  const uint32_t TS = 16;
+  const uint32_t tx = (w + TS - 1) / TS, ty = (h + TS - 1) / TS;
+  std::memset(rgb, 0, sizeof(double) * 3 * w * h);
+  std::memset(counts, 0, sizeof(uint32_t) * w * h);
+  ParallelFor(tx * ty, threads, [&](uint32_t t) {
+    const uint32_t bx = (t % tx) * TS, by = (t / tx) * TS;
+    for (uint32_t y = by; y < std::min(by + TS, h); ++y)
+      for (uint32_t x = bx; x < std::min(bx + TS, w); ++x) {
+        const uint32_t pixel = (y + y0) * W + (x + x0);
+        for (uint32_t smp = d->spp_begin; smp < d->spp_end; ++smp) {
+          float lam[4];
+          if (rp.mode == HERO4) HeroWavelengths(WavelengthU(pixel, smp, rp.seed), lam);
+          float l0;
+          const Spec L = RenderSample(c, rp, d->camera, W, H, pixel, smp, &l0);
+          double o[3];
+          ToRGB(L, (int)rp.mode, lam, o);
+          double *px = rgb + 3 * ((size_t)y * w + x);
+          px[0] += o[0]; px[1] += o[1]; px[2] += o[2];
+          counts[(size_t)y * w + x]++;
+        }
+      }
+  });
+  return 0;
+}
+
+int oracle_intersect(oracle_scene s, uint32_t n, const float *rays, uint32_t *hits) {
+  if (!s || !s->committed) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *r = rays + 8 * i;
+    float t, u, v;
+    uint32_t prim;
+    if (ClosestHit(*s, V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), r[3], r[7], &t, &u, &v, &prim)) {
+      std::memcpy(&hits[4 * i], &t, 4); std::memcpy(&hits[4 * i + 1], &u, 4); std::memcpy(&hits[4 * i + 2], &v, 4);
+      hits[4 * i + 3] = prim;
+    } else {
+      const float inf = INFINITY;
+      std::memcpy(&hits[4 * i], &inf, 4); hits[4 * i + 1] = 0; hits[4 * i + 2] = 0; hits[4 * i + 3] = LGPU_NONE;
+    }
+  }
+  return 0;
+}
+int oracle_occluded(oracle_scene s, uint32_t n, const float *rays, uint32_t *occ) {
+  if (!s || !s->committed) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *r = rays + 8 * i;
+    occ[i] = AnyHit(*s, V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), r[3], r[7]) ? 1u : 0u;
+  }
+  return 0;
+}
+
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  Philox(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+float oracle_fr_dielectric(float c, float a, float b) { return FrDielectric(c, a, b); }
+void oracle_fr_conductor(float cosI, const float ei[3], const float et[3], const float k[3], float out[3]) {
+  Spec a, b, kk;
+  for (int i = 0; i < 3; ++i) { a.c[i] = ei[i]; b.c[i] = et[i]; kk.c[i] = k[i]; }
+  a.c[3] = b.c[3] = kk.c[3] = 1;
+  const Spec r = FrConductor(cosI, a, b, kk);
+  for (int i = 0; i < 3; ++i) out[i] = r.c[i];
+}
+float oracle_microfacet_d(int ggx, const float wh[3], float ax, float ay) { return MfD(ggx, V3(wh), ax, ay); }
+float oracle_microfacet_lambda(int ggx, const float w[3], float ax, float ay) { return MfLambda(ggx, V3(w), ax, ay); }
+void oracle_blackbody_rgb(float T, float out[3]) { BlackbodyRGB(T, out); }
+float oracle_blackbody_normalised(float l, float T) { return BlackbodyNormalised(l, T); }
+void oracle_cmf(float l, float out[3]) { CMF(l, out); }
+float oracle_uplift(const float rgb[3], float l) { return Uplift(rgb, l); }
+float oracle_hg_p(float cosT, float g) {  // HenyeyGreenstein.cpp:8-12
+  const float d = 1 + g * g + 2 * g * cosT;
+  return INV_PI4 * (1 - g * g) / (d * std::sqrt(d));
+}
+uint32_t oracle_piecewise1d_sample(const float *f, uint32_t n, float u, float *pdf) {
+  Piecewise1D d(f, n);
+  return d.SampleDiscrete(u, pdf);
+}
+
+}  // extern "C"

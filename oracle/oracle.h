@@ -1,0 +1,63 @@
+/*
+ * oracle.h — CPU restatement of the Zoophish/Lambda render path, used ONLY as
+ * test infrastructure: tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it as the checker. The product library never links it.
+ *
+ * It consumes the same scene description structs as the product's C ABI
+ * (include/lambda_gpu.h) so that both are driven from one scene builder, and
+ * re-implements the reference's algorithm independently: per-path recursive
+ * PathIntegrator::Li, pull-evaluated shader graphs, pointer-based light tree,
+ * median-split BVH.
+ */
+#ifndef LAMBDA_ORACLE_H
+#define LAMBDA_ORACLE_H
+#include <stdint.h>
+#include "../include/lambda_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_scene_s *oracle_scene;
+
+oracle_scene oracle_scene_create(void);
+void oracle_scene_destroy(oracle_scene s);
+int oracle_scene_add_texture(oracle_scene s, const lgpu_texture_desc *d);
+int oracle_scene_add_medium(oracle_scene s, const lgpu_medium_desc *d);
+int oracle_scene_add_material(oracle_scene s, const lgpu_material_desc *d);
+int oracle_scene_add_mesh(oracle_scene s, const lgpu_mesh_desc *d);
+int oracle_scene_add_light(oracle_scene s, const lgpu_light_desc *d);
+int oracle_scene_commit(oracle_scene s, uint32_t light_strategy);
+int oracle_light_tree(oracle_scene s, float *nodes, uint32_t max_nodes);
+
+/* Per-path radiance, 5 floats per path (see lgpu_trace_paths). */
+int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *desc, uint32_t width, uint32_t height, uint32_t n,
+                       const uint32_t *pixels, const uint32_t *samples, float *out, int threads);
+/* Film render over a pixel window [x0,x0+w) x [y0,y0+h) of a width x height
+ * image: FP64 RGB sums (3 per window pixel) and counts. Tiles of 16x16 are
+ * distributed dynamically over `threads` worker threads. */
+int oracle_render(oracle_scene s, const lgpu_render_desc *desc, uint32_t width, uint32_t height,
+                  uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, int threads,
+                  double *rgb_sum, uint32_t *counts);
+/* Closest hit / occlusion on the oracle BVH (same record layout as lgpu_intersect). */
+int oracle_intersect(oracle_scene s, uint32_t n, const float *rays, uint32_t *hits);
+int oracle_occluded(oracle_scene s, uint32_t n, const float *rays, uint32_t *occ);
+
+/* Building blocks exposed for closed-form tests. */
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float oracle_fr_dielectric(float cos_i, float eta_i, float eta_t);
+void oracle_fr_conductor(float cos_i, const float eta_i[3], const float eta_t[3], const float k[3], float out[3]);
+float oracle_microfacet_d(int ggx, const float wh[3], float ax, float ay);
+float oracle_microfacet_lambda(int ggx, const float w[3], float ax, float ay);
+void oracle_blackbody_rgb(float temperature, float out[3]);
+float oracle_blackbody_normalised(float lambda_nm, float temperature);
+void oracle_cmf(float lambda_nm, float out_xyz[3]);
+float oracle_uplift(const float rgb[3], float lambda_nm);
+float oracle_hg_p(float cos_theta, float g);
+/* Piecewise1D (Piecewise.cpp:9-44): returns offset; pdf, remapped out. */
+uint32_t oracle_piecewise1d_sample(const float *f, uint32_t n, float u, float *pdf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
