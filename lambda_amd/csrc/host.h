@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -18,10 +19,18 @@ lgpu_status Fail(lgpu_status s, const std::string &msg);
     if (e_ != hipSuccess) return ::lgpu::Fail(LGPU_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-// A device allocation owned by the host object.
+// A device allocation owned by the host object (move-only).
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
     if (ptr) (void)hipFree(ptr);
@@ -97,7 +106,7 @@ struct lgpu_scene_s {
   uint32_t num_tlights = 0;
 
   lgpu::DevScene dsc{};
-  std::vector<lgpu::DevBuf> bufs;  // owns every array referenced by dsc
+  std::deque<lgpu::DevBuf> bufs;  // owns every array referenced by dsc (stable addresses)
 };
 
 namespace lgpu {

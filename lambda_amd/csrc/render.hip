@@ -775,6 +775,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   lgpu_scene s = d->scene;
   lgpu_device dev = s->dev;
   LGPU_HIP(hipSetDevice(dev->device));
+  (void)hipGetLastError();  // clear a sticky error left by an earlier call
   const auto t0 = std::chrono::steady_clock::now();
   uint64_t cap = d->path_capacity ? d->path_capacity : (1u << 22);
   if (cap > total) cap = total;
@@ -898,6 +899,7 @@ extern "C" lgpu_status lgpu_trace_paths(const lgpu_render_desc *d, uint32_t num,
   LGPU_HIP(ds.upload(samples, 4ull * num));
   LGPU_HIP(dout.alloc(20ull * num));
   st = RunWaves(d, f->width, f->height, num, dp.as<uint32_t>(), ds.as<uint32_t>(), nullptr, nullptr, dout.as<float>(), nullptr);
+  (void)hipStreamSynchronize(d->scene->dev->stream);  // no kernel may outlive the list buffers
   if (st != LGPU_OK) return st;
   LGPU_HIP(hipMemcpy(out, dout.ptr, 20ull * num, hipMemcpyDeviceToHost));
   return LGPU_OK;
@@ -908,14 +910,17 @@ static lgpu_status Query(lgpu_scene s, uint32_t n, const float *rays, uint32_t *
   if (!s->committed) return Fail(LGPU_ERR_STATE, "query: scene not committed");
   if (!n) return LGPU_OK;
   LGPU_HIP(hipSetDevice(s->dev->device));
+  (void)hipGetLastError();
   DevBuf dr, dh, dsp;
   LGPU_HIP(dr.upload(rays, 32ull * n));
   LGPU_HIP(dh.alloc(occ ? 4ull * n : 16ull * n));
   LGPU_HIP(dsp.alloc(4ull * (kMaxBvhDepth - kLdsStack) * n));
   hipLaunchKernelGGL(k_query, dim3(Blocks(n)), dim3(kBlock), 0, s->dev->stream, s->dsc, dr.as<float4>(), n,
                      occ ? nullptr : dh.as<uint4>(), occ ? dh.as<uint32_t>() : nullptr, dsp.as<uint32_t>());
-  LGPU_HIP(hipGetLastError());
-  LGPU_HIP(hipStreamSynchronize(s->dev->stream));
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(s->dev->stream);  // before the buffers leave scope
+  LGPU_HIP(le);
+  LGPU_HIP(se);
   LGPU_HIP(hipMemcpy(occ ? (void *)occ : (void *)hits, dh.ptr, occ ? 4ull * n : 16ull * n, hipMemcpyDeviceToHost));
   return LGPU_OK;
 }
@@ -934,8 +939,10 @@ extern "C" lgpu_status lgpu_philox(lgpu_device dev, uint32_t n, const uint32_t *
   LGPU_HIP(dc.upload(counters, 16ull * n));
   LGPU_HIP(dout.alloc(16ull * n));
   hipLaunchKernelGGL(k_philox, dim3((n + 255) / 256), dim3(256), 0, dev->stream, n, dc.as<uint4>(), key[0], key[1], dout.as<uint4>());
-  LGPU_HIP(hipGetLastError());
-  LGPU_HIP(hipStreamSynchronize(dev->stream));
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(dev->stream);
+  LGPU_HIP(le);
+  LGPU_HIP(se);
   LGPU_HIP(hipMemcpy(out, dout.ptr, 16ull * n, hipMemcpyDeviceToHost));
   return LGPU_OK;
 }
