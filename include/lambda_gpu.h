@@ -301,6 +301,11 @@ lgpu_status lgpu_occluded(lgpu_scene scene, uint32_t num_rays, const float *rays
 /* Philox4x32-10 on the device: counters (4 words each) and one key (2 words) -> 4 words each. */
 lgpu_status lgpu_philox(lgpu_device dev, uint32_t n, const uint32_t *counters, const uint32_t key[2], uint32_t *out);
 
+/* Measurement utility, no reference counterpart: `iters` launches of a plain
+ * 16-byte-per-lane device copy of `bytes` bytes; *ms receives the mean launch
+ * time. Used to calibrate HBM byte counters and the achievable bandwidth. */
+lgpu_status lgpu_copy_bench(lgpu_device dev, uint64_t bytes, uint32_t iters, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -946,3 +946,33 @@ extern "C" lgpu_status lgpu_philox(lgpu_device dev, uint32_t n, const uint32_t *
   LGPU_HIP(hipMemcpy(out, dout.ptr, 16ull * n, hipMemcpyDeviceToHost));
   return LGPU_OK;
 }
+
+namespace lgpu {
+__global__ void __launch_bounds__(kBlock) k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+}  // namespace lgpu
+
+extern "C" lgpu_status lgpu_copy_bench(lgpu_device dev, uint64_t bytes, uint32_t iters, double *ms) {
+  if (!dev || !ms || bytes < 16 || !iters) return Fail(LGPU_ERR_INVALID_ARGUMENT, "copy_bench: bad argument");
+  LGPU_HIP(hipSetDevice(dev->device));
+  const uint64_t n = bytes / 16;
+  DevBuf a, b;
+  LGPU_HIP(a.alloc(16 * n));
+  LGPU_HIP(b.alloc(16 * n));
+  LGPU_HIP(hipMemsetAsync(a.ptr, 0, 16 * n, dev->stream));
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 256ull * 64);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, dev->stream, a.as<float4>(), b.as<float4>(), n);
+  LGPU_HIP(hipEventRecord(dev->ev[0], dev->stream));
+  for (uint32_t i = 0; i < iters; ++i)
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, dev->stream, a.as<float4>(), b.as<float4>(), n);
+  LGPU_HIP(hipEventRecord(dev->ev[1], dev->stream));
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(dev->stream);
+  LGPU_HIP(le);
+  LGPU_HIP(se);
+  float t;
+  LGPU_HIP(hipEventElapsedTime(&t, dev->ev[0], dev->ev[1]));
+  *ms = t / iters;
+  return LGPU_OK;
+}

@@ -23,7 +23,7 @@ EXPORTS = [
     "lgpu_scene_add_mesh", "lgpu_scene_add_light", "lgpu_scene_commit", "lgpu_scene_info", "lgpu_scene_light_tree",
     "lgpu_camera_init", "lgpu_camera_set_rotation", "lgpu_camera_set_forwards", "lgpu_camera_set_fov",
     "lgpu_film_create", "lgpu_film_destroy", "lgpu_film_clear", "lgpu_film_read", "lgpu_film_read_rgba",
-    "lgpu_render", "lgpu_trace_paths", "lgpu_intersect", "lgpu_occluded", "lgpu_philox",
+    "lgpu_render", "lgpu_trace_paths", "lgpu_intersect", "lgpu_occluded", "lgpu_philox", "lgpu_copy_bench",
 ]
 
 
@@ -50,6 +50,7 @@ def lib():
             "lgpu_film_read": [vp, vp, vp], "lgpu_film_read_rgba": [vp, vp], "lgpu_render": [vp, vp],
             "lgpu_trace_paths": [vp, u32, vp, vp, vp], "lgpu_intersect": [vp, u32, vp, vp],
             "lgpu_occluded": [vp, u32, vp, vp], "lgpu_philox": [vp, u32, vp, vp, vp],
+            "lgpu_copy_bench": [vp, C.c_uint64, u32, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -89,6 +90,12 @@ class Device:
 
     def __del__(self):
         self.close()
+
+    def copy_bench(self, nbytes, iters=10):
+        """Mean time (ms) of a plain device copy of nbytes (read + write)."""
+        ms = C.c_double()
+        _check(lib().lgpu_copy_bench(self.h, nbytes, iters, C.byref(ms)))
+        return ms.value
 
     def philox(self, counters, key):
         counters = np.ascontiguousarray(counters, dtype=np.uint32).reshape(-1, 4)

@@ -263,3 +263,198 @@ def cornell_box(temperature=6500.0, intensity=10.0):
 def cornell_camera(width=256, height=256):
     """Pinhole at (0,0,3.8), SetRotation(pi, 0) (looking down -z), SetFov(0.6911)."""
     return camera_pinhole((0.0, 0.0, 3.8), width, height, math.pi, 0.0, 0.6911)
+
+
+# --------------------------------------------------------------------- procedural helpers
+def _perm(seed):
+    p = np.random.default_rng(seed).permutation(256)
+    return np.concatenate([p, p]).astype(np.int64)
+
+
+def _fade(t):
+    return t * t * t * (t * (t * 6 - 15) + 10)
+
+
+_GRAD = np.array([[1, 1, 0], [-1, 1, 0], [1, -1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, 1], [1, 0, -1], [-1, 0, -1],
+                  [0, 1, 1], [0, -1, 1], [0, 1, -1], [0, -1, -1], [1, 1, 0], [-1, 1, 0], [0, -1, 1], [0, -1, -1]],
+                 dtype=np.float64)
+
+
+def perlin3(p, seed=0):
+    """Improved Perlin noise (Perlin 2002) at points p (N x 3), values in about [-1, 1]."""
+    perm = _perm(seed)
+    p = np.asarray(p, dtype=np.float64)
+    pi = np.floor(p).astype(np.int64)
+    pf = p - pi
+    pi &= 255
+    u, v, w = _fade(pf[:, 0]), _fade(pf[:, 1]), _fade(pf[:, 2])
+
+    def g(ix, iy, iz, dx, dy, dz):
+        h = perm[perm[perm[ix] + iy] + iz] & 15
+        gr = _GRAD[h]
+        return gr[:, 0] * dx + gr[:, 1] * dy + gr[:, 2] * dz
+
+    x, y, z = pi[:, 0], pi[:, 1], pi[:, 2]
+    fx, fy, fz = pf[:, 0], pf[:, 1], pf[:, 2]
+    n000 = g(x, y, z, fx, fy, fz)
+    n100 = g(x + 1, y, z, fx - 1, fy, fz)
+    n010 = g(x, y + 1, z, fx, fy - 1, fz)
+    n110 = g(x + 1, y + 1, z, fx - 1, fy - 1, fz)
+    n001 = g(x, y, z + 1, fx, fy, fz - 1)
+    n101 = g(x + 1, y, z + 1, fx - 1, fy, fz - 1)
+    n011 = g(x, y + 1, z + 1, fx, fy - 1, fz - 1)
+    n111 = g(x + 1, y + 1, z + 1, fx - 1, fy - 1, fz - 1)
+    x00 = n000 + u * (n100 - n000)
+    x10 = n010 + u * (n110 - n010)
+    x01 = n001 + u * (n101 - n001)
+    x11 = n011 + u * (n111 - n011)
+    y0 = x00 + v * (x10 - x00)
+    y1 = x01 + v * (x11 - x01)
+    return y0 + w * (y1 - y0)
+
+
+def icosphere(level):
+    """Unit icosphere by repeated midpoint subdivision (20 * 4^level faces)."""
+    t = (1 + 5 ** 0.5) / 2
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    verts = [np.array(x, dtype=np.float64) / np.linalg.norm(x) for x in v]
+    faces = f
+    for _ in range(level):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        faces = nf
+    return np.array(verts), np.array(faces, dtype=np.uint32)
+
+
+def _vertex_normals(v, f):
+    n = np.zeros_like(v)
+    fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def _tangents(n):
+    up = np.where(np.abs(n[:, 1:2]) < 0.99, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(up, n)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    return np.concatenate([t, np.ones((len(n), 1))], axis=1)
+
+
+COPPER_ETA = (0.200438, 0.924033, 1.102212)  # RGB-fitted copper optical constants
+COPPER_K = (3.912949, 2.452848, 2.142188)
+
+
+def ggx_conductor(alpha, eta=COPPER_ETA, k=COPPER_K):
+    """MicrofacetBRDF + TrowbridgeReitz + FresnelConductor with alpha given directly."""
+    p = [1.0, 1.0, 1.0, eta[0], eta[1], eta[2], k[0], k[1], k[2], 1.0]
+    return Material([node(abi.NODE_RGB, p=(1, 1, 1)), node(abi.NODE_SCALAR, p=[alpha]),
+                     node(abi.NODE_MICROFACET_BRDF, [0, 1], p=p)], bxdf=2)
+
+
+def rough_dielectric(alpha, ior=1.5):
+    return Material([node(abi.NODE_RGB, p=(1, 1, 1)), node(abi.NODE_SCALAR, p=[alpha]),
+                     node(abi.NODE_ROUGH_DIELECTRIC, [0, 1], p=[ior])], bxdf=2)
+
+
+def oren_nayar(rgb, sigma):
+    return Material([node(abi.NODE_RGB, p=rgb), node(abi.NODE_SCALAR, p=[sigma]),
+                     node(abi.NODE_OREN_NAYAR_BRDF, [0, 1])], bxdf=2)
+
+
+def sky(width=1024, height=512):
+    """Procedural FP32 sky: vertical gradient plus a sun disk (equirectangular)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * np.pi
+    phi = u * 2 * np.pi
+    th, ph = np.meshgrid(theta, phi, indexing="ij")
+    up = np.cos(th)
+    horizon = np.array([0.9, 0.85, 0.8])
+    zenith = np.array([0.25, 0.45, 0.9])
+    t = np.clip(up, 0, 1)[..., None]
+    img = horizon * (1 - t) + zenith * t
+    img = np.where(up[..., None] < 0, np.array([0.12, 0.1, 0.08]), img)
+    d = np.stack([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)], axis=-1)
+    sun = np.array([0.4, 0.75, -0.5])
+    sun /= np.linalg.norm(sun)
+    img = img + (d @ sun > np.cos(0.03))[..., None] * np.array([400.0, 380.0, 340.0])
+    return img.astype(np.float32)
+
+
+def many_light_scene(n_spheres=200, sphere_level=4, ground_res=297, n_light_quads=2048, env=True, seed=0):
+    """The many-light glossy mesh workload: displaced icospheres on a 40 x 40 ground,
+    emissive quads sampled through the light tree, an HDR sky.
+
+    Level-4 icospheres (5,120 faces) make 200 spheres 1,024,000 triangles; with the
+    297 x 297-quad ground (176,418 triangles) the scene has 1,200,418 triangles
+    (level 6 would give 16.4M, inconsistent with the 1.2M figure; the count is
+    what is honoured)."""
+    rng = np.random.default_rng(seed)
+    sd = SceneDesc()
+    # ground: [-20, 20]^2 at y = 0, normal +y
+    g = ground_res
+    xs = np.linspace(-20, 20, g + 1)
+    gx, gz = np.meshgrid(xs, xs, indexing="ij")
+    gv = np.stack([gx.ravel(), np.zeros(gx.size), gz.ravel()], axis=1)
+    idx = np.arange((g + 1) * (g + 1)).reshape(g + 1, g + 1)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
+    gf = np.concatenate([np.stack([a, d, c], 1), np.stack([a, c, b], 1)])  # (d-a)x(c-a) points +y
+    ground = sd.add_material(oren_nayar((0.5, 0.5, 0.5), 0.3491))
+    sd.add_mesh(gv, gf, ground)
+    # spheres on the integer grid points of [-20, 19]^2
+    base_v, base_f = icosphere(sphere_level)
+    cells = rng.choice(1600, size=n_spheres, replace=False)
+    for k, cell in enumerate(cells):
+        cx, cz = -20 + (cell % 40) + 0.5, -20 + (cell // 40) + 0.5
+        r = rng.uniform(0.3, 0.8)
+        centre = np.array([cx, r, cz])
+        p = base_v * r + centre
+        disp = perlin3(p * 4.0, seed=seed + 1) * 0.05
+        p = p + base_v * disp[:, None]
+        n = _vertex_normals(p, base_f)
+        kind = rng.integers(0, 3)
+        if kind == 0:
+            mat = ggx_conductor(float(rng.uniform(0.05, 0.5)))
+        elif kind == 1:
+            mat = rough_dielectric(float(rng.uniform(0.05, 0.5)))
+        else:
+            mat = oren_nayar(tuple(rng.uniform(0.2, 0.9, size=3)), 0.3491)
+        mi = sd.add_material(mat)
+        sd.add_mesh(p, base_f, mi, normals=n, tangents=_tangents(n))
+    # emissive quads: side 0.05 at heights U[1, 3], facing down
+    for q in range(n_light_quads):
+        T = float(rng.uniform(2000, 9000))
+        inten = float(10 ** rng.uniform(0, 3))
+        c = np.array([rng.uniform(-20, 20), rng.uniform(1, 3), rng.uniform(-20, 20)])
+        s = 0.025
+        v = np.array([[c[0] - s, c[1], c[2] - s], [c[0] + s, c[1], c[2] - s], [c[0] + s, c[1], c[2] + s],
+                      [c[0] - s, c[1], c[2] + s]])
+        m = sd.add_material(emitter_blackbody(T, inten))
+        sd.add_mesh(v, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32), m)  # (v1-v0)x(v2-v0) = -y
+    if env:
+        sd.add_env_light(sky(), intensity=1.0)
+    return sd
+
+
+def many_light_camera(width=1920, height=1080):
+    """Thin lens at (0, 6, 24) looking at the field centre, aperture 0.05, focused on the centre."""
+    origin = (0.0, 6.0, 24.0)
+    theta = -math.atan2(6.0, 24.0)
+    return camera_thin_lens(origin, width, height, math.pi, theta, 0.9, math.hypot(6.0, 24.0), 0.05)

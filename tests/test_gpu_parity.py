@@ -163,6 +163,28 @@ def test_many_light_per_path_parity(dev):
             assert ok.mean() >= 0.999
 
 
+@pytest.mark.parametrize("estimator", [abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED])
+@pytest.mark.parametrize("spectral", [abi.SPECTRAL_RGB, abi.SPECTRAL_HERO4])
+def test_glossy_many_light_per_path_parity(dev, spectral, estimator):
+    # GGX conductor, rough dielectric and Oren-Nayar spheres with smooth normals,
+    # emissive quads through the light tree, HDR sky, thin-lens camera
+    sd = scene.many_light_scene(n_spheres=40, sphere_level=2, ground_res=16, n_light_quads=64, seed=5)
+    W, H = 96, 54
+    s, k1 = gpu_scene(dev, sd)
+    o, k2 = helpers.oracle_scene(sd)
+    cam = scene.many_light_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    d = gpu.render_desc(s, film, cam, 0, 2, max_bounces=16, min_bounces=3, spectral=spectral, estimator=estimator)
+    g = gpu.trace_paths(d, pixels, samples)
+    c = o.trace_paths(helpers.desc(cam, 0, 2, 16, 3, spectral, estimator), W, H, pixels, samples)
+    ok = _compare_paths(g, c)
+    print("matching", ok.mean(), "finite", np.isfinite(g).all())
+    assert np.isfinite(g).all()
+    assert ok.mean() >= 0.999
+
+
 def test_film_render_matches_oracle(dev):
     W = H = 64
     spp = 8
