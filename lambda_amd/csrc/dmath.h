@@ -129,18 +129,20 @@ LGPU_HD float U32ToFloat(uint32_t u) { return (float)(u >> 8) * (1.f / 16777216.
 // Per-path sampler: dimension d is word (d & 3) of Philox(pixel, sample, d >> 2, 0; seed, 0).
 struct Rng {
   uint32_t pixel, sample, dim, seed;
-  uint32_t blk, cache[4];
+  uint32_t blk, c0, c1, c2, c3;  // the current Philox block (no indexed array: registers, not scratch)
   LGPU_HD void init(uint32_t p, uint32_t s, uint32_t d, uint32_t sd) {
     pixel = p; sample = s; dim = d; seed = sd; blk = 0xFFFFFFFFu;
   }
   LGPU_HD float Get1D() {
     const uint32_t b = dim >> 2;
     if (b != blk) {
-      cache[0] = pixel; cache[1] = sample; cache[2] = b; cache[3] = 0;
-      Philox(cache, seed, 0u);
+      uint32_t c[4] = {pixel, sample, b, 0u};
+      Philox(c, seed, 0u);
+      c0 = c[0]; c1 = c[1]; c2 = c[2]; c3 = c[3];
       blk = b;
     }
-    return U32ToFloat(cache[dim++ & 3]);
+    const uint32_t k = dim++ & 3;
+    return U32ToFloat(k == 0 ? c0 : (k == 1 ? c1 : (k == 2 ? c2 : c3)));
   }
 };
 LGPU_HD float WavelengthU(uint32_t pixel, uint32_t sample, uint32_t seed) {

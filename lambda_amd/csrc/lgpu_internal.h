@@ -28,14 +28,20 @@ struct BvhNode {
 constexpr uint32_t kLeafBit = 0x80000000u;
 inline uint32_t MakeLeaf(uint32_t first, uint32_t count) { return kLeafBit | (count << 26) | first; }
 
-// Triangles in BVH order: {v0.xyz, prim}, {e1.xyz, 0}, {e2.xyz, 0} with
-// e1 = v1 - v0, e2 = v2 - v0 (48 B).
+// Triangles in BVH order: {v0.xyz, prim}, {e1.xyz, mesh}, {e2.xyz, 0} with
+// e1 = v1 - v0, e2 = v2 - v0 (48 B). Ng = e1 x e2 is bit-identical to
+// (v1 - v0) x (v2 - v0).
 struct BvhTri {
   float v0[3];
   uint32_t prim;
-  float e1[4];
+  float e1[3];
+  uint32_t mesh;
   float e2[4];
 };
+// Shading attributes per triangle in BVH order (7 x float4 = 112 B), present
+// for meshes with smooth normals or uvs:
+//   {n0, uv0.x} {n1, uv0.y} {n2, uv1.x} {t0} {t1} {t2} {uv1.y, uv2.x, uv2.y, 0}
+constexpr int kTriAttrF4 = 7;
 
 // ---------------------------------------------------------------- geometry for shading
 struct MeshRec {
@@ -171,31 +177,38 @@ enum PathFlags : uint32_t {
   kFlagDead = 32u
 };
 
+// Active paths are compacted: slot i of the current buffer holds the i-th
+// live path; shade appends survivors to the other buffer. Radiance lives in a
+// per-sample array indexed by the sample slot `sid` of the wave.
 struct PathState {
-  float4 *ray_o;     // xyz
-  float4 *ray_d;     // xyz
+  float4 *ray_o;     // xyz, w = lambda0 (hero mode)
+  float4 *ray_d;     // xyz, w = ScatterEvent::eta (persists along the path)
   float4 *beta;
-  float4 *L;
-  float4 *hit;       // {t, u, v, prim bits}
+  float4 *hit;       // {t, u, v, BVH-order triangle index bits}
   uint4 *ids;        // {pixel, sample, dim, (depth << 8) | flags}
-  float4 *mis;       // {scatter pdf, light dist pdf, q, lambda0}
-  uint32_t *nee_light;  // {kind:1 | index} of the light picked for NEE at the previous vertex
+  uint4 *aux;        // {sid, NEE light of the previous vertex, pass-through count, medium}
+  float4 *mis;       // {scatter pdf, light dist pdf, q, -}
   float4 *prev_p;    // shading point of the previous vertex
   float4 *prev_n;    // shading normal of the previous vertex
-  uint32_t *pass;    // pass-through count
+};
+struct SampleState {
+  float4 *L;         // radiance per sample slot
+  uint32_t *pixel;
+  float *lambda0;
 };
 
 struct ShadowRay {
   float o[3];
   float tmax;
   float d[3];
-  uint32_t path;
+  uint32_t sid;
   float contrib[4];
 };
 
 struct DevScene {
   const BvhNode *nodes;
   const BvhTri *tris;
+  const float4 *tri_attr;    // kTriAttrF4 per BVH-order triangle (smooth / uv meshes)
   const uint4 *tri_vtx;      // {i0, i1, i2, mesh} global vertex indices
   const float4 *vtx_pos;
   const float4 *vtx_nrm;

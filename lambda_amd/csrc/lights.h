@@ -1,0 +1,228 @@
+// Device-side light sampling: light selection (power CDF or light tree),
+// Light::Sample_Li for each light kind, emitter pdfs and the environment map.
+#pragma once
+#include "dmath.h"
+#include "lgpu_internal.h"
+#include "shading.h"
+#include "vm.h"
+
+namespace lgpu {
+
+// ---------------------------------------------------------------- lights
+__device__ inline float PowerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }  // Integrator.h:22-29
+constexpr uint32_t kTreeBit = 0x80000000u;
+
+__device__ inline const LightRec &LightOf(const DevScene &sc, uint32_t pick) {
+  return (pick & kTreeBit) ? sc.tlights[pick & ~kTreeBit] : sc.lights[pick];
+}
+
+// LightSampler::Sample: PowerLightSampler (LightSampler.cpp:14-16) or ManyLightSampler (ManyLightSampler.cpp:17-27)
+__device__ inline uint32_t SampleLight(const DevScene &sc, V3 point, V3 normalS, Rng &rng, float *pdf) {
+  if (sc.strategy == LGPU_LIGHTS_POWER) {
+    if (!sc.num_lights) { *pdf = 0.f; return kNone; }
+    return PwSampleDiscrete(sc.dist, sc.power_dist, rng.Get1D(), pdf);
+  }
+  const float eps = rng.Get1D();
+  const float pTree = sc.tree_power / (sc.inf_power + sc.tree_power);
+  if (eps < pTree) {
+    *pdf *= pTree;
+    float e = rng.Get1D();
+    uint32_t n = 0;
+    TreeNode nd = sc.tree[0];
+    while (nd.child[0] != kNone) {  // PickLight (ManyLightSampler.cpp:355-378)
+      const TreeNode l = sc.tree[nd.child[0]], r = sc.tree[nd.child[1]];
+      const float IL = TreeImportance(l, point, normalS), IR = TreeImportance(r, point, normalS);
+      const float PL = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
+      const float PR = 1 - PL;
+      if (e < PL) { e /= PL; *pdf *= PL; n = nd.child[0]; nd = l; }
+      else { e = (e - PL) / PR; *pdf *= PR; n = nd.child[1]; nd = r; }
+    }
+    (void)n;
+    float lp = 1.f;
+    const uint32_t i = PwSampleDiscrete(sc.dist, nd.leaf_dist, e, &lp);
+    *pdf *= lp;
+    return kTreeBit | sc.tree_order[nd.first + i];
+  }
+  if (!sc.has_env) { *pdf = 0.f; return kNone; }
+  *pdf *= 1 - pTree;
+  return sc.env_light;
+}
+
+// LightSampler::Pdf for the light hit by a BSDF sample (ManyLightSampler.cpp:29-35, 380-393; LightSampler.cpp:18-20)
+__device__ inline float LightPdf(const DevScene &sc, uint32_t pick, V3 point, V3 normalS, bool corrected) {
+  if (sc.strategy == LGPU_LIGHTS_POWER) return sc.lights[pick].power * sc.inv_total_power;
+  if (!(pick & kTreeBit)) return sc.inf_power / (sc.inf_power + sc.tree_power);
+  const uint32_t tl = pick & ~kTreeBit;
+  const uint32_t leaf = sc.tlight_leaf[tl], slot = sc.tlight_slot[tl];
+  const uint32_t off = sc.tree[leaf].leaf_dist;
+  float p = corrected ? PwPdfNormalised(sc.dist, off, slot) : PwPdfRaw(sc.dist, off, slot);
+  uint32_t child = leaf;
+  for (uint32_t n = sc.tree[leaf].parent; n != kNone; child = n, n = sc.tree[n].parent) {
+    const TreeNode nd = sc.tree[n];
+    const float IL = TreeImportance(sc.tree[nd.child[0]], point, normalS);
+    const float IR = TreeImportance(sc.tree[nd.child[1]], point, normalS);
+    const float p0 = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
+    const float p1 = corrected ? 1.f - p0 : 1.f - IR;  // faithful: RecursivePDF :387
+    p *= (child == nd.child[0]) ? p0 : p1;
+  }
+  if (corrected) p *= sc.tree_power / (sc.inf_power + sc.tree_power);
+  return p;
+}
+
+__device__ inline Spec EnvLookup(const DevScene &sc, float u, float v, const VmCtx &vm) {
+  float c[3];
+  TexBilinear(sc.texels + sc.env.tex_off, sc.env.w, sc.env.h, 3, Fract(u), Fract(v), c);
+  return S4(UpliftV(c[0], c[1], c[2], vm));
+}
+__device__ inline Spec EnvLe(const DevScene &sc, V3 w, const VmCtx &vm) {  // EnvironmentLight.h:44-47
+  const float u = Fract((SphericalPhi(w) - sc.env.offset[0]) * kInvPi2);
+  const float v = Fract((SphericalTheta(w) - sc.env.offset[1]) * kInvPi);
+  return EnvLookup(sc, u, v, vm) * sc.env.intensity;
+}
+__device__ inline float Env2DPdf(const DevScene &sc, float u, float v) {  // Piecewise2D::PDF (Piecewise.cpp:70-76)
+  const int nu = (int)sc.env.w, nv = (int)sc.env.h;
+  int iu = (int)(u * (float)nu), iv = (int)(v * (float)nv);
+  iu = iu < 0 ? 0 : (iu > nu - 1 ? nu - 1 : iu);
+  iv = iv < 0 ? 0 : (iv > nv - 1 ? nv - 1 : iv);
+  return PwPdfRaw(sc.dist, sc.env.cond_off + (uint32_t)iv * sc.env.cond_stride, (uint32_t)iu) / PwIntegral(sc.dist, sc.env.marg_off);
+}
+__device__ inline float EnvPdfLi(const DevScene &sc, V3 wi) {  // EnvironmentLight.cpp:60-67
+  const float theta = SphericalTheta(wi) - sc.env.offset[1];
+  const float phi = SphericalPhi(wi) - sc.env.offset[0];
+  const float sinT = sinf(theta);
+  if (sinT == 0) return 0.f;
+  return Env2DPdf(sc, Fract(phi * kInvPi2), Fract(theta * kInvPi)) / (2.f * kPi * kPi * sinT);
+}
+__device__ inline V3 PointInTriangle(V3 v0, V3 v1, V3 v2, float ux, float uy, bool corrected) {
+  if (corrected && ux + uy > 1.f) { ux = 1.f - ux; uy = 1.f - uy; }  // parallelogram in the reference (TriangleMesh.h:73-79)
+  return v0 + (v1 - v0) * ux + (v2 - v0) * uy;
+}
+__device__ inline void TriAreaNormal(V3 v0, V3 v1, V3 v2, float *area, V3 *normal) {  // TriangleMesh.h:60-64
+  const V3 c = Cross(v1 - v0, v2 - v0);
+  *area = Mag(c) * .5f;
+  if (normal) *normal = c / (*area * 2.f);
+}
+__device__ inline Spec EmissionOf(const DevScene &sc, uint32_t mesh, const VmCtx &vm) {
+  const MaterialRec m = sc.materials[sc.meshes[mesh].material];
+  if (m.emission == kNone) return S(0.f);
+  return SocketSpectrum(sc.sockets[m.emission], vm);
+}
+__device__ inline Spec LightColour(const LightRec &L, const VmCtx &vm) {
+  if (L.temperature > 0) {
+    if (vm.mode) {
+      Spec s;
+      for (int i = 0; i < 4; ++i) s.c[i] = BlackbodyNormalised(vm.lam[i], L.temperature);
+      return s;
+    }
+    Spec s; s.c[0] = L.rgb[0]; s.c[1] = L.rgb[1]; s.c[2] = L.rgb[2]; s.c[3] = 0.f; return s;
+  }
+  return S4(UpliftV(L.rgb[0], L.rgb[1], L.rgb[2], vm));
+}
+
+struct ShadowSpec {
+  V3 o, d;
+  float tmax;
+};
+
+// Light::Sample_Li for the light kinds (MeshLight.cpp:22-41,134-151; PointLight.cpp:18-30;
+// Spotlight.cpp:29-42; EnvironmentLight.cpp:24-45). Visibility is deferred to the shadow stage:
+// `sh` receives the occlusion ray of Light::MutualVisibility / RayEscapes.
+__device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit &h, Ev &ev, Rng &rng, const VmCtx &vm,
+                                bool corrected, float &pdf, ShadowSpec &sh) {
+  switch (L.kind) {
+    case kLightMesh:
+    case kLightTri: {
+      const MeshRec m = sc.meshes[L.mesh];
+      const LightRec &ML = L.kind == kLightTri ? sc.lights[L.parent] : L;
+      uint32_t tri;
+      if (L.kind == kLightMesh) tri = PwSampleDiscrete(sc.dist, L.cdf_off, rng.Get1D(), &pdf);
+      else tri = L.tri;
+      const float ux = rng.Get1D(), uy = rng.Get1D();
+      const uint4 tv = sc.tri_vtx[m.first_tri + tri];
+      const V3 v0 = ToV3(sc.vtx_pos[tv.x]), v1 = ToV3(sc.vtx_pos[tv.y]), v2 = ToV3(sc.vtx_pos[tv.z]);
+      const V3 pL = PointInTriangle(v0, v1, v2, ux, uy, corrected);
+      const V3 pS = h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
+      const V3 diff = pL - pS;  // Scene::MutualVisibility (Scene.cpp:83-95)
+      const float mag = Mag(diff);
+      ev.wi = diff / mag;
+      sh.o = pS; sh.d = ev.wi; sh.tmax = (float)((double)mag - .00001);
+      float area;
+      V3 normal;
+      TriAreaNormal(v0, v1, v2, &area, &normal);
+      const float dn = Dot(normal, -ev.wi);
+      const float cosT = fabsf(dn);
+      if (cosT > 0 && !(corrected && dn < 0)) {
+        ev.wiL = ToLocal(h, ev.wi);
+        if (L.kind == kLightMesh) pdf *= DistSq(h.point, pL) / (cosT * area);
+        else if (corrected) pdf = DistSq(h.point, pL) / (cosT * area);
+        else pdf = DistSq(h.point, pL) * (cosT * area);  // MeshLight.cpp:145 as written
+        return EmissionOf(sc, L.mesh, vm) * S(1.f) * ML.intensity * kInvPi;
+      }
+      pdf = 0.f;
+      return S(0.f);
+    }
+    case kLightPoint:
+    case kLightSpot: {
+      const V3 p = h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
+      const V3 pos = ToV3(L.pos);
+      const V3 diff = pos - p;
+      const float mag = Mag(diff);
+      ev.wi = diff / mag;
+      sh.o = p; sh.d = ev.wi; sh.tmax = (float)((double)mag - .00001);
+      pdf = 1.f;
+      const float d2 = DistSq(pos, h.point);
+      ev.wiL = ToLocal(h, ev.wi);
+      Spec e = LightColour(L, vm) * S(1.f);
+      if (L.kind == kLightSpot) {  // Spotlight::Falloff (Spotlight.cpp:21-27)
+        const V3 w = -ev.wi;
+        const float cosT = Dot(w, ToV3(L.axis));
+        float fall;
+        if (cosT < L.cos_cone) fall = 0.f;
+        else if (cosT > L.cos_falloff) fall = 1.f;
+        else { const float dd = (cosT - L.cos_cone) / (L.cos_falloff - L.cos_cone); fall = (dd * dd) * (dd * dd); }
+        e = e * fall;
+      }
+      return e * L.intensity / d2;
+    }
+    case kLightEnv: {
+      const float u0 = rng.Get1D(), u1 = rng.Get1D();
+      // Piecewise2D::SampleContinuous (Piecewise.cpp:61-68)
+      float p1, p0;
+      uint32_t vi;
+      const float d1 = PwSampleContinuous(sc.dist, sc.env.marg_off, u1, &p1, &vi);
+      const float d0 = PwSampleContinuous(sc.dist, sc.env.cond_off + vi * sc.env.cond_stride, u0, &p0, nullptr);
+      pdf = p0 * p1;
+      if (pdf == 0) return S(0.f);
+      const float theta = d1 * kPi + sc.env.offset[1];
+      const float phi = d0 * kPi2 + sc.env.offset[0];
+      const float cosT = cosf(theta), sinT = sinf(theta);
+      if (sinT == 0) { pdf = 0.f; return S(0.f); }
+      ev.wi = SphericalDirection(sinT, cosT, phi);
+      sh.o = h.point + h.normalG * kSurfaceEpsilon;
+      sh.d = ev.wi;
+      sh.tmax = INFINITY;
+      ev.wiL = ToLocal(h, ev.wi);
+      pdf /= 2 * kPi * kPi * sinT;
+      return EnvLookup(sc, d0, d1, vm) * S(1.f) * sc.env.intensity;
+    }
+  }
+  pdf = 0.f;
+  return S(0.f);
+}
+
+// Light::PDF_Li(event) for a hit on an emitter (MeshLight.cpp:56-64,157-164)
+__device__ inline float PdfLiHit(const DevScene &sc, const LightRec &L, V3 wi, const Hit &hit, uint32_t bi, bool corrected) {
+  if (Dot(wi, hit.normalG) > 0) return 0.f;
+  const float distSq = hit.tFar * hit.tFar;
+  const float4 *tp = reinterpret_cast<const float4 *>(sc.tris + bi);
+  const float4 w1 = tp[1], w2 = tp[2];
+  const float area = Mag(Cross(v3(w1.x, w1.y, w1.z), v3(w2.x, w2.y, w2.z))) * .5f;  // TriangleMesh.h:60-64
+  const float cosT = fabsf(Dot(hit.normalG, -wi));
+  if (L.kind == kLightMesh) {
+    const float triPdf = corrected ? PwPdfNormalised(sc.dist, L.cdf_off, hit.local) : PwPdfRaw(sc.dist, L.cdf_off, hit.local);
+    return triPdf * distSq / (cosT * area);
+  }
+  return distSq / (cosT * area);
+}
+
+}  // namespace lgpu

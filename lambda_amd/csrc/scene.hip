@@ -761,14 +761,29 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   // The traversal stack holds at most one entry per level (LDS + spill = kMaxBvhDepth).
   if (bb.maxDepth + 1 >= (uint32_t)kMaxBvhDepth) return Fail(LGPU_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
   std::vector<BvhTri> btris(nt);
+  bool anyAttr = false;
+  for (const MeshRec &m : meshes) anyAttr |= (m.flags & (kMeshSmooth | kMeshUV)) != 0;
+  std::vector<float4> attr(anyAttr ? (size_t)nt * kTriAttrF4 : 0);
   for (uint32_t i = 0; i < nt; ++i) {
     const uint32_t t = bb.idx[i];
-    const V3 a = P(triVtx[t].x), b = P(triVtx[t].y), c = P(triVtx[t].z);
+    const uint4 tv = triVtx[t];
+    const V3 a = P(tv.x), b = P(tv.y), c = P(tv.z);
     const V3 e1 = b - a, e2 = c - a;
     BvhTri &o = btris[i];
     o.v0[0] = a.x; o.v0[1] = a.y; o.v0[2] = a.z; o.prim = t;
-    o.e1[0] = e1.x; o.e1[1] = e1.y; o.e1[2] = e1.z; o.e1[3] = 0;
+    o.e1[0] = e1.x; o.e1[1] = e1.y; o.e1[2] = e1.z; o.mesh = tv.w;
     o.e2[0] = e2.x; o.e2[1] = e2.y; o.e2[2] = e2.z; o.e2[3] = 0;
+    if (anyAttr) {
+      float4 *r = &attr[(size_t)i * kTriAttrF4];
+      const float2 u0 = uv[tv.x], u1 = uv[tv.y], u2 = uv[tv.z];
+      r[0] = make_float4(nrm[tv.x].x, nrm[tv.x].y, nrm[tv.x].z, u0.x);
+      r[1] = make_float4(nrm[tv.y].x, nrm[tv.y].y, nrm[tv.y].z, u0.y);
+      r[2] = make_float4(nrm[tv.z].x, nrm[tv.z].y, nrm[tv.z].z, u1.x);
+      r[3] = tan[tv.x];
+      r[4] = tan[tv.y];
+      r[5] = tan[tv.z];
+      r[6] = make_float4(u1.y, u2.x, u2.y, 0.f);
+    }
   }
 
   // ------------------------------------------------ textures
@@ -1046,6 +1061,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
 #define UP(vec, field) if ((st = Upload(s, vec, &d.field)) != LGPU_OK) return st
   UP(bb.nodes, nodes);
   UP(btris, tris);
+  UP(attr, tri_attr);
   UP(triVtx, tri_vtx);
   UP(pos, vtx_pos);
   UP(nrm, vtx_nrm);

@@ -22,29 +22,32 @@ struct Ev {  // ScatterEvent.h:18-40
 __device__ inline V3 ToLocal(const Hit &h, V3 v) { return WorldToLocal(v, h.tangent, h.normalS, h.bitangent); }
 __device__ inline V3 ToWorld(const Hit &h, V3 v) { return LocalToWorld(v, h.tangent, h.normalS, h.bitangent); }
 
-// Object::Hit + TriangleMesh::ProcessHit (Object.h:27-35, TriangleMesh.cpp:80-114)
-__device__ inline void ProcessHit(const DevScene &sc, V3 o, V3 d, float t, float bu, float bv, uint32_t prim, bool corrected,
+// Object::Hit + TriangleMesh::ProcessHit (Object.h:27-35, TriangleMesh.cpp:80-114).
+// `bi` is the BVH-order triangle index recorded by the traversal.
+__device__ inline void ProcessHit(const DevScene &sc, V3 o, V3 d, float t, float bu, float bv, uint32_t bi, bool corrected,
                                   Hit &h) {
-  const uint4 tv = sc.tri_vtx[prim];
-  const MeshRec m = sc.meshes[tv.w];
-  const V3 p0 = ToV3(sc.vtx_pos[tv.x]), p1 = ToV3(sc.vtx_pos[tv.y]), p2 = ToV3(sc.vtx_pos[tv.z]);
+  const float4 *tp = reinterpret_cast<const float4 *>(sc.tris + bi);
+  const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+  const uint32_t prim = __float_as_uint(w0.w), mesh = __float_as_uint(w1.w);
+  const MeshRec m = sc.meshes[mesh];
   h.tFar = t;
   h.point = v3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
-  h.normalG = Norm(Cross(p1 - p0, p2 - p0));
+  h.normalG = Norm(Cross(v3(w1.x, w1.y, w1.z), v3(w2.x, w2.y, w2.z)));
   h.prim = prim;
   h.local = prim - m.first_tri;
-  h.mesh = tv.w;
+  h.mesh = mesh;
+  const float4 *at = sc.tri_attr + (size_t)bi * kTriAttrF4;
   if (m.flags & kMeshUV) {
-    const float2 a = sc.vtx_uv[tv.x], b = sc.vtx_uv[tv.y], c = sc.vtx_uv[tv.z];
-    h.u = a.x + (b.x - a.x) * bu + (c.x - a.x) * bv;
-    h.v = a.y + (b.y - a.y) * bu + (c.y - a.y) * bv;
+    const float4 a0 = at[0], a1 = at[1], a2 = at[2], a6 = at[6];
+    h.u = a0.w + (a2.w - a0.w) * bu + (a6.y - a0.w) * bv;
+    h.v = a1.w + (a6.x - a1.w) * bu + (a6.z - a1.w) * bv;
   } else {
     h.u = 0.f;
     h.v = 0.f;
   }
   if (m.flags & kMeshSmooth) {
-    const V3 n0 = ToV3(sc.vtx_nrm[tv.x]), n1 = ToV3(sc.vtx_nrm[tv.y]), n2 = ToV3(sc.vtx_nrm[tv.z]);
-    const float4 t0 = sc.vtx_tan[tv.x], t1 = sc.vtx_tan[tv.y], t2 = sc.vtx_tan[tv.z];
+    const V3 n0 = ToV3(at[0]), n1 = ToV3(at[1]), n2 = ToV3(at[2]);
+    const float4 t0 = at[3], t1 = at[4], t2 = at[5];
     const V3 T0 = ToV3(t0), T1 = ToV3(t1), T2 = ToV3(t2);
     h.normalS = Norm(n0 + (n1 - n0) * bu + (n2 - n0) * bv);
     h.tangent = Norm(T0 + (T1 - T0) * bu + (T2 - T0) * bv);

@@ -94,8 +94,11 @@ LGPU_HD float4 AsSpectrumV(float4 r, uint32_t t, const VmCtx &c) {
 }
 LGPU_HD float Comp(float4 r, uint32_t i) { return i == 0 ? r.x : (i == 1 ? r.y : (i == 2 ? r.z : r.w)); }
 
-// Straight-line program; returns r[res].
-LGPU_HD float4 RunProgram(const VmCtx &c, uint32_t off, uint32_t len, uint32_t res) {
+// Straight-line program; returns r[res]. Kept out of line: its register file
+// is an indexed array (scratch), which must not burden the shading kernels'
+// common path where every socket is a folded constant.
+static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmCtx &c, uint32_t off, uint32_t len,
+                                                                        uint32_t res) {
   float4 r[kMaxRegs];
   for (uint32_t pc = off; pc < off + len; ++pc) {
     const Instr in = c.code[pc];
