@@ -260,6 +260,104 @@ def cornell_box(temperature=6500.0, intensity=10.0):
     return sd
 
 
+def passthrough(interior=abi.NONE, exterior=abi.NONE):
+    """Material with neither BxDF nor emission: a media boundary that rays cross
+    (MediaBoundary, Media.h:31-57; skipped by Scene::IntersectTr, Scene.cpp:45-59)."""
+    return Material([], medium_interior=interior, medium_exterior=exterior)
+
+
+def aabb(bmin, bmax):
+    """Axis-aligned box, 12 triangles, Ng = (v1-v0)x(v2-v0) facing outwards, so
+    that MediaBoundary::GetMedium gives the interior for entering rays."""
+    c = [(a + b) * 0.5 for a, b in zip(bmin, bmax)]
+    size = [b - a for a, b in zip(bmin, bmax)]
+    return _box(c, size, 0.0)
+
+
+def homogeneous(sd, sigma_a, sigma_s, g=0.0):
+    return sd.add_medium(type=abi.MEDIUM_HOMOGENEOUS, sigma_a=sigma_a, sigma_s=sigma_s, g=g)
+
+
+def grid_medium(sd, density, sigma_a, sigma_s, g, bmin, bmax):
+    """Piecewise-constant density grid (z-major, shape (nz, ny, nx)) over [bmin, bmax];
+    sigma = density * sigma_{a,s}."""
+    return sd.add_medium(type=abi.MEDIUM_GRID, sigma_a=sigma_a, sigma_s=sigma_s, g=g, density=density,
+                         bounds_min=bmin, bounds_max=bmax)
+
+
+def fbm_density(res=128, octaves=5, freq=2.0, seed=7):
+    """d = max(0, fBm) of 3-D Perlin noise sampled at voxel centres of [-1,1]^3
+    (lacunarity 2, gain 0.5); z-major float32 (res, res, res)."""
+    c = (np.arange(res, dtype=np.float64) + 0.5) / res * 2.0 - 1.0
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    acc = np.zeros(len(p))
+    amp, f = 1.0, freq
+    for o in range(octaves):
+        acc += amp * perlin3(p * f + 17.31 * o, seed=seed)
+        amp *= 0.5
+        f *= 2.0
+    return np.maximum(acc, 0.0).reshape(res, res, res).astype(np.float32)
+
+
+def fog_cornell(kind="homogeneous", res=16, point_light=True, sigma_scale=1.0):
+    """Cornell room (walls, ceiling quad light) with a pass-through fog box
+    [-0.95,0.95] x [-0.95,0.6] x [-0.95,0.95] holding a chromatic medium
+    (homogeneous, or a res^3 random-density grid), an inner Lambertian box whose
+    exterior medium is the fog, and optionally a point light inside the fog."""
+    sd = cornell_box()
+    bmin, bmax = (-0.95, -0.95, -0.95), (0.95, 0.6, 0.95)
+    sa = [0.05 * sigma_scale, 0.1 * sigma_scale, 0.2 * sigma_scale]
+    ss = [0.6 * sigma_scale, 0.45 * sigma_scale, 0.3 * sigma_scale]
+    if kind == "homogeneous":
+        med = homogeneous(sd, sa, ss, g=0.3)
+    else:
+        dens = np.random.default_rng(3).uniform(0.0, 2.0, size=(res, res, res)).astype(np.float32)
+        dens[dens < 0.4] = 0.0
+        med = grid_medium(sd, dens, sa, ss, 0.3, bmin, bmax)
+    fog = sd.add_material(passthrough(interior=med))
+    v, i = aabb(bmin, bmax)
+    sd.add_mesh(v, i, fog)
+    inner = sd.add_material(Material([node(abi.NODE_RGB, p=(0.6, 0.6, 0.5)), node(abi.NODE_LAMBERTIAN, [0])],
+                                     bxdf=1, medium_exterior=med))
+    v, i = _box((-0.3, -0.6, 0.2), (0.4, 0.4, 0.4), 20.0)
+    sd.add_mesh(v, i, inner)
+    if point_light:
+        sd.add_point_light((0.35, 0.1, 0.3), 0.6, rgb=(1.0, 0.8, 0.6))
+    return sd
+
+
+def media_scene(res=128, sigma_t=4.0, albedo=0.9, g=0.6, n_points=64, octaves=5):
+    """Heterogeneous-media workload: a pass-through box [-1,1]^3 whose interior is a
+    res^3 FP32 density grid d = max(0, 5-octave Perlin fBm) with sigma_t = 4 d,
+    sigma_s = 0.9 sigma_t, HG g = 0.6; one spotlight at (0,3,0) aiming down
+    (cone 30 deg, falloff from 20 deg) and 64 point lights on a radius-3 sphere
+    (Fibonacci layout)."""
+    sd = SceneDesc()
+    dens = fbm_density(res, octaves)
+    ss = sigma_t * albedo
+    sa = sigma_t - ss
+    med = grid_medium(sd, dens, [sa] * 3, [ss] * 3, g, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    box = sd.add_material(passthrough(interior=med))
+    v, i = aabb((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    sd.add_mesh(v, i, box)
+    sd.add_spot_light((0.0, 3.0, 0.0), (0.0, -1.0, 0.0), math.radians(30.0), math.radians(20.0), 30.0,
+                      temperature=5000.0)
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    for k in range(n_points):
+        y = 1.0 - 2.0 * (k + 0.5) / n_points
+        r = math.sqrt(max(0.0, 1.0 - y * y))
+        phi = golden * k
+        pos = (3.0 * r * math.cos(phi), 3.0 * y, 3.0 * r * math.sin(phi))
+        sd.add_point_light(pos, 1.5, temperature=3000.0 + 6000.0 * (k % 8) / 7.0)
+    return sd
+
+
+def media_camera(width=1024, height=1024):
+    """Pinhole at (0, 0.4, 3.8) looking down -z, SetFov(0.6911)."""
+    return camera_pinhole((0.0, 0.4, 3.8), width, height, math.pi, 0.0, 0.6911)
+
+
 def cornell_camera(width=256, height=256):
     """Pinhole at (0,0,3.8), SetRotation(pi, 0) (looking down -z), SetFov(0.6911)."""
     return camera_pinhole((0.0, 0.0, 3.8), width, height, math.pi, 0.0, 0.6911)

@@ -55,6 +55,15 @@ def lib():
         L.oracle_hg_p.restype = f32
         L.oracle_piecewise1d_sample.argtypes = [vp, u32, f32, vp]
         L.oracle_piecewise1d_sample.restype = u32
+        L.oracle_medium_tr.argtypes = [vp, u32, vp, vp, f32, C.c_int, vp, vp]
+        L.oracle_medium_sample_distance.argtypes = [vp, u32, vp, vp, f32, f32, f32, C.c_int, vp, vp]
+        L.oracle_medium_pdf_distance.argtypes = [vp, u32, vp, vp, f32, C.c_int, vp, C.c_int]
+        L.oracle_medium_pdf_distance.restype = f32
+        L.oracle_medium_sample_equiangular.argtypes = [vp, u32, vp, vp, vp, f32, vp, f32, C.c_int, vp, C.c_int, vp]
+        L.oracle_medium_pdf_equiangular.argtypes = [vp, vp, vp, f32, f32, C.c_int]
+        L.oracle_medium_pdf_equiangular.restype = f32
+        L.oracle_hg_sample.argtypes = [f32, vp, f32, f32, C.c_int, vp]
+        L.oracle_hg_sample.restype = f32
         _LIB = L
     return _LIB
 
@@ -130,6 +139,55 @@ class OracleScene:
         occ = np.zeros(len(rays), dtype=np.uint32)
         self._chk(self.L.oracle_occluded(self.h, len(rays), _ptr(rays), _ptr(occ)), "occluded")
         return occ
+
+
+def _v3(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(3))
+
+
+class Medium:
+    """Closed-form access to one medium of an OracleScene (mode 0 RGB, 1 HERO4)."""
+
+    def __init__(self, scene, med, mode=0, lambdas=(0, 0, 0, 0)):
+        self.s, self.med, self.mode = scene, med, mode
+        self.lam = np.ascontiguousarray(np.asarray(lambdas, dtype=np.float32))
+
+    def tr(self, o, d, t_far):
+        out = np.zeros(4, dtype=np.float32)
+        o, d = _v3(o), _v3(d)
+        lib().oracle_medium_tr(self.s.h, self.med, _ptr(o), _ptr(d), t_far, self.mode, _ptr(self.lam), _ptr(out))
+        return out
+
+    def sample_distance(self, o, d, t_far, u0, u1):
+        out = np.zeros(7, dtype=np.float32)
+        o, d = _v3(o), _v3(d)
+        lib().oracle_medium_sample_distance(self.s.h, self.med, _ptr(o), _ptr(d), t_far, u0, u1, self.mode,
+                                            _ptr(self.lam), _ptr(out))
+        return out
+
+    def pdf_distance(self, o, d, t, corrected=True):
+        o, d = _v3(o), _v3(d)
+        return lib().oracle_medium_pdf_distance(self.s.h, self.med, _ptr(o), _ptr(d), t, self.mode, _ptr(self.lam),
+                                                int(corrected))
+
+    def sample_equiangular(self, o, d, hit_point, t_far, light, eps, corrected=True):
+        out = np.zeros(7, dtype=np.float32)
+        o, d, hp, lp = _v3(o), _v3(d), _v3(hit_point), _v3(light)
+        lib().oracle_medium_sample_equiangular(self.s.h, self.med, _ptr(o), _ptr(d), _ptr(hp), t_far, _ptr(lp), eps,
+                                               self.mode, _ptr(self.lam), int(corrected), _ptr(out))
+        return out
+
+
+def pdf_equiangular(o, d, light, t_far, t, corrected=True):
+    o, d, lp = _v3(o), _v3(d), _v3(light)
+    return lib().oracle_medium_pdf_equiangular(_ptr(o), _ptr(d), _ptr(lp), t_far, t, int(corrected))
+
+
+def hg_sample(g, wo, u0, u1, corrected=True):
+    wi = np.zeros(3, dtype=np.float32)
+    wo = _v3(wo)
+    p = lib().oracle_hg_sample(g, _ptr(wo), u0, u1, int(corrected), _ptr(wi))
+    return p, wi
 
 
 def philox(ctr, key):

@@ -397,11 +397,15 @@ struct Hit {
   int mesh = -1;
 };
 
+struct Sampler;
 struct Event {  // ScatterEvent.h:18-40
   V3 wo, wi, woL, wiL;
   Hit *hit = nullptr;
+  Sampler *sampler = nullptr;  // the integrator's sampler (LdMediumPoint draws from it)
   int sidedness = 1;
   float eta = 1.001f;
+  int medium = -1;                 // ScatterEvent::medium
+  bool mediumInteraction = false;
   Lanes lanes;
   V3 ToLocal(const V3 &v) const { return WorldToLocal(v, hit->tangent, hit->normalS, hit->bitangent); }
   V3 ToWorld(const V3 &v) const { return LocalToWorld(v, hit->tangent, hit->normalS, hit->bitangent); }
@@ -423,13 +427,24 @@ struct LNode {
 
 struct BVHNode { Bounds b; uint32_t left = 0, right = 0, first = 0, count = 0; };
 
+// HomogeneousMedium (HomogeneousMedium.cpp) or the density-grid medium (project extension:
+// piecewise-constant voxels, sigma = density * sigma_base).
+struct MediumO {
+  int type = LGPU_MEDIUM_HOMOGENEOUS;
+  float sa[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
+  float g = 0;
+  uint32_t nx = 0, ny = 0, nz = 0;
+  std::vector<float> dens;
+  V3 bmin, bmax;
+};
+
 }  // namespace orc
 
 using namespace orc;
 
 struct oracle_scene_s {
   std::vector<Texture> textures;
-  std::vector<lgpu_medium_desc> media;
+  std::vector<MediumO> media;
   std::vector<Material> materials;
   std::vector<Mesh> meshes;
   std::vector<uint32_t> triMesh;  // global tri -> mesh
@@ -907,6 +922,7 @@ struct Ctx {
   const Scene *sc;
   bool corrected;
   int mode;
+  bool volumes = false;  // Scene::hasVolumes: never set in the reference (A42); corrected: scene has media
 };
 
 static Spec BxDF_f(const Ctx &c, const Material &m, uint32_t node, Event &ev);
@@ -1204,7 +1220,265 @@ static Spec BxDF_Sample(const Ctx &c, const Material &m, uint32_t node, Event &e
   return Spec(0);
 }
 
-// ---------------------------------------------------------------- lights
+// ---------------------------------------------------------------- media (HomogeneousMedium.cpp, Media.h, Scene.cpp:45-81)
+static inline int NSamples(const Lanes &l) { return l.mode == HERO4 ? 4 : 3; }
+static inline Spec ExpS(const Spec &s) { Spec r; for (int i = 0; i < 4; ++i) r.c[i] = std::exp(s.c[i]); return r; }
+static Spec SigmaT(const MediumO &m, const Lanes &l) { return FromRGB(m.sa, l) + FromRGB(m.ss, l); }
+static Spec SigmaS(const MediumO &m, const Lanes &l) { return FromRGB(m.ss, l); }
+// MediaBoundary::GetMedium (Media.h:53-55)
+static int GetMedium(const Material &mat, const V3 &w, const V3 &n) {
+  const uint32_t id = Dot(w, n) < 0 ? mat.medIn : mat.medOut;
+  return id == LGPU_NONE ? -1 : (int)id;
+}
+
+// Grid traversal: visits the voxels of [t0, t1] along o + t d in order; fn(density, ta, tb) returns
+// true to stop. Piecewise-constant (nearest-voxel) density.
+template <class F>
+static void GridWalk(const MediumO &m, const V3 &o, const V3 &d, float t0, float t1, const F &fn) {
+  float tn = t0, tf = t1;
+  for (int a = 0; a < 3; ++a) {
+    const float inv = 1.f / d[a];
+    float ta = (m.bmin[a] - o[a]) * inv, tb = (m.bmax[a] - o[a]) * inv;
+    if (ta > tb) std::swap(ta, tb);
+    if (ta > tn) tn = ta;
+    if (tb < tf) tf = tb;
+  }
+  if (!(tn < tf)) return;
+  const int n[3] = {(int)m.nx, (int)m.ny, (int)m.nz};
+  float cs[3], tNext[3], tDelta[3];
+  int cell[3], step[3];
+  const V3 p = o + d * tn;
+  for (int a = 0; a < 3; ++a) {
+    cs[a] = (m.bmax[a] - m.bmin[a]) / (float)n[a];
+    int c = (int)std::floor((p[a] - m.bmin[a]) / cs[a]);
+    cell[a] = std::min(std::max(c, 0), n[a] - 1);
+    if (d[a] > 0) { step[a] = 1; tNext[a] = (m.bmin[a] + (float)(cell[a] + 1) * cs[a] - o[a]) / d[a]; tDelta[a] = cs[a] / d[a]; }
+    else if (d[a] < 0) { step[a] = -1; tNext[a] = (m.bmin[a] + (float)cell[a] * cs[a] - o[a]) / d[a]; tDelta[a] = -cs[a] / d[a]; }
+    else { step[a] = 0; tNext[a] = INFINITY; tDelta[a] = INFINITY; }
+  }
+  float t = tn;
+  for (int it = 0; it < n[0] + n[1] + n[2] + 3; ++it) {
+    const int ax = (tNext[0] <= tNext[1] && tNext[0] <= tNext[2]) ? 0 : (tNext[1] <= tNext[2] ? 1 : 2);
+    const float tc = std::min(tNext[ax], tf);
+    const float rho = m.dens[((size_t)cell[2] * m.ny + cell[1]) * m.nx + cell[0]];
+    if (fn(rho, t, tc)) return;
+    t = tc;
+    if (t >= tf) return;
+    cell[ax] += step[ax];
+    if (cell[ax] < 0 || cell[ax] >= n[ax]) return;
+    tNext[ax] += tDelta[ax];
+  }
+}
+// Optical-depth density integral over [0, tFar].
+static float DensityIntegral(const MediumO &m, const V3 &o, const V3 &d, float tFar) {
+  float acc = 0.f;
+  GridWalk(m, o, d, 0.f, tFar, [&](float rho, float ta, float tb) { acc += rho * (tb - ta); return false; });
+  return acc;
+}
+// Medium::Tr (HomogeneousMedium.cpp:12-15)
+static Spec MediumTr(const MediumO &m, const V3 &o, const V3 &d, float tFar, const Lanes &l) {
+  const Spec st = SigmaT(m, l);
+  if (m.type == LGPU_MEDIUM_HOMOGENEOUS) return ExpS(st * -1.f * std::min(tFar, 3.40282347e38f));
+  return ExpS(st * -1.f * DensityIntegral(m, o, d, tFar));
+}
+// HomogeneousMedium::SampleDistance (HomogeneousMedium.cpp:17-38); the grid walks voxels to the
+// sampled optical depth of the chosen channel.
+static Spec SampleDistanceU(const MediumO &m, const V3 &o, const V3 &d, float tFarHit, const V2 &u, const Lanes &l,
+                            bool *interaction, float *tOut, float *pdfOut) {
+  const Spec st = SigmaT(m, l), ss = SigmaS(m, l);
+  const int ns = NSamples(l);
+  const unsigned ch = std::min((unsigned)(u.x * (float)ns), (unsigned)ns - 1);
+  float t, rhoAt = 1.f;
+  float D = 0.f;
+  if (m.type == LGPU_MEDIUM_HOMOGENEOUS) {
+    const float dist = -std::log(1 - u.y) / st.c[ch];
+    t = std::min(dist, tFarHit);
+  } else {
+    const float target = -std::log(1 - u.y);
+    t = tFarHit;
+    float acc = 0.f;
+    bool found = false;
+    GridWalk(m, o, d, 0.f, tFarHit, [&](float rho, float ta, float tb) {
+      const float k = st.c[ch] * rho;
+      const float seg = k * (tb - ta);
+      if (k > 0 && acc + seg >= target) {
+        t = ta + (target - acc) / (st.c[ch] * rho);
+        D += rho * (t - ta);
+        rhoAt = rho;
+        found = true;
+        return true;
+      }
+      acc += seg;
+      D += rho * (tb - ta);
+      return false;
+    });
+    if (!found) t = tFarHit;
+  }
+  *interaction = t < tFarHit;
+  const Spec Tr = m.type == LGPU_MEDIUM_HOMOGENEOUS ? ExpS(st * -1.f * std::min(t, 3.40282347e38f)) : ExpS(st * -1.f * D);
+  const Spec stl = st * rhoAt, ssl = ss * rhoAt;
+  const Spec dens = *interaction ? stl * Tr : Tr;
+  float pdf = 0;
+  for (int i = 0; i < ns; ++i) pdf += dens.c[i];
+  pdf *= 1 / (float)ns;
+  if (pdf == 0) pdf = 1;
+  *tOut = t;
+  *pdfOut = pdf;
+  return *interaction ? Tr * ssl / pdf : Tr / pdf;
+}
+static Spec SampleDistance(const MediumO &m, const V3 &o, const V3 &d, float tFarHit, Sampler &smp, const Lanes &l,
+                           bool *interaction, float *tOut, float *pdfOut) {
+  return SampleDistanceU(m, o, d, tFarHit, smp.Get2D(), l, interaction, tOut, pdfOut);
+}
+// HomogeneousMedium::PDFDistance (:40-47): the reference multiplies by `1 / Spectrum::nSamples`
+// (integer division, 0); corrected returns the channel-averaged pdf.
+static float PdfDistance(const MediumO &m, const V3 &o, const V3 &d, float t, const Lanes &l, bool corrected) {
+  if (!corrected) return 0.f;
+  const Spec st = SigmaT(m, l);
+  float rho = 1.f, D = t;
+  if (m.type != LGPU_MEDIUM_HOMOGENEOUS) {
+    D = 0.f;
+    rho = 0.f;
+    GridWalk(m, o, d, 0.f, t, [&](float r, float ta, float tb) { D += r * (tb - ta); rho = r; return false; });
+  }
+  const Spec Tr = ExpS(st * -1.f * D);
+  const int ns = NSamples(l);
+  float pdf = 0;
+  for (int i = 0; i < ns; ++i) pdf += st.c[i] * rho * Tr.c[i];
+  return pdf * (1 / (float)ns);
+}
+// HomogeneousMedium::SampleEquiangular (:49-78). faithful: origin from the surface hit point
+// (A14), t = D tan(theta) used as the ray distance, the unused Get2D kept; corrected: origin
+// ray.o and t = l + D tan(theta).
+static Spec SampleEquiangularU(const MediumO &m, const V3 &o, const V3 &d, const V3 &hitPoint, float tFarHit,
+                               const V3 &lightPoint, float eps, const Lanes &l, bool corrected, bool *interaction,
+                               float *tOut, float *pdfOut) {
+  const V3 delta = lightPoint - (corrected ? o : hitPoint);
+  const float lp = Dot(d, delta);
+  const float D = std::sqrt(corrected ? std::max(0.f, Dot(delta, delta) - lp * lp) : Dot(delta, delta) - lp * lp);
+  if (corrected && !(D > 0)) {  // light point on the ray's line: no equiangular sample
+    *interaction = false;
+    *tOut = tFarHit;
+    *pdfOut = 0;
+    return Spec(0);
+  }
+  const float thetaA = -std::atan2(lp, D);
+  const float thetaB = std::atan2(tFarHit - lp, D);
+  const float th = thetaB + (thetaA - thetaB) * eps;
+  const float dist = D * std::tan(th);
+  float t = corrected ? std::min(lp + dist, tFarHit) : std::min(dist, tFarHit);
+  *interaction = t < tFarHit;
+  const float x = corrected ? t - lp : t;
+  const float pdf = D / ((thetaB - thetaA) * (D * D + x * x));
+  *tOut = t;
+  *pdfOut = pdf;
+  const Spec st = SigmaT(m, l), ss = SigmaS(m, l);
+  float Dn = t, rho = 1.f;
+  if (m.type != LGPU_MEDIUM_HOMOGENEOUS) {
+    Dn = 0.f;
+    rho = 0.f;
+    GridWalk(m, o, d, 0.f, t, [&](float r, float ta, float tb) { Dn += r * (tb - ta); rho = r; return false; });
+  }
+  const Spec Tr = ExpS(st * -1.f * std::min(Dn, 3.40282347e38f));
+  return Tr * (ss * rho) / pdf;
+}
+static Spec SampleEquiangular(const MediumO &m, const V3 &o, const V3 &d, const V3 &hitPoint, float tFarHit,
+                              const V3 &lightPoint, Sampler &smp, const Lanes &l, bool corrected, bool *interaction,
+                              float *tOut, float *pdfOut) {
+  const float eps = smp.Get1D();
+  smp.Get2D();  // channel draw, unused (HomogeneousMedium.cpp:70)
+  return SampleEquiangularU(m, o, d, hitPoint, tFarHit, lightPoint, eps, l, corrected, interaction, tOut, pdfOut);
+}
+// HomogeneousMedium::PDFEquiangular (:80-92)
+static float PdfEquiangular(const V3 &o, const V3 &d, const V3 &lightPoint, float tFar, float t, bool corrected) {
+  const V3 delta = lightPoint - o;
+  const float lp = Dot(d, delta);
+  const float D = std::sqrt(Dot(delta, delta) - lp * lp);
+  const float thetaA = -std::atan2(lp, D);
+  const float thetaB = std::atan2(tFar - lp, D);
+  const float x = corrected ? t - lp : t;
+  return D / ((thetaB - thetaA) * (D * D + x * x));
+}
+// Henyey-Greenstein (HenyeyGreenstein.cpp:8-36). faithful: frame from the unnormalised
+// cross(-wo, y) (A21); corrected: orthonormal frame.
+static inline float PhaseHG(float cosT, float g) {
+  const float g2 = g * g;
+  const float denom = 1 + g2 + 2 * g * cosT;
+  return INV_PI4 * (1 - g2) / (denom * std::sqrt(denom));
+}
+static float HGSampleU(float g, const V3 &wo, V3 *wi, const V2 &u, bool corrected) {
+  float cosT;
+  if (std::abs(g) < 1e-3f) cosT = 1 - 2 * u.x;
+  else {
+    const float sq = (1 - g * g) / (1 - g + 2 * g * u.x);
+    cosT = (1 + g * g - sq * sq) / (2 * g);
+  }
+  const float sinT = std::sqrt(std::max(0.f, 1 - cosT * cosT));
+  const float phi = 2 * PI * u.y;
+  const V3 woNeg = -wo;
+  V3 v1 = Cross(woNeg, V3(0, 1, 0));
+  if (corrected) {
+    if (Dot(v1, v1) < 1e-12f) v1 = Cross(woNeg, V3(1, 0, 0));
+    v1 = Norm(v1);
+  }
+  const V3 v2 = Cross(woNeg, v1);
+  const V3 dd = SphericalDirection(sinT, cosT, phi);
+  *wi = v1 * dd.x + woNeg * dd.y + v2 * dd.z;
+  return PhaseHG(-cosT, g);
+}
+static float HGSample(float g, const V3 &wo, V3 *wi, Sampler &smp, bool corrected) {
+  return HGSampleU(g, wo, wi, smp.Get2D(), corrected);
+}
+
+// Scene::IntersectTr (Scene.cpp:45-81, 280-300): skips surfaces without BxDF or light,
+// multiplying the transmittance of the medium each segment crosses. maxT < 0: no limit.
+static bool IntersectTr(const Scene &sc, V3 o, V3 d, Hit &h, int med, Spec *Tr, const Lanes &l, float maxT,
+                        bool corrected, bool *leftAll) {
+  float tFar = 0;
+  if (leftAll) *leftAll = false;
+  for (int k = 0; k < 256; ++k) {
+    if (!Intersect(sc, o, d, h, corrected)) {
+      if (leftAll) *leftAll = true;
+      return false;
+    }
+    tFar += h.tFar;
+    const bool terminate = maxT >= 0 && tFar > maxT;
+    if (terminate) { const float delta = tFar - maxT; tFar -= delta; h.tFar -= delta; }
+    if (med >= 0 && Tr) *Tr = *Tr * MediumTr(sc.media[med], o, d, h.tFar, l);
+    const Material &mat = sc.materials[sc.meshes[h.mesh].material];
+    if (terminate || mat.bxdf != LGPU_NONE || mat.emission != LGPU_NONE) { h.tFar = tFar; return true; }
+    med = GetMedium(mat, d, h.normalG);
+    o = h.point + h.normalG * (Dot(h.normalG, d) < 0 ? -SURFACE_EPSILON : SURFACE_EPSILON);
+  }
+  return false;
+}
+// Light::MutualVisibility / PointMutualVisibility / RayEscapes (Light.cpp:10-49) on the
+// hasVolumes branch, reached only in corrected mode (A42). The ray starts in the medium the
+// path is in at a medium vertex, or in the one GetMedium gives for the shading surface. A ray
+// that leaves every boundary without another hit is unoccluded (the reference compares the
+// last segment's tFar with minT there, A41).
+static int NeeMedium(const Scene &sc, const Event &ev, const V3 &dir) {
+  if (ev.mediumInteraction) return ev.medium;
+  const Hit &h = *ev.hit;
+  return GetMedium(sc.materials[sc.meshes[h.mesh].material], dir, h.normalG);
+}
+static bool VisibleTr(const Ctx &c, const V3 &p1, const V3 &p2, Event &ev, Spec *Tr, bool point) {
+  const V3 diff = p2 - p1;
+  const float mag = Mag(diff);
+  const V3 dir = diff / mag;
+  const float minT = mag - SURFACE_EPSILON * 10;
+  ev.wi = dir;
+  Hit h;
+  bool left;
+  IntersectTr(*c.sc, p1, dir, h, NeeMedium(*c.sc, ev, dir), Tr, ev.lanes, point ? mag : -1.f, true, &left);
+  if (left) return true;
+  return h.tFar > minT;
+}
+static bool EscapesTr(const Ctx &c, const V3 &o, const V3 &d, const Event &ev, Spec *Tr) {
+  Hit h;
+  return !IntersectTr(*c.sc, o, d, h, NeeMedium(*c.sc, ev, d), Tr, ev.lanes, -1.f, true, nullptr);
+}
+
 static Spec Emission(const Ctx &c, const Material &m, const Event &ev) {
   if (m.emission == LGPU_NONE) return Spec(0);
   return AsSpectrum(EvalNode(*c.sc, m, m.emission, ev), ev.lanes);
@@ -1254,8 +1528,10 @@ static Spec SampleLi(const Ctx &c, const Light &L, Event &ev, Sampler &smp, floa
       const V2 u = smp.Get2D();
       const V3 v0 = m.p[m.idx[3 * tri]], v1 = m.p[m.idx[3 * tri + 1]], v2 = m.p[m.idx[3 * tri + 2]];
       const V3 pL = SamplePointInTriangle(v0, v1, v2, u, c.corrected);
-      const V3 pS = h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
-      if (MutualVisibility(sc, pS, pL, &ev.wi)) {
+      const bool atMedium = c.volumes && ev.mediumInteraction;
+      const V3 pS = atMedium ? h.point : h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+      Spec Tr(1.f);
+      if (c.volumes ? VisibleTr(c, pS, pL, ev, &Tr, false) : MutualVisibility(sc, pS, pL, &ev.wi)) {
         float area;
         V3 normal;
         TriAreaNormal(v0, v1, v2, &area, &normal);
@@ -1268,7 +1544,7 @@ static Spec SampleLi(const Ctx &c, const Light &L, Event &ev, Sampler &smp, floa
           else if (c.corrected) pdf = DistSq(h.point, pL) / (cosT * area);
           else pdf = DistSq(h.point, pL) * (cosT * area);  // A6
           const Material &mat = sc.materials[m.material];
-          return Emission(c, mat, ev) * Spec(1.f) * ML.intensity * INV_PI;
+          return Emission(c, mat, ev) * Tr * ML.intensity * INV_PI;
         }
       }
       pdf = 0;
@@ -1276,12 +1552,14 @@ static Spec SampleLi(const Ctx &c, const Light &L, Event &ev, Sampler &smp, floa
     }
     case L_POINT:
     case L_SPOT: {  // PointLight.cpp:18-30, Spotlight.cpp:29-42
-      const V3 p = h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
-      if (MutualVisibility(sc, p, L.position, &ev.wi)) {
+      const bool atMedium = c.volumes && ev.mediumInteraction;
+      const V3 p = atMedium ? h.point : h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+      Spec Tr(1.f);
+      if (c.volumes ? VisibleTr(c, p, L.position, ev, &Tr, true) : MutualVisibility(sc, p, L.position, &ev.wi)) {
         pdf = 1;
         const float d2 = DistSq(L.position, h.point);
         ev.wiL = ev.ToLocal(ev.wi);
-        Spec e = LightColour(L, ev.lanes) * Spec(1.f);
+        Spec e = LightColour(L, ev.lanes) * Tr;
         if (L.kind == L_SPOT) {
           const V3 w = -ev.wi;
           const float cosT = Dot(w, L.axis);  // Spotlight.cpp:21-27
@@ -1304,10 +1582,13 @@ static Spec SampleLi(const Ctx &c, const Light &L, Event &ev, Sampler &smp, floa
       const float cosT = std::cos(theta), sinT = std::sin(theta);
       if (sinT == 0) { pdf = 0; return Spec(0); }
       ev.wi = SphericalDirection(sinT, cosT, phi);
-      if (RayEscapes(sc, h.point + h.normalG * SURFACE_EPSILON, ev.wi)) {
+      const bool atMedium = c.volumes && ev.mediumInteraction;
+      const V3 pe = atMedium ? h.point : h.point + h.normalG * SURFACE_EPSILON;
+      Spec Tr(1.f);
+      if (c.volumes ? EscapesTr(c, pe, ev.wi, ev, &Tr) : RayEscapes(sc, pe, ev.wi)) {
         ev.wiL = ev.ToLocal(ev.wi);
         pdf /= 2 * PI * PI * sinT;
-        return EnvLookup(L, uv, ev.lanes) * Spec(1.f) * L.intensity;
+        return EnvLookup(L, uv, ev.lanes) * Tr * L.intensity;
       }
       pdf = 0;
       return Spec(0);
@@ -1546,6 +1827,8 @@ static float TreeImportance(const LNode *nd, const V3 &point, const V3 &normalS)
   const float thetaDash = std::max(theta - nd->cone.thetaO - thetaU, 0.f);
   if (thetaDash < nd->cone.thetaE) {
     const float E = nd->power;
+    // a zero normal marks a medium vertex (corrected mode): no surface-cosine bound
+    if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * std::cos(thetaDash);
     const float thetaI = std::acos(Dot(delta, normalS));
     return (std::cos(std::max(thetaI - thetaU, 0.f)) * E) / d2 * std::cos(thetaDash);
   }
@@ -1554,7 +1837,7 @@ static float TreeImportance(const LNode *nd, const V3 &point, const V3 &normalS)
 
 struct LightPick { int kind; int index; };  // kind 0: scene light index, 1: tree light index, -1 none
 
-static LightPick SampleLight(const Ctx &c, const Event &ev, Sampler &smp, float *pdf) {
+static LightPick SampleLightAt(const Ctx &c, const V3 &refP, const V3 &refN, Sampler &smp, float *pdf) {
   const Scene &sc = *c.sc;
   if (sc.strategy == LGPU_LIGHTS_POWER) {  // LightSampler.cpp:14-16
     if (sc.lights.empty()) { *pdf = 0; return {-1, -1}; }
@@ -1569,8 +1852,8 @@ static LightPick SampleLight(const Ctx &c, const Event &ev, Sampler &smp, float 
     float e = smp.Get1D();  // A5: the rescaled epsilon is discarded
     LNode *n = sc.root;
     while (!n->IsLeaf()) {  // PickLight :355-378
-      const float IL = TreeImportance(n->child[0], ev.hit->point, ev.hit->normalS);
-      const float IR = TreeImportance(n->child[1], ev.hit->point, ev.hit->normalS);
+      const float IL = TreeImportance(n->child[0], refP, refN);
+      const float IR = TreeImportance(n->child[1], refP, refN);
       const float PL = (IL + IR == 0.f) ? .5f : IL / (IL + IR);  // A10
       const float PR = 1 - PL;
       if (e < PL) { e /= PL; *pdf *= PL; n = n->child[0]; }
@@ -1584,6 +1867,9 @@ static LightPick SampleLight(const Ctx &c, const Event &ev, Sampler &smp, float 
   if (sc.envLight < 0) { *pdf = 0; return {-1, -1}; }
   *pdf *= 1 - pTree;
   return {0, sc.envLight};
+}
+static LightPick SampleLight(const Ctx &c, const Event &ev, Sampler &smp, float *pdf) {
+  return SampleLightAt(c, ev.hit->point, ev.hit->normalS, smp, pdf);
 }
 
 // LightSampler::Pdf for the light hit by a BSDF sample.
@@ -1734,6 +2020,390 @@ static Spec PathLi(const Ctx &c, const RenderParams &rp, V3 ro, V3 rd, Sampler &
   return L;
 }
 
+// ---------------------------------------------------------------- volumetric integrators
+// Light reached by a BSDF- or phase-sampled ray (the MIS half of PathIntegrator.cpp:52-63,
+// VolumetricPathIntegrator.cpp:127-139, MISVolumetricPathIntegrator.cpp:52-66). `h` is that
+// ray's hit (si: it hit something). Returns Le * w; the caller applies f (or Tr) and 1/pdf.
+// envNoLightPdf: the light technique cannot produce the environment (SamplePoint, corrected).
+static Spec MisLight(const Ctx &c, const LightPick &l, float lightDistPdf, const V3 &refP, const V3 &refN, bool si,
+                     Hit &h, Event &ev, const V3 &rd, float scatPdf, bool specular, bool envNoLightPdf) {
+  const Scene &sc = *c.sc;
+  LightPick nl{-1, -1};
+  if (si) {
+    const Mesh &hm = sc.meshes[h.mesh];
+    if (hm.meshLight >= 0) nl = sc.strategy == LGPU_LIGHTS_TREE ? LightPick{1, sc.triToTLight[h.prim]} : LightPick{0, hm.meshLight};
+  } else if (sc.envLight >= 0) {
+    nl = {0, sc.envLight};
+  }
+  if (nl.kind < 0) return Spec(0);
+  Hit *saved = ev.hit;
+  ev.hit = &h;
+  bool same = SameLight(nl, l);
+  if (!c.corrected && sc.strategy == LGPU_LIGHTS_TREE && nl.kind == 1) same = false;  // MeshLight != TriangleLight
+  float ldp = lightDistPdf;
+  if (!same) ldp = c.corrected ? LightPdf(c, nl, refP, refN) : LightPdf(c, nl, h.point, h.normalS);  // A8
+  const Light &NL = LightOf(sc, nl);
+  float pl;
+  Spec Le;
+  if (si) {
+    if (!c.corrected && sc.strategy == LGPU_LIGHTS_TREE) pl = PdfLiHit(c, sc.lights[NL.parentMeshLight], ev, h);
+    else pl = PdfLiHit(c, NL, ev, h);
+    const Material &hmat = sc.materials[sc.meshes[h.mesh].material];
+    Le = Emission(c, hmat, ev) * sc.lights[sc.meshes[h.mesh].meshLight].intensity * INV_PI;
+    if (c.corrected && Dot(rd, h.normalG) > 0) Le = Spec(0);
+  } else {
+    pl = envNoLightPdf ? 0.f : EnvPdfLi(NL, ev.wi);
+    Le = EnvLe(NL, rd, ev.lanes);
+  }
+  ev.hit = saved;
+  const float lightPDF = ldp * pl;
+  const float w = (c.corrected && specular) ? 1.f : PowerHeuristic(scatPdf, lightPDF);
+  if (!IsBlack(Le, ev.lanes) && (lightPDF > 0 || c.corrected)) return Le * w;
+  return Spec(0);
+}
+
+// PartialLightSample (Light.h:107-113). pdf starts at 1: the reference leaves it uninitialised
+// before ManyLightSampler::Sample multiplies into it (MISVolumetricPathIntegrator.cpp:95-96).
+struct PartialLS {
+  V3 point, normal;
+  V2 uv;
+  Spec Le;
+  LightPick light{-1, -1};
+  float pdf = 1, selPdf = 1;
+};
+// Light::SamplePoint (MeshLight.cpp:66-81, 166-180; PointLight.cpp:40-44; Spotlight.cpp:50-54;
+// EnvironmentLight.cpp:69-74). Faithful writes the sampled uv into the event's hit as written.
+static Spec SamplePointL(const Ctx &c, const Light &L, Event &ev, Sampler &smp, PartialLS &ls) {
+  const Scene &sc = *c.sc;
+  switch (L.kind) {
+    case L_MESH:
+    case L_TRI: {
+      const Mesh &m = sc.meshes[L.mesh];
+      const Light &ML = L.kind == L_TRI ? sc.lights[L.parentMeshLight] : L;
+      uint32_t tri = L.tri;
+      float area;
+      if (L.kind == L_MESH) {
+        float distPdf;
+        tri = L.triDist.SampleDiscrete(smp.Get1D(), &distPdf);
+        TriAreaNormal(m.p[m.idx[3 * tri]], m.p[m.idx[3 * tri + 1]], m.p[m.idx[3 * tri + 2]], &area, &ls.normal);
+        ls.pdf *= distPdf / area;
+      } else {
+        TriAreaNormal(m.p[m.idx[3 * tri]], m.p[m.idx[3 * tri + 1]], m.p[m.idx[3 * tri + 2]], &area, &ls.normal);
+        ls.pdf /= area;
+      }
+      const V2 u = smp.Get2D();
+      const uint32_t i0 = m.idx[3 * tri], i1 = m.idx[3 * tri + 1], i2 = m.idx[3 * tri + 2];
+      V2 uv(0, 0);
+      if (!m.uv.empty()) {  // maths::BarycentricInterpolation (maths.h:58-60)
+        uv.x = m.uv[i0].x + (m.uv[i1].x - m.uv[i0].x) * u.x + (m.uv[i2].x - m.uv[i0].x) * u.y;
+        uv.y = m.uv[i0].y + (m.uv[i1].y - m.uv[i0].y) * u.x + (m.uv[i2].y - m.uv[i0].y) * u.y;
+      }
+      ls.uv = uv;
+      ls.point = SamplePointInTriangle(m.p[i0], m.p[i1], m.p[i2], u, c.corrected);
+      Hit tmp;
+      Hit *saved = ev.hit;
+      if (c.corrected) { tmp.uv = uv; ev.hit = &tmp; }
+      else ev.hit->uv = uv;
+      const Spec Le = Emission(c, sc.materials[m.material], ev) * ML.intensity * INV_PI;
+      ev.hit = saved;
+      return Le;
+    }
+    case L_POINT:
+    case L_SPOT:
+      ls.point = L.position;
+      return LightColour(L, ev.lanes) * L.intensity;
+    case L_ENV:
+      ls.pdf *= 0;
+      ls.point = V3(0, 0, 0);
+      return Spec(0);
+  }
+  return Spec(0);
+}
+// Light::Visibility (MeshLight.cpp:83-92, 182-191; PointLight.cpp:46-55; Spotlight.cpp:56-65;
+// EnvironmentLight.cpp:76-84): completes the pdf in ls and sets ev.wi. `eh` is the event's hit.
+static Spec VisibilityL(const Ctx &c, const Light &L, const V3 &sp, Event &ev, Hit &eh, PartialLS &ls) {
+  const Scene &sc = *c.sc;
+  Spec Tr(1.f);
+  switch (L.kind) {
+    case L_MESH:
+    case L_TRI: {
+      if (c.volumes ? VisibleTr(c, sp, ls.point, ev, &Tr, false) : MutualVisibility(sc, sp, ls.point, &ev.wi)) {
+        // TriangleLight uses the event hit's normal as written; corrected uses the light's
+        const V3 n = (L.kind == L_TRI && !c.corrected) ? eh.normalG : ls.normal;
+        const float dn = Dot(n, -ev.wi);
+        const float cosT = std::abs(dn);
+        if (c.corrected && !(dn > 0)) break;  // A29: one-sided emission
+        ls.pdf *= DistSq(sp, ls.point) / cosT;
+        return Tr;
+      }
+      break;
+    }
+    case L_POINT:
+    case L_SPOT: {
+      const V3 p = c.corrected ? sp : sp + eh.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+      if (c.volumes ? VisibleTr(c, p, ls.point, ev, &Tr, true) : MutualVisibility(sc, p, ls.point, &ev.wi)) {
+        const float d2 = DistSq(L.position, eh.point);
+        if (L.kind == L_SPOT) {
+          const float cosT = Dot(-ev.wi, L.axis);
+          float fall;
+          if (cosT < L.cosCone) fall = 0;
+          else if (cosT > L.cosFalloff) fall = 1;
+          else { const float d = (cosT - L.cosCone) / (L.cosFalloff - L.cosCone); fall = (d * d) * (d * d); }
+          return Tr * fall / d2;
+        }
+        return Tr / d2;
+      }
+      break;
+    }
+    case L_ENV: {
+      const V3 wi = Norm(ls.point - sp);
+      if (c.volumes ? EscapesTr(c, sp, wi, ev, &Tr) : RayEscapes(sc, sp, wi)) return Tr;
+      break;
+    }
+  }
+  ls.pdf = 0;
+  return Spec(0);
+}
+
+// MISVolumetricPathIntegrator::LdMediumPoint (MISVolumetricPathIntegrator.cpp:23-69).
+// faithful: `eh` is the path's hit (the reference's event.hit, overwritten here) and ls carries
+// the pdf compounded by earlier calls; corrected works on copies.
+static Spec LdMediumPoint(const Ctx &c, const MediumO &M, Event &ev, Hit &eh, const V3 &p, PartialLS &ls,
+                          const V3 &refP, const V3 &refN, V3 *wiOut) {
+  const Scene &sc = *c.sc;
+  float partialLightPDF = c.corrected ? ls.selPdf : ls.pdf;
+  eh.point = p;
+  Hit *saved = ev.hit;
+  ev.hit = &eh;
+  const Light &LL = LightOf(sc, ls.light);
+  const bool delta = LL.kind == L_POINT || LL.kind == L_SPOT;
+  Spec Li = ls.Le * VisibilityL(c, LL, p, ev, eh, ls);
+  float lightPDF = ls.pdf;
+  Spec Ld(0);
+  if (lightPDF > 0 && !IsBlack(Li, ev.lanes)) {
+    const float ph = PhaseHG(Dot(ev.wo, ev.wi), M.g);
+    if (ph > 0) {
+      const float w = (c.corrected && delta) ? 1.f : PowerHeuristic(lightPDF, ph);
+      Ld += Li * ph * w / lightPDF;
+    }
+  }
+  V3 wi;
+  const float scatPdf = HGSample(M.g, ev.wo, &wi, *ev.sampler, c.corrected);
+  ev.wi = wi;
+  *wiOut = wi;
+  Spec Tr(1.f);
+  const bool si = IntersectTr(sc, p, wi, eh, ev.medium, &Tr, ev.lanes, -1.f, c.corrected, nullptr);
+  if (scatPdf > 0) {
+    const Spec C = MisLight(c, ls.light, partialLightPDF, refP, refN, si, eh, ev, wi, scatPdf, false, c.corrected);
+    Ld += c.corrected ? C * Tr : C * Tr / scatPdf;  // A43: the phase value is missing as written
+  }
+  ev.hit = saved;
+  return Ld;
+}
+
+// VolumetricPathIntegrator::Li (VolumetricPathIntegrator.cpp:20-162) and
+// MISVolumetricPathIntegrator::Li (MISVolumetricPathIntegrator.cpp:71-215).
+static Spec VolLi(const Ctx &c, const RenderParams &rp, V3 ro, V3 rd, Sampler &smp, const Lanes &lanes, bool misVol) {
+  const Scene &sc = *c.sc;
+  Spec L(0), beta(1);
+  Hit hit;
+  Event ev;
+  ev.hit = &hit;
+  ev.lanes = lanes;
+  ev.sampler = &smp;
+  ev.wo = -rd;
+  ev.medium = -1;  // InMedium returns nullptr (:15-18)
+  bool scatterIntersect = false;
+  int passThrough = 0;
+  for (int bounces = 0; bounces < (int)rp.maxBounces; ++bounces) {
+    if (!(bounces == 0 ? Intersect(sc, ro, rd, hit, c.corrected) : scatterIntersect)) {
+      if (bounces == 0 && sc.envLight >= 0) L += EnvLe(sc.lights[sc.envLight], rd, lanes) * beta;
+      break;
+    }
+    const Material &mat = sc.materials[sc.meshes[hit.mesh].material];
+    if (bounces == 0 && mat.emission != LGPU_NONE && (!c.corrected || Dot(rd, hit.normalG) < 0)) {
+      Spec Le = Emission(c, mat, ev) * sc.lights[sc.meshes[hit.mesh].meshLight].intensity * INV_PI;
+      if (ev.medium >= 0) Le = Le * MediumTr(sc.media[ev.medium], ro, rd, hit.tFar, lanes);
+      L += c.corrected ? beta * Le : Le;  // beta is taken as 1 as written
+    }
+    ev.mediumInteraction = false;
+    if (ev.medium >= 0) {
+      const MediumO &M = sc.media[ev.medium];
+      if (!misVol) {
+        bool inter;
+        float tD, pdfD;
+        beta = beta * SampleDistance(M, ro, rd, hit.tFar, smp, lanes, &inter, &tD, &pdfD);
+        ev.mediumInteraction = inter;
+        if (inter) {  // :99-148
+          ev.wo = -rd;
+          const V3 sp = ro + rd * tD;
+          hit.point = sp;
+          const V3 refP = sp, refN = c.corrected ? V3(0, 0, 0) : hit.normalS;
+          float lightDistPdf = 1;
+          const LightPick l = SampleLightAt(c, refP, refN, smp, &lightDistPdf);
+          Spec Ld(0), Li(0);
+          float lightPDF = 0;
+          bool delta = false;
+          if (l.kind >= 0) {
+            const Light &LL = LightOf(sc, l);
+            delta = LL.kind == L_POINT || LL.kind == L_SPOT;
+            Li = SampleLi(c, LL, ev, smp, lightPDF);
+          }
+          lightPDF *= lightDistPdf;
+          if (lightPDF > 0 && !IsBlack(Li, lanes)) {
+            const float ph = PhaseHG(Dot(ev.wo, ev.wi), M.g);
+            if (ph > 0) {
+              const float w = (c.corrected && delta) ? 1.f : PowerHeuristic(lightPDF, ph);
+              Ld += Li * ph * w / lightPDF;
+            }
+          }
+          V3 wi;
+          const float scatPdf = HGSample(M.g, ev.wo, &wi, smp, c.corrected);
+          ev.wi = wi;
+          Spec Tr(1.f);
+          const bool si = IntersectTr(sc, sp, wi, hit, ev.medium, &Tr, lanes, -1.f, c.corrected, nullptr);
+          if (scatPdf > 0) {
+            const Spec C = MisLight(c, l, lightDistPdf, refP, refN, si, hit, ev, wi, scatPdf, false, false);
+            Ld += c.corrected ? C * Tr : C * Tr / scatPdf;  // A43
+          } else {
+            if (c.corrected) L += beta * Ld;
+            break;
+          }
+          ro = sp;
+          rd = wi;
+          scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+          L += beta * Ld;
+        }
+      } else {  // :92-147
+        PartialLS ls;
+        const V3 refP = c.corrected ? ro : hit.point, refN = c.corrected ? V3(0, 0, 0) : hit.normalS;
+        ls.light = SampleLightAt(c, refP, refN, smp, &ls.pdf);
+        ls.selPdf = ls.pdf;
+        if (ls.light.kind >= 0) ls.Le = SamplePointL(c, LightOf(sc, ls.light), ev, smp, ls);
+        else { ls.pdf = 0; ls.Le = Spec(0); }
+        const int choice = smp.Get1D() > .5f ? 0 : 1;  // 0 distance, 1 equiangular
+        Spec mTr[2];
+        bool inter[2];
+        float tS[2], pdfS[2];
+        mTr[0] = SampleDistance(M, ro, rd, hit.tFar, smp, lanes, &inter[0], &tS[0], &pdfS[0]);
+        mTr[1] = SampleEquiangular(M, ro, rd, hit.point, hit.tFar, ls.point, smp, lanes, c.corrected, &inter[1], &tS[1], &pdfS[1]);
+        if (!c.corrected) {
+          ev.mediumInteraction = inter[0] && inter[1];
+          if (!ev.mediumInteraction) beta = beta * mTr[0];
+          else {
+            ev.wo = -rd;
+            const V3 mp0 = ro + rd * tS[0], mp1 = ro + rd * tS[1];
+            const float pdfD2 = PdfDistance(M, ro, rd, tS[1], lanes, false);
+            const float pdfE2 = PdfEquiangular(ro, rd, ls.point, hit.tFar, tS[0], false);
+            const float wD = PowerHeuristic(pdfS[0], pdfE2), wE = PowerHeuristic(pdfS[1], pdfD2);
+            V3 wi0, wi1;
+            Spec Ld(0);
+            Ld += LdMediumPoint(c, M, ev, hit, mp0, ls, refP, refN, &wi0) * mTr[0] * wD;
+            Ld += LdMediumPoint(c, M, ev, hit, mp1, ls, refP, refN, &wi1) * mTr[1] * wE;
+            ro = choice == 0 ? mp0 : mp1;
+            rd = choice == 0 ? wi0 : wi1;
+            scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+            L += Ld;  // A15: beta missing as written
+            beta = beta * mTr[choice];
+          }
+        } else {
+          // both points estimate direct light unconditionally (power-heuristic MIS over the two
+          // distance techniques); the path continues with the distance sample
+          ev.wo = -rd;
+          ev.mediumInteraction = true;  // NEE rays start in this medium
+          const V3 mp0 = ro + rd * tS[0], mp1 = ro + rd * tS[1];
+          Spec Ld(0);
+          V3 wi0(0, 0, 0), wi1;
+          if (inter[0]) {
+            PartialLS l0 = ls;
+            Hit eh = hit;
+            const float wD = PowerHeuristic(pdfS[0], PdfEquiangular(ro, rd, ls.point, hit.tFar, tS[0], true));
+            Ld += LdMediumPoint(c, M, ev, eh, mp0, l0, refP, refN, &wi0) * mTr[0] * wD;
+          }
+          if (inter[1]) {
+            PartialLS l1 = ls;
+            Hit eh = hit;
+            const float wE = PowerHeuristic(pdfS[1], PdfDistance(M, ro, rd, tS[1], lanes, true));
+            Ld += LdMediumPoint(c, M, ev, eh, mp1, l1, refP, refN, &wi1) * mTr[1] * wE;
+          }
+          L += beta * Ld;
+          beta = beta * mTr[0];
+          ev.mediumInteraction = inter[0];
+          if (inter[0]) {
+            ro = mp0;
+            rd = wi0;
+            scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+          }
+        }
+        (void)choice;
+      }
+    }
+    if (!ev.mediumInteraction) {
+      if (mat.bxdf != LGPU_NONE) {  // :45-90 / :150-193
+        ev.wo = -rd;
+        ev.SurfaceLocalise();
+        const V3 shadeP = hit.point, shadeN = hit.normalS;
+        float lightDistPdf = 1;
+        const LightPick l = SampleLight(c, ev, smp, &lightDistPdf);
+        Spec Ld(0), Li(0);
+        float lightPDF = 0, scatteringPDF = 0;
+        bool delta = false;
+        if (l.kind >= 0) {
+          const Light &LL = LightOf(sc, l);
+          delta = LL.kind == L_POINT || LL.kind == L_SPOT;
+          Li = SampleLi(c, LL, ev, smp, lightPDF);
+        }
+        lightPDF *= lightDistPdf;
+        if (lightPDF > 0 && !IsBlack(Li, lanes)) {
+          const Spec f = BxDF_f(c, mat, mat.bxdf, ev) * std::abs(ev.wiL.y);
+          scatteringPDF = BxDF_Pdf(c, mat, mat.bxdf, ev.woL, ev.wiL, ev);
+          if (!IsBlack(f, lanes) && scatteringPDF > 0) {
+            const float w = (c.corrected && delta) ? 1.f : PowerHeuristic(lightPDF, scatteringPDF);
+            Ld += Li * f * w / lightPDF;
+          }
+        }
+        bool specular;
+        Spec f = BxDF_Sample(c, mat, mat.bxdf, ev, smp, scatteringPDF, &specular);
+        f = f * std::abs(ev.wiL.y);
+        ro = hit.point;
+        rd = ev.wi;
+        ev.medium = GetMedium(mat, ev.wi, hit.normalG);
+        scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+        if (scatteringPDF > 0 && !IsBlack(f, lanes)) {
+          if (c.volumes) {  // corrected: the light behind pass-through surfaces, with Tr
+            Hit th;
+            Spec Tr(1.f);
+            const bool si = IntersectTr(sc, ro, rd, th, ev.medium, &Tr, lanes, -1.f, true, nullptr);
+            Ld += MisLight(c, l, lightDistPdf, shadeP, shadeN, si, th, ev, rd, scatteringPDF, specular, false) * f * Tr /
+                  scatteringPDF;
+          } else {
+            Ld += MisLight(c, l, lightDistPdf, shadeP, shadeN, scatterIntersect, hit, ev, rd, scatteringPDF, specular, false) *
+                  f / scatteringPDF;
+          }
+        } else {
+          if (c.corrected) L += beta * Ld;
+          break;
+        }
+        L += beta * Ld;
+        beta = beta * (f / scatteringPDF);
+      } else {  // pass-through (:91-97)
+        ev.medium = GetMedium(mat, rd, hit.normalG);
+        ro = hit.point + hit.normalG * (Dot(hit.normalG, rd) < 0 ? -SURFACE_EPSILON : SURFACE_EPSILON);
+        if (bounces > 0) scatterIntersect = Intersect(sc, ro, rd, hit, c.corrected);
+        bounces--;
+        if (++passThrough > 1024) break;  // A27 cap
+        continue;
+      }
+    }
+    if (bounces > (int)rp.minBounces) {
+      const float q = std::max(.05f, 1 - Luminance(beta, lanes));
+      if (smp.Get1D() < q) break;
+      beta = beta / Spec(1.f - q);
+    }
+  }
+  return L;
+}
+
 // Camera::GenerateRay (Camera.cpp:41-71) with the aperture draws of Aperture.cpp:10-27
 static void GenerateRay(const lgpu_camera &cam, float u, float v, Sampler &smp, V3 *o, V3 *d, bool corrected) {
   const V3 origin(cam.origin), xh(cam.x_hat), yh(cam.y_hat), zh(cam.z_hat);
@@ -1790,6 +2460,8 @@ static Spec RenderSample(const Ctx &c, const RenderParams &rp, const lgpu_camera
   V3 o, d;
   GenerateRay(cam, u, v, smp, &o, &d, c.corrected);
   if (lambda0) *lambda0 = l0;
+  if (rp.integrator == LGPU_INTEGRATOR_VOLPATH) return VolLi(c, rp, o, d, smp, lanes, false);
+  if (rp.integrator == LGPU_INTEGRATOR_MIS_VOLPATH) return VolLi(c, rp, o, d, smp, lanes, true);
   return PathLi(c, rp, o, d, smp, lanes);
 }
 
@@ -1836,7 +2508,17 @@ int oracle_scene_add_texture(oracle_scene s, const lgpu_texture_desc *d) {
 }
 int oracle_scene_add_medium(oracle_scene s, const lgpu_medium_desc *d) {
   if (!s || !d) return -1;
-  s->media.push_back(*d);
+  MediumO m;
+  m.type = (int)d->type;
+  for (int k = 0; k < 3; ++k) { m.sa[k] = d->sigma_a[k]; m.ss[k] = d->sigma_s[k]; }
+  m.g = d->g;
+  if (d->type == LGPU_MEDIUM_GRID) {
+    if (!d->density || !d->nx || !d->ny || !d->nz) return -1;
+    m.nx = d->nx; m.ny = d->ny; m.nz = d->nz;
+    m.dens.assign(d->density, d->density + (size_t)d->nx * d->ny * d->nz);
+    m.bmin = V3(d->bounds_min); m.bmax = V3(d->bounds_max);
+  }
+  s->media.push_back(std::move(m));
   return (int)s->media.size() - 1;
 }
 int oracle_scene_add_material(oracle_scene s, const lgpu_material_desc *d) {
@@ -2042,9 +2724,10 @@ static RenderParams Params(const lgpu_render_desc *d) {
 int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width, uint32_t height, uint32_t n,
                        const uint32_t *pixels, const uint32_t *samples, float *out, int threads) {
   if (!s || !s->committed || !d || !width || !height) return -1;
-  if (d->integrator != LGPU_INTEGRATOR_PATH) return -3;
+  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH || d->integrator == LGPU_INTEGRATOR_DIRECT) return -3;
   const RenderParams rp = Params(d);
-  const Ctx c{s, rp.corrected, (int)rp.mode};
+  Ctx c{s, rp.corrected, (int)rp.mode};
+  c.volumes = rp.corrected && !s->media.empty();
   ParallelFor(n, threads, [&](uint32_t i) {
     float l0;
     const Spec L = RenderSample(c, rp, d->camera, width, height, pixels[i], samples[i], &l0);
@@ -2058,9 +2741,10 @@ int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width
 int oracle_render(oracle_scene s, const lgpu_render_desc *d, uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                   uint32_t w, uint32_t h, int threads, double *rgb, uint32_t *counts) {
   if (!s || !s->committed || !d || x0 + w > W || y0 + h > H) return -1;
-  if (d->integrator != LGPU_INTEGRATOR_PATH) return -3;
+  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH || d->integrator == LGPU_INTEGRATOR_DIRECT) return -3;
   const RenderParams rp = Params(d);
-  const Ctx c{s, rp.corrected, (int)rp.mode};
+  Ctx c{s, rp.corrected, (int)rp.mode};
+  c.volumes = rp.corrected && !s->media.empty();
   const uint32_t TS = 16;
   const uint32_t tx = (w + TS - 1) / TS, ty = (h + TS - 1) / TS;
   std::memset(rgb, 0, sizeof(double) * 3 * w * h);
@@ -2129,6 +2813,60 @@ float oracle_microfacet_lambda(int ggx, const float w[3], float ax, float ay) { 
 void oracle_blackbody_rgb(float T, float out[3]) { BlackbodyRGB(T, out); }
 float oracle_blackbody_normalised(float l, float T) { return BlackbodyNormalised(l, T); }
 void oracle_cmf(float l, float out[3]) { CMF(l, out); }
+
+// Medium closed forms (test hooks). lanes: mode 0 RGB, 1 HERO4 with lambda[4].
+static Lanes MkLanes(int mode, const float *lambda) {
+  Lanes l;
+  l.mode = mode;
+  for (int i = 0; i < 4; ++i) l.lambda[i] = lambda ? lambda[i] : 0.f;
+  return l;
+}
+int oracle_medium_tr(oracle_scene s, uint32_t med, const float o[3], const float d[3], float tFar, int mode,
+                     const float *lambda, float out[4]) {
+  if (!s || med >= s->media.size()) return -1;
+  const Spec r = MediumTr(s->media[med], V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), tFar, MkLanes(mode, lambda));
+  for (int i = 0; i < 4; ++i) out[i] = r.c[i];
+  return 0;
+}
+int oracle_medium_sample_distance(oracle_scene s, uint32_t med, const float o[3], const float d[3], float tFar,
+                                  float u0, float u1, int mode, const float *lambda, float out[7]) {
+  if (!s || med >= s->media.size()) return -1;
+  bool inter;
+  float t, pdf;
+  const Spec w = SampleDistanceU(s->media[med], V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), tFar, V2(u0, u1),
+                                 MkLanes(mode, lambda), &inter, &t, &pdf);
+  out[0] = t; out[1] = pdf; out[2] = inter ? 1.f : 0.f;
+  for (int i = 0; i < 4; ++i) out[3 + i] = w.c[i];
+  return 0;
+}
+float oracle_medium_pdf_distance(oracle_scene s, uint32_t med, const float o[3], const float d[3], float t, int mode,
+                                 const float *lambda, int corrected) {
+  return PdfDistance(s->media[med], V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), t, MkLanes(mode, lambda), corrected != 0);
+}
+int oracle_medium_sample_equiangular(oracle_scene s, uint32_t med, const float o[3], const float d[3],
+                                     const float hitPoint[3], float tFar, const float light[3], float eps, int mode,
+                                     const float *lambda, int corrected, float out[7]) {
+  if (!s || med >= s->media.size()) return -1;
+  bool inter;
+  float t, pdf;
+  const Spec w = SampleEquiangularU(s->media[med], V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]),
+                                    V3(hitPoint[0], hitPoint[1], hitPoint[2]), tFar, V3(light[0], light[1], light[2]),
+                                    eps, MkLanes(mode, lambda), corrected != 0, &inter, &t, &pdf);
+  out[0] = t; out[1] = pdf; out[2] = inter ? 1.f : 0.f;
+  for (int i = 0; i < 4; ++i) out[3 + i] = w.c[i];
+  return 0;
+}
+float oracle_medium_pdf_equiangular(const float o[3], const float d[3], const float light[3], float tFar, float t,
+                                    int corrected) {
+  return PdfEquiangular(V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), V3(light[0], light[1], light[2]), tFar, t,
+                        corrected != 0);
+}
+float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]) {
+  V3 w;
+  const float p = HGSampleU(g, V3(wo[0], wo[1], wo[2]), &w, V2(u0, u1), corrected != 0);
+  wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+  return p;
+}
 float oracle_uplift(const float rgb[3], float l) { return Uplift(rgb, l); }
 float oracle_hg_p(float cosT, float g) {  // HenyeyGreenstein.cpp:8-12
   const float d = 1 + g * g + 2 * g * cosT;

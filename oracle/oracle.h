@@ -56,6 +56,21 @@ float oracle_uplift(const float rgb[3], float lambda_nm);
 float oracle_hg_p(float cos_theta, float g);
 /* Piecewise1D (Piecewise.cpp:9-44): returns offset; pdf, remapped out. */
 uint32_t oracle_piecewise1d_sample(const float *f, uint32_t n, float u, float *pdf);
+/* Media (HomogeneousMedium.cpp:12-92 plus the grid medium): mode 0 RGB, 1 HERO4 (lambda[4]).
+ * sample_* out: t, pdf, interaction (0/1), weight[4]. */
+int oracle_medium_tr(oracle_scene s, uint32_t med, const float o[3], const float d[3], float t_far, int mode,
+                     const float *lambda, float out[4]);
+int oracle_medium_sample_distance(oracle_scene s, uint32_t med, const float o[3], const float d[3], float t_far,
+                                  float u0, float u1, int mode, const float *lambda, float out[7]);
+float oracle_medium_pdf_distance(oracle_scene s, uint32_t med, const float o[3], const float d[3], float t, int mode,
+                                 const float *lambda, int corrected);
+int oracle_medium_sample_equiangular(oracle_scene s, uint32_t med, const float o[3], const float d[3],
+                                     const float hit_point[3], float t_far, const float light[3], float eps, int mode,
+                                     const float *lambda, int corrected, float out[7]);
+float oracle_medium_pdf_equiangular(const float o[3], const float d[3], const float light[3], float t_far, float t,
+                                    int corrected);
+/* HenyeyGreenstein::Sample_p (HenyeyGreenstein.cpp:21-36) with explicit u; returns pdf. */
+float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]);
 
 #ifdef __cplusplus
 }
