@@ -167,6 +167,16 @@ struct MaterialRec {
   uint32_t pad[3];
 };
 
+// Medium (HomogeneousMedium.h; the grid is piecewise constant over [bmin, bmax], z-major).
+struct MediumRec {
+  uint32_t type;
+  float sa[3], ss[3];
+  float g;
+  uint32_t nx, ny, nz;
+  uint32_t dens_off;  // offset into DevScene::density
+  float bmin[3], bmax[3];
+};
+
 // ---------------------------------------------------------------- path state (SoA)
 enum PathFlags : uint32_t {
   kFlagCamera = 1u,        // ray came from the camera (or pass-through at depth 0)
@@ -174,7 +184,9 @@ enum PathFlags : uint32_t {
   kFlagSpecular = 4u,      // last BSDF sample was a delta lobe
   kFlagNoEmit = 8u,        // pass-through at depth > 0: no emission at the next hit
   kFlagRR = 16u,           // RR survived: divide beta by (1 - q) at the next vertex
-  kFlagDead = 32u
+  kFlagDead = 32u,
+  kFlagSideNeg = 64u,      // ScatterEvent::sidedness == -1 (kept along volumetric paths)
+  kFlagTrMIS = 128u        // the sampled light is looked up through pass-through surfaces with Tr
 };
 
 // Active paths are compacted: slot i of the current buffer holds the i-th
@@ -187,7 +199,7 @@ struct PathState {
   float4 *hit;       // {t, u, v, BVH-order triangle index bits}
   uint4 *ids;        // {pixel, sample, dim, (depth << 8) | flags}
   uint4 *aux;        // {sid, NEE light of the previous vertex, pass-through count, medium}
-  float4 *mis;       // {scatter pdf, light dist pdf, q, -}
+  float4 *mis;       // {scatter pdf, light dist pdf, q, weight factor (volumetric paths)}
   float4 *prev_p;    // shading point of the previous vertex
   float4 *prev_n;    // shading normal of the previous vertex
 };
@@ -240,6 +252,9 @@ struct DevScene {
   float tree_power, inf_power;
   uint32_t num_tree_nodes;
   uint32_t num_tris;
+  const MediumRec *media;
+  const float *density;      // grid densities of all media
+  uint32_t num_media;
 };
 
 struct CameraRec {
@@ -252,6 +267,13 @@ struct CameraRec {
   float offset_phi, offset_theta;
 };
 
+// Output queues of a shade launch.
+struct Queues {
+  PathState next;      // compacted survivors
+  ShadowRay *shadow;
+  uint32_t *counters;  // [0] next count, [1] shadow count
+};
+
 struct RenderParams {
   CameraRec cam;
   uint32_t width, height;
@@ -262,5 +284,9 @@ struct RenderParams {
   uint32_t seed;
   uint32_t integrator;
 };
+
+// Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator.
+void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
+                    SampleState ss, uint32_t n, Queues q, uint32_t *spill);
 
 }  // namespace lgpu

@@ -3,6 +3,7 @@
 #pragma once
 #include "dmath.h"
 #include "lgpu_internal.h"
+#include "media.h"
 #include "shading.h"
 #include "vm.h"
 
@@ -125,10 +126,14 @@ struct ShadowSpec {
 };
 
 // Light::Sample_Li for the light kinds (MeshLight.cpp:22-41,134-151; PointLight.cpp:18-30;
-// Spotlight.cpp:29-42; EnvironmentLight.cpp:24-45). Visibility is deferred to the shadow stage:
-// `sh` receives the occlusion ray of Light::MutualVisibility / RayEscapes.
+// Spotlight.cpp:29-42; EnvironmentLight.cpp:24-45). kInline = false: visibility is deferred to
+// the shadow stage, `sh` receives the occlusion ray of Light::MutualVisibility / RayEscapes.
+// kInline = true: the ray is resolved here through `vc` (volumetric paths), Li carries Tr.
+template <bool kInline>
 __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit &h, Ev &ev, Rng &rng, const VmCtx &vm,
-                                bool corrected, float &pdf, ShadowSpec &sh) {
+                                bool corrected, float &pdf, ShadowSpec &sh, const VisCtx *vc) {
+  const bool noOff = kInline && vc->volumes && vc->atMedium;  // medium vertex: no surface offset
+  Spec Tr = S(1.f);
   switch (L.kind) {
     case kLightMesh:
     case kLightTri: {
@@ -141,11 +146,12 @@ __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit
       const uint4 tv = sc.tri_vtx[m.first_tri + tri];
       const V3 v0 = ToV3(sc.vtx_pos[tv.x]), v1 = ToV3(sc.vtx_pos[tv.y]), v2 = ToV3(sc.vtx_pos[tv.z]);
       const V3 pL = PointInTriangle(v0, v1, v2, ux, uy, corrected);
-      const V3 pS = h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
+      const V3 pS = noOff ? h.point : h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
       const V3 diff = pL - pS;  // Scene::MutualVisibility (Scene.cpp:83-95)
       const float mag = Mag(diff);
       ev.wi = diff / mag;
       sh.o = pS; sh.d = ev.wi; sh.tmax = (float)((double)mag - .00001);
+      if (kInline && !Visible(sc, *vc, pS, ev.wi, mag, false, &Tr, vm)) { pdf = 0.f; return S(0.f); }
       float area;
       V3 normal;
       TriAreaNormal(v0, v1, v2, &area, &normal);
@@ -156,23 +162,24 @@ __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit
         if (L.kind == kLightMesh) pdf *= DistSq(h.point, pL) / (cosT * area);
         else if (corrected) pdf = DistSq(h.point, pL) / (cosT * area);
         else pdf = DistSq(h.point, pL) * (cosT * area);  // MeshLight.cpp:145 as written
-        return EmissionOf(sc, L.mesh, vm) * S(1.f) * ML.intensity * kInvPi;
+        return EmissionOf(sc, L.mesh, vm) * Tr * ML.intensity * kInvPi;
       }
       pdf = 0.f;
       return S(0.f);
     }
     case kLightPoint:
     case kLightSpot: {
-      const V3 p = h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
+      const V3 p = noOff ? h.point : h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
       const V3 pos = ToV3(L.pos);
       const V3 diff = pos - p;
       const float mag = Mag(diff);
       ev.wi = diff / mag;
       sh.o = p; sh.d = ev.wi; sh.tmax = (float)((double)mag - .00001);
+      if (kInline && !Visible(sc, *vc, p, ev.wi, mag, true, &Tr, vm)) { pdf = 0.f; return S(0.f); }
       pdf = 1.f;
       const float d2 = DistSq(pos, h.point);
       ev.wiL = ToLocal(h, ev.wi);
-      Spec e = LightColour(L, vm) * S(1.f);
+      Spec e = LightColour(L, vm) * Tr;
       if (L.kind == kLightSpot) {  // Spotlight::Falloff (Spotlight.cpp:21-27)
         const V3 w = -ev.wi;
         const float cosT = Dot(w, ToV3(L.axis));
@@ -198,12 +205,13 @@ __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit
       const float cosT = cosf(theta), sinT = sinf(theta);
       if (sinT == 0) { pdf = 0.f; return S(0.f); }
       ev.wi = SphericalDirection(sinT, cosT, phi);
-      sh.o = h.point + h.normalG * kSurfaceEpsilon;
+      sh.o = noOff ? h.point : h.point + h.normalG * kSurfaceEpsilon;
       sh.d = ev.wi;
       sh.tmax = INFINITY;
+      if (kInline && !Escapes(sc, *vc, sh.o, ev.wi, &Tr, vm)) { pdf = 0.f; return S(0.f); }
       ev.wiL = ToLocal(h, ev.wi);
       pdf /= 2 * kPi * kPi * sinT;
-      return EnvLookup(sc, d0, d1, vm) * S(1.f) * sc.env.intensity;
+      return EnvLookup(sc, d0, d1, vm) * Tr * sc.env.intensity;
     }
   }
   pdf = 0.f;
@@ -223,6 +231,152 @@ __device__ inline float PdfLiHit(const DevScene &sc, const LightRec &L, V3 wi, c
     return triPdf * distSq / (cosT * area);
   }
   return distSq / (cosT * area);
+}
+
+// Light reached by a BSDF- or phase-sampled ray: the MIS half of PathIntegrator.cpp:52-63,
+// VolumetricPathIntegrator.cpp:127-139 and MISVolumetricPathIntegrator.cpp:52-66. `h`/`bi` is that
+// ray's hit (si: it hit something); refP/refN is the corrected light-selection reference point.
+// Returns Le * w; the caller applies f or Tr and 1/pdf. envNoLightPdf: the light technique
+// cannot produce the environment (point sampling at medium vertices, corrected).
+__device__ inline Spec MisLight(const DevScene &sc, uint32_t l, float lightDistPdf, V3 refP, V3 refN, bool si, const Hit &h,
+                                uint32_t bi, V3 rd, float scatPdf, bool specular, bool envNoLightPdf, VmCtx vm,
+                                bool corrected) {
+  uint32_t nl = kNone;
+  if (si) {
+    const uint32_t ml = sc.meshes[h.mesh].mesh_light;
+    if (ml != kNone) nl = sc.strategy == LGPU_LIGHTS_TREE ? (kTreeBit | sc.tri_tlight[h.prim]) : ml;
+  } else if (sc.has_env) {
+    nl = sc.env_light;
+  }
+  if (nl == kNone) return S(0.f);
+  bool same = nl == l;
+  if (!corrected && sc.strategy == LGPU_LIGHTS_TREE && (nl & kTreeBit)) same = false;  // MeshLight != TriangleLight
+  float ldp = lightDistPdf;
+  if (!same) ldp = corrected ? LightPdf(sc, nl, refP, refN, true) : LightPdf(sc, nl, h.point, h.normalS, false);
+  float pl;
+  Spec Le;
+  if (si) {
+    const LightRec &NL = LightOf(sc, nl);
+    if (!corrected && sc.strategy == LGPU_LIGHTS_TREE) pl = PdfLiHit(sc, sc.lights[NL.parent], rd, h, bi, false);
+    else pl = PdfLiHit(sc, NL, rd, h, bi, corrected);
+    vm.u = h.u;
+    vm.v = h.v;
+    Le = EmissionOf(sc, h.mesh, vm) * sc.lights[sc.meshes[h.mesh].mesh_light].intensity * kInvPi;
+    if (corrected && Dot(rd, h.normalG) > 0) Le = S(0.f);
+  } else {
+    pl = envNoLightPdf ? 0.f : EnvPdfLi(sc, rd);
+    Le = EnvLe(sc, rd, vm);
+  }
+  const float lightPDF = ldp * pl;
+  const float w = (corrected && specular) ? 1.f : PowerHeuristic(scatPdf, lightPDF);
+  if (!IsBlack(Le, vm.mode) && (lightPDF > 0 || corrected)) return Le * w;
+  return S(0.f);
+}
+
+// PartialLightSample (Light.h:107-113); pdf starts at 1 (left uninitialised by the reference).
+struct PartialLS {
+  V3 point, normal;
+  Spec Le;
+  uint32_t light;
+  float pdf, selPdf;
+};
+// Light::SamplePoint (MeshLight.cpp:66-81,166-180; PointLight.cpp:40-44; Spotlight.cpp:50-54;
+// EnvironmentLight.cpp:69-74). Faithful writes the sampled uv into the event hit (h, vm) as written.
+__device__ inline Spec SamplePointL(const DevScene &sc, const LightRec &L, Hit &h, Rng &rng, VmCtx &vm, bool corrected,
+                                    PartialLS &ls) {
+  switch (L.kind) {
+    case kLightMesh:
+    case kLightTri: {
+      const MeshRec m = sc.meshes[L.mesh];
+      const LightRec &ML = L.kind == kLightTri ? sc.lights[L.parent] : L;
+      uint32_t tri = L.tri;
+      float area;
+      if (L.kind == kLightMesh) {
+        float distPdf;
+        tri = PwSampleDiscrete(sc.dist, L.cdf_off, rng.Get1D(), &distPdf);
+        const uint4 tv = sc.tri_vtx[m.first_tri + tri];
+        TriAreaNormal(ToV3(sc.vtx_pos[tv.x]), ToV3(sc.vtx_pos[tv.y]), ToV3(sc.vtx_pos[tv.z]), &area, &ls.normal);
+        ls.pdf *= distPdf / area;
+      } else {
+        const uint4 tv = sc.tri_vtx[m.first_tri + tri];
+        TriAreaNormal(ToV3(sc.vtx_pos[tv.x]), ToV3(sc.vtx_pos[tv.y]), ToV3(sc.vtx_pos[tv.z]), &area, &ls.normal);
+        ls.pdf /= area;
+      }
+      const float ux = rng.Get1D(), uy = rng.Get1D();
+      const uint4 tv = sc.tri_vtx[m.first_tri + tri];
+      float uu = 0.f, uv = 0.f;
+      if (m.flags & kMeshUV) {  // maths::BarycentricInterpolation (maths.h:58-60)
+        const float2 a = sc.vtx_uv[tv.x], b = sc.vtx_uv[tv.y], c = sc.vtx_uv[tv.z];
+        uu = a.x + (b.x - a.x) * ux + (c.x - a.x) * uy;
+        uv = a.y + (b.y - a.y) * ux + (c.y - a.y) * uy;
+      }
+      ls.point = PointInTriangle(ToV3(sc.vtx_pos[tv.x]), ToV3(sc.vtx_pos[tv.y]), ToV3(sc.vtx_pos[tv.z]), ux, uy, corrected);
+      VmCtx ve = vm;
+      ve.u = uu;
+      ve.v = uv;
+      if (!corrected) { h.u = uu; h.v = uv; vm.u = uu; vm.v = uv; }
+      return EmissionOf(sc, L.mesh, ve) * ML.intensity * kInvPi;
+    }
+    case kLightPoint:
+    case kLightSpot:
+      ls.point = ToV3(L.pos);
+      return LightColour(L, vm) * L.intensity;
+    case kLightEnv:
+      ls.pdf *= 0.f;
+      ls.point = v3(0, 0, 0);
+      return S(0.f);
+  }
+  return S(0.f);
+}
+// Light::Visibility (MeshLight.cpp:83-92,182-191; PointLight.cpp:46-55; Spotlight.cpp:56-65;
+// EnvironmentLight.cpp:76-84): completes ls.pdf and sets ev.wi. `eh` is the event's hit.
+__device__ inline Spec VisibilityL(const DevScene &sc, const LightRec &L, V3 sp, Ev &ev, const Hit &eh, PartialLS &ls,
+                                   const VmCtx &vm, bool corrected, const VisCtx &vc) {
+  Spec Tr = S(1.f);
+  switch (L.kind) {
+    case kLightMesh:
+    case kLightTri: {
+      const V3 diff = ls.point - sp;
+      const float mag = Mag(diff);
+      ev.wi = diff / mag;
+      if (Visible(sc, vc, sp, ev.wi, mag, false, &Tr, vm)) {
+        const V3 n = (L.kind == kLightTri && !corrected) ? eh.normalG : ls.normal;  // TriangleLight: event normal as written
+        const float dn = Dot(n, -ev.wi);
+        const float cosT = fabsf(dn);
+        if (corrected && !(dn > 0)) break;
+        ls.pdf *= DistSq(sp, ls.point) / cosT;
+        return Tr;
+      }
+      break;
+    }
+    case kLightPoint:
+    case kLightSpot: {
+      const V3 p = corrected ? sp : sp + eh.normalG * kSurfaceEpsilon * (float)ev.sidedness;
+      const V3 diff = ls.point - p;
+      const float mag = Mag(diff);
+      ev.wi = diff / mag;
+      if (Visible(sc, vc, p, ev.wi, mag, true, &Tr, vm)) {
+        const float d2 = DistSq(ToV3(L.pos), eh.point);
+        if (L.kind == kLightSpot) {
+          const float cosT = Dot(-ev.wi, ToV3(L.axis));
+          float fall;
+          if (cosT < L.cos_cone) fall = 0.f;
+          else if (cosT > L.cos_falloff) fall = 1.f;
+          else { const float dd = (cosT - L.cos_cone) / (L.cos_falloff - L.cos_cone); fall = (dd * dd) * (dd * dd); }
+          return Tr * fall / d2;
+        }
+        return Tr / d2;
+      }
+      break;
+    }
+    case kLightEnv: {
+      const V3 wi = Norm(ls.point - sp);
+      if (Escapes(sc, vc, sp, wi, &Tr, vm)) return Tr;
+      break;
+    }
+  }
+  ls.pdf = 0.f;
+  return S(0.f);
 }
 
 }  // namespace lgpu

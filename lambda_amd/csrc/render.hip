@@ -139,12 +139,6 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene sc, SampleState ss, 
 }
 
 // ---------------------------------------------------------------- shade
-struct Queues {
-  PathState next;      // compacted survivors
-  ShadowRay *shadow;
-  uint32_t *counters;  // [0] next count, [1] shadow count
-};
-
 __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_t sid, Spec c) {
   const uint32_t slot = atomicAdd(&q.counters[1], 1u);
   ShadowRay r;
@@ -297,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, RenderParams rp, 
   if (pick != kNone) {
     const LightRec &LL = LightOf(sc, pick);
     delta = LL.kind == kLightPoint || LL.kind == kLightSpot;
-    Li = SampleLi(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh);
+    Li = SampleLi<false>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, nullptr);
   }
   lightPDF *= lightDistPdf;
   bool hasNee = false;
@@ -439,7 +433,9 @@ static inline uint32_t Blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) /
 static lgpu_status ValidateRender(const lgpu_render_desc *d) {
   if (!d || !d->scene) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: null descriptor or scene");
   if (!d->scene->committed) return Fail(LGPU_ERR_STATE, "render: scene not committed");
-  if (d->integrator != LGPU_INTEGRATOR_PATH) return Fail(LGPU_ERR_UNSUPPORTED, "render: only the path integrator is built");
+  if (d->integrator != LGPU_INTEGRATOR_PATH && d->integrator != LGPU_INTEGRATOR_VOLPATH &&
+      d->integrator != LGPU_INTEGRATOR_MIS_VOLPATH)
+    return Fail(LGPU_ERR_UNSUPPORTED, "render: integrator not built (path, volpath, mis_volpath are)");
   if (d->spectral_mode > LGPU_SPECTRAL_HERO4 || d->estimator_mode > LGPU_ESTIMATOR_CORRECTED)
     return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: bad spectral or estimator mode");
   if (d->max_bounces == 0) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: max_bounces must be > 0");
@@ -520,7 +516,10 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       LGPU_HIP(hipGetLastError());
       LGPU_HIP(hipEventRecord(ev[3], stream));
       Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
-      hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, s->dsc, rp, W.ps[cur], W.ss, count, q);
+      if (rp.integrator == LGPU_INTEGRATOR_PATH)
+        hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, s->dsc, rp, W.ps[cur], W.ss, count, q);
+      else
+        LaunchShadeVol(Blocks(count), stream, s->dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       LGPU_HIP(hipGetLastError());
       LGPU_HIP(hipEventRecord(ev[4], stream));
       uint32_t cnt[2];

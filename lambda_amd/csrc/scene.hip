@@ -812,8 +812,33 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
     r.med_in = hm.desc.medium_interior;
     r.med_out = hm.desc.medium_exterior;
     if (!c.err.empty()) return Fail(LGPU_ERR_GRAPH, "material " + std::to_string(mats.size()) + ": " + c.err);
+    for (uint32_t id : {r.med_in, r.med_out})
+      if (id != kNone && id >= s->media.size())
+        return Fail(LGPU_ERR_INVALID_ARGUMENT, "material " + std::to_string(mats.size()) + ": unknown medium id");
     mats.push_back(r);
   }
+
+  // ------------------------------------------------ media
+  std::vector<MediumRec> media;
+  std::vector<float> density;
+  for (size_t k = 0; k < s->media.size(); ++k) {
+    const lgpu_medium_desc &md = s->media[k];
+    MediumRec r;
+    std::memset(&r, 0, sizeof(r));
+    r.type = md.type;
+    for (int a = 0; a < 3; ++a) {
+      r.sa[a] = md.sigma_a[a]; r.ss[a] = md.sigma_s[a];
+      r.bmin[a] = md.bounds_min[a]; r.bmax[a] = md.bounds_max[a];
+    }
+    r.g = md.g;
+    if (md.type == LGPU_MEDIUM_GRID) {
+      r.nx = md.nx; r.ny = md.ny; r.nz = md.nz;
+      r.dens_off = (uint32_t)density.size();
+      density.insert(density.end(), s->media_density[k].begin(), s->media_density[k].end());
+    }
+    media.push_back(r);
+  }
+  if (density.empty()) density.push_back(0.f);
 
   // ------------------------------------------------ lights
   Pool pool;
@@ -922,6 +947,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   std::memset(&d, 0, sizeof(d));
   d.strategy = strategy;
   d.num_lights = (uint32_t)lights.size();
+  d.num_media = (uint32_t)media.size();
   d.has_env = hasEnv;
   d.env_light = envLight;
   d.env = env;
@@ -1083,6 +1109,8 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   UP(tlLeaf, tlight_leaf);
   UP(tlSlot, tlight_slot);
   UP(pool.f, dist);
+  UP(media, media);
+  UP(density, density);
 #undef UP
   s->committed = true;
   return LGPU_OK;

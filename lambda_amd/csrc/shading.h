@@ -457,6 +457,8 @@ __device__ inline float TreeImportance(const TreeNode &nd, V3 point, V3 normalS)
   const float thetaDash = fmaxf(theta - nd.thetaO - thetaU, 0.f);
   if (thetaDash < nd.thetaE) {
     const float E = nd.power;
+    // a zero normal marks a medium vertex (corrected mode): no surface-cosine bound
+    if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * cosf(thetaDash);
     const float thetaI = acosf(Dot(delta, normalS));
     return (cosf(fmaxf(thetaI - thetaU, 0.f)) * E) / d2 * cosf(thetaDash);
   }

@@ -1259,7 +1259,7 @@ static void GridWalk(const MediumO &m, const V3 &o, const V3 &d, float t0, float
   float t = tn;
   for (int it = 0; it < n[0] + n[1] + n[2] + 3; ++it) {
     const int ax = (tNext[0] <= tNext[1] && tNext[0] <= tNext[2]) ? 0 : (tNext[1] <= tNext[2] ? 1 : 2);
-    const float tc = std::min(tNext[ax], tf);
+    const float tc = std::max(t, std::min(tNext[ax], tf));  // entry rounding can put tNext before t
     const float rho = m.dens[((size_t)cell[2] * m.ny + cell[1]) * m.nx + cell[0]];
     if (fn(rho, t, tc)) return;
     t = tc;
@@ -2175,9 +2175,15 @@ static Spec LdMediumPoint(const Ctx &c, const MediumO &M, Event &ev, Hit &eh, co
   eh.point = p;
   Hit *saved = ev.hit;
   ev.hit = &eh;
-  const Light &LL = LightOf(sc, ls.light);
-  const bool delta = LL.kind == L_POINT || LL.kind == L_SPOT;
-  Spec Li = ls.Le * VisibilityL(c, LL, p, ev, eh, ls);
+  Spec Li(0);
+  bool delta = false;
+  if (ls.light.kind >= 0) {
+    const Light &LL = LightOf(sc, ls.light);
+    delta = LL.kind == L_POINT || LL.kind == L_SPOT;
+    Li = ls.Le * VisibilityL(c, LL, p, ev, eh, ls);
+  } else {
+    ls.pdf = 0;
+  }
   float lightPDF = ls.pdf;
   Spec Ld(0);
   if (lightPDF > 0 && !IsBlack(Li, ev.lanes)) {
