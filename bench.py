@@ -11,6 +11,12 @@ call. Synthetic procedural data (no assets).
 Prints one JSON line with the metric, a roofline entry for the dominant kernel
 (BVH extension traversal) and a CPU baseline (the CPU checker on the host
 cores over a bounded crop of the same workload).
+
+Other workloads (--workload): "media" (128^3 FP32 fBm density grid in a 2x2x2
+pass-through box, 1 spotlight + 64 point lights, MIS distance + equiangular
+volumetric integrator, 1024x1024, 128 spp, depth 32) and "cornell" (36 triangles,
+256x256, depth 8). --relmse measures time-to-relMSE <= 1e-3 of progressive
+rendering against an FP64 GPU reference at --ref-spp with another seed.
 """
 import argparse
 import json
@@ -28,6 +34,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+DEFAULT_STEPS = {"many-light": 256, "media": 128, "cornell": 64}
+
+
 def build_workload(name):
     from lambda_amd import scene
     if name == "cornell":
@@ -35,15 +44,30 @@ def build_workload(name):
         W, H = 256, 256
         cam = scene.cornell_camera(W, H)
         cfg = {"workload": "cornell-box 36 triangles, pinhole 256x256, max depth 8",
-               "width": W, "height": H, "max_bounces": 8, "triangles": sd.n_tris}
+               "width": W, "height": H, "max_bounces": 8, "triangles": sd.n_tris, "integrator": "path"}
+        return sd, cam, cfg
+    if name == "media":
+        sd = scene.media_scene(res=128)
+        W, H = 1024, 1024
+        cam = scene.media_camera(W, H)
+        cfg = {"workload": "heterogeneous media: 128^3 FP32 fBm density grid (sigma_t 4 d, albedo 0.9, HG g 0.6) "
+                           "in a pass-through 2x2x2 box, 1 spotlight + 64 point lights, MIS distance + equiangular "
+                           "volumetric integrator, pinhole 1024x1024, depth 32",
+               "width": W, "height": H, "max_bounces": 32, "triangles": sd.n_tris, "integrator": "mis_volpath"}
         return sd, cam, cfg
     sd = scene.many_light_scene()
     W, H = 1920, 1080
     cam = scene.many_light_camera(W, H)
     cfg = {"workload": "many-light glossy mesh: 1.2M triangles (200 displaced icospheres + ground), 4096 "
                        "emissive triangles via the light tree, 1024x512 HDR sky, thin lens 1920x1080, depth 16",
-           "width": W, "height": H, "max_bounces": 16, "triangles": sd.n_tris}
+           "width": W, "height": H, "max_bounces": 16, "triangles": sd.n_tris, "integrator": "path"}
     return sd, cam, cfg
+
+
+def integrator_id(cfg):
+    from lambda_amd import abi
+    return {"path": abi.INTEGRATOR_PATH, "volpath": abi.INTEGRATOR_VOLPATH,
+            "mis_volpath": abi.INTEGRATOR_MIS_VOLPATH}[cfg["integrator"]]
 
 
 def cpu_baseline(sd, cam, cfg, spectral, estimator, budget_s):
@@ -59,7 +83,7 @@ def cpu_baseline(sd, cam, cfg, spectral, estimator, budget_s):
     win = ((W - cw) // 2, (H - ch) // 2, cw, ch)
     d = abi.RenderDesc()
     d.camera = cam
-    d.integrator, d.max_bounces, d.min_bounces = abi.INTEGRATOR_PATH, cfg["max_bounces"], 3
+    d.integrator, d.max_bounces, d.min_bounces = integrator_id(cfg), cfg["max_bounces"], 3
     d.spectral_mode, d.estimator_mode, d.seed = spectral, estimator, 0
     d.spp_begin, d.spp_end = 0, 1
     t0 = time.perf_counter()
@@ -80,9 +104,9 @@ def cpu_baseline(sd, cam, cfg, spectral, estimator, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=0, help="spp timed (default: 256 many-light, 128 media, 64 cornell)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="many-light", choices=["many-light", "cornell"])
+    ap.add_argument("--workload", default="many-light", choices=["many-light", "media", "cornell"])
     ap.add_argument("--spectral", default="hero4", choices=["hero4", "rgb"])
     ap.add_argument("--estimator", default="corrected", choices=["corrected", "faithful"])
     ap.add_argument("--capacity", type=int, default=0)
@@ -91,7 +115,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
     ap.add_argument("--save-image", default="")
+    ap.add_argument("--relmse", action="store_true", help="measure time-to-relMSE <= 1e-3 instead")
+    ap.add_argument("--ref-spp", type=int, default=16384)
     args = ap.parse_args()
+    if not args.steps:
+        args.steps = DEFAULT_STEPS[args.workload]
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X "
               "(multi-GPU sample partitioning is a design note, not built); run with --gpus 1", file=sys.stderr)
@@ -121,16 +149,27 @@ def main():
     W, H = cfg["width"], cfg["height"]
     film = gpu.Film(dev, W, H)
 
-    def desc(b, e, flags=0):
-        return gpu.render_desc(s, film, cam, b, e, max_bounces=cfg["max_bounces"], min_bounces=3,
-                               spectral=spectral, estimator=estimator, capacity=args.capacity, flags=flags)
+    def desc(b, e, flags=0, f=None, seed=0):
+        return gpu.render_desc(s, f or film, cam, b, e, max_bounces=cfg["max_bounces"], min_bounces=3,
+                               spectral=spectral, estimator=estimator, capacity=args.capacity, flags=flags,
+                               integrator=integrator_id(cfg), seed=seed)
+
+    if args.relmse:
+        relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev)
+        del keep
+        film.close()
+        s.close()
+        dev.close()
+        return
 
     # traversal work per extension ray (untimed counting pass, 1 spp)
     cst = gpu.render(desc(10000, 10001, abi.RENDER_COUNT_TRAVERSAL))
     fetch_per_ray = cst.nodes_visited / max(1, cst.extension_rays)
     tris_per_ray = cst.tris_tested / max(1, cst.extension_rays)
     film.clear()
-    log("counting pass: %.2f node fetches, %.2f triangles per extension ray" % (fetch_per_ray, tris_per_ray))
+    voxels_per_vertex = cst.voxel_steps / max(1, cst.extension_rays)
+    log("counting pass: %.2f node fetches, %.2f triangles, %.1f voxel steps per extension ray"
+        % (fetch_per_ray, tris_per_ray, voxels_per_vertex))
 
     if args.warmup:
         gpu.render(desc(0, args.warmup))
@@ -149,8 +188,11 @@ def main():
     # Rooflines (HIP-event kernel times on the render stream).
     # extend: algorithmic bytes per ray = 32 B per binary-BVH node examined (two per
     #   BVH2 fetch) + 36 B per triangle tested + 32 B ray read + 20 B hit write.
-    # shade: algorithmic bytes per path vertex = 2 x 96 B path state (read + write).
+    # shade: algorithmic bytes per path vertex = 2 x 96 B path state (read + write),
+    #   plus 4 B per density-grid voxel visited by the vertex's medium walks.
     bytes_per_ray = 32.0 * 2.0 * fetch_per_ray + 36.0 * tris_per_ray + 32.0 + 20.0
+    bytes_per_vertex = 192.0 + 4.0 * voxels_per_vertex
+    shade_name = "k_shade" if cfg["integrator"] == "path" else "k_shade_vol"
     ext_s = st.ms_extend / 1000.0
     sh_s = st.ms_shade / 1000.0
     ext = {"bound": "hbm", "achieved": round(bytes_per_ray * st.extension_rays / ext_s / 1e9, 1),
@@ -158,15 +200,17 @@ def main():
            "launches": st.launches_extend, "avg_launch_ms": round(st.ms_extend / max(1, st.launches_extend), 4),
            "alg_bytes_per_unit": round(bytes_per_ray, 1), "unit_of_work": "extension ray",
            "units_per_s": round(st.extension_rays / ext_s, 1)}
-    shd = {"bound": "hbm", "achieved": round(192.0 * st.extension_rays / sh_s / 1e9, 1),
-           "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": "k_shade",
+    shd = {"bound": "hbm", "achieved": round(bytes_per_vertex * st.extension_rays / sh_s / 1e9, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": shade_name,
            "launches": st.launches_shade, "avg_launch_ms": round(st.ms_shade / max(1, st.launches_shade), 4),
-           "alg_bytes_per_unit": 192.0, "unit_of_work": "path vertex",
+           "alg_bytes_per_unit": round(bytes_per_vertex, 1), "unit_of_work": "path vertex",
            "units_per_s": round(st.extension_rays / sh_s, 1)}
     for r in (ext, shd):
         r["frac"] = round(r["achieved"] / HBM_PEAK_GBS, 4)
     roofline = dict(shd if st.ms_shade >= st.ms_extend else ext)
     roofline["other_kernels"] = [shd if roofline["kernel"] == "k_extend" else ext]
+    if args.workload != "many-light":
+        cfg["note"] = "secondary workload; the headline line is --workload many-light"
     if args.save_image:
         rgba = film.read_rgba()
         import numpy as np
@@ -195,6 +239,75 @@ def main():
     film.close()
     s.close()
     dev.close()
+
+
+def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
+    """time-to-relMSE <= 1e-3: progressive rendering (2^k spp checkpoints, resumable sample
+    ranges, cumulative render wall time) against an FP64 reference at --ref-spp rendered with
+    seed 1; the reference is checked against the CPU checker on a 64x64 centre crop."""
+    import numpy as np
+    from lambda_amd import gpu
+    W, H = cfg["width"], cfg["height"]
+    ref = gpu.Film(dev, W, H)
+    chunk = 256
+    t0 = time.perf_counter()
+    for b in range(0, args.ref_spp, chunk):
+        gpu.render(desc(b, min(args.ref_spp, b + chunk), f=ref, seed=1))
+        log("reference %d / %d spp (%.0f s)" % (min(args.ref_spp, b + chunk), args.ref_spp, time.perf_counter() - t0))
+    ref_s = time.perf_counter() - t0
+    rsum, rcnt = ref.read()
+    r = rsum / rcnt[..., None]
+    film.clear()
+    checkpoints = []
+    spent, done, k = 0.0, 0, 0
+    hit = None
+    while (1 << k) <= args.ref_spp // 2:
+        target = 1 << k
+        t = time.perf_counter()
+        gpu.render(desc(done, target))
+        spent += time.perf_counter() - t
+        done = target
+        xs, xc = film.read()
+        x = xs / xc[..., None]
+        rel = float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+        checkpoints.append({"spp": target, "seconds": round(spent, 4), "relmse": rel})
+        log("spp %d: %.3f s, relMSE %.3e" % (target, spent, rel))
+        if rel <= 1e-3:
+            hit = checkpoints[-1]
+            break
+        k += 1
+    # CPU validation of the reference on a 64x64 centre crop
+    valid = None
+    if not args.no_cpu:
+        import oracle
+        from lambda_amd import abi
+        o = oracle.OracleScene()
+        keep = sd.push(o, abi.LIGHTS_TREE)
+        cw = 64
+        win = ((W - cw) // 2, (H - cw) // 2, cw, cw)
+        d = abi.RenderDesc()
+        d.camera = cam
+        d.integrator, d.max_bounces, d.min_bounces = integrator_id(cfg), cfg["max_bounces"], 3
+        d.spectral_mode, d.estimator_mode, d.seed = spectral, estimator, 2
+        d.spp_begin, d.spp_end = 0, 1024
+        t = time.perf_counter()
+        crgb, ccnt = o.render(d, W, H, window=win, threads=max(1, min(os.cpu_count() or 1, 16)))
+        c = crgb / ccnt[..., None]
+        rc = r[win[1]:win[1] + cw, win[0]:win[0] + cw]
+        valid = {"crop": "64x64 centre", "cpu_spp": 1024, "cpu_s": round(time.perf_counter() - t, 1),
+                 "cpu_mean_rgb": [round(float(v), 6) for v in c.mean(axis=(0, 1))],
+                 "gpu_ref_mean_rgb": [round(float(v), 6) for v in rc.mean(axis=(0, 1))],
+                 "relmse_cpu_vs_ref": float(np.mean((c - rc) ** 2 / (rc * rc + 0.01)))}
+        log("reference check: %s" % valid)
+        del keep
+    line = {"metric": "time-to-relMSE<=1e-3 (plain wavefront, 1 MI355X)",
+            "value": hit["seconds"] if hit else None, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+            "spp_at_target": hit["spp"] if hit else None, "dtype": "f32", "data": "synthetic (procedural scene)",
+            "config": dict(cfg, spectral=args.spectral, estimator=args.estimator, reference_spp=args.ref_spp,
+                           reference_seed=1, reference_s=round(ref_s, 1)),
+            "checkpoints": checkpoints, "reference_check": valid}
+    print(json.dumps(line), flush=True)
+    ref.close()
 
 
 if __name__ == "__main__":

@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 1u
+#define LGPU_ABI_VERSION 2u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -237,6 +237,7 @@ typedef struct lgpu_render_stats {
   uint64_t tris_tested;     /* with LGPU_RENDER_COUNT_TRAVERSAL: triangles tested (extension rays) */
   uint32_t waves;           /* path-pool refills */
   uint32_t launches_extend, launches_shade, launches_shadow;
+  uint64_t voxel_steps;     /* with LGPU_RENDER_COUNT_TRAVERSAL: density-grid voxels visited by medium walks */
 } lgpu_render_stats;
 
 /* ---------------- device (Lambda.h:37-40 lambdaCreateDevice/Release) ---- */

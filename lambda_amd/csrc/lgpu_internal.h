@@ -255,6 +255,7 @@ struct DevScene {
   const MediumRec *media;
   const float *density;      // grid densities of all media
   uint32_t num_media;
+  unsigned long long *voxel_count;  // non-null in counting renders: voxels visited by grid walks
 };
 
 struct CameraRec {

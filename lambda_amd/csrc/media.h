@@ -67,18 +67,21 @@ __device__ inline void GridWalk(const DevScene &sc, const MediumRec &m, V3 o, V3
   const float *dens = sc.density + m.dens_off;
   float t = tn;
   const int iters = nx + ny + nz + 3;
-  for (int it = 0; it < iters; ++it) {
+  int it = 0;
+  while (it < iters) {
+    ++it;
     const int ax = (tNx <= tNy && tNx <= tNz) ? 0 : (tNy <= tNz ? 1 : 2);
     const float tnext = ax == 0 ? tNx : (ax == 1 ? tNy : tNz);
     const float tc = StdMax(t, StdMin(tnext, tf));
     const float rho = dens[((size_t)cz * m.ny + cy) * m.nx + cx];
-    if (fn(rho, t, tc)) return;
+    if (fn(rho, t, tc)) break;
     t = tc;
-    if (t >= tf) return;
-    if (ax == 0) { cx += sx; if (cx < 0 || cx >= nx) return; tNx += dX; }
-    else if (ax == 1) { cy += sy; if (cy < 0 || cy >= ny) return; tNy += dY; }
-    else { cz += sz; if (cz < 0 || cz >= nz) return; tNz += dZ; }
+    if (t >= tf) break;
+    if (ax == 0) { cx += sx; if (cx < 0 || cx >= nx) break; tNx += dX; }
+    else if (ax == 1) { cy += sy; if (cy < 0 || cy >= ny) break; tNy += dY; }
+    else { cz += sz; if (cz < 0 || cz >= nz) break; tNz += dZ; }
   }
+  if (sc.voxel_count) atomicAdd(sc.voxel_count, (unsigned long long)it);
 }
 struct DensAcc {
   float D, rho;

@@ -484,11 +484,13 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   if (st != LGPU_OK) return st;
   const RenderParams rp = MakeParams(d, w, h);
   const bool countTrav = (d->flags & LGPU_RENDER_COUNT_TRAVERSAL) != 0;
+  DevScene dsc = s->dsc;
+  if (countTrav) dsc.voxel_count = W.tcount.as<unsigned long long>() + 2;
   hipStream_t stream = dev->stream;
   hipEvent_t *ev = dev->ev;
   lgpu_render_stats S;
   std::memset(&S, 0, sizeof(S));
-  if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount.ptr, 0, 16, stream));
+  if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount.ptr, 0, 24, stream));
   for (uint64_t base = 0; base < total; base += cap) {
     const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - base);
     int cur = 0;
@@ -508,18 +510,18 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       LGPU_HIP(hipMemsetAsync(W.counters.ptr, 0, 8, stream));
       LGPU_HIP(hipEventRecord(ev[2], stream));
       if (countTrav)
-        hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, s->dsc, W.ps[cur], count,
+        hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count,
                            W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
       else
-        hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, s->dsc, W.ps[cur], count,
+        hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count,
                            W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
       LGPU_HIP(hipGetLastError());
       LGPU_HIP(hipEventRecord(ev[3], stream));
       Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
       if (rp.integrator == LGPU_INTEGRATOR_PATH)
-        hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, s->dsc, rp, W.ps[cur], W.ss, count, q);
+        hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, rp, W.ps[cur], W.ss, count, q);
       else
-        LaunchShadeVol(Blocks(count), stream, s->dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       LGPU_HIP(hipGetLastError());
       LGPU_HIP(hipEventRecord(ev[4], stream));
       uint32_t cnt[2];
@@ -535,7 +537,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       if (cnt[0] > count || cnt[1] > count) return Fail(LGPU_ERR_STATE, "render: queue overflow");
       if (cnt[1]) {
         LGPU_HIP(hipEventRecord(ev[5], stream));
-        hipLaunchKernelGGL(k_shadow, dim3(Blocks(cnt[1])), dim3(kBlock), 0, stream, s->dsc, W.ss, W.shadow.as<ShadowRay>(),
+        hipLaunchKernelGGL(k_shadow, dim3(Blocks(cnt[1])), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(),
                            cnt[1], W.spill.as<uint32_t>());
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[6], stream));
@@ -556,10 +558,11 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   }
   LGPU_HIP(hipStreamSynchronize(stream));
   if (countTrav) {
-    unsigned long long tc[2];
-    LGPU_HIP(hipMemcpy(tc, W.tcount.ptr, 16, hipMemcpyDeviceToHost));
+    unsigned long long tc[3];
+    LGPU_HIP(hipMemcpy(tc, W.tcount.ptr, 24, hipMemcpyDeviceToHost));
     S.nodes_visited = tc[0];
     S.tris_tested = tc[1];
+    S.voxel_steps = tc[2];
   }
   S.samples = total;
   S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -62,6 +62,9 @@ def lib():
         L.lgpu_camera_set_rotation.argtypes = [vp, f32, f32]
         L.lgpu_camera_set_forwards.argtypes = [vp, vp]
         L.lgpu_camera_set_fov.argtypes = [vp, f32]
+        if L.lgpu_abi_version() != abi.ABI_VERSION:
+            raise ImportError("liblambda_gpu.so has ABI %d, this binding expects %d: rebuild it"
+                              % (L.lgpu_abi_version(), abi.ABI_VERSION))
         _LIB = L
     return _LIB
 

@@ -27,13 +27,13 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert gpu.lib().lgpu_abi_version() == 1
+    assert gpu.lib().lgpu_abi_version() == abi.ABI_VERSION
 
 
 def test_struct_sizes_match_header_layout():
     # descriptors are passed by pointer; their sizes must match the C layout
     assert C.sizeof(abi.Node) == 4 + 16 + 48
-    assert C.sizeof(abi.RenderStats) == 5 * 8 + 5 * 8 + 4 * 4
+    assert C.sizeof(abi.RenderStats) == 5 * 8 + 5 * 8 + 4 * 4 + 8
     assert C.sizeof(abi.Camera) == 4 + 12 * 4 + 3 * 4 + 4 + 4 + 4 + 8
 
 
