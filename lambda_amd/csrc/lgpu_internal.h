@@ -11,7 +11,9 @@ constexpr float kPi = 3.14159265358979323846264338f;
 constexpr float kSurfaceEpsilon = 1e-5f;  // ScatterEvent.h:12
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kLdsStack = 32;             // per-thread BVH stack entries held in LDS
-constexpr int kMaxBvhDepth = 64;          // builder guarantees depth <= this
+constexpr int kMaxBvhDepth = 64;          // binary-BVH depth bound (builder guarantees)
+constexpr int kMaxStack = 96;             // 4-wide traversal stack entries per thread (LDS + spill)
+constexpr int kSpillWords = kMaxStack - kLdsStack;  // per-thread global spill
 
 // ---------------------------------------------------------------- BVH2
 // Two child boxes per node (64 B, one node = 4 x 16-B loads):
@@ -27,6 +29,21 @@ struct BvhNode {
 };
 constexpr uint32_t kLeafBit = 0x80000000u;
 inline uint32_t MakeLeaf(uint32_t first, uint32_t count) { return kLeafBit | (count << 26) | first; }
+
+// 4-wide node with 8-bit quantized child boxes (64 B = 4 x 16-B loads). Child k's
+// box on axis a is [p_a + qlo_a[k] * s_a, p_a + qhi_a[k] * s_a] with s_a a power
+// of two, so the decode q * s is exact; boxes are quantized outwards from
+// padded child bounds. q[2a] / q[2a+1] hold the lo / hi bytes of the four
+// children (byte k = child k). child[k]: inner node index, leaf ref (as
+// BvhNode), or kNone for an empty slot.
+//   load 0 = {p.x, p.y, p.z, s.x}, load 1 = {s.y, s.z, qlo.x, qhi.x},
+//   load 2 = {qlo.y, qhi.y, qlo.z, qhi.z}, load 3 = {child0..3}
+struct Bvh4Node {
+  float p[3];
+  float s[3];
+  uint32_t q[6];
+  uint32_t child[4];
+};
 
 // Triangles in BVH order: {v0.xyz, prim}, {e1.xyz, mesh}, {e2.xyz, 0} with
 // e1 = v1 - v0, e2 = v2 - v0 (48 B). Ng = e1 x e2 is bit-identical to
@@ -218,7 +235,8 @@ struct ShadowRay {
 };
 
 struct DevScene {
-  const BvhNode *nodes;
+  const BvhNode *nodes;       // binary BVH (traversal-count passes)
+  const Bvh4Node *nodes4;     // 4-wide quantized BVH (rendering)
   const BvhTri *tris;
   const float4 *tri_attr;    // kTriAttrF4 per BVH-order triangle (smooth / uv meshes)
   const uint4 *tri_vtx;      // {i0, i1, i2, mesh} global vertex indices

@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, ui
   uint32_t prim, bi;
   TraceStats ts{0, 0};
   const bool hit = Trace<false, kCount>(sc, o, d, 0.f, INFINITY, &t, &u, &v, &prim, &bi, stk,
-                                        spill + (size_t)i * (kMaxBvhDepth - kLdsStack), &ts);
+                                        spill + (size_t)i * kSpillWords, &ts);
   ps.hit[i] = hit ? make_float4(t, u, v, __uint_as_float(bi)) : make_float4(INFINITY, 0.f, 0.f, __uint_as_float(kNone));
   if (kCount) {
     atomicAdd(&counts[0], (unsigned long long)ts.nodes);
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene sc, SampleState ss, 
   float t, u, v;
   uint32_t prim, bi;
   const bool occ = Trace<true, false>(sc, ToV3(r.o), ToV3(r.d), 0.f, r.tmax, &t, &u, &v, &prim, &bi, stk,
-                                      spill + (size_t)i * (kMaxBvhDepth - kLdsStack), nullptr);
+                                      spill + (size_t)i * kSpillWords, nullptr);
   if (!occ) {  // one shadow ray per path per bounce: no other writer of this slot runs concurrently
     float4 L = ss.L[r.sid];
     L.x += r.contrib[0]; L.y += r.contrib[1]; L.z += r.contrib[2]; L.w += r.contrib[3];
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_query(DevScene sc, const float4 *ray
   const float4 a = rays[2 * i], b = rays[2 * i + 1];
   float t, u, v;
   uint32_t prim, bi;
-  uint32_t *sp = spill + (size_t)i * (kMaxBvhDepth - kLdsStack);
+  uint32_t *sp = spill + (size_t)i * kSpillWords;
   if (occ) {
     occ[i] = Trace<true, false>(sc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &t, &u, &v, &prim, &bi, stk, sp, nullptr) ? 1u : 0u;
   } else if (Trace<false, false>(sc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &t, &u, &v, &prim, &bi, stk, sp, nullptr)) {
@@ -421,7 +421,7 @@ struct Wave {
     LGPU_HIP(L.alloc(16ull * n)); LGPU_HIP(pix.alloc(4ull * n)); LGPU_HIP(lam.alloc(4ull * n));
     LGPU_HIP(shadow.alloc(sizeof(ShadowRay) * (size_t)n));
     LGPU_HIP(counters.alloc(64)); LGPU_HIP(tcount.alloc(64));
-    LGPU_HIP(spill.alloc(4ull * (kMaxBvhDepth - kLdsStack) * n));
+    LGPU_HIP(spill.alloc(4ull * kSpillWords * n));
     ss.L = L.as<float4>(); ss.pixel = pix.as<uint32_t>(); ss.lambda0 = lam.as<float>();
     cap = n;
     return LGPU_OK;
@@ -616,7 +616,7 @@ static lgpu_status Query(lgpu_scene s, uint32_t n, const float *rays, uint32_t *
   DevBuf dr, dh, dsp;
   LGPU_HIP(dr.upload(rays, 32ull * n));
   LGPU_HIP(dh.alloc(occ ? 4ull * n : 16ull * n));
-  LGPU_HIP(dsp.alloc(4ull * (kMaxBvhDepth - kLdsStack) * n));
+  LGPU_HIP(dsp.alloc(4ull * kSpillWords * n));
   hipLaunchKernelGGL(k_query, dim3(Blocks(n)), dim3(kBlock), 0, s->dev->stream, s->dsc, dr.as<float4>(), n,
                      occ ? nullptr : dh.as<uint4>(), occ ? dh.as<uint32_t>() : nullptr, dsp.as<uint32_t>());
   const hipError_t le = hipGetLastError();

@@ -4,11 +4,13 @@
 // (ManyLightSampler.cpp:37-303), PowerLightSampler::Commit (LightSampler.cpp:22-35),
 // EnvironmentLight ctor (EnvironmentLight.cpp:7-22).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
 #include <memory>
 
+#include "bvh_build.h"
 #include "host.h"
 #include "vm.h"
 
@@ -36,130 +38,6 @@ struct Pool {
     f[off + 1] = integral;
     for (uint32_t i = 0; i < n + 1; ++i) f.push_back(cdf[i]);
     return off;
-  }
-};
-
-// ---------------------------------------------------------------- BVH (binned SAH)
-struct Box {
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  void grow(const float *p) { for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); } }
-  void grow(const Box &b) { for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], b.lo[a]); hi[a] = std::max(hi[a], b.hi[a]); } }
-  float area() const {
-    if (lo[0] > hi[0]) return 0.f;
-    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-    return 2.f * (dx * dy + dy * dz + dz * dx);
-  }
-};
-
-struct BvhBuilder {
-  std::vector<Box> tb;
-  std::vector<float> cen;  // 3 per tri
-  std::vector<uint32_t> idx;
-  std::vector<BvhNode> nodes;
-  uint32_t maxDepth = 0;
-  float pad = 0;
-  static constexpr int kBins = 16;
-  static constexpr uint32_t kMaxLeaf = 31;
-
-  void storeChild(BvhNode &n, int slot, const Box &b) {
-    // Conservative padding so that rounding in the slab test never culls a
-    // triangle lying on a box face.
-    float lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = b.lo[a] - (std::fabs(b.lo[a]) * 4e-7f + pad);
-      hi[a] = b.hi[a] + (std::fabs(b.hi[a]) * 4e-7f + pad);
-    }
-    if (b.lo[0] > b.hi[0]) {  // empty child: a box no ray enters
-      for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
-    }
-    if (slot == 0) {
-      n.a[0] = lo[0]; n.a[1] = hi[0]; n.a[2] = lo[1]; n.a[3] = hi[1];
-      n.c[0] = lo[2]; n.c[1] = hi[2];
-    } else {
-      n.b[0] = lo[0]; n.b[1] = hi[0]; n.b[2] = lo[1]; n.b[3] = hi[1];
-      n.c[2] = lo[2]; n.c[3] = hi[2];
-    }
-  }
-
-  Box boundsOf(uint32_t first, uint32_t count) const {
-    Box b;
-    for (uint32_t i = first; i < first + count; ++i) b.grow(tb[idx[i]]);
-    return b;
-  }
-
-  // Returns the child reference for range [first, first+count).
-  uint32_t build(uint32_t first, uint32_t count, uint32_t depth) {
-    maxDepth = std::max(maxDepth, depth);
-    Box cb;
-    for (uint32_t i = first; i < first + count; ++i) cb.grow(&cen[3 * idx[i]]);
-    int bestAxis = -1, bestSplit = -1;
-    float bestCost = INFINITY;
-    const Box pb = boundsOf(first, count);
-    const float pa = pb.area();
-    if (count > 2 && depth < (uint32_t)kMaxBvhDepth - 2) {
-      for (int a = 0; a < 3; ++a) {
-        const float ext = cb.hi[a] - cb.lo[a];
-        if (!(ext > 0)) continue;
-        Box bb[kBins];
-        uint32_t bc[kBins] = {0};
-        const float k = (float)kBins / ext;
-        for (uint32_t i = first; i < first + count; ++i) {
-          int b = (int)((cen[3 * idx[i] + a] - cb.lo[a]) * k);
-          b = std::min(std::max(b, 0), kBins - 1);
-          bb[b].grow(tb[idx[i]]);
-          bc[b]++;
-        }
-        float rightArea[kBins];
-        uint32_t rightCount[kBins];
-        Box acc;
-        uint32_t n = 0;
-        for (int b = kBins - 1; b > 0; --b) {
-          acc.grow(bb[b]);
-          n += bc[b];
-          rightArea[b] = acc.area();
-          rightCount[b] = n;
-        }
-        Box la;
-        uint32_t ln = 0;
-        for (int b = 0; b < kBins - 1; ++b) {
-          la.grow(bb[b]);
-          ln += bc[b];
-          if (!ln || !rightCount[b + 1]) continue;
-          const float cost = 1.f + (la.area() * (float)ln + rightArea[b + 1] * (float)rightCount[b + 1]) / pa;
-          if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = b; }
-        }
-      }
-    }
-    const bool canLeaf = count <= kMaxLeaf;
-    const bool forceSplit = !canLeaf;
-    if (canLeaf && (count <= 2 || bestAxis < 0 || bestCost >= (float)count || depth >= (uint32_t)kMaxBvhDepth - 2)) {
-      return MakeLeaf(first, count);
-    }
-    uint32_t mid;
-    if (bestAxis >= 0) {
-      const float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
-      const float k = (float)kBins / ext;
-      auto it = std::partition(idx.begin() + first, idx.begin() + first + count, [&](uint32_t t) {
-        int b = (int)((cen[3 * t + bestAxis] - cb.lo[bestAxis]) * k);
-        b = std::min(std::max(b, 0), kBins - 1);
-        return b <= bestSplit;
-      });
-      mid = (uint32_t)(it - idx.begin());
-      if (mid == first || mid == first + count) mid = first + count / 2;
-    } else {
-      (void)forceSplit;
-      mid = first + count / 2;  // degenerate centroids: split by index
-    }
-    const uint32_t me = (uint32_t)nodes.size();
-    nodes.push_back(BvhNode());
-    const uint32_t l = build(first, mid - first, depth + 1);
-    const uint32_t r = build(mid, first + count - mid, depth + 1);
-    nodes[me].child[0] = l;
-    nodes[me].child[1] = r;
-    nodes[me].pad[0] = nodes[me].pad[1] = 0;
-    storeChild(nodes[me], 0, boundsOf(first, mid - first));
-    storeChild(nodes[me], 1, boundsOf(mid, first + count - mid));
-    return me;
   }
 };
 
@@ -738,6 +616,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   bb.pad = ext * 1e-7f;
   bb.nodes.reserve(nt ? 2 * nt / 2 + 1 : 1);
   bb.nodes.push_back(BvhNode());  // root
+  bb.cbox.emplace_back();
   if (nt == 0) {
     Box empty;
     bb.nodes[0].child[0] = MakeLeaf(0, 0);
@@ -749,17 +628,31 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
     if (r & kLeafBit) {  // whole scene is one leaf: root holds it next to an empty child
       bb.nodes[0].child[0] = r;
       bb.nodes[0].child[1] = MakeLeaf(0, 0);
+      bb.cbox[0][0] = bb.boundsOf(0, nt);
+      bb.cbox[0][1] = Box();
       bb.storeChild(bb.nodes[0], 0, bb.boundsOf(0, nt));
       bb.storeChild(bb.nodes[0], 1, Box());
     } else {
       // build() appended the real root at index 1; copy it into slot 0
       bb.nodes[0] = bb.nodes[r];
+      bb.cbox[0] = bb.cbox[r];
     }
   }
   s->num_bvh_nodes = (uint32_t)bb.nodes.size();
   s->bvh_depth = bb.maxDepth;
-  // The traversal stack holds at most one entry per level (LDS + spill = kMaxBvhDepth).
+  // The binary traversal stack holds at most one entry per level (LDS + spill = kMaxBvhDepth).
   if (bb.maxDepth + 1 >= (uint32_t)kMaxBvhDepth) return Fail(LGPU_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+  // 4-wide tree: node 0 is the root (a scene that is one leaf sits in slot 0 of it).
+  if (nt == 0) {
+    Bvh4Node root;
+    std::memset(&root, 0, sizeof(root));
+    for (int k = 0; k < 4; ++k) root.child[k] = kNone;
+    bb.nodes4.push_back(root);
+  } else {
+    bb.collapse(0, 0);
+  }
+  // each 4-wide level pushes at most three entries
+  if (3 * (bb.depth4 + 1) + 1 > (uint32_t)kMaxStack) return Fail(LGPU_ERR_UNSUPPORTED, "4-wide BVH deeper than the traversal stack");
   std::vector<BvhTri> btris(nt);
   bool anyAttr = false;
   for (const MeshRec &m : meshes) anyAttr |= (m.flags & (kMeshSmooth | kMeshUV)) != 0;
@@ -1086,6 +979,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   lgpu_status st;
 #define UP(vec, field) if ((st = Upload(s, vec, &d.field)) != LGPU_OK) return st
   UP(bb.nodes, nodes);
+  UP(bb.nodes4, nodes4);
   UP(btris, tris);
   UP(attr, tri_attr);
   UP(triVtx, tri_vtx);

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
   __shared__ uint32_t stk[kLdsStack][kBlock];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const TraceCtx tc{stk, spillBase + (size_t)i * (kMaxBvhDepth - kLdsStack)};
+  const TraceCtx tc{stk, spillBase + (size_t)i * kSpillWords};
   const bool corrected = rp.corrected != 0;
   const bool volumes = corrected && sc.num_media > 0;  // Scene::hasVolumes: never set as written
   const bool misVol = rp.integrator == LGPU_INTEGRATOR_MIS_VOLPATH;
