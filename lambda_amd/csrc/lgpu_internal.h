@@ -10,7 +10,7 @@ namespace lgpu {
 constexpr float kPi = 3.14159265358979323846264338f;
 constexpr float kSurfaceEpsilon = 1e-5f;  // ScatterEvent.h:12
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kLdsStack = 32;             // per-thread BVH stack entries held in LDS
+constexpr int kLdsStack = 16;             // per-thread BVH stack entries held in LDS
 constexpr int kMaxBvhDepth = 64;          // binary-BVH depth bound (builder guarantees)
 constexpr int kMaxStack = 96;             // 4-wide traversal stack entries per thread (LDS + spill)
 constexpr int kSpillWords = kMaxStack - kLdsStack;  // per-thread global spill
@@ -104,6 +104,18 @@ struct TreeNode {
   uint32_t first;       // leaf: first position in the tree light order
   uint32_t count;
   uint32_t parent;      // kNone at the root
+  uint32_t leaf_dist;   // leaf: offset of its Piecewise1D in the dist pool
+};
+
+// Light-tree node as the pick walk reads it (64 B): bounds centre and half diagonal,
+// cone axis and power, cos/sin of the cone's thetaO and cos of thetaE (the importance
+// measure without inverse trigonometry), children and leaf data.
+struct TreeImp {
+  float c[3], r;
+  float axis[3], power;
+  float cosO, sinO, cosE, pad;
+  uint32_t child[2];    // kNone for leaves
+  uint32_t first;       // leaf: first position in the tree light order
   uint32_t leaf_dist;   // leaf: offset of its Piecewise1D in the dist pool
 };
 
@@ -256,6 +268,7 @@ struct DevScene {
   const LightRec *tlights;   // tree lights
   const uint32_t *tree_order;// leaf position -> tree light index
   const TreeNode *tree;
+  const TreeImp *timp;       // same nodes, pick-walk layout
   const uint32_t *tri_tlight;// global triangle -> tree light (kNone)
   const uint32_t *tlight_leaf; // tree light -> (leaf node index)
   const uint32_t *tlight_slot; // tree light -> position inside its leaf

@@ -28,17 +28,15 @@ __device__ inline uint32_t SampleLight(const DevScene &sc, V3 point, V3 normalS,
   if (eps < pTree) {
     *pdf *= pTree;
     float e = rng.Get1D();
-    uint32_t n = 0;
-    TreeNode nd = sc.tree[0];
+    TreeImp nd = sc.timp[0];
     while (nd.child[0] != kNone) {  // PickLight (ManyLightSampler.cpp:355-378)
-      const TreeNode l = sc.tree[nd.child[0]], r = sc.tree[nd.child[1]];
-      const float IL = TreeImportance(l, point, normalS), IR = TreeImportance(r, point, normalS);
+      const TreeImp l = sc.timp[nd.child[0]], r = sc.timp[nd.child[1]];
+      const float IL = TreeImportanceFast(l, point, normalS), IR = TreeImportanceFast(r, point, normalS);
       const float PL = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
       const float PR = 1 - PL;
-      if (e < PL) { e /= PL; *pdf *= PL; n = nd.child[0]; nd = l; }
-      else { e = (e - PL) / PR; *pdf *= PR; n = nd.child[1]; nd = r; }
+      if (e < PL) { e /= PL; *pdf *= PL; nd = l; }
+      else { e = (e - PL) / PR; *pdf *= PR; nd = r; }
     }
-    (void)n;
     float lp = 1.f;
     const uint32_t i = PwSampleDiscrete(sc.dist, nd.leaf_dist, e, &lp);
     *pdf *= lp;
@@ -60,8 +58,8 @@ __device__ inline float LightPdf(const DevScene &sc, uint32_t pick, V3 point, V3
   uint32_t child = leaf;
   for (uint32_t n = sc.tree[leaf].parent; n != kNone; child = n, n = sc.tree[n].parent) {
     const TreeNode nd = sc.tree[n];
-    const float IL = TreeImportance(sc.tree[nd.child[0]], point, normalS);
-    const float IR = TreeImportance(sc.tree[nd.child[1]], point, normalS);
+    const float IL = TreeImportanceFast(sc.timp[nd.child[0]], point, normalS);
+    const float IR = TreeImportanceFast(sc.timp[nd.child[1]], point, normalS);
     const float p0 = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
     const float p1 = corrected ? 1.f - p0 : 1.f - IR;  // faithful: RecursivePDF :387
     p *= (child == nd.child[0]) ? p0 : p1;

@@ -999,6 +999,19 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   UP(tl, tlights);
   UP(treeOrder, tree_order);
   UP(tree, tree);
+  std::vector<TreeImp> timp(tree.size());
+  for (size_t k = 0; k < tree.size(); ++k) {
+    const TreeNode &t = tree[k];
+    TreeImp &m = timp[k];
+    std::memset(&m, 0, sizeof(m));
+    for (int a = 0; a < 3; ++a) { m.c[a] = t.center[a]; m.axis[a] = t.axis[a]; }
+    m.r = t.diag * .5f;
+    m.power = t.power;
+    m.cosO = std::cos(t.thetaO); m.sinO = std::sin(t.thetaO); m.cosE = std::cos(t.thetaE);
+    m.child[0] = t.child[0]; m.child[1] = t.child[1];
+    m.first = t.first; m.leaf_dist = t.leaf_dist;
+  }
+  UP(timp, timp);
   UP(triTL, tri_tlight);
   UP(tlLeaf, tlight_leaf);
   UP(tlSlot, tlight_slot);

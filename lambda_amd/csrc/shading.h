@@ -445,6 +445,34 @@ struct Shader {
 };
 
 // ---------------------------------------------------------------- light tree importance (ManyLightSampler.cpp:305-326)
+// ManyLightSampler::ImportanceMeasure (ManyLightSampler.cpp:305-326) without inverse
+// trigonometry: with c = cos(theta) = delta.axis, tan(thetaU) = r/d and the node's
+// cos/sin(thetaO), cos(max(theta - thetaO - thetaU, 0)) = cos(theta - alpha) by the
+// angle-difference identity (1 when theta <= alpha), the cone test thetaDash < thetaE
+// is cos(thetaDash) > cos(thetaE), and cos(max(thetaI - thetaU, 0)) likewise.
+// A |cos| above 1 (acos would give NaN) yields 0 as the reference's NaN comparison does.
+__device__ inline float TreeImportanceFast(const TreeImp &nd, V3 point, V3 normalS) {
+  V3 delta = v3(nd.c[0], nd.c[1], nd.c[2]) - point;
+  const float d2 = Dot(delta, delta);
+  const float d = sqrtf(d2);
+  const float invD = 1.f / d;
+  delta = delta * invD;
+  const float cT = Dot(delta, v3(nd.axis[0], nd.axis[1], nd.axis[2]));
+  if (!(fabsf(cT) <= 1.f)) return 0.f;
+  const float tU = nd.r * invD;
+  const float cU = 1.f / sqrtf(1.f + tU * tU), sU = tU * cU;
+  const float cA = nd.cosO * cU - nd.sinO * sU, sA = nd.sinO * cU + nd.cosO * sU;  // alpha = thetaO + thetaU
+  float cD = 1.f;
+  if (!(sA < 0.f) && cT < cA) cD = cT * cA + sqrtf(fmaxf(0.f, 1.f - cT * cT)) * sA;
+  if (!(cD > nd.cosE)) return 0.f;
+  const float E = nd.power;
+  if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * cD;
+  const float cI = Clampf(Dot(delta, normalS), -1.f, 1.f);
+  float cIU = 1.f;
+  if (cI < cU) cIU = cI * cU + sqrtf(fmaxf(0.f, 1.f - cI * cI)) * sU;
+  return (cIU * E) / d2 * cD;
+}
+
 __device__ inline float TreeImportance(const TreeNode &nd, V3 point, V3 normalS) {
   V3 delta = ToV3(nd.center) - point;
   const float diam = nd.diag;

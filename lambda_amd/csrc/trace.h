@@ -54,6 +54,7 @@ __device__ inline bool Trace4(const DevScene &sc, V3 o, V3 d, float tmin, float 
   const int lane = threadIdx.x;
   const float ix = 1.f / SafeDir(d.x), iy = 1.f / SafeDir(d.y), iz = 1.f / SafeDir(d.z);
   const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+  const bool nx = ix < 0.f, ny = iy < 0.f, nz = iz < 0.f;  // near plane = hi bound on negative axes
   float best = tmax;
   uint32_t bestPrim = kNone, bestBi = kNone;
   float bu = 0.f, bv = 0.f;
@@ -69,15 +70,17 @@ __device__ inline bool Trace4(const DevScene &sc, V3 o, V3 d, float tmin, float 
       const float Ay = b.x * iy, By = a.y * iy - oy;
       const float Az = b.y * iz, Bz = a.z * iz - oz;
       const uint32_t qxl = __float_as_uint(b.z), qxh = __float_as_uint(b.w);
+      const uint32_t qxn = nx ? qxh : qxl, qxf = nx ? qxl : qxh;
+      const uint32_t qyn = ny ? qc.y : qc.x, qyf = ny ? qc.x : qc.y;
+      const uint32_t qzn = nz ? qc.w : qc.z, qzf = nz ? qc.z : qc.w;
       float tn[4];
       uint32_t cr[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float x0 = fmaf(ByteF(qxl, k), Ax, Bx), x1 = fmaf(ByteF(qxh, k), Ax, Bx);
-        const float y0 = fmaf(ByteF(qc.x, k), Ay, By), y1 = fmaf(ByteF(qc.y, k), Ay, By);
-        const float z0 = fmaf(ByteF(qc.z, k), Az, Bz), z1 = fmaf(ByteF(qc.w, k), Az, Bz);
-        const float lo = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-        const float hi = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best));
+        const float lo = fmaxf(fmaxf(fmaf(ByteF(qxn, k), Ax, Bx), fmaf(ByteF(qyn, k), Ay, By)),
+                               fmaxf(fmaf(ByteF(qzn, k), Az, Bz), tmin));
+        const float hi = fminf(fminf(fmaf(ByteF(qxf, k), Ax, Bx), fmaf(ByteF(qyf, k), Ay, By)),
+                               fminf(fmaf(ByteF(qzf, k), Az, Bz), best));
         const uint32_t r = k == 0 ? ch.x : (k == 1 ? ch.y : (k == 2 ? ch.z : ch.w));
         const bool h = lo <= hi && r != kNone;
         tn[k] = h ? lo : INFINITY;
