@@ -119,6 +119,8 @@ struct TreeImp {
   uint32_t leaf_dist;   // leaf: offset of its Piecewise1D in the dist pool
 };
 
+constexpr int kTreeLds = 255;  // light-tree records cached in LDS per shading block (8 levels)
+
 struct EnvRec {
   uint32_t w, h;
   uint32_t tex_off;     // float offset of RGB data in the texel pool
@@ -268,7 +270,9 @@ struct DevScene {
   const LightRec *tlights;   // tree lights
   const uint32_t *tree_order;// leaf position -> tree light index
   const TreeNode *tree;
-  const TreeImp *timp;       // same nodes, pick-walk layout
+  const TreeImp *timp;       // same nodes, pick-walk layout, breadth-first order
+  const uint32_t *tree_bfs;  // tree (depth-first) index -> timp (breadth-first) index
+  uint32_t tree_lds;         // leading timp records a shading block caches in LDS
   const uint32_t *tri_tlight;// global triangle -> tree light (kNone)
   const uint32_t *tlight_leaf; // tree light -> (leaf node index)
   const uint32_t *tlight_slot; // tree light -> position inside its leaf
@@ -320,5 +324,8 @@ struct RenderParams {
 // Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator.
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                     SampleState ss, uint32_t n, Queues q, uint32_t *spill);
+// Direct-lighting shade stage (direct.hip): DirectLightingIntegrator.
+void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
+                       SampleState ss, uint32_t n, Queues q, uint32_t *spill);
 
 }  // namespace lgpu

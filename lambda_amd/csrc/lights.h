@@ -17,8 +17,23 @@ __device__ inline const LightRec &LightOf(const DevScene &sc, uint32_t pick) {
   return (pick & kTreeBit) ? sc.tlights[pick & ~kTreeBit] : sc.lights[pick];
 }
 
-// LightSampler::Sample: PowerLightSampler (LightSampler.cpp:14-16) or ManyLightSampler (ManyLightSampler.cpp:17-27)
-__device__ inline uint32_t SampleLight(const DevScene &sc, V3 point, V3 normalS, Rng &rng, float *pdf) {
+// Copies the leading light-tree records into the block's LDS cache; every thread of the block
+// must call it before any of them returns.
+__device__ inline void LoadTreeLds(const DevScene &sc, TreeImp *lds) {
+  const float4 *src = reinterpret_cast<const float4 *>(sc.timp);
+  float4 *dst = reinterpret_cast<float4 *>(lds);
+  const uint32_t n4 = sc.tree_lds * 4u;
+  for (uint32_t k = threadIdx.x; k < n4; k += blockDim.x) dst[k] = src[k];
+  __syncthreads();
+}
+__device__ inline TreeImp TreeRec(const DevScene &sc, const TreeImp *lds, uint32_t i) {
+  return i < sc.tree_lds ? lds[i] : sc.timp[i];
+}
+
+// LightSampler::Sample: PowerLightSampler (LightSampler.cpp:14-16) or ManyLightSampler (ManyLightSampler.cpp:17-27).
+// lds: the block's cached top of the light tree (LoadTreeLds).
+__device__ inline uint32_t SampleLight(const DevScene &sc, V3 point, V3 normalS, Rng &rng, float *pdf, const TreeImp *lds,
+                                       bool corrected) {
   if (sc.strategy == LGPU_LIGHTS_POWER) {
     if (!sc.num_lights) { *pdf = 0.f; return kNone; }
     return PwSampleDiscrete(sc.dist, sc.power_dist, rng.Get1D(), pdf);
@@ -28,10 +43,10 @@ __device__ inline uint32_t SampleLight(const DevScene &sc, V3 point, V3 normalS,
   if (eps < pTree) {
     *pdf *= pTree;
     float e = rng.Get1D();
-    TreeImp nd = sc.timp[0];
+    TreeImp nd = TreeRec(sc, lds, 0);
     while (nd.child[0] != kNone) {  // PickLight (ManyLightSampler.cpp:355-378)
-      const TreeImp l = sc.timp[nd.child[0]], r = sc.timp[nd.child[1]];
-      const float IL = TreeImportanceFast(l, point, normalS), IR = TreeImportanceFast(r, point, normalS);
+      const TreeImp l = TreeRec(sc, lds, nd.child[0]), r = TreeRec(sc, lds, nd.child[1]);
+      const float IL = TreeImportanceFast(l, point, normalS, corrected), IR = TreeImportanceFast(r, point, normalS, corrected);
       const float PL = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
       const float PR = 1 - PL;
       if (e < PL) { e /= PL; *pdf *= PL; nd = l; }
@@ -58,8 +73,8 @@ __device__ inline float LightPdf(const DevScene &sc, uint32_t pick, V3 point, V3
   uint32_t child = leaf;
   for (uint32_t n = sc.tree[leaf].parent; n != kNone; child = n, n = sc.tree[n].parent) {
     const TreeNode nd = sc.tree[n];
-    const float IL = TreeImportanceFast(sc.timp[nd.child[0]], point, normalS);
-    const float IR = TreeImportanceFast(sc.timp[nd.child[1]], point, normalS);
+    const float IL = TreeImportanceFast(sc.timp[sc.tree_bfs[nd.child[0]]], point, normalS, corrected);
+    const float IR = TreeImportanceFast(sc.timp[sc.tree_bfs[nd.child[1]]], point, normalS, corrected);
     const float p0 = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
     const float p1 = corrected ? 1.f - p0 : 1.f - IR;  // faithful: RecursivePDF :387
     p *= (child == nd.child[0]) ? p0 : p1;

@@ -150,6 +150,8 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
 
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, Queues q) {
+  __shared__ TreeImp treeLds[kTreeLds];
+  LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const bool corrected = rp.corrected != 0;
@@ -283,7 +285,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, RenderParams rp, 
   shd.vm = vm;
   const V3 shadeP = h.point, shadeN = h.normalS;
   float lightDistPdf = 1.f;
-  const uint32_t pick = SampleLight(sc, h.point, h.normalS, rng, &lightDistPdf);
+  const uint32_t pick = SampleLight(sc, h.point, h.normalS, rng, &lightDistPdf, treeLds, corrected);
   float lightPDF = 0.f;
   Spec Li = S(0.f);
   ShadowSpec sh;
@@ -433,9 +435,7 @@ static inline uint32_t Blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) /
 static lgpu_status ValidateRender(const lgpu_render_desc *d) {
   if (!d || !d->scene) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: null descriptor or scene");
   if (!d->scene->committed) return Fail(LGPU_ERR_STATE, "render: scene not committed");
-  if (d->integrator != LGPU_INTEGRATOR_PATH && d->integrator != LGPU_INTEGRATOR_VOLPATH &&
-      d->integrator != LGPU_INTEGRATOR_MIS_VOLPATH)
-    return Fail(LGPU_ERR_UNSUPPORTED, "render: integrator not built (path, volpath, mis_volpath are)");
+  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH) return Fail(LGPU_ERR_UNSUPPORTED, "render: unknown integrator");
   if (d->spectral_mode > LGPU_SPECTRAL_HERO4 || d->estimator_mode > LGPU_ESTIMATOR_CORRECTED)
     return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: bad spectral or estimator mode");
   if (d->max_bounces == 0) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: max_bounces must be > 0");
@@ -520,6 +520,8 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
       if (rp.integrator == LGPU_INTEGRATOR_PATH)
         hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, rp, W.ps[cur], W.ss, count, q);
+      else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
+        LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       else
         LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       LGPU_HIP(hipGetLastError());

@@ -999,19 +999,33 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   UP(tl, tlights);
   UP(treeOrder, tree_order);
   UP(tree, tree);
-  std::vector<TreeImp> timp(tree.size());
-  for (size_t k = 0; k < tree.size(); ++k) {
-    const TreeNode &t = tree[k];
-    TreeImp &m = timp[k];
+  // pick-walk records in breadth-first order: the top levels form a prefix that
+  // shading blocks cache in LDS
+  std::vector<uint32_t> bfsOf(tree.size(), kNone), bfsOrder;
+  if (!tree.empty()) {
+    bfsOrder.push_back(0);
+    for (size_t q = 0; q < bfsOrder.size(); ++q) {
+      const TreeNode &t = tree[bfsOrder[q]];
+      for (int c = 0; c < 2; ++c)
+        if (t.child[c] != kNone) bfsOrder.push_back(t.child[c]);
+    }
+    for (size_t q = 0; q < bfsOrder.size(); ++q) bfsOf[bfsOrder[q]] = (uint32_t)q;
+  }
+  std::vector<TreeImp> timp(bfsOrder.size());
+  for (size_t q = 0; q < bfsOrder.size(); ++q) {
+    const TreeNode &t = tree[bfsOrder[q]];
+    TreeImp &m = timp[q];
     std::memset(&m, 0, sizeof(m));
     for (int a = 0; a < 3; ++a) { m.c[a] = t.center[a]; m.axis[a] = t.axis[a]; }
     m.r = t.diag * .5f;
     m.power = t.power;
     m.cosO = std::cos(t.thetaO); m.sinO = std::sin(t.thetaO); m.cosE = std::cos(t.thetaE);
-    m.child[0] = t.child[0]; m.child[1] = t.child[1];
+    for (int c = 0; c < 2; ++c) m.child[c] = t.child[c] == kNone ? kNone : bfsOf[t.child[c]];
     m.first = t.first; m.leaf_dist = t.leaf_dist;
   }
+  d.tree_lds = (uint32_t)std::min<size_t>(timp.size(), kTreeLds);
   UP(timp, timp);
+  UP(bfsOf, tree_bfs);
   UP(triTL, tri_tlight);
   UP(tlLeaf, tlight_leaf);
   UP(tlSlot, tlight_slot);

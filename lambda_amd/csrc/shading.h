@@ -451,13 +451,15 @@ struct Shader {
 // angle-difference identity (1 when theta <= alpha), the cone test thetaDash < thetaE
 // is cos(thetaDash) > cos(thetaE), and cos(max(thetaI - thetaU, 0)) likewise.
 // A |cos| above 1 (acos would give NaN) yields 0 as the reference's NaN comparison does.
-__device__ inline float TreeImportanceFast(const TreeImp &nd, V3 point, V3 normalS) {
+// corrected: cone angle from the emitter towards the point and |cos thetaI| (see the CPU checker).
+__device__ inline float TreeImportanceFast(const TreeImp &nd, V3 point, V3 normalS, bool corrected) {
   V3 delta = v3(nd.c[0], nd.c[1], nd.c[2]) - point;
   const float d2 = Dot(delta, delta);
   const float d = sqrtf(d2);
   const float invD = 1.f / d;
   delta = delta * invD;
-  const float cT = Dot(delta, v3(nd.axis[0], nd.axis[1], nd.axis[2]));
+  const float cA0 = Dot(delta, v3(nd.axis[0], nd.axis[1], nd.axis[2]));
+  const float cT = corrected ? -cA0 : cA0;
   if (!(fabsf(cT) <= 1.f)) return 0.f;
   const float tU = nd.r * invD;
   const float cU = 1.f / sqrtf(1.f + tU * tU), sU = tU * cU;
@@ -467,30 +469,11 @@ __device__ inline float TreeImportanceFast(const TreeImp &nd, V3 point, V3 norma
   if (!(cD > nd.cosE)) return 0.f;
   const float E = nd.power;
   if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * cD;
-  const float cI = Clampf(Dot(delta, normalS), -1.f, 1.f);
+  const float cI = Clampf(corrected ? fabsf(Dot(delta, normalS)) : Dot(delta, normalS), -1.f, 1.f);
   float cIU = 1.f;
   if (cI < cU) cIU = cI * cU + sqrtf(fmaxf(0.f, 1.f - cI * cI)) * sU;
   return (cIU * E) / d2 * cD;
 }
 
-__device__ inline float TreeImportance(const TreeNode &nd, V3 point, V3 normalS) {
-  V3 delta = ToV3(nd.center) - point;
-  const float diam = nd.diag;
-  const float d2 = Dot(delta, delta);
-  const float d = sqrtf(d2);
-  const float invD = 1.f / d;
-  delta = delta * invD;
-  const float theta = acosf(Dot(delta, ToV3(nd.axis)));
-  const float thetaU = atanf((diam * .5f) * invD);
-  const float thetaDash = fmaxf(theta - nd.thetaO - thetaU, 0.f);
-  if (thetaDash < nd.thetaE) {
-    const float E = nd.power;
-    // a zero normal marks a medium vertex (corrected mode): no surface-cosine bound
-    if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * cosf(thetaDash);
-    const float thetaI = acosf(Dot(delta, normalS));
-    return (cosf(fmaxf(thetaI - thetaU, 0.f)) * E) / d2 * cosf(thetaDash);
-  }
-  return 0.f;
-}
 
 }  // namespace lgpu

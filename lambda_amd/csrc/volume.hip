@@ -88,6 +88,8 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
 __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                       uint32_t n, Queues q, uint32_t *spillBase) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
+  __shared__ TreeImp treeLds[kTreeLds];
+  LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const TraceCtx tc{stk, spillBase + (size_t)i * kSpillWords};
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
           h.point = sp;
           const V3 refP = sp, refN = corrected ? v3(0, 0, 0) : h.normalS;
           float lightDistPdf = 1.f;
-          const uint32_t pick = SampleLight(sc, refP, refN, rng, &lightDistPdf);
+          const uint32_t pick = SampleLight(sc, refP, refN, rng, &lightDistPdf, treeLds, corrected);
           Spec Li = S(0.f);
           float lightPDF = 0.f;
           bool delta = false;
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         const V3 refP = corrected ? ro : h.point, refN = corrected ? v3(0, 0, 0) : h.normalS;
         PartialLS ls;
         ls.pdf = 1.f;
-        ls.light = SampleLight(sc, refP, refN, rng, &ls.pdf);
+        ls.light = SampleLight(sc, refP, refN, rng, &ls.pdf, treeLds, corrected);
         ls.selPdf = ls.pdf;
         ls.point = v3(0, 0, 0);
         ls.normal = v3(0, 0, 0);
@@ -334,7 +336,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         const V3 shadeP = h.point, shadeN = h.normalS;
         const VisCtx vc{tc, volumes, false, med, &mat, h.normalG};
         float lightDistPdf = 1.f;
-        const uint32_t pick = SampleLight(sc, h.point, h.normalS, rng, &lightDistPdf);
+        const uint32_t pick = SampleLight(sc, h.point, h.normalS, rng, &lightDistPdf, treeLds, corrected);
         float lightPDF = 0.f;
         Spec Li = S(0.f);
         bool delta = false;

@@ -1815,21 +1815,25 @@ static float LightPower(const Scene &sc, const Light &L) {  // Light::Power = Ir
 }
 
 // ---------------------------------------------------------------- light samplers
-static float TreeImportance(const LNode *nd, const V3 &point, const V3 &normalS) {
+// ManyLightSampler::ImportanceMeasure (ManyLightSampler.cpp:305-326). corrected: the cone angle
+// is taken from the emitter towards the point (the reference measures it against the direction
+// towards the light, giving emitters that face the point zero importance, A47) and the receiver
+// cosine is |cos thetaI| (it goes negative below the horizon as written, A48).
+static float TreeImportance(const LNode *nd, const V3 &point, const V3 &normalS, bool corrected) {
   V3 delta = nd->bounds.Center() - point;
   const float diam = nd->bounds.Diagonal();
   const float d2 = Dot(delta, delta);
   const float d = std::sqrt(d2);
   const float invD = 1.f / d;
   delta = delta * invD;
-  const float theta = std::acos(Dot(delta, nd->cone.axis));
+  const float theta = std::acos(corrected ? -Dot(delta, nd->cone.axis) : Dot(delta, nd->cone.axis));
   const float thetaU = std::atan((diam * .5f) * invD);
   const float thetaDash = std::max(theta - nd->cone.thetaO - thetaU, 0.f);
   if (thetaDash < nd->cone.thetaE) {
     const float E = nd->power;
     // a zero normal marks a medium vertex (corrected mode): no surface-cosine bound
     if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * std::cos(thetaDash);
-    const float thetaI = std::acos(Dot(delta, normalS));
+    const float thetaI = std::acos(corrected ? std::abs(Dot(delta, normalS)) : Dot(delta, normalS));
     return (std::cos(std::max(thetaI - thetaU, 0.f)) * E) / d2 * std::cos(thetaDash);
   }
   return 0.f;
@@ -1852,8 +1856,8 @@ static LightPick SampleLightAt(const Ctx &c, const V3 &refP, const V3 &refN, Sam
     float e = smp.Get1D();  // A5: the rescaled epsilon is discarded
     LNode *n = sc.root;
     while (!n->IsLeaf()) {  // PickLight :355-378
-      const float IL = TreeImportance(n->child[0], refP, refN);
-      const float IR = TreeImportance(n->child[1], refP, refN);
+      const float IL = TreeImportance(n->child[0], refP, refN, c.corrected);
+      const float IR = TreeImportance(n->child[1], refP, refN, c.corrected);
       const float PL = (IL + IR == 0.f) ? .5f : IL / (IL + IR);  // A10
       const float PR = 1 - PL;
       if (e < PL) { e /= PL; *pdf *= PL; n = n->child[0]; }
@@ -1884,8 +1888,8 @@ static float LightPdf(const Ctx &c, const LightPick &lp, const V3 &point, const 
   float p = c.corrected ? ld.PDFNormalised(ref.second) : ld.PDF(ref.second);  // A28
   const LNode *child = ref.first;
   for (const LNode *n = ref.first->parent; n; child = n, n = n->parent) {
-    const float IL = TreeImportance(n->child[0], point, normalS);
-    const float IR = TreeImportance(n->child[1], point, normalS);
+    const float IL = TreeImportance(n->child[0], point, normalS, c.corrected);
+    const float IR = TreeImportance(n->child[1], point, normalS, c.corrected);
     const float p0 = (IL + IR == 0.f) ? .5f : IL / (IL + IR);
     const float p1 = c.corrected ? 1.f - p0 : 1.f - IR;  // A7
     p *= (child == n->child[0]) ? p0 : p1;
@@ -2410,6 +2414,136 @@ static Spec VolLi(const Ctx &c, const RenderParams &rp, V3 ro, V3 rd, Sampler &s
   return L;
 }
 
+// ---------------------------------------------------------------- direct lighting
+// Light::PDF_Li(event, sampler) for the light chosen by SampleOneLight (MeshLight.cpp:44-55,
+// :153-155, EnvironmentLight.cpp:44-58, PointLight/Spotlight: 0). Faithful fires the
+// reference's own rays; corrected uses the MIS ray's hit (see DirectLi).
+static float PdfLiRay(const Ctx &c, const LightPick &lp, const Event &ev) {
+  const Scene &sc = *c.sc;
+  const Light &L = LightOf(sc, lp);
+  const Hit &h = *ev.hit;
+  if (L.kind == L_POINT || L.kind == L_SPOT) return 0.f;
+  if (L.kind == L_ENV) {
+    const float theta = SphericalTheta(ev.wi) - L.offset.y;
+    const float phi = SphericalPhi(ev.wi) - L.offset.x;
+    const float sinT = std::sin(theta);
+    if (sinT == 0) return 0.f;
+    const V3 wiOffset = SphericalDirection(sinT, std::cos(theta), phi);
+    if (!RayEscapes(sc, h.point, wiOffset)) return 0.f;
+    return L.envDist.PDF(V2(Fract(phi * INV_PI2), Fract(theta * INV_PI))) / (2.f * PI * PI * sinT);
+  }
+  // MeshLight::PDF_Li(event, sampler); a TriangleLight forwards to its mesh light
+  const Light &ML = L.kind == L_TRI ? sc.lights[L.parentMeshLight] : L;
+  Hit lh;
+  const V3 o = h.point + h.normalG * SURFACE_EPSILON * (float)ev.sidedness;
+  if (!Intersect(sc, o, ev.wi, lh, c.corrected)) return 0.f;
+  if (sc.meshes[lh.mesh].meshLight != (int)(&ML - &sc.lights[0])) return 0.f;
+  V3 v0, v1, v2;
+  TriVerts(sc, lh.prim, v0, v1, v2);
+  float area;
+  V3 normal;
+  TriAreaNormal(v0, v1, v2, &area, &normal);
+  const float cosT = std::abs(Dot(normal, -ev.wi));
+  if (cosT > 0) return DistSq(h.point, lh.point) / (cosT * area);
+  return 0.f;
+}
+
+// Integrator::SampleOneLight / EstimateDirect (Integrator.cpp:5-55) and
+// DirectLightingIntegrator::Li (DirectLightingIntegrator.cpp:13-29). Corrected adds the
+// emission of the first surface (front face), weighs delta lights and specular samples 1,
+// offsets the MIS ray to the side of wi, and takes the MIS pdf and emission from that ray's
+// own hit on the sampled light (triangle-selection pdf included).
+static Spec DirectLi(const Ctx &c, V3 ro, V3 rd, Sampler &smp, const Lanes &lanes) {
+  const Scene &sc = *c.sc;
+  Hit hit;
+  Event ev;
+  ev.hit = &hit;
+  ev.lanes = lanes;
+  ev.sampler = &smp;
+  Spec L(0);
+  for (int pass = 0; pass <= 1024; ++pass) {
+    if (!Intersect(sc, ro, rd, hit, c.corrected)) {
+      if (sc.envLight >= 0) L += EnvLe(sc.lights[sc.envLight], rd, lanes);  // A3: no env light gives 0
+      return L;
+    }
+    const Material &mat = sc.materials[sc.meshes[hit.mesh].material];
+    if (c.corrected && mat.emission != LGPU_NONE && Dot(rd, hit.normalG) < 0)
+      L += Emission(c, mat, ev) * sc.lights[sc.meshes[hit.mesh].meshLight].intensity * INV_PI;
+    if (mat.bxdf == LGPU_NONE) {  // :24-26
+      ro = hit.point + rd * .0001f;
+      continue;
+    }
+    ev.wo = -rd;
+    ev.SurfaceLocalise();
+    float lightPdf = 1;
+    const LightPick l = SampleLight(c, ev, smp, &lightPdf);
+    if (l.kind < 0) return L;
+    const Light &LL = LightOf(sc, l);
+    const bool delta = LL.kind == L_POINT || LL.kind == L_SPOT;
+    Spec Ld(0);
+    float lightPDF = 0, scatteringPDF = 0;
+    Spec Li = SampleLi(c, LL, ev, smp, lightPDF);
+    if (lightPDF > 0 && !IsBlack(Li, lanes)) {
+      const Spec f = BxDF_f(c, mat, mat.bxdf, ev) * std::abs(ev.wiL.y);
+      scatteringPDF = BxDF_Pdf(c, mat, mat.bxdf, ev.woL, ev.wiL, ev);
+      if (!IsBlack(f, lanes) && scatteringPDF > 0) {
+        const float w = (c.corrected && delta) ? 1.f : PowerHeuristic(lightPDF, scatteringPDF);
+        Ld += Li * f * w / lightPDF;
+      }
+    }
+    bool specular;
+    Spec f = BxDF_Sample(c, mat, mat.bxdf, ev, smp, scatteringPDF, &specular);
+    f = f * std::abs(ev.wiL.y);
+    if (scatteringPDF > 0 && !IsBlack(f, lanes)) {
+      Spec Lb(0);
+      float weight = 1;
+      if (!c.corrected) {
+        const float lp = PdfLiRay(c, l, ev);
+        if (lp == 0) return L + Ld / lightPdf;
+        weight = PowerHeuristic(scatteringPDF, lp);
+        Hit lh;
+        const V3 o = hit.point + hit.normalG * .00001f;
+        if (Intersect(sc, o, ev.wi, lh, c.corrected)) {
+          // material->light == &_light: a mesh's light against the sampled light (a TriangleLight
+          // under the tree never equals it)
+          const int ml = sc.meshes[lh.mesh].meshLight;
+          if (ml >= 0 && l.kind == 0 && ml == l.index) {
+            Hit *saved = ev.hit;
+            ev.hit = &lh;
+            Lb = Emission(c, sc.materials[sc.meshes[lh.mesh].material], ev) * sc.lights[ml].intensity * INV_PI;
+            ev.hit = saved;
+          }
+        } else if (LL.kind == L_ENV) {
+          Lb = EnvLe(LL, ev.wi, lanes);
+        }
+      } else if (!delta) {
+        Hit lh;
+        const V3 o = hit.point + hit.normalG * (Dot(ev.wi, hit.normalG) < 0 ? -SURFACE_EPSILON : SURFACE_EPSILON);
+        float lp = 0;
+        if (Intersect(sc, o, ev.wi, lh, true)) {
+          const int ml = sc.meshes[lh.mesh].meshLight;
+          const bool same = ml >= 0 && (l.kind == 1 ? sc.triToTLight[lh.prim] == l.index : ml == l.index);
+          if (same && Dot(ev.wi, lh.normalG) < 0) {
+            lp = PdfLiHit(c, LL, ev, lh);
+            Hit *saved = ev.hit;
+            ev.hit = &lh;
+            Lb = Emission(c, sc.materials[sc.meshes[lh.mesh].material], ev) * sc.lights[ml].intensity * INV_PI;
+            ev.hit = saved;
+          }
+        } else if (LL.kind == L_ENV) {
+          lp = EnvPdfLi(LL, ev.wi);
+          Lb = EnvLe(LL, ev.wi, lanes);
+        }
+        weight = specular ? 1.f : PowerHeuristic(scatteringPDF, lp);
+        if (lp == 0 && !specular) Lb = Spec(0);
+      }
+      if (!IsBlack(Lb, lanes)) Ld += f * Lb * weight / scatteringPDF;
+    }
+    return L + Ld / lightPdf;
+  }
+  return L;
+}
+
 // Camera::GenerateRay (Camera.cpp:41-71) with the aperture draws of Aperture.cpp:10-27
 static void GenerateRay(const lgpu_camera &cam, float u, float v, Sampler &smp, V3 *o, V3 *d, bool corrected) {
   const V3 origin(cam.origin), xh(cam.x_hat), yh(cam.y_hat), zh(cam.z_hat);
@@ -2466,6 +2600,7 @@ static Spec RenderSample(const Ctx &c, const RenderParams &rp, const lgpu_camera
   V3 o, d;
   GenerateRay(cam, u, v, smp, &o, &d, c.corrected);
   if (lambda0) *lambda0 = l0;
+  if (rp.integrator == LGPU_INTEGRATOR_DIRECT) return DirectLi(c, o, d, smp, lanes);
   if (rp.integrator == LGPU_INTEGRATOR_VOLPATH) return VolLi(c, rp, o, d, smp, lanes, false);
   if (rp.integrator == LGPU_INTEGRATOR_MIS_VOLPATH) return VolLi(c, rp, o, d, smp, lanes, true);
   return PathLi(c, rp, o, d, smp, lanes);
@@ -2730,7 +2865,7 @@ static RenderParams Params(const lgpu_render_desc *d) {
 int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width, uint32_t height, uint32_t n,
                        const uint32_t *pixels, const uint32_t *samples, float *out, int threads) {
   if (!s || !s->committed || !d || !width || !height) return -1;
-  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH || d->integrator == LGPU_INTEGRATOR_DIRECT) return -3;
+  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH) return -3;
   const RenderParams rp = Params(d);
   Ctx c{s, rp.corrected, (int)rp.mode};
   c.volumes = rp.corrected && !s->media.empty();
@@ -2747,7 +2882,7 @@ int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width
 int oracle_render(oracle_scene s, const lgpu_render_desc *d, uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                   uint32_t w, uint32_t h, int threads, double *rgb, uint32_t *counts) {
   if (!s || !s->committed || !d || x0 + w > W || y0 + h > H) return -1;
-  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH || d->integrator == LGPU_INTEGRATOR_DIRECT) return -3;
+  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH) return -3;
   const RenderParams rp = Params(d);
   Ctx c{s, rp.corrected, (int)rp.mode};
   c.volumes = rp.corrected && !s->media.empty();

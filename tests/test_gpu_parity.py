@@ -248,3 +248,58 @@ def test_invalid_render_arguments(dev):
     d = gpu.render_desc(s, film, cam, 0, 1)
     with pytest.raises(gpu.LambdaError):
         gpu.trace_paths(d, np.array([64], dtype=np.uint32), np.array([0], dtype=np.uint32))
+
+
+@pytest.mark.parametrize("spectral,estimator,strategy", MODES)
+def test_direct_lighting_per_path_parity(dev, spectral, estimator, strategy):
+    """DirectLightingIntegrator (SampleOneLight / EstimateDirect) on the Cornell box."""
+    W = H = 64
+    sd = scene.cornell_box()
+    s, k1 = gpu_scene(dev, sd, strategy)
+    o, k2 = helpers.oracle_scene(sd, strategy)
+    cam = scene.cornell_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    d = gpu.render_desc(s, film, cam, 0, 2, spectral=spectral, estimator=estimator, integrator=abi.INTEGRATOR_DIRECT)
+    g = gpu.trace_paths(d, pixels, samples)
+    cd = helpers.desc(cam, 0, 2, 8, 3, spectral, estimator)
+    cd.integrator = abi.INTEGRATOR_DIRECT
+    c = o.trace_paths(cd, W, H, pixels, samples)
+    ok = _compare_paths(g, c)
+    print("direct matching", ok.mean(), "nonzero", (np.abs(c[:, :3]).sum(axis=1) > 0).mean())
+    assert ok.mean() >= 0.999
+
+
+@pytest.mark.parametrize("estimator", [abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED])
+def test_direct_lighting_glossy_parity(dev, estimator):
+    sd = scene.many_light_scene(n_spheres=40, sphere_level=2, ground_res=16, n_light_quads=64, seed=5)
+    W, H = 96, 54
+    s, k1 = gpu_scene(dev, sd)
+    o, k2 = helpers.oracle_scene(sd)
+    cam = scene.many_light_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    d = gpu.render_desc(s, film, cam, 0, 2, spectral=abi.SPECTRAL_HERO4, estimator=estimator,
+                        integrator=abi.INTEGRATOR_DIRECT)
+    g = gpu.trace_paths(d, pixels, samples)
+    cd = helpers.desc(cam, 0, 2, 8, 3, abi.SPECTRAL_HERO4, estimator)
+    cd.integrator = abi.INTEGRATOR_DIRECT
+    c = o.trace_paths(cd, W, H, pixels, samples)
+    ok = _compare_paths(g, c)
+    print("direct glossy matching", ok.mean())
+    assert np.isfinite(g).all()
+    assert ok.mean() >= 0.999
+
+
+def test_direct_lighting_furnace_on_gpu(dev):
+    rho = 0.5
+    sd = helpers.furnace_box(albedo=rho, emit=1.0, subdivide=2)
+    s, k = gpu_scene(dev, sd)
+    cam = scene.camera_pinhole((0.1, -0.2, 0.05), 64, 64, 0.3, 0.2, 1.2)
+    film = gpu.Film(dev, 64, 64)
+    gpu.render(gpu.render_desc(s, film, cam, 0, 64, estimator=abi.ESTIMATOR_CORRECTED,
+                               integrator=abi.INTEGRATOR_DIRECT))
+    rgb, cnt = film.read()
+    np.testing.assert_allclose((rgb / cnt[..., None]).mean(axis=(0, 1)), 1 + rho, rtol=0.01)

@@ -263,3 +263,46 @@ def test_faithful_tree_mode_reproduces_reference_bias():
     rgb, cnt = o.render(d, 16, 16, threads=8)
     mean = (rgb / cnt[..., None]).mean()
     assert abs(mean - 2.0) > 0.2
+
+
+@pytest.mark.parametrize("strategy", [abi.LIGHTS_TREE, abi.LIGHTS_POWER])
+@pytest.mark.parametrize("spectral", [abi.SPECTRAL_RGB, abi.SPECTRAL_HERO4])
+def test_direct_lighting_furnace(strategy, spectral):
+    """Closed Lambertian box of emission Le and albedo rho: the irradiance is pi Le everywhere,
+    so direct lighting reflects rho Le and (corrected) the first surface adds Le."""
+    rho, emit = 0.5, 1.0
+    o, _ = helpers.oracle_scene(helpers.furnace_box(albedo=rho, emit=emit, subdivide=2), strategy)
+    W = 16
+    cam = scene.camera_pinhole((0.1, -0.2, 0.05), W, W, 0.3, 0.2, 1.2)
+    d = helpers.desc(cam, 0, 64, estimator=abi.ESTIMATOR_CORRECTED, spectral=spectral)
+    d.integrator = abi.INTEGRATOR_DIRECT
+    rgb, cnt = o.render(d, W, W)
+    mean = (rgb / cnt[..., None]).mean(axis=(0, 1))
+    expect = emit * (1 + rho)
+    if spectral == abi.SPECTRAL_HERO4:  # constant spectrum: sRGB coordinates of illuminant E
+        np.testing.assert_allclose(mean, np.array([1.2049, 0.9483, 0.9091]) * expect, rtol=0.03)
+    else:
+        np.testing.assert_allclose(mean, expect, rtol=0.01)
+
+
+def test_faithful_tree_importance_drops_facing_emitters():
+    """A47: as written the cone angle is measured against the direction towards the light,
+    so emitters facing the shading point get zero importance and a direct-lighting estimate
+    through the tree misses their light; the corrected orientation recovers Le (1 + rho)."""
+    o, _ = helpers.oracle_scene(helpers.furnace_box(albedo=0.5, emit=1.0, subdivide=2), abi.LIGHTS_TREE)
+    W = 16
+    cam = scene.camera_pinhole((0.1, -0.2, 0.05), W, W, 0.3, 0.2, 1.2)
+    d = helpers.desc(cam, 0, 64, estimator=abi.ESTIMATOR_FAITHFUL)
+    d.integrator = abi.INTEGRATOR_DIRECT
+    rgb, cnt = o.render(d, W, W)
+    assert (rgb / cnt[..., None]).mean() < 0.8 * 1.5
+
+
+def test_direct_lighting_faithful_runs():
+    o, _ = helpers.oracle_scene(scene.cornell_box())
+    W = 16
+    for est in (abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED):
+        d = helpers.desc(scene.cornell_camera(W, W), 0, 4, estimator=est)
+        d.integrator = abi.INTEGRATOR_DIRECT
+        rgb, cnt = o.render(d, W, W)
+        assert np.isfinite(rgb).all() and rgb.sum() > 0
