@@ -62,8 +62,12 @@ def lib():
         L.oracle_medium_sample_equiangular.argtypes = [vp, u32, vp, vp, vp, f32, vp, f32, C.c_int, vp, C.c_int, vp]
         L.oracle_medium_pdf_equiangular.argtypes = [vp, vp, vp, f32, f32, C.c_int]
         L.oracle_medium_pdf_equiangular.restype = f32
+        L.oracle_light_pdfs.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.oracle_light_sample.argtypes = [vp, vp, vp, C.c_int, u32, u32, vp]
         L.oracle_hg_sample.argtypes = [f32, vp, f32, f32, C.c_int, vp]
         L.oracle_hg_sample.restype = f32
+        L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
+        L.oracle_tree_importance.restype = f32
         _LIB = L
     return _LIB
 
@@ -134,6 +138,18 @@ class OracleScene:
         self._chk(self.L.oracle_intersect(self.h, len(rays), _ptr(rays), _ptr(hits)), "intersect")
         return hits
 
+    def light_pdfs(self, p, n, corrected=True, max_lights=1 << 16):
+        out = np.zeros(max_lights + 1, dtype=np.float32)
+        p, n = _v3(p), _v3(n)
+        k = self._chk(self.L.oracle_light_pdfs(self.h, _ptr(p), _ptr(n), int(corrected), _ptr(out)), "light_pdfs")
+        return out[:k], float(out[k])
+
+    def light_sample(self, p, n, pixel, sample, corrected=True):
+        pdf = np.zeros(1, dtype=np.float32)
+        p, n = _v3(p), _v3(n)
+        idx = self.L.oracle_light_sample(self.h, _ptr(p), _ptr(n), int(corrected), pixel, sample, _ptr(pdf))
+        return idx, float(pdf[0])
+
     def occluded(self, rays):
         rays = np.ascontiguousarray(rays, dtype=np.float32)
         occ = np.zeros(len(rays), dtype=np.uint32)
@@ -188,6 +204,12 @@ def hg_sample(g, wo, u0, u1, corrected=True):
     wo = _v3(wo)
     p = lib().oracle_hg_sample(g, _ptr(wo), u0, u1, int(corrected), _ptr(wi))
     return p, wi
+
+
+def tree_importance(center, diag, axis, theta_o, theta_e, power, p, n, corrected=True):
+    """Light-tree node importance at point p with shading normal n (zero normal: medium point)."""
+    node = np.array([*center, diag, *axis, theta_o, theta_e, power], dtype=np.float32)
+    return float(lib().oracle_tree_importance(_ptr(node), _ptr(_v3(p)), _ptr(_v3(n)), int(corrected)))
 
 
 def philox(ctr, key):

@@ -3002,6 +3002,46 @@ float oracle_medium_pdf_equiangular(const float o[3], const float d[3], const fl
   return PdfEquiangular(V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), V3(light[0], light[1], light[2]), tFar, t,
                         corrected != 0);
 }
+// Light-tree pick and pdf at a shading point (test hooks). pdfs: LightPdf for every tree light
+// and, last, the environment light (0 when absent). Returns the number of tree lights.
+int oracle_light_pdfs(oracle_scene s, const float p[3], const float n[3], int corrected, float *pdfs) {
+  if (!s || !s->committed || s->strategy != LGPU_LIGHTS_TREE) return -1;
+  const Ctx c{s, corrected != 0, 0};
+  const V3 P(p[0], p[1], p[2]), N(n[0], n[1], n[2]);
+  for (size_t i = 0; i < s->tlights.size(); ++i) pdfs[i] = LightPdf(c, LightPick{1, (int)i}, P, N);
+  pdfs[s->tlights.size()] = s->envLight >= 0 ? LightPdf(c, LightPick{0, s->envLight}, P, N) : 0.f;
+  return (int)s->tlights.size();
+}
+// SampleLight with the sampler of (pixel, sample): picked tree light index (-1 for the environment
+// light, -2 none) and the pick pdf.
+int oracle_light_sample(oracle_scene s, const float p[3], const float n[3], int corrected, uint32_t pixel,
+                        uint32_t sample, float *pdf) {
+  if (!s || !s->committed || s->strategy != LGPU_LIGHTS_TREE) return -3;
+  const Ctx c{s, corrected != 0, 0};
+  Sampler smp;
+  smp.pixel = pixel; smp.sample = sample; smp.seed = 0; smp.dim = 0;
+  float pd = 1;
+  const LightPick lp = SampleLightAt(c, V3(p[0], p[1], p[2]), V3(n[0], n[1], n[2]), smp, &pd);
+  *pdf = pd;
+  if (lp.kind == 1) return lp.index;
+  return lp.kind == 0 ? -1 : -2;
+}
+
+// TreeImportance of one node (test hook): node[] = center xyz, diagonal, axis xyz, thetaO, thetaE,
+// power. The node's box is the cube of that diagonal around the center.
+float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected) {
+  LNode nd;
+  const V3 c(node[0], node[1], node[2]);
+  const float h = node[3] / (2.f * std::sqrt(3.f));
+  nd.bounds.mn = c - V3(h, h, h);
+  nd.bounds.mx = c + V3(h, h, h);
+  nd.cone.axis = V3(node[4], node[5], node[6]);
+  nd.cone.thetaO = node[7];
+  nd.cone.thetaE = node[8];
+  nd.power = node[9];
+  return TreeImportance(&nd, V3(p[0], p[1], p[2]), V3(n[0], n[1], n[2]), corrected != 0);
+}
+
 float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]) {
   V3 w;
   const float p = HGSampleU(g, V3(wo[0], wo[1], wo[2]), &w, V2(u0, u1), corrected != 0);

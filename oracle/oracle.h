@@ -69,7 +69,12 @@ int oracle_medium_sample_equiangular(oracle_scene s, uint32_t med, const float o
                                      const float *lambda, int corrected, float out[7]);
 float oracle_medium_pdf_equiangular(const float o[3], const float d[3], const float light[3], float t_far, float t,
                                     int corrected);
+/* ManyLightSampler pick and pdf at a point (tree strategy): pdfs[nTreeLights + 1] (last = env). */
+int oracle_light_pdfs(oracle_scene s, const float p[3], const float n[3], int corrected, float *pdfs);
+int oracle_light_sample(oracle_scene s, const float p[3], const float n[3], int corrected, uint32_t pixel,
+                        uint32_t sample, float *pdf);
 /* HenyeyGreenstein::Sample_p (HenyeyGreenstein.cpp:21-36) with explicit u; returns pdf. */
+float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected);
 float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]);
 
 #ifdef __cplusplus

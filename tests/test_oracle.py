@@ -285,17 +285,81 @@ def test_direct_lighting_furnace(strategy, spectral):
         np.testing.assert_allclose(mean, expect, rtol=0.01)
 
 
-def test_faithful_tree_importance_drops_facing_emitters():
-    """A47: as written the cone angle is measured against the direction towards the light,
-    so emitters facing the shading point get zero importance and a direct-lighting estimate
-    through the tree misses their light; the corrected orientation recovers Le (1 + rho)."""
-    o, _ = helpers.oracle_scene(helpers.furnace_box(albedo=0.5, emit=1.0, subdivide=2), abi.LIGHTS_TREE)
-    W = 16
-    cam = scene.camera_pinhole((0.1, -0.2, 0.05), W, W, 0.3, 0.2, 1.2)
-    d = helpers.desc(cam, 0, 64, estimator=abi.ESTIMATOR_FAITHFUL)
-    d.integrator = abi.INTEGRATOR_DIRECT
-    rgb, cnt = o.render(d, W, W)
-    assert (rgb / cnt[..., None]).mean() < 0.8 * 1.5
+def _many_light_quads(n_quads, seed):
+    """Small emissive quads in random positions and orientations (blackbody, varied power)."""
+    rng = np.random.default_rng(seed)
+    sd = scene.SceneDesc()
+    sd.add_material(scene.lambertian((0.5, 0.5, 0.5)))
+    for _ in range(n_quads):
+        m = sd.add_material(scene.emitter_blackbody(float(rng.uniform(2000, 9000)), float(10 ** rng.uniform(0, 2))))
+        c = rng.uniform(-1.5, 1.5, size=3)
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        a = 0.1 * a / np.linalg.norm(a)
+        b = np.cross(a, b)
+        b = 0.1 * b / np.linalg.norm(b)
+        v = np.array([c - a - b, c + a - b, c + a + b, c - a + b], dtype=np.float32)
+        sd.add_mesh(v, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32), m)
+    return sd
+
+
+def test_light_tree_pdfs_sum_to_one_and_match_sampling():
+    """Corrected tree: sum over lights of P(pick) = 1 at any point, and pick frequencies follow
+    the pdf (brute-force enumeration plus a chi-square-style check)."""
+    o, _ = helpers.oracle_scene(_many_light_quads(24, seed=4), abi.LIGHTS_TREE)
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        p = rng.uniform(-2, 2, size=3)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        pdfs, env = o.light_pdfs(p, n, corrected=True)
+        assert pdfs.sum() + env == pytest.approx(1.0, abs=1e-5)
+        counts = np.zeros(len(pdfs))
+        N = 20000
+        for k in range(N):
+            idx, pk = o.light_sample(p, n, k, 0, corrected=True)
+            assert idx >= 0
+            assert pk == pytest.approx(pdfs[idx], rel=1e-4)
+            counts[idx] += 1
+        expected = pdfs * N
+        m = expected > 5
+        chi2 = ((counts[m] - expected[m]) ** 2 / expected[m]).sum()
+        assert chi2 < m.sum() + 6 * np.sqrt(2 * m.sum()) + 10
+
+
+def test_tree_importance_closed_forms():
+    """Node importance (ManyLightSampler.cpp:305-330) against hand-computed values. A node of power E
+    one unit above the origin, facing down onto it (axis -y, thetaO 0, thetaE pi/2), point normal +y:
+    the corrected orientation measures the emitter cone against the direction from the light to the
+    point and gives E / d^2 = E; as written it measures it against the direction towards the light,
+    the angle is pi and the node gets nothing (A47). Moving the node to (1, 1, 0) puts both the
+    emitter angle and the incidence angle at pi/4: E/2 * cos^2(pi/4) = E/4. A node of diagonal D
+    widens both angles by atan(D / 2d). A zero normal (medium point) drops the incidence term. As
+    written the incidence cosine is signed, so a node below the shading horizon gets a negative
+    importance (A48); corrected takes |cos|."""
+    E = 3.0
+    down = ((0, 1, 0), 0.0, (0, -1, 0), 0.0, math.pi / 2, E)
+    assert oracle.tree_importance(*down, (0, 0, 0), (0, 1, 0), True) == pytest.approx(E, rel=1e-6)
+    assert oracle.tree_importance(*down, (0, 0, 0), (0, 1, 0), False) == 0.0
+    up = ((0, 1, 0), 0.0, (0, 1, 0), 0.0, math.pi / 2, E)
+    assert oracle.tree_importance(*up, (0, 0, 0), (0, 1, 0), True) == 0.0
+    assert oracle.tree_importance(*up, (0, 0, 0), (0, 1, 0), False) == pytest.approx(E, rel=1e-6)
+    side = ((1, 1, 0), 0.0, (0, -1, 0), 0.0, math.pi / 2, E)
+    assert oracle.tree_importance(*side, (0, 0, 0), (0, 1, 0), True) == pytest.approx(E / 4, rel=1e-5)
+    # diagonal D at distance sqrt(2): both angles shrink by thetaU = atan(D / (2 sqrt 2))
+    D = 0.5
+    tu = math.atan(D / (2 * math.sqrt(2)))
+    wide = ((1, 1, 0), D, (0, -1, 0), 0.0, math.pi / 2, E)
+    want = math.cos(math.pi / 4 - tu) ** 2 * E / 2
+    assert oracle.tree_importance(*wide, (0, 0, 0), (0, 1, 0), True) == pytest.approx(want, rel=1e-5)
+    # medium point: no incidence term
+    assert oracle.tree_importance(*side, (0, 0, 0), (0, 0, 0), True) == pytest.approx(E / 2 * math.cos(math.pi / 4), rel=1e-5)
+    # shading normal facing away from the node: |cos| when corrected, cos(pi) = -1 as written
+    assert oracle.tree_importance(*down, (0, 0, 0), (0, -1, 0), True) == pytest.approx(E, rel=1e-6)
+    assert oracle.tree_importance(*up, (0, 0, 0), (0, -1, 0), False) == pytest.approx(-E, rel=1e-6)
+    # outside the emission cone: thetaO 0, thetaE pi/4, emitter angle pi/4 + 0.1
+    a = math.pi / 4 + 0.1
+    off = ((math.sin(a), math.cos(a), 0), 0.0, (0, -1, 0), 0.0, math.pi / 4, E)
+    assert oracle.tree_importance(*off, (0, 0, 0), (0, 1, 0), True) == 0.0
 
 
 def test_direct_lighting_faithful_runs():
