@@ -58,16 +58,34 @@ typedef enum lgpu_node_type {
   LGPU_NODE_IMAGE_COLOUR = 6,   /* out colour; p[0] = texture id */
   LGPU_NODE_IMAGE_SCALAR = 7,   /* out scalar; p[0] = texture id */
   LGPU_NODE_UV = 8,             /* out vec2 = surface uv (SurfaceInput) */
-  LGPU_NODE_FRESNEL = 9,        /* out scalar = FrDielectric(woL.y, 1, p[0]) */
+  LGPU_NODE_FRESNEL = 9,        /* FresnelInput (GraphInputs.cpp:187-199): out scalar = Schlick
+                                   with n1 = the path's eta, n2 = in[0] (p[0] when unset),
+                                   cos = woL.y */
+  LGPU_NODE_SPECTRAL = 10,      /* SpectralInput (GraphInputs.cpp:52-66): out spectrum; samples
+                                   in texture p[0] (1 channel, width x height values in order),
+                                   evenly spaced over [p[1], p[2]] nm */
+  LGPU_NODE_IMAGE_CHANNEL = 11, /* ImageTextureChannelInput: out scalar; p[0] texture, p[1]
+                                   channel 0..3 (alpha 1 unless the texture is RGBA) */
+  LGPU_NODE_SPECTRAL_TEXTURE = 12, /* SpectralTextureInput: out spectrum; p[0] texture of RGB spectra */
+  LGPU_NODE_NORMAL = 13,        /* SurfaceInfoInput: out vec3 = shading normal */
   /* maths / converters (GraphMaths.cpp, GraphConverters.cpp) */
   LGPU_NODE_SCALAR_MATHS = 16,  /* out scalar = in[0] op in[1]; p[0] = op */
   LGPU_NODE_VECTOR_MATHS = 17,  /* out vec3 = in[0] op in[1]; p[0] = op */
   LGPU_NODE_DOT = 18,           /* out scalar */
   LGPU_NODE_LENGTH = 19,        /* out scalar */
-  LGPU_NODE_MERGE_RGB = 20,     /* out colour from three scalars */
-  LGPU_NODE_SEPARATE_R = 21,    /* out scalar: channel p[0] of a colour */
-  /* textures (GraphTexture.cpp) */
+  LGPU_NODE_MERGE_RGB = 20,     /* MergeRGBA: out colour from scalars in[0..2], alpha in[3] (1 when unset) */
+  LGPU_NODE_SEPARATE_R = 21,    /* SeparateRGBA: out scalar: channel p[0] (0..3) of a colour */
+  LGPU_NODE_CROSS = 22,         /* out vec3 = in[0] x in[1] */
+  LGPU_NODE_SEPARATE_XYZ = 23,  /* out scalar: component p[0] of a vec3 */
+  LGPU_NODE_MERGE_XYZ = 24,     /* out vec3 from three scalars */
+  LGPU_NODE_SCALAR_TO_COLOUR = 25, /* out colour (x, x, x, 1) */
+  /* textures (GraphTexture.cpp); at a medium interaction the 3-D variant on the event point */
   LGPU_NODE_CHECKER = 32,       /* out scalar; p[0] = scale */
+  LGPU_NODE_PERLIN = 33,        /* out scalar; p[0] = scale (3-D variant: 0, as written) */
+  LGPU_NODE_VALUE_NOISE = 34,   /* out scalar; p[0] = scale (3-D variant: 0, as written) */
+  LGPU_NODE_OCTAVE_NOISE = 35,  /* out scalar; p[0] scale, p[1] octaves, p[2] base texture node
+                                   type (CHECKER/PERLIN/VALUE_NOISE/VORONOI), p[3] base scale */
+  LGPU_NODE_VORONOI = 36,       /* out scalar; p[0] = scale */
   /* BxDF nodes (GraphBxDF.cpp) */
   LGPU_NODE_LAMBERTIAN = 64,    /* in[0] albedo */
   LGPU_NODE_OREN_NAYAR_BRDF = 65, /* in[0] albedo, in[1] sigma (radians) */
@@ -160,7 +178,7 @@ typedef struct lgpu_medium_desc {
 } lgpu_medium_desc;
 
 typedef struct lgpu_texture_desc {
-  uint32_t width, height, channels; /* channels: 1 or 3 */
+  uint32_t width, height, channels; /* channels: 1, 3 or 4 (RGBA) */
   const float *data;                /* row-major */
 } lgpu_texture_desc;
 

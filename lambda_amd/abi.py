@@ -14,8 +14,12 @@ OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_STATE, ERR_OUT_OF_MEMORY, ERR_UNSUPPORTED
 # node types
 NODE_SCALAR, NODE_VEC2, NODE_VEC3, NODE_RGB, NODE_BLACKBODY = 1, 2, 3, 4, 5
 NODE_IMAGE_COLOUR, NODE_IMAGE_SCALAR, NODE_UV, NODE_FRESNEL = 6, 7, 8, 9
+NODE_SPECTRAL, NODE_IMAGE_CHANNEL, NODE_SPECTRAL_TEXTURE, NODE_NORMAL = 10, 11, 12, 13
 NODE_SCALAR_MATHS, NODE_VECTOR_MATHS, NODE_DOT, NODE_LENGTH, NODE_MERGE_RGB, NODE_SEPARATE_R = 16, 17, 18, 19, 20, 21
-NODE_CHECKER = 32
+NODE_CROSS, NODE_SEPARATE_XYZ, NODE_MERGE_XYZ, NODE_SCALAR_TO_COLOUR = 22, 23, 24, 25
+NODE_CHECKER, NODE_PERLIN, NODE_VALUE_NOISE, NODE_OCTAVE_NOISE, NODE_VORONOI = 32, 33, 34, 35, 36
+# value types reported by the oracle's node evaluation
+VT_SCALAR, VT_VEC2, VT_VEC3, VT_COLOUR, VT_SPECTRUM = 0, 1, 2, 3, 4
 NODE_LAMBERTIAN, NODE_OREN_NAYAR_BRDF, NODE_OREN_NAYAR_BTDF, NODE_MICROFACET_BRDF = 64, 65, 66, 67
 NODE_FRESNEL_BSDF, NODE_SPECULAR_BRDF, NODE_SPECULAR_BTDF, NODE_GHOST_BTDF = 68, 69, 70, 71
 NODE_MIX_BSDF, NODE_ROUGH_DIELECTRIC = 72, 73

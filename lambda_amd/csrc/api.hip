@@ -81,7 +81,7 @@ lgpu_status lgpu_scene_destroy(lgpu_scene s) {
 lgpu_status lgpu_scene_add_texture(lgpu_scene s, const lgpu_texture_desc *d, uint32_t *id) {
   CHECK_ARG(s && d && d->data, "add_texture: null argument");
   CHECK_ARG(!s->committed, "add_texture: scene already committed");
-  CHECK_ARG(d->width && d->height && (d->channels == 1 || d->channels == 3), "add_texture: bad size or channel count");
+  CHECK_ARG(d->width && d->height && (d->channels == 1 || d->channels == 3 || d->channels == 4), "add_texture: bad size or channel count");
   s->textures.emplace_back(d->data, d->data + (size_t)d->width * d->height * d->channels);
   s->tex_dims.push_back(d->width);
   s->tex_dims.push_back(d->height);

@@ -15,8 +15,7 @@ namespace lgpu {
 namespace {
 
 __device__ inline Spec EmissionAt(const DevScene &sc, const Hit &lh, VmCtx vm) {
-  vm.u = lh.u;
-  vm.v = lh.v;
+  VmAtHit(vm, lh.u, lh.v, lh.point, lh.normalS);
   return EmissionOf(sc, lh.mesh, vm) * sc.lights[sc.meshes[lh.mesh].mesh_light].intensity * kInvPi;
 }
 
@@ -85,14 +84,16 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
     vm.lam[0] = vm.lam[1] = vm.lam[2] = vm.lam[3] = 0.f;
   }
   vm.woLy = 0.f;
+  vm.eta = rd4.w;
+  vm.medium = 0;
+  VmAtHit(vm, 0.f, 0.f, v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f));
   Spec Lacc = S(0.f);
   if (bi == kNone) {  // :28 (no environment light: 0)
     if (sc.has_env) Lacc = EnvLe(sc, rd, vm);
   } else {
     Hit h;
     ProcessHit(sc, ro, rd, hr.x, hr.y, hr.z, bi, corrected, h);
-    vm.u = h.u;
-    vm.v = h.v;
+    VmAtHit(vm, h.u, h.v, h.point, h.normalS);
     const MeshRec m = sc.meshes[h.mesh];
     const MaterialRec mat = sc.materials[m.material];
     if (corrected && m.mesh_light != kNone && Dot(rd, h.normalG) < 0)  // first-surface emission (corrected)

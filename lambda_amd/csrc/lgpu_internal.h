@@ -144,7 +144,7 @@ enum Op : uint32_t {
   kOpUV = 2,         // r[dst] = (u, v, 0, 0)
   kOpTex = 3,        // r[dst] = bilinear(texture imm, fract(uv))
   kOpChecker = 4,    // r[dst].x = checker(uv * consts[imm].x)
-  kOpFresnel = 5,    // r[dst].x = FrDielectric(woL.y, 1, consts[imm].x)
+  kOpFresnel = 5,    // r[dst].x = Schlick(eta, r[a].x, woL.y)
   kOpSMaths = 6,     // r[dst].x = r[a].x op r[b].x ; op in c
   kOpVMaths = 7,     // r[dst].xyz = r[a] op r[b] ; op in c
   kOpDot = 8,
@@ -155,7 +155,16 @@ enum Op : uint32_t {
   kOpToReal = 13,    // r[dst].x = AsReal(r[a], type c)
   kOpToColour = 14,  // r[dst] = AsColour(r[a], type c)
   kOpToVec3 = 15,
-  kOpToSpectrum = 16 // r[dst] = AsSpectrum(r[a], type c)
+  kOpToSpectrum = 16, // r[dst] = AsSpectrum(r[a], type c)
+  kOpNormal = 17,    // r[dst].xyz = shading normal
+  kOpTexChannel = 18, // r[dst].x = channel c of bilinear(texture imm, fract(uv))
+  kOpCross = 19,     // r[dst].xyz = r[a] x r[b]
+  kOpNoise = 20,     // r[dst].x = noise c (node type) at uv (or the point at a medium
+                     //   interaction) scaled by consts[imm].x, .y, .z in turn
+  kOpSpectral = 21,  // r[dst] = sampled SPD of texture consts[imm].z over [consts[imm].x, .y] nm
+                     //   at the hero wavelengths; RGB mode: consts[imm + 1]
+  kOpSpecTex = 22,   // r[dst] = RGB texel of texture imm as a spectrum
+  kOpAlpha = 23      // r[dst] = (r[a].xyz, r[b].x)
 };
 struct Instr {
   uint32_t op, dst, a, b, c, imm, pad[2];

@@ -272,8 +272,7 @@ __device__ inline Spec MisLight(const DevScene &sc, uint32_t l, float lightDistP
     const LightRec &NL = LightOf(sc, nl);
     if (!corrected && sc.strategy == LGPU_LIGHTS_TREE) pl = PdfLiHit(sc, sc.lights[NL.parent], rd, h, bi, false);
     else pl = PdfLiHit(sc, NL, rd, h, bi, corrected);
-    vm.u = h.u;
-    vm.v = h.v;
+    VmAtHit(vm, h.u, h.v, h.point, h.normalS);
     Le = EmissionOf(sc, h.mesh, vm) * sc.lights[sc.meshes[h.mesh].mesh_light].intensity * kInvPi;
     if (corrected && Dot(rd, h.normalG) > 0) Le = S(0.f);
   } else {

@@ -185,10 +185,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, RenderParams rp, 
     vm.lam[0] = vm.lam[1] = vm.lam[2] = vm.lam[3] = 0.f;
   }
   vm.woLy = 0.f;
+  vm.eta = rd4.w;
+  vm.medium = 0;
   Hit h;
   if (hit) ProcessHit(sc, ro, rd, hr.x, hr.y, hr.z, bi, corrected, h);
-  vm.u = hit ? h.u : 0.f;
-  vm.v = hit ? h.v : 0.f;
+  if (hit) VmAtHit(vm, h.u, h.v, h.point, h.normalS);
+  else VmAtHit(vm, 0.f, 0.f, v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f));
   bool alive = true;
 
   // ---- emission reaching this vertex

@@ -236,17 +236,22 @@ struct Compiler {
   std::vector<float4> &consts;
   const std::vector<lgpu_node> &nodes;
   const std::vector<uint32_t> &texIds;  // texture id -> index in DevScene textures
+  const std::vector<std::vector<float>> &texData;  // host texels per texture id
+  const std::vector<uint32_t> &texDims;            // w, h, channels per texture id
   std::string err;
   uint32_t nextReg = 0;
 
   uint32_t OutType(uint32_t t) const {
     switch (t) {
       case LGPU_NODE_SCALAR: case LGPU_NODE_IMAGE_SCALAR: case LGPU_NODE_FRESNEL: case LGPU_NODE_SCALAR_MATHS:
-      case LGPU_NODE_DOT: case LGPU_NODE_LENGTH: case LGPU_NODE_SEPARATE_R: case LGPU_NODE_CHECKER: return kScalar;
+      case LGPU_NODE_DOT: case LGPU_NODE_LENGTH: case LGPU_NODE_SEPARATE_R: case LGPU_NODE_CHECKER:
+      case LGPU_NODE_IMAGE_CHANNEL: case LGPU_NODE_SEPARATE_XYZ: case LGPU_NODE_PERLIN: case LGPU_NODE_VALUE_NOISE:
+      case LGPU_NODE_OCTAVE_NOISE: case LGPU_NODE_VORONOI: return kScalar;
       case LGPU_NODE_VEC2: case LGPU_NODE_UV: return kVec2;
-      case LGPU_NODE_VEC3: case LGPU_NODE_VECTOR_MATHS: return kVec3;
-      case LGPU_NODE_RGB: case LGPU_NODE_IMAGE_COLOUR: case LGPU_NODE_MERGE_RGB: return kColour;
-      case LGPU_NODE_BLACKBODY: return kSpectrum;
+      case LGPU_NODE_VEC3: case LGPU_NODE_VECTOR_MATHS: case LGPU_NODE_NORMAL: case LGPU_NODE_CROSS:
+      case LGPU_NODE_MERGE_XYZ: return kVec3;
+      case LGPU_NODE_RGB: case LGPU_NODE_IMAGE_COLOUR: case LGPU_NODE_MERGE_RGB: case LGPU_NODE_SCALAR_TO_COLOUR: return kColour;
+      case LGPU_NODE_BLACKBODY: case LGPU_NODE_SPECTRAL: case LGPU_NODE_SPECTRAL_TEXTURE: return kSpectrum;
     }
     return kBxdf;
   }
@@ -256,11 +261,15 @@ struct Compiler {
     const lgpu_node &n = nodes[i];
     switch (n.type) {
       case LGPU_NODE_SCALAR: case LGPU_NODE_VEC2: case LGPU_NODE_VEC3: case LGPU_NODE_RGB: return true;
-      case LGPU_NODE_SCALAR_MATHS: case LGPU_NODE_VECTOR_MATHS: case LGPU_NODE_DOT:
+      case LGPU_NODE_SCALAR_MATHS: case LGPU_NODE_VECTOR_MATHS: case LGPU_NODE_DOT: case LGPU_NODE_CROSS:
         return IsConst(n.in[0], depth + 1) && IsConst(n.in[1], depth + 1);
-      case LGPU_NODE_LENGTH: case LGPU_NODE_SEPARATE_R: return IsConst(n.in[0], depth + 1);
-      case LGPU_NODE_MERGE_RGB:
+      case LGPU_NODE_LENGTH: case LGPU_NODE_SEPARATE_R: case LGPU_NODE_SEPARATE_XYZ: case LGPU_NODE_SCALAR_TO_COLOUR:
+        return IsConst(n.in[0], depth + 1);
+      case LGPU_NODE_MERGE_XYZ:
         return IsConst(n.in[0], depth + 1) && IsConst(n.in[1], depth + 1) && IsConst(n.in[2], depth + 1);
+      case LGPU_NODE_MERGE_RGB:
+        return IsConst(n.in[0], depth + 1) && IsConst(n.in[1], depth + 1) && IsConst(n.in[2], depth + 1) &&
+               (n.in[3] == LGPU_NONE || IsConst(n.in[3], depth + 1));
     }
     return false;
   }
@@ -300,14 +309,35 @@ struct Compiler {
         r.x = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
         break;
       }
-      case LGPU_NODE_MERGE_RGB:
+      case LGPU_NODE_MERGE_RGB: {
+        uint32_t td;
         r = make_float4(AsRealV(EvalConst(n.in[0], &ta), ta, 0), AsRealV(EvalConst(n.in[1], &tb), tb, 0),
                         AsRealV(EvalConst(n.in[2], &tc), tc, 0), 1);
+        if (n.in[3] != LGPU_NONE) r.w = AsRealV(EvalConst(n.in[3], &td), td, 0);
+        break;
+      }
+      case LGPU_NODE_MERGE_XYZ:
+        r = make_float4(AsRealV(EvalConst(n.in[0], &ta), ta, 0), AsRealV(EvalConst(n.in[1], &tb), tb, 0),
+                        AsRealV(EvalConst(n.in[2], &tc), tc, 0), 0);
         break;
       case LGPU_NODE_SEPARATE_R: {
         const float4 c = AsColourV(EvalConst(n.in[0], &ta), ta);
-        const int ch = std::min(std::max((int)n.p[0], 0), 2);
-        r.x = ch == 0 ? c.x : (ch == 1 ? c.y : c.z);
+        r.x = Comp(c, (uint32_t)std::min(std::max((int)n.p[0], 0), 3));
+        break;
+      }
+      case LGPU_NODE_SEPARATE_XYZ: {
+        const float4 a = AsVec3V(EvalConst(n.in[0], &ta), ta);
+        r.x = Comp(a, (uint32_t)std::min(std::max((int)n.p[0], 0), 2));
+        break;
+      }
+      case LGPU_NODE_SCALAR_TO_COLOUR: {
+        const float x = AsRealV(EvalConst(n.in[0], &ta), ta, 0);
+        r = make_float4(x, x, x, 1);
+        break;
+      }
+      case LGPU_NODE_CROSS: {
+        const float4 a = AsVec3V(EvalConst(n.in[0], &ta), ta), b = AsVec3V(EvalConst(n.in[1], &tb), tb);
+        r = make_float4(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, 0);
         break;
       }
     }
@@ -365,8 +395,80 @@ struct Compiler {
         break;
       }
       case LGPU_NODE_UV: Emit(kOpUV, dst); break;
-      case LGPU_NODE_FRESNEL: Emit(kOpFresnel, dst, 0, 0, 0, AddConst(make_float4(n.p[0], 0, 0, 0))); break;
-      case LGPU_NODE_CHECKER: Emit(kOpChecker, dst, 0, 0, 0, AddConst(make_float4(n.p[0], 0, 0, 0))); break;
+      case LGPU_NODE_NORMAL: Emit(kOpNormal, dst); break;
+      case LGPU_NODE_FRESNEL: {
+        uint32_t a;
+        if (n.in[0] == LGPU_NONE) { a = NewReg(); Emit(kOpConst, a, 0, 0, 0, AddConst(make_float4(n.p[0], 0, 0, 0))); }
+        else a = asReal(n.in[0]);
+        Emit(kOpFresnel, dst, a);
+        break;
+      }
+      case LGPU_NODE_IMAGE_CHANNEL:
+      case LGPU_NODE_SPECTRAL_TEXTURE: {
+        const uint32_t t = (uint32_t)n.p[0];
+        if (t >= texIds.size()) { err = "image node names a missing texture"; return dst; }
+        if (n.type == LGPU_NODE_IMAGE_CHANNEL) Emit(kOpTexChannel, dst, 0, 0, (uint32_t)std::min(std::max((int)n.p[1], 0), 3), texIds[t]);
+        else Emit(kOpSpecTex, dst, 0, 0, 0, texIds[t]);
+        break;
+      }
+      case LGPU_NODE_SPECTRAL: {
+        const uint32_t t = (uint32_t)n.p[0];
+        if (t >= texIds.size()) { err = "spectral node names a missing texture"; return dst; }
+        const std::vector<float> &d = texData[t];
+        const uint32_t ch = texDims[3 * t + 2], cnt = (uint32_t)(d.size() / ch);
+        if (cnt < 2 || !(n.p[2] > n.p[1])) { err = "spectral node needs >= 2 samples over an increasing range"; return dst; }
+        std::vector<float> lam(cnt), v(cnt);
+        for (uint32_t k = 0; k < cnt; ++k) { lam[k] = n.p[1] + (n.p[2] - n.p[1]) / (float)(cnt - 1) * (float)k; v[k] = d[k * ch]; }
+        float rgb[3];
+        SampledRGBHost(lam.data(), v.data(), cnt, rgb);
+        const uint32_t k0 = AddConst(make_float4(n.p[1], n.p[2], (float)texIds[t], 0));
+        AddConst(make_float4(rgb[0], rgb[1], rgb[2], 0));
+        Emit(kOpSpectral, dst, 0, 0, 0, k0);
+        break;
+      }
+      case LGPU_NODE_CHECKER: case LGPU_NODE_PERLIN: case LGPU_NODE_VALUE_NOISE: case LGPU_NODE_VORONOI:
+        Emit(kOpNoise, dst, 0, 0, n.type, AddConst(make_float4(n.p[0], 1, 1, 0)));
+        break;
+      case LGPU_NODE_OCTAVE_NOISE: {  // OctaveNoise::Get2D/3D (GraphTexture.cpp:156-174), unrolled
+        const uint32_t base = (uint32_t)n.p[2];
+        if (base != LGPU_NODE_CHECKER && base != LGPU_NODE_PERLIN && base != LGPU_NODE_VALUE_NOISE && base != LGPU_NODE_VORONOI) {
+          err = "octave noise needs a checker, Perlin, value-noise or Voronoi base"; return dst;
+        }
+        if (!(n.p[1] >= 0.f && n.p[1] <= 32.f)) { err = "octave noise: 0..32 octaves"; return dst; }
+        const uint32_t oct = (uint32_t)n.p[1];
+        Emit(kOpConst, dst, 0, 0, 0, AddConst(make_float4(0, 0, 0, 0)));
+        float amp = .5f, scl = 1.f;
+        for (uint32_t k = 0; k < oct; ++k) {
+          const uint32_t nr = NewReg(), ar = NewReg(), pr = NewReg();
+          Emit(kOpNoise, nr, 0, 0, base, AddConst(make_float4(n.p[0], scl, n.p[3], 0)));
+          Emit(kOpConst, ar, 0, 0, 0, AddConst(make_float4(amp, 0, 0, 0)));
+          Emit(kOpSMaths, pr, nr, ar, LGPU_OP_MULTIPLY);
+          Emit(kOpSMaths, dst, dst, pr, LGPU_OP_ADD);
+          nextReg -= 3;
+          scl *= 2.f;
+          amp *= .5f;
+        }
+        break;
+      }
+      case LGPU_NODE_CROSS: { const uint32_t a = asVec3(n.in[0]), b = asVec3(n.in[1]); Emit(kOpCross, dst, a, b); break; }
+      case LGPU_NODE_SEPARATE_XYZ: {
+        const uint32_t a = asVec3(n.in[0]);
+        Emit(kOpSeparate, dst, a, 0, (uint32_t)std::min(std::max((int)n.p[0], 0), 2));
+        break;
+      }
+      case LGPU_NODE_MERGE_XYZ: {
+        const uint32_t a = asReal(n.in[0]), b = asReal(n.in[1]), c = asReal(n.in[2]);
+        Emit(kOpMerge, dst, a, b, c);
+        const uint32_t z = NewReg();
+        Emit(kOpConst, z, 0, 0, 0, AddConst(make_float4(0, 0, 0, 0)));
+        Emit(kOpAlpha, dst, dst, z);
+        break;
+      }
+      case LGPU_NODE_SCALAR_TO_COLOUR: {
+        const uint32_t a = asReal(n.in[0]);
+        Emit(kOpToColour, dst, a, 0, kScalar);
+        break;
+      }
       case LGPU_NODE_SCALAR_MATHS: {
         const uint32_t a = asReal(n.in[0]), b = asReal(n.in[1]);
         Emit(kOpSMaths, dst, a, b, (uint32_t)n.p[0]);
@@ -382,6 +484,7 @@ struct Compiler {
       case LGPU_NODE_MERGE_RGB: {
         const uint32_t a = asReal(n.in[0]), b = asReal(n.in[1]), c = asReal(n.in[2]);
         Emit(kOpMerge, dst, a, b, c);
+        if (n.in[3] != LGPU_NONE) { const uint32_t al = asReal(n.in[3]); Emit(kOpAlpha, dst, dst, al); }
         break;
       }
       case LGPU_NODE_SEPARATE_R: {
@@ -389,7 +492,7 @@ struct Compiler {
         const uint32_t r = EmitNode(n.in[0], &t, depth + 1);
         const uint32_t c = NewReg();
         Emit(kOpToColour, c, r, 0, t);
-        Emit(kOpSeparate, dst, c, 0, (uint32_t)std::min(std::max((int)n.p[0], 0), 2));
+        Emit(kOpSeparate, dst, c, 0, (uint32_t)std::min(std::max((int)n.p[0], 0), 3));
         break;
       }
       default: err = "node type cannot feed a socket"; break;
@@ -483,6 +586,10 @@ struct Compiler {
     float lam[3], v[3];
     for (unsigned i = 0; i < n; ++i) lam[i] = 400.f + 300.f * invSamples * (float)i;
     for (unsigned i = 0; i < n; ++i) v[i] = BlackbodyNormalised(lam[i], T);
+    SampledRGBHost(lam, v, n, out);
+  }
+  // RGBSpectrum::FromSampled (Spectrum.h:353-373): n samples at increasing wavelengths -> RGB.
+  static void SampledRGBHost(const float *lam, const float *v, unsigned n, float out[3]) {
     float xyz[3] = {0, 0, 0};
     for (unsigned i = 0; i < 471; ++i) {
       const float li = 360.f + (float)i;
@@ -696,7 +803,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   std::vector<float4> consts;
   std::vector<MaterialRec> mats;
   for (const HostMaterial &hm : s->materials) {
-    Compiler c{sockets, lobes, code, consts, hm.nodes, texIds};
+    Compiler c{sockets, lobes, code, consts, hm.nodes, texIds, s->textures, s->tex_dims};
     MaterialRec r;
     std::memset(&r, 0, sizeof(r));
     r.lobe = hm.desc.bxdf_root == LGPU_NONE ? kNone : c.CompileLobe(hm.desc.bxdf_root);
@@ -744,6 +851,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   const float sceneMaxLen = nt ? std::max(std::max(sb.hi[0] - sb.lo[0], sb.hi[1] - sb.lo[1]), sb.hi[2] - sb.lo[2]) : 0.f;
   VmCtx hostVm;
   std::memset(&hostVm, 0, sizeof(hostVm));
+  hostVm.eta = 1.001f;  // a default ScatterEvent
   hostVm.texels = texels.data();
   hostVm.textures = texRecs.data();
   hostVm.consts = consts.data();

@@ -30,6 +30,7 @@ struct PathOut {  // continuation written to the next queue
   uint32_t flags, depth, pick, pass, med;
   float4 mis;
   V3 prevP, prevN;
+  bool medInter = false;  // the vertex is a medium interaction (its deferred light lookup reads 3-D textures)
 };
 
 __device__ inline void Push(const Queues &q, const uint4 &ids, uint32_t sid, uint32_t dim, float l0, const PathOut &p) {
@@ -41,7 +42,7 @@ __device__ inline void Push(const Queues &q, const uint4 &ids, uint32_t sid, uin
   q.next.aux[j] = make_uint4(sid, p.pick, p.pass, p.med);
   q.next.mis[j] = p.mis;
   q.next.prev_p[j] = ToF4(p.prevP, 0.f);
-  q.next.prev_n[j] = ToF4(p.prevN, 0.f);
+  q.next.prev_n[j] = ToF4(p.prevN, p.medInter ? 1.f : 0.f);
 }
 
 // MISVolumetricPathIntegrator::LdMediumPoint (MISVolumetricPathIntegrator.cpp:23-69). `eh` is the
@@ -50,6 +51,8 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
                                      PartialLS &ls, V3 refP, V3 refN, V3 *wiOut, Rng &rng, const VmCtx &vm,
                                      bool corrected, const VisCtx &vc) {
   const float partialLightPDF = corrected ? ls.selPdf : ls.pdf;
+  VmCtx vmm = vm;  // the phase-sampled light is looked up at a medium interaction
+  vmm.medium = 1;
   eh.point = p;
   Spec Li = S(0.f);
   bool delta = false;
@@ -78,7 +81,7 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
   uint32_t tbi = kNone;
   const bool si = IntersectTr(sc, p, wi, med, -1.f, &Tr, vm, corrected, vc.tc, eh, tbi, nullptr, false);
   if (scatPdf > 0) {
-    const Spec C = MisLight(sc, ls.light, partialLightPDF, refP, refN, si, eh, tbi, wi, scatPdf, false, corrected, vm,
+    const Spec C = MisLight(sc, ls.light, partialLightPDF, refP, refN, si, eh, tbi, wi, scatPdf, false, corrected, vmm,
                             corrected);
     Ld = Ld + (corrected ? C * Tr : C * Tr / scatPdf);  // the phase value is missing as written
   }
@@ -127,10 +130,12 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
     vm.lam[0] = vm.lam[1] = vm.lam[2] = vm.lam[3] = 0.f;
   }
   vm.woLy = 0.f;
+  vm.eta = rd4.w;
+  vm.medium = 0;
   Hit h;
   if (hit) ProcessHit(sc, ro, rd, hr.x, hr.y, hr.z, bi, corrected, h);
-  vm.u = hit ? h.u : 0.f;
-  vm.v = hit ? h.v : 0.f;
+  if (hit) VmAtHit(vm, h.u, h.v, h.point, h.normalS);
+  else VmAtHit(vm, 0.f, 0.f, v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f));
   bool alive = true;
 
   // ---- 1. light reached by the previous vertex's sampled direction
@@ -141,8 +146,11 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
     uint32_t tbi = bi;
     Spec Tr = S(1.f);
     if ((flags & kFlagTrMIS) && hit) si = IntersectTr(sc, ro, rd, med, -1.f, &Tr, vm, corrected, tc, th, tbi, nullptr, true);
-    const Spec C = MisLight(sc, aux.y, mis.y, ToV3(ps.prev_p[i]), ToV3(ps.prev_n[i]), si, th, tbi, rd, mis.x,
-                            (flags & kFlagSpecular) != 0, false, vm, corrected);
+    const float4 pn = ps.prev_n[i];
+    VmCtx vmm = vm;
+    vmm.medium = pn.w != 0.f;  // looked up for a medium interaction
+    const Spec C = MisLight(sc, aux.y, mis.y, ToV3(ps.prev_p[i]), ToV3(pn), si, th, tbi, rd, mis.x,
+                            (flags & kFlagSpecular) != 0, false, vmm, corrected);
     Lacc = Lacc + beta * (C * Tr * mis.w);
     if ((flags & kFlagEmitOnly) || !hit) alive = false;
   } else if (!(flags & kFlagCamera) && !hit) {
@@ -189,6 +197,8 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
           medInter = true;
           const V3 sp = ro + rd * ds.t;
           h.point = sp;
+          vm.medium = 1;
+          vm.px = sp.x; vm.py = sp.y; vm.pz = sp.z;
           const V3 refP = sp, refN = corrected ? v3(0, 0, 0) : h.normalS;
           float lightDistPdf = 1.f;
           const uint32_t pick = SampleLight(sc, refP, refN, rng, &lightDistPdf, treeLds, corrected);
@@ -222,6 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
             out.pick = pick;
             out.mis = make_float4(scatPdf, lightDistPdf, 0.f, corrected ? 1.f : 1.f / scatPdf);
             out.prevP = refP; out.prevN = refN;
+            out.medInter = true;
             out.med = med;
             push = true;
             rrCheck = true;

@@ -358,6 +358,82 @@ def media_camera(width=1024, height=1024):
     return camera_pinhole((0.0, 0.4, 3.8), width, height, math.pi, 0.0, 0.6911)
 
 
+def _quad_uv(p0, p1, p2, p3):
+    v, i = _quad(p0, p1, p2, p3)
+    return v, i, np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float32)
+
+
+def procedural_textures(seed=7):
+    """Texture set of the procedural-material scene: 0 RGBA (alpha ramp), 1 RGB spectra,
+    2 a 41-sample SPD (380-780 nm)."""
+    rng = np.random.default_rng(seed)
+    rgba = rng.uniform(0.1, 0.9, size=(16, 16, 4)).astype(np.float32)
+    rgba[..., 3] = np.linspace(0.05, 0.95, 16)[None, :]
+    rgb = rng.uniform(0.05, 0.8, size=(8, 8, 3)).astype(np.float32)
+    lam = np.linspace(380, 780, 41)
+    spd = (0.6 + 0.4 * np.sin(lam / 37.0)).astype(np.float32).reshape(1, 41)
+    return rgba, rgb, spd
+
+
+def procedural_materials(sd):
+    """Materials built from every procedural, converter and input node (GraphTexture.cpp,
+    GraphConverters.cpp, GraphMaths.cpp, GraphInputs.cpp). Returns the material ids
+    (floor, back, left, right, tall, short, emitter_spd, emitter_voronoi)."""
+    rgba, rgb, spd = procedural_textures()
+    t_rgba, t_rgb, t_spd = sd.add_texture(rgba), sd.add_texture(rgb), sd.add_texture(spd)
+    N = abi
+    floor = sd.add_material(Material([
+        node(N.NODE_VORONOI, p=[6]), node(N.NODE_OCTAVE_NOISE, p=[2, 3, N.NODE_VORONOI, 3]),
+        node(N.NODE_PERLIN, p=[5]), node(N.NODE_VALUE_NOISE, p=[7]),
+        node(N.NODE_MERGE_RGB, [2, 3, 1]), node(N.NODE_LAMBERTIAN, [4])], bxdf=5))
+    back = sd.add_material(Material([
+        node(N.NODE_SCALAR, p=[2.333]), node(N.NODE_FRESNEL, [0]), node(N.NODE_RGB, p=[.7, .7, .7]),
+        node(N.NODE_LAMBERTIAN, [2]), node(N.NODE_SPECULAR_BRDF, [2]), node(N.NODE_MIX_BSDF, [3, 4, 1])], bxdf=5))
+    left = sd.add_material(Material([
+        node(N.NODE_NORMAL), node(N.NODE_VEC3, p=[.25, .25, .25]), node(N.NODE_VECTOR_MATHS, [0, 1], p=[N.OP_MULTIPLY]),
+        node(N.NODE_VEC3, p=[.5, .5, .5]), node(N.NODE_VECTOR_MATHS, [2, 3], p=[N.OP_ADD]),
+        node(N.NODE_VEC3, p=[0, 0, 1]), node(N.NODE_CROSS, [0, 5]), node(N.NODE_LENGTH, [6]),
+        node(N.NODE_SEPARATE_XYZ, [4], p=[0]), node(N.NODE_SCALAR, p=[.5]),
+        node(N.NODE_SCALAR_MATHS, [7, 9], p=[N.OP_MULTIPLY]), node(N.NODE_MERGE_RGB, [8, 10, 9, 9]),
+        node(N.NODE_LAMBERTIAN, [11])], bxdf=12))
+    right = sd.add_material(Material([
+        node(N.NODE_IMAGE_CHANNEL, p=[t_rgba, 3]), node(N.NODE_SCALAR_TO_COLOUR, [0]),
+        node(N.NODE_IMAGE_COLOUR, p=[t_rgba]), node(N.NODE_SEPARATE_R, [2], p=[3]), node(N.NODE_SCALAR, p=[.35]),
+        node(N.NODE_OREN_NAYAR_BRDF, [1, 4]), node(N.NODE_LAMBERTIAN, [2]), node(N.NODE_MIX_BSDF, [5, 6, 3])], bxdf=7))
+    tall = sd.add_material(Material([node(N.NODE_SPECTRAL_TEXTURE, p=[t_rgb]), node(N.NODE_LAMBERTIAN, [0])], bxdf=1))
+    short = sd.add_material(Material([
+        node(N.NODE_CHECKER, p=[8]), node(N.NODE_SCALAR_TO_COLOUR, [0]), node(N.NODE_LAMBERTIAN, [1])], bxdf=2))
+    e_spd = sd.add_material(Material([node(N.NODE_SPECTRAL, p=[t_spd, 380, 780])], emission=0, intensity=8.0))
+    e_vor = sd.add_material(Material([node(N.NODE_VORONOI, p=[4]), node(N.NODE_CHECKER, p=[3]),
+                                      node(N.NODE_SCALAR_MATHS, [0, 1], p=[N.OP_ADD])], emission=2, intensity=4.0))
+    return floor, back, left, right, tall, short, e_spd, e_vor
+
+
+def procedural_cornell():
+    """Cornell room whose surfaces (with uvs) use the procedural, converter and input nodes: a
+    ceiling quad light with a sampled-SPD emission and a small back-wall emitter with a
+    Voronoi + checker emission."""
+    sd = SceneDesc()
+    floor, back, left, right, tall, short, e_spd, e_vor = procedural_materials(sd)
+    for quad, mat in [(((-1, -1, -1), (-1, -1, 1), (1, -1, 1), (1, -1, -1)), floor),
+                      (((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1)), floor),
+                      (((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)), back),
+                      (((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1)), left),
+                      (((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1)), right)]:
+        v, i, uv = _quad_uv(*quad)
+        sd.add_mesh(v, i, mat, uvs=uv)
+    for (c, size, rot), mat in [(((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 15.0), tall),
+                                (((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -18.0), short)]:
+        v, i = _box(c, size, rot)
+        uv = np.stack([v[:, 0] + v[:, 2], v[:, 1]], axis=1).astype(np.float32)
+        sd.add_mesh(v, i, mat, uvs=uv)
+    v, i, uv = _quad_uv((-0.25, 0.999, -0.25), (0.25, 0.999, -0.25), (0.25, 0.999, 0.25), (-0.25, 0.999, 0.25))
+    sd.add_mesh(v, i, e_spd, uvs=uv)
+    v, i, uv = _quad_uv((-0.6, 0.2, -0.999), (-0.2, 0.2, -0.999), (-0.2, 0.6, -0.999), (-0.6, 0.6, -0.999))
+    sd.add_mesh(v, i, e_vor, uvs=uv)
+    return sd
+
+
 def cornell_camera(width=256, height=256):
     """Pinhole at (0,0,3.8), SetRotation(pi, 0) (looking down -z), SetFov(0.6911)."""
     return camera_pinhole((0.0, 0.0, 3.8), width, height, math.pi, 0.0, 0.6911)

@@ -66,6 +66,7 @@ def lib():
         L.oracle_light_sample.argtypes = [vp, vp, vp, C.c_int, u32, u32, vp]
         L.oracle_hg_sample.argtypes = [f32, vp, f32, f32, C.c_int, vp]
         L.oracle_hg_sample.restype = f32
+        L.oracle_eval_node.argtypes = [vp, u32, u32, vp, C.c_int, vp, vp]
         L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
         L.oracle_tree_importance.restype = f32
         _LIB = L
@@ -149,6 +150,17 @@ class OracleScene:
         p, n = _v3(p), _v3(n)
         idx = self.L.oracle_light_sample(self.h, _ptr(p), _ptr(n), int(corrected), pixel, sample, _ptr(pdf))
         return idx, float(pdf[0])
+
+    def eval_node(self, material, node, uv=(0, 0), point=(0, 0, 0), normal=(0, 1, 0), eta=1.0, wo_y=1.0,
+                  medium=False, mode=0, lambdas=None):
+        """Value of one shader node at an event: (type, 4 floats)."""
+        ev = np.array([*uv, *point, *normal, eta, wo_y, 1.0 if medium else 0.0], dtype=np.float32)
+        out = np.zeros(4, dtype=np.float32)
+        lam = np.asarray(lambdas if lambdas is not None else (0, 0, 0, 0), dtype=np.float32)
+        t = self.L.oracle_eval_node(self.h, material, node, _ptr(ev), mode, _ptr(lam), _ptr(out))
+        if t < 0:
+            raise ValueError("eval_node: bad material or node")
+        return t, out
 
     def occluded(self, rays):
         rays = np.ascontiguousarray(rays, dtype=np.float32)

@@ -197,12 +197,7 @@ static float BlackbodyNormalised(float lambdaNm, float T) {
 }
 // BlackbodyInput::MakeBlackbodySpectrum with Spectrum::nSamples = 3 (GraphInputs.cpp:106-116)
 // then RGBSpectrum::FromSampled (Spectrum.h:353-373) over 471 CIE wavelengths.
-static void BlackbodyRGB(float T, float out[3]) {
-  const unsigned n = 3;
-  const float invSamples = 1.f / (float)n;
-  float lam[3], v[3];
-  for (unsigned i = 0; i < n; ++i) lam[i] = 400.f + 300.f * invSamples * (float)i;
-  for (unsigned i = 0; i < n; ++i) v[i] = BlackbodyNormalised(lam[i], T);
+static void SampledToRGB(const float *lam, const float *v, unsigned n, float out[3]) {
   float xyz[3] = {0, 0, 0};
   for (unsigned i = 0; i < 471; ++i) {
     const float li = 360.f + (float)i;
@@ -226,6 +221,14 @@ static void BlackbodyRGB(float T, float out[3]) {
   out[0] = (float)(3.240479 * xyz[0] - 1.537150 * xyz[1] - 0.498535 * xyz[2]);
   out[1] = (float)(-0.969256 * xyz[0] + 1.875991 * xyz[1] + 0.041556 * xyz[2]);
   out[2] = (float)(0.055648 * xyz[0] - 0.204043 * xyz[1] + 1.057311 * xyz[2]);
+}
+static void BlackbodyRGB(float T, float out[3]) {
+  const unsigned n = 3;
+  const float invSamples = 1.f / (float)n;
+  float lam[3], v[3];
+  for (unsigned i = 0; i < n; ++i) lam[i] = 400.f + 300.f * invSamples * (float)i;
+  for (unsigned i = 0; i < n; ++i) v[i] = BlackbodyNormalised(lam[i], T);
+  SampledToRGB(lam, v, n, out);
 }
 
 // ---------------------------------------------------------------- Philox4x32-10 sampler
@@ -345,6 +348,22 @@ struct Texture {
       const float yb = p01 + (p11 - p01) * tx;
       out[c] = ya + (yb - ya) * ty;
     }
+  }
+  // The same lookup with alpha: channel 3 of an RGBA texture, 1 for 1- and 3-channel textures.
+  void Bilinear4(float u, float v, float out[4]) const {
+    Bilinear(u, v, out);
+    out[3] = 1.f;
+    if (ch != 4) return;
+    const float fx = std::min((float)(w - 1), Clampf(u, 0.f, 1.f) * (float)w);
+    const float fy = std::min((float)(h - 1), Clampf(v, 0.f, 1.f) * (float)h);
+    const unsigned x = (unsigned)fx, y = (unsigned)fy;
+    const unsigned x1 = std::min(x + 1, w - 1), y1 = std::min(y + 1, h - 1);
+    const float tx = fx - (float)x, ty = fy - (float)y;
+    const float p00 = data[(y * w + x) * 4 + 3], p10 = data[(y * w + x1) * 4 + 3];
+    const float p01 = data[(y1 * w + x) * 4 + 3], p11 = data[(y1 * w + x1) * 4 + 3];
+    const float ya = p00 + (p10 - p00) * tx;
+    const float yb = p01 + (p11 - p01) * tx;
+    out[3] = ya + (yb - ya) * ty;
   }
 };
 
@@ -717,6 +736,138 @@ static Spec AsSpectrum(const Val &x, const Lanes &l) {  // ShaderGraph.h:232-251
   }
   return Spec(0);
 }
+// Alpha of a value read as a Colour: its own for colours, 1 for every conversion (Colour.h:22-33).
+static float AlphaOf(const Val &x) { return x.t == VT_COLOUR ? x.v[3] : 1.f; }
+static V2 AsVec2(const Val &x) {  // GetAs<Vec2> (ShaderGraph.h:176-199)
+  switch (x.t) {
+    case VT_VEC2: return V2(x.v[0], x.v[1]);
+    case VT_VEC3: return V2(x.v[0], x.v[1]);
+    case VT_SCALAR: return V2(x.v[0], x.v[0]);
+  }
+  return V2(0, 0);
+}
+
+// ---- procedural textures (GraphTexture.cpp:6-240). The hashes take unsigned arguments: a float
+// lattice coordinate converts as x86-64 compiles float -> unsigned (through a 64-bit signed
+// integer, so negative values wrap modulo 2^32).
+static inline uint32_t LatticeU32(float f) { return (uint32_t)(int64_t)f; }
+static inline uint32_t Hash21(uint32_t x, uint32_t y) {  // :25-29
+  x *= 1597334677u;
+  y *= 3812015801u;
+  return (x ^ y) * 1597334677u;
+}
+static inline float HashUnit(uint32_t h) { return (float)((double)h * (1. / 4294967296.)); }  // Hash * inv32
+static inline V2 Hash22(V2 v) {  // :31-34
+  const float a = v.x * 127.1f + v.y * 311.7f, b = v.x * 269.5f + v.y * 183.3f;
+  const float sa = std::sin(a) * 18.5453f, sb = std::sin(b) * 18.5453f;
+  return V2(sa - std::floor(sa), sb - std::floor(sb));
+}
+static inline V3 Hash33(V3 p) {  // :37-41
+  p = V3(p.x * .1031f, p.y * .1030f, p.z * .0973f);
+  p = V3(p.x - std::floor(p.x), p.y - std::floor(p.y), p.z - std::floor(p.z));
+  const float d = (p.x * (p.y + 33.33f) + p.y * (p.x + 33.33f)) + p.z * (p.z + 33.33f);
+  p = V3(p.x + d, p.y + d, p.z + d);
+  const V3 r((p.x + p.y) * p.z, (p.x + p.x) * p.y, (p.y + p.x) * p.z);
+  return V3(r.x - std::floor(r.x), r.y - std::floor(r.y), r.z - std::floor(r.z));
+}
+static inline float Quintic(float x) { return x * x * x * (x * (x * 6.f - 15.f) + 10.f); }
+static inline float CubicS(float x) { return x * x * (3 - 2 * x); }
+static inline float LerpF(float a, float b, float r) { return a + (b - a) * r; }
+
+static float Perlin2D(V2 t) {  // :61-79 (t already scaled)
+  const float gx = std::floor(t.x), gy = std::floor(t.y);
+  auto grad = [&](float px, float py) {
+    float u = HashUnit(Hash21(LatticeU32(px), LatticeU32(py)));
+    u *= PI2;
+    return V2(std::cos(u), std::sin(u));
+  };
+  const V2 g00 = grad(gx, gy), g10 = grad(gx + 1, gy), g11 = grad(gx + 1, gy + 1), g01 = grad(gx, gy + 1);
+  const float dg00 = (t.x - gx) * g00.x + (t.y - gy) * g00.y;
+  const float dg10 = (t.x - (gx + 1)) * g10.x + (t.y - gy) * g10.y;
+  const float dg11 = (t.x - (gx + 1)) * g11.x + (t.y - (gy + 1)) * g11.y;
+  const float dg01 = (t.x - gx) * g01.x + (t.y - (gy + 1)) * g01.y;
+  const float vx0 = LerpF(dg00, dg10, Quintic(t.x - gx));
+  const float vx1 = LerpF(dg01, dg11, Quintic(t.x - gx));
+  return LerpF(vx0, vx1, Quintic(t.y - gy)) + .5f;
+}
+static float Value2D(V2 t) {  // :126-138
+  const float x0 = std::floor(t.x), y0 = std::floor(t.y);
+  const V2 p(t.x - x0, t.y - y0);
+  const float v00 = HashUnit(Hash21(LatticeU32(x0), LatticeU32(y0)));
+  const float v10 = HashUnit(Hash21(LatticeU32(x0 + 1), LatticeU32(y0)));
+  const float v11 = HashUnit(Hash21(LatticeU32(x0 + 1), LatticeU32(y0 + 1)));
+  const float v01 = HashUnit(Hash21(LatticeU32(x0), LatticeU32(y0 + 1)));
+  const float vx0 = LerpF(v00, v10, CubicS(p.x));
+  const float vx1 = LerpF(v01, v11, CubicS(p.x));
+  return LerpF(vx0, vx1, CubicS(p.y));
+}
+static float Checker2D(V2 t) {  // :105-108 (t already scaled)
+  const float u = t.x - std::floor(t.x), v = t.y - std::floor(t.y);
+  return (u > .5f) == (v > .5f) ? 0.f : 1.f;
+}
+static float Checker3D(V3 t) {  // :110-113: ((x > .5) == (y > .5)) == (z > .5)
+  const float x = t.x - std::floor(t.x), y = t.y - std::floor(t.y), z = t.z - std::floor(t.z);
+  return ((x > .5f) == (y > .5f)) == (z > .5f) ? 0.f : 1.f;
+}
+static float Voronoi2D(V2 t) {  // :187-201
+  const float x0 = std::floor(t.x), y0 = std::floor(t.y);
+  float m = INFINITY;
+  for (int y = -1; y < 2; ++y)
+    for (int x = -1; x < 2; ++x) {
+      const V2 o(x0 + (float)x, y0 + (float)y);
+      const V2 h = Hash22(o);
+      const float rx = (o.x + h.x) - t.x, ry = (o.y + h.y) - t.y;
+      const float d2 = rx * rx + ry * ry;
+      if (d2 < m) m = d2;
+    }
+  return std::sqrt(m);
+}
+static float Voronoi3D(V3 t) {  // :203-219
+  const V3 p0(std::floor(t.x), std::floor(t.y), std::floor(t.z));
+  float m = INFINITY;
+  for (int z = -1; z < 2; ++z)
+    for (int y = -1; y < 2; ++y)
+      for (int x = -1; x < 2; ++x) {
+        const V3 o(p0.x + (float)x, p0.y + (float)y, p0.z + (float)z);
+        const V3 h = Hash33(o);
+        const V3 r((o.x + h.x) - t.x, (o.y + h.y) - t.y, (o.z + h.z) - t.z);
+        const float d2 = (r.x * r.x + r.y * r.y) + r.z * r.z;
+        if (d2 < m) m = d2;
+      }
+  return std::sqrt(m);
+}
+// One noise lookup: texture type, coordinates scaled by s0 then s1 then s2 (the octave node's
+// scale, octave factor and the base node's scale). At a medium interaction the 3-D variants
+// are used on the event point: Perlin and value noise return 0 there as written (:81-84, :140-142).
+static float NoiseAt(uint32_t type, const Event &ev, float s0, float s1, float s2) {
+  if (ev.mediumInteraction) {
+    const V3 &p = ev.hit->point;
+    const V3 t(p.x * s0 * s1 * s2, p.y * s0 * s1 * s2, p.z * s0 * s1 * s2);
+    if (type == LGPU_NODE_CHECKER) return Checker3D(t);
+    if (type == LGPU_NODE_VORONOI) return Voronoi3D(t);
+    return 0.f;
+  }
+  const V2 t(ev.hit->uv.x * s0 * s1 * s2, ev.hit->uv.y * s0 * s1 * s2);
+  switch (type) {
+    case LGPU_NODE_CHECKER: return Checker2D(t);
+    case LGPU_NODE_PERLIN: return Perlin2D(t);
+    case LGPU_NODE_VALUE_NOISE: return Value2D(t);
+    default: return Voronoi2D(t);
+  }
+}
+// RGBSpectrum::FromSampled (Spectrum.h:353-373) of n samples at increasing wavelengths.
+static void SampledToRGB(const float *lam, const float *v, unsigned n, float out[3]);
+static float SpdAt(const Texture &t, float l0, float l1, float lambda) {  // InterpolateSpectrumSamples
+  const unsigned n = t.w * t.h;
+  const float step = (l1 - l0) / (float)(n - 1);
+  if (lambda <= l0) return t.data[0];
+  if (lambda >= l1) return t.data[(n - 1) * t.ch];
+  unsigned k = (unsigned)((lambda - l0) / step);
+  if (k > n - 2) k = n - 2;
+  const float a = l0 + step * (float)k;
+  const float f = (lambda - a) / step;
+  return t.data[k * t.ch] + (t.data[(k + 1) * t.ch] - t.data[k * t.ch]) * f;
+}
 static Val Input(const Scene &sc, const Material &m, uint32_t node, int k, const Event &ev) {
   const uint32_t j = m.nodes[node].in[k];
   if (j == LGPU_NONE || j >= m.nodes.size()) return Val();
@@ -752,12 +903,90 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
       const Texture &t = sc.textures[(uint32_t)n.p[0]];
       float c[3];
       t.Bilinear(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c);
-      if (n.type == LGPU_NODE_IMAGE_COLOUR) { r.t = VT_COLOUR; r.v[0] = c[0]; r.v[1] = c[1]; r.v[2] = c[2]; }
+      if (n.type == LGPU_NODE_IMAGE_COLOUR) {
+        float c4[4];
+        t.Bilinear4(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c4);
+        r.t = VT_COLOUR; r.v[0] = c[0]; r.v[1] = c[1]; r.v[2] = c[2]; r.v[3] = c4[3];
+      }
       else { r.t = VT_SCALAR; r.v[0] = c[0]; }
       break;
     }
     case LGPU_NODE_UV: r.t = VT_VEC2; r.v[0] = ev.hit->uv.x; r.v[1] = ev.hit->uv.y; break;
-    case LGPU_NODE_FRESNEL: r.t = VT_SCALAR; r.v[0] = FrDielectric(ev.woL.y, 1.f, n.p[0]); break;
+    case LGPU_NODE_FRESNEL: {  // FresnelInput::SchlickApprox (GraphInputs.cpp:187-199)
+      const float n1 = ev.eta;
+      const float n2 = n.in[0] == LGPU_NONE ? n.p[0] : AsReal(Input(sc, m, i, 0, ev), ev.lanes);
+      float R0 = (n1 - n2) / (n1 + n2);
+      R0 *= R0;
+      const float cc = 1 - ev.woL.y;
+      r.t = VT_SCALAR; r.v[0] = R0 + (1 - R0) * cc * cc * cc * cc * cc;
+      break;
+    }
+    case LGPU_NODE_SPECTRAL: {  // SpectralInput (GraphInputs.cpp:52-66): a sampled SPD
+      const Texture &t = sc.textures[(uint32_t)n.p[0]];
+      r.t = VT_SPECTRUM;
+      if (ev.lanes.mode == HERO4) {
+        for (int k = 0; k < 4; ++k) r.s.c[k] = SpdAt(t, n.p[1], n.p[2], ev.lanes.lambda[k]);
+      } else {
+        const unsigned cnt = t.w * t.h;
+        std::vector<float> lam(cnt), v(cnt);
+        for (unsigned k = 0; k < cnt; ++k) { lam[k] = n.p[1] + (n.p[2] - n.p[1]) / (float)(cnt - 1) * (float)k; v[k] = t.data[k * t.ch]; }
+        float rgb[3];
+        SampledToRGB(lam.data(), v.data(), cnt, rgb);
+        r.s.c[0] = rgb[0]; r.s.c[1] = rgb[1]; r.s.c[2] = rgb[2]; r.s.c[3] = 0;
+      }
+      break;
+    }
+    case LGPU_NODE_IMAGE_CHANNEL: {  // ImageTextureChannelInput (GraphInputs.cpp:150-160)
+      const Texture &t = sc.textures[(uint32_t)n.p[0]];
+      float c[4];
+      t.Bilinear4(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c);
+      r.t = VT_SCALAR; r.v[0] = c[std::min(std::max((int)n.p[1], 0), 3)];
+      break;
+    }
+    case LGPU_NODE_SPECTRAL_TEXTURE: {  // SpectralTextureInput (GraphInputs.cpp:162-171): RGB spectra
+      const Texture &t = sc.textures[(uint32_t)n.p[0]];
+      float c[3];
+      t.Bilinear(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c);
+      r.t = VT_SPECTRUM; r.s = FromRGB(c, ev.lanes);
+      break;
+    }
+    case LGPU_NODE_NORMAL:  // SurfaceInfoInput (GraphInputs.cpp:173-181): shading normal
+      r.t = VT_VEC3; r.v[0] = ev.hit->normalS.x; r.v[1] = ev.hit->normalS.y; r.v[2] = ev.hit->normalS.z;
+      break;
+    case LGPU_NODE_CROSS: {  // CrossProductNode (GraphMaths.cpp:77-86)
+      const V3 o = Cross(AsVec3(Input(sc, m, i, 0, ev)), AsVec3(Input(sc, m, i, 1, ev)));
+      r.t = VT_VEC3; r.v[0] = o.x; r.v[1] = o.y; r.v[2] = o.z;
+      break;
+    }
+    case LGPU_NODE_SEPARATE_XYZ: {  // SeparateXYZ (GraphConverters.cpp:8-28)
+      const V3 v = AsVec3(Input(sc, m, i, 0, ev));
+      const int k = std::min(std::max((int)n.p[0], 0), 2);
+      r.t = VT_SCALAR; r.v[0] = k == 0 ? v.x : (k == 1 ? v.y : v.z);
+      break;
+    }
+    case LGPU_NODE_MERGE_XYZ:  // MergeXYZ (GraphConverters.cpp:63-75)
+      r.t = VT_VEC3;
+      for (int k = 0; k < 3; ++k) r.v[k] = AsReal(Input(sc, m, i, k, ev), ev.lanes);
+      break;
+    case LGPU_NODE_SCALAR_TO_COLOUR: {  // ScalarToColour (GraphConverters.cpp:95-103)
+      const float x = AsReal(Input(sc, m, i, 0, ev), ev.lanes);
+      r.t = VT_COLOUR; r.v[0] = r.v[1] = r.v[2] = x; r.v[3] = 1;
+      break;
+    }
+    case LGPU_NODE_PERLIN: case LGPU_NODE_VALUE_NOISE: case LGPU_NODE_VORONOI:
+      r.t = VT_SCALAR; r.v[0] = NoiseAt(n.type, ev, n.p[0], 1.f, 1.f);
+      break;
+    case LGPU_NODE_OCTAVE_NOISE: {  // OctaveNoise (GraphTexture.cpp:156-174)
+      float out = 0, amp = .5f, scl = 1;
+      const unsigned oct = (unsigned)n.p[1];
+      for (unsigned k = 0; k < oct; ++k) {
+        out += NoiseAt((uint32_t)n.p[2], ev, n.p[0], scl, n.p[3]) * amp;
+        scl *= 2;
+        amp *= .5f;
+      }
+      r.t = VT_SCALAR; r.v[0] = out;
+      break;
+    }
     case LGPU_NODE_SCALAR_MATHS: {  // GraphMaths.cpp:17-30
       const float a = AsReal(Input(sc, m, i, 0, ev), ev.lanes), b = AsReal(Input(sc, m, i, 1, ev), ev.lanes);
       r.t = VT_SCALAR;
@@ -783,20 +1012,21 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
     }
     case LGPU_NODE_DOT: r.t = VT_SCALAR; r.v[0] = Dot(AsVec3(Input(sc, m, i, 0, ev)), AsVec3(Input(sc, m, i, 1, ev))); break;
     case LGPU_NODE_LENGTH: r.t = VT_SCALAR; r.v[0] = Mag(AsVec3(Input(sc, m, i, 0, ev))); break;
-    case LGPU_NODE_MERGE_RGB:
+    case LGPU_NODE_MERGE_RGB:  // MergeRGBA (GraphConverters.cpp:78-92); alpha 1 when unconnected
       r.t = VT_COLOUR;
       for (int k = 0; k < 3; ++k) r.v[k] = AsReal(Input(sc, m, i, k, ev), ev.lanes);
-      r.v[3] = 1;
+      r.v[3] = n.in[3] == LGPU_NONE ? 1.f : AsReal(Input(sc, m, i, 3, ev), ev.lanes);
       break;
-    case LGPU_NODE_SEPARATE_R: {
-      float c[3];
-      AsColour(Input(sc, m, i, 0, ev), c);
-      r.t = VT_SCALAR; r.v[0] = c[std::min(std::max((int)n.p[0], 0), 2)];
+    case LGPU_NODE_SEPARATE_R: {  // SeparateRGBA (GraphConverters.cpp:31-60)
+      const Val x = Input(sc, m, i, 0, ev);
+      float c[4];
+      AsColour(x, c);
+      c[3] = AlphaOf(x);
+      r.t = VT_SCALAR; r.v[0] = c[std::min(std::max((int)n.p[0], 0), 3)];
       break;
     }
-    case LGPU_NODE_CHECKER: {  // GraphTexture.cpp:100-108 (surface branch)
-      const float u = Fract(ev.hit->uv.x * n.p[0]), v = Fract(ev.hit->uv.y * n.p[0]);
-      r.t = VT_SCALAR; r.v[0] = (u > .5f) == (v > .5f) ? 0.f : 1.f;
+    case LGPU_NODE_CHECKER: {  // GraphTexture.cpp:95-113
+      r.t = VT_SCALAR; r.v[0] = NoiseAt(LGPU_NODE_CHECKER, ev, n.p[0], 1.f, 1.f);
       break;
     }
     default: r.t = VT_BXDF; break;
@@ -2640,7 +2870,7 @@ oracle_scene oracle_scene_create(void) { return new oracle_scene_s(); }
 void oracle_scene_destroy(oracle_scene s) { delete s; }
 
 int oracle_scene_add_texture(oracle_scene s, const lgpu_texture_desc *d) {
-  if (!s || !d || !d->data || !d->width || !d->height || (d->channels != 1 && d->channels != 3)) return -1;
+  if (!s || !d || !d->data || !d->width || !d->height || (d->channels != 1 && d->channels != 3 && d->channels != 4)) return -1;
   Texture t;
   t.w = d->width; t.h = d->height; t.ch = d->channels;
   t.data.assign(d->data, d->data + (size_t)d->width * d->height * d->channels);
@@ -2668,6 +2898,20 @@ int oracle_scene_add_material(oracle_scene s, const lgpu_material_desc *d) {
   m.nodes.assign(d->nodes, d->nodes + d->num_nodes);
   m.bxdf = d->bxdf_root; m.emission = d->emission_root; m.intensity = d->emission_intensity;
   m.medIn = d->medium_interior; m.medOut = d->medium_exterior;
+  for (const lgpu_node &n : m.nodes) {  // texture references and noise parameters
+    const bool tex = n.type == LGPU_NODE_IMAGE_COLOUR || n.type == LGPU_NODE_IMAGE_SCALAR || n.type == LGPU_NODE_SPECTRAL ||
+                     n.type == LGPU_NODE_IMAGE_CHANNEL || n.type == LGPU_NODE_SPECTRAL_TEXTURE;
+    if (tex && !(n.p[0] >= 0.f && (size_t)n.p[0] < s->textures.size())) return -1;
+    if (n.type == LGPU_NODE_SPECTRAL) {
+      const Texture &t = s->textures[(size_t)n.p[0]];
+      if (t.w * t.h < 2 || !(n.p[2] > n.p[1])) return -1;
+    }
+    if (n.type == LGPU_NODE_OCTAVE_NOISE) {
+      const uint32_t b = (uint32_t)n.p[2];
+      if (!(n.p[1] >= 0.f && n.p[1] <= 32.f)) return -1;
+      if (b != LGPU_NODE_CHECKER && b != LGPU_NODE_PERLIN && b != LGPU_NODE_VALUE_NOISE && b != LGPU_NODE_VORONOI) return -1;
+    }
+  }
   m.bbRGB.assign(3 * m.nodes.size(), 0.f);
   m.bbConst.assign(m.nodes.size(), 0);
   for (size_t i = 0; i < m.nodes.size(); ++i) {  // constant-temperature blackbody: evaluate once
@@ -2961,6 +3205,26 @@ static Lanes MkLanes(int mode, const float *lambda) {
   l.mode = mode;
   for (int i = 0; i < 4; ++i) l.lambda[i] = lambda ? lambda[i] : 0.f;
   return l;
+}
+// One shader-node evaluation (test hook). ev[11] = {u, v, point xyz, shading normal xyz, eta,
+// woL.y, medium interaction (0/1)}; out[4] receives the value (a spectrum's lanes, or the
+// components of a scalar / vec / colour). Returns the value type (VT_*) or -1.
+int oracle_eval_node(oracle_scene s, uint32_t material, uint32_t node, const float ev[11], int mode, const float *lambda,
+                     float out[4]) {
+  if (!s || material >= s->materials.size() || node >= s->materials[material].nodes.size()) return -1;
+  Hit h;
+  h.uv = V2(ev[0], ev[1]);
+  h.point = V3(ev[2], ev[3], ev[4]);
+  h.normalS = V3(ev[5], ev[6], ev[7]);
+  Event e;
+  e.hit = &h;
+  e.eta = ev[8];
+  e.woL = V3(0, ev[9], 0);
+  e.mediumInteraction = ev[10] != 0.f;
+  e.lanes = MkLanes(mode, lambda);
+  const Val v = EvalNode(*s, s->materials[material], node, e);
+  for (int i = 0; i < 4; ++i) out[i] = v.t == VT_SPECTRUM ? v.s.c[i] : v.v[i];
+  return v.t;
 }
 int oracle_medium_tr(oracle_scene s, uint32_t med, const float o[3], const float d[3], float tFar, int mode,
                      const float *lambda, float out[4]) {

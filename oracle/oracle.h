@@ -74,6 +74,8 @@ int oracle_light_pdfs(oracle_scene s, const float p[3], const float n[3], int co
 int oracle_light_sample(oracle_scene s, const float p[3], const float n[3], int corrected, uint32_t pixel,
                         uint32_t sample, float *pdf);
 /* HenyeyGreenstein::Sample_p (HenyeyGreenstein.cpp:21-36) with explicit u; returns pdf. */
+int oracle_eval_node(oracle_scene s, uint32_t material, uint32_t node, const float ev[11], int mode, const float *lambda,
+                     float out[4]);
 float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected);
 float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]);
 
