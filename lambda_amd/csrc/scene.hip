@@ -497,6 +497,9 @@ struct Compiler {
       }
       default: err = "node type cannot feed a socket"; break;
     }
+    // The inputs' registers are dead once this node's value is written: a node keeps one
+    // register, so a program needs about (depth x fan-in) registers, not one per node.
+    nextReg = dst + 1;
     return dst;
   }
   // XYZ->RGB weights of the 3-sample blackbody spectrum (400/500/600 nm) for RGB mode.
