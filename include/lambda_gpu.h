@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 2u
+#define LGPU_ABI_VERSION 3u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -214,7 +214,10 @@ typedef enum lgpu_integrator {
   LGPU_INTEGRATOR_PATH = 0,          /* PathIntegrator.cpp:15-88 */
   LGPU_INTEGRATOR_DIRECT = 1,        /* DirectLightingIntegrator.cpp:13-29 */
   LGPU_INTEGRATOR_VOLPATH = 2,       /* VolumetricPathIntegrator.cpp:20-162 */
-  LGPU_INTEGRATOR_MIS_VOLPATH = 3    /* MISVolumetricPathIntegrator.cpp:71-215 */
+  LGPU_INTEGRATOR_MIS_VOLPATH = 3,   /* MISVolumetricPathIntegrator.cpp:71-215 */
+  LGPU_INTEGRATOR_DEPTH = 4,         /* DepthPass::P (UtilityIntegrators.cpp:26-33) */
+  LGPU_INTEGRATOR_NORMAL = 5,        /* NormalPass::P (UtilityIntegrators.cpp:46-53) */
+  LGPU_INTEGRATOR_AOV = 6            /* AOVPass::P (UtilityIntegrators.cpp:66-77) */
 } lgpu_integrator;
 
 typedef enum lgpu_spectral_mode {
@@ -241,6 +244,12 @@ typedef struct lgpu_render_desc {
   uint32_t seed;            /* Philox key word 0 */
   uint32_t path_capacity;   /* paths in flight; 0 = choose */
   uint32_t flags;           /* LGPU_RENDER_COUNT_TRAVERSAL: count BVH nodes / triangles visited */
+  /* utility passes (UtilityIntegrators.cpp:26-53); the reference's defaults are power 1,
+   * background (1, 0, 1), shading normals */
+  float pass_max_depth;     /* DepthPass::maxDepth (> 0) */
+  float pass_power;         /* DepthPass::power */
+  float pass_background[3]; /* DepthPass::background, returned on a miss */
+  uint32_t pass_geometric_normal; /* NormalPass::normalG */
 } lgpu_render_desc;
 
 #define LGPU_RENDER_COUNT_TRAVERSAL 1u

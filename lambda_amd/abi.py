@@ -5,7 +5,7 @@ binding, so both are driven from the same scene description.
 """
 import ctypes as C
 
-ABI_VERSION = 2  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
+ABI_VERSION = 3  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
 NONE = 0xFFFFFFFF
 
 # lgpu_status
@@ -32,6 +32,7 @@ LIGHTS_POWER, LIGHTS_TREE = 0, 1
 CAMERA_PINHOLE, CAMERA_THIN_LENS, CAMERA_SPHERICAL = 0, 1, 2
 APERTURE_NONE, APERTURE_CIRCULAR, APERTURE_BLADE = 0, 1, 2
 INTEGRATOR_PATH, INTEGRATOR_DIRECT, INTEGRATOR_VOLPATH, INTEGRATOR_MIS_VOLPATH = 0, 1, 2, 3
+INTEGRATOR_DEPTH, INTEGRATOR_NORMAL, INTEGRATOR_AOV = 4, 5, 6
 SPECTRAL_RGB, SPECTRAL_HERO4 = 0, 1
 ESTIMATOR_FAITHFUL, ESTIMATOR_CORRECTED = 0, 1
 
@@ -83,7 +84,9 @@ class RenderDesc(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("film", C.c_void_p), ("camera", Camera), ("integrator", C.c_uint32),
                 ("spp_begin", C.c_uint32), ("spp_end", C.c_uint32), ("max_bounces", C.c_uint32),
                 ("min_bounces", C.c_uint32), ("spectral_mode", C.c_uint32), ("estimator_mode", C.c_uint32),
-                ("seed", C.c_uint32), ("path_capacity", C.c_uint32), ("flags", C.c_uint32)]
+                ("seed", C.c_uint32), ("path_capacity", C.c_uint32), ("flags", C.c_uint32),
+                ("pass_max_depth", C.c_float), ("pass_power", C.c_float), ("pass_background", C.c_float * 3),
+                ("pass_geometric_normal", C.c_uint32)]
 
 RENDER_COUNT_TRAVERSAL = 1
 

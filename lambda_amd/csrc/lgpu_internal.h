@@ -328,11 +328,17 @@ struct RenderParams {
   uint32_t corrected;   // 0 faithful, 1 corrected
   uint32_t seed;
   uint32_t integrator;
+  float pass_max_depth, pass_power;  // utility passes
+  float pass_bg[3];
+  uint32_t pass_normal_g;
 };
 
 // Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator.
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                     SampleState ss, uint32_t n, Queues q, uint32_t *spill);
+// Utility-pass shade stage (utility.hip): DepthPass, NormalPass, AOVPass.
+void LaunchShadeUtility(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
+                        SampleState ss, uint32_t n);
 // Direct-lighting shade stage (direct.hip): DirectLightingIntegrator.
 void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                        SampleState ss, uint32_t n, Queues q, uint32_t *spill);

@@ -437,7 +437,9 @@ static inline uint32_t Blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) /
 static lgpu_status ValidateRender(const lgpu_render_desc *d) {
   if (!d || !d->scene) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: null descriptor or scene");
   if (!d->scene->committed) return Fail(LGPU_ERR_STATE, "render: scene not committed");
-  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH) return Fail(LGPU_ERR_UNSUPPORTED, "render: unknown integrator");
+  if (d->integrator > LGPU_INTEGRATOR_AOV) return Fail(LGPU_ERR_UNSUPPORTED, "render: unknown integrator");
+  if (d->integrator == LGPU_INTEGRATOR_DEPTH && !(d->pass_max_depth > 0.f))
+    return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: depth pass needs pass_max_depth > 0");
   if (d->spectral_mode > LGPU_SPECTRAL_HERO4 || d->estimator_mode > LGPU_ESTIMATOR_CORRECTED)
     return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: bad spectral or estimator mode");
   if (d->max_bounces == 0) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: max_bounces must be > 0");
@@ -463,6 +465,9 @@ static RenderParams MakeParams(const lgpu_render_desc *d, uint32_t w, uint32_t h
   rp.max_bounces = d->max_bounces; rp.min_bounces = d->min_bounces;
   rp.mode = d->spectral_mode; rp.corrected = d->estimator_mode == LGPU_ESTIMATOR_CORRECTED;
   rp.seed = d->seed; rp.integrator = d->integrator;
+  rp.pass_max_depth = d->pass_max_depth; rp.pass_power = d->pass_power;
+  for (int k = 0; k < 3; ++k) rp.pass_bg[k] = d->pass_background[k];
+  rp.pass_normal_g = d->pass_geometric_normal;
   return rp;
 }
 
@@ -524,6 +529,8 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, rp, W.ps[cur], W.ss, count, q);
       else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
         LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+      else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
+        LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
       else
         LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       LGPU_HIP(hipGetLastError());

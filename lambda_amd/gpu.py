@@ -206,7 +206,10 @@ class Film:
 
 def render_desc(scene, film, camera, spp_begin=0, spp_end=1, max_bounces=8, min_bounces=3,
                 spectral=abi.SPECTRAL_RGB, estimator=abi.ESTIMATOR_CORRECTED, seed=0, capacity=0, flags=0,
-                integrator=abi.INTEGRATOR_PATH):
+                integrator=abi.INTEGRATOR_PATH, max_depth=1.0, depth_power=1.0, background=(1.0, 0.0, 1.0),
+                geometric_normal=False):
+    """lgpu_render_desc. The utility-pass fields default to the reference's DepthPass/NormalPass
+    defaults (UtilityIntegrators.h:18-30)."""
     d = abi.RenderDesc()
     d.scene = scene.h.value if scene is not None and hasattr(scene, "h") else None
     d.film = film.h.value if film is not None else None
@@ -216,6 +219,9 @@ def render_desc(scene, film, camera, spp_begin=0, spp_end=1, max_bounces=8, min_
     d.max_bounces, d.min_bounces = max_bounces, min_bounces
     d.spectral_mode, d.estimator_mode = spectral, estimator
     d.seed, d.path_capacity, d.flags = seed, capacity, flags
+    d.pass_max_depth, d.pass_power = max_depth, depth_power
+    d.pass_background[:] = [float(x) for x in background]
+    d.pass_geometric_normal = 1 if geometric_normal else 0
     return d
 
 

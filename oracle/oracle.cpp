@@ -2134,6 +2134,8 @@ static inline float PowerHeuristic(float f, float g) { return (f * f) / (f * f +
 struct RenderParams {
   uint32_t integrator, maxBounces, minBounces, mode, seed;
   bool corrected;
+  float maxDepth = 1, power = 1, background[3] = {1, 0, 1};  // DepthPass
+  bool normalG = false;                                      // NormalPass
 };
 
 static const Light &LightOf(const Scene &sc, const LightPick &lp) { return lp.kind == 1 ? sc.tlights[lp.index] : sc.lights[lp.index]; }
@@ -2814,6 +2816,29 @@ static void GenerateRay(const lgpu_camera &cam, float u, float v, Sampler &smp, 
 }
 
 // TileRenderers::UniformSpp per-sample body (Render.cpp:53-60): returns the path's spectrum.
+// UtilityIntegrator::Li (UtilityIntegrators.cpp:7-11) over DepthPass::P (:26-33), NormalPass::P
+// (:46-53) and AOVPass::P (:66-77): a colour of the closest hit, as a reflectance spectrum.
+static Spec UtilityLi(const Ctx &c, const RenderParams &rp, const V3 &ro, const V3 &rd, const Lanes &lanes) {
+  Hit hit;
+  float col[3] = {0, 0, 0};
+  const bool h = Intersect(*c.sc, ro, rd, hit, c.corrected);
+  if (rp.integrator == LGPU_INTEGRATOR_DEPTH) {
+    if (h) {
+      const float d = std::pow(Mag(hit.point - ro) / rp.maxDepth, rp.power);
+      col[0] = col[1] = col[2] = d;
+    } else {
+      col[0] = rp.background[0]; col[1] = rp.background[1]; col[2] = rp.background[2];
+    }
+  } else if (rp.integrator == LGPU_INTEGRATOR_NORMAL && h) {
+    const V3 n = rp.normalG ? hit.normalG : hit.normalS;
+    col[0] = n.x * .5f + .5f;
+    col[1] = n.y * .5f + (c.corrected ? .5f : 1.f);  // A50: the y channel is offset by 1 as written
+    col[2] = n.z * .5f + .5f;
+  }
+  // AOVPass: the material AOV lookup is commented out (:72-75); the colour stays black.
+  return FromRGB(col, lanes);
+}
+
 static Spec RenderSample(const Ctx &c, const RenderParams &rp, const lgpu_camera &cam, uint32_t W, uint32_t H,
                          uint32_t pixel, uint32_t sample, float *lambda0) {
   Sampler smp;
@@ -2831,6 +2856,7 @@ static Spec RenderSample(const Ctx &c, const RenderParams &rp, const lgpu_camera
   GenerateRay(cam, u, v, smp, &o, &d, c.corrected);
   if (lambda0) *lambda0 = l0;
   if (rp.integrator == LGPU_INTEGRATOR_DIRECT) return DirectLi(c, o, d, smp, lanes);
+  if (rp.integrator >= LGPU_INTEGRATOR_DEPTH) return UtilityLi(c, rp, o, d, lanes);
   if (rp.integrator == LGPU_INTEGRATOR_VOLPATH) return VolLi(c, rp, o, d, smp, lanes, false);
   if (rp.integrator == LGPU_INTEGRATOR_MIS_VOLPATH) return VolLi(c, rp, o, d, smp, lanes, true);
   return PathLi(c, rp, o, d, smp, lanes);
@@ -3103,13 +3129,17 @@ static RenderParams Params(const lgpu_render_desc *d) {
   RenderParams rp;
   rp.integrator = d->integrator; rp.maxBounces = d->max_bounces; rp.minBounces = d->min_bounces;
   rp.mode = d->spectral_mode; rp.seed = d->seed; rp.corrected = d->estimator_mode == LGPU_ESTIMATOR_CORRECTED;
+  rp.maxDepth = d->pass_max_depth; rp.power = d->pass_power;
+  for (int k = 0; k < 3; ++k) rp.background[k] = d->pass_background[k];
+  rp.normalG = d->pass_geometric_normal != 0;
   return rp;
 }
 
 int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width, uint32_t height, uint32_t n,
                        const uint32_t *pixels, const uint32_t *samples, float *out, int threads) {
   if (!s || !s->committed || !d || !width || !height) return -1;
-  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH) return -3;
+  if (d->integrator > LGPU_INTEGRATOR_AOV) return -3;
+  if (d->integrator == LGPU_INTEGRATOR_DEPTH && !(d->pass_max_depth > 0.f)) return -3;
   const RenderParams rp = Params(d);
   Ctx c{s, rp.corrected, (int)rp.mode};
   c.volumes = rp.corrected && !s->media.empty();
@@ -3126,7 +3156,8 @@ int oracle_trace_paths(oracle_scene s, const lgpu_render_desc *d, uint32_t width
 int oracle_render(oracle_scene s, const lgpu_render_desc *d, uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                   uint32_t w, uint32_t h, int threads, double *rgb, uint32_t *counts) {
   if (!s || !s->committed || !d || x0 + w > W || y0 + h > H) return -1;
-  if (d->integrator > LGPU_INTEGRATOR_MIS_VOLPATH) return -3;
+  if (d->integrator > LGPU_INTEGRATOR_AOV) return -3;
+  if (d->integrator == LGPU_INTEGRATOR_DEPTH && !(d->pass_max_depth > 0.f)) return -3;
   const RenderParams rp = Params(d);
   Ctx c{s, rp.corrected, (int)rp.mode};
   c.volumes = rp.corrected && !s->media.empty();

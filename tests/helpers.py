@@ -19,6 +19,8 @@ def desc(camera, spp_begin=0, spp_end=1, max_bounces=8, min_bounces=3, spectral=
     d.spp_begin, d.spp_end = spp_begin, spp_end
     d.max_bounces, d.min_bounces = max_bounces, min_bounces
     d.spectral_mode, d.estimator_mode, d.seed = spectral, estimator, seed
+    d.pass_max_depth, d.pass_power = 1.0, 1.0
+    d.pass_background[:] = [1.0, 0.0, 1.0]
     return d
 
 

@@ -303,3 +303,34 @@ def test_direct_lighting_furnace_on_gpu(dev):
                                integrator=abi.INTEGRATOR_DIRECT))
     rgb, cnt = film.read()
     np.testing.assert_allclose((rgb / cnt[..., None]).mean(axis=(0, 1)), 1 + rho, rtol=0.01)
+
+
+PASSES = [(i, sp, e) for i in (abi.INTEGRATOR_DEPTH, abi.INTEGRATOR_NORMAL, abi.INTEGRATOR_AOV)
+          for sp in (abi.SPECTRAL_RGB, abi.SPECTRAL_HERO4)
+          for e in (abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED)]
+
+
+@pytest.mark.parametrize("integrator,spectral,estimator", PASSES)
+def test_utility_pass_per_path_parity(dev, integrator, spectral, estimator):
+    """DepthPass / NormalPass / AOVPass on the glossy many-light scene (smooth normals, sky misses)."""
+    sd = scene.many_light_scene(n_spheres=20, sphere_level=2, ground_res=8, n_light_quads=16, seed=3)
+    W, H = 64, 36
+    s, k1 = gpu_scene(dev, sd)
+    o, k2 = helpers.oracle_scene(sd)
+    cam = scene.many_light_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.arange(W * H, dtype=np.uint32)
+    samples = np.zeros(W * H, dtype=np.uint32)
+    for geometric in (False, True):
+        d = gpu.render_desc(s, film, cam, 0, 1, spectral=spectral, estimator=estimator, integrator=integrator,
+                            max_depth=12.0, depth_power=1.5, background=(0.1, 0.2, 0.3), geometric_normal=geometric)
+        g = gpu.trace_paths(d, pixels, samples)
+        cd = helpers.desc(cam, 0, 1, 8, 3, spectral, estimator)
+        cd.integrator = integrator
+        cd.pass_max_depth, cd.pass_power = 12.0, 1.5
+        cd.pass_background[:] = [0.1, 0.2, 0.3]
+        cd.pass_geometric_normal = 1 if geometric else 0
+        c = o.trace_paths(cd, W, H, pixels, samples)
+        ok = _compare_paths(g, c)
+        print(integrator, spectral, estimator, geometric, "matching", ok.mean())
+        assert ok.mean() >= 0.999

@@ -370,3 +370,45 @@ def test_direct_lighting_faithful_runs():
         d.integrator = abi.INTEGRATOR_DIRECT
         rgb, cnt = o.render(d, W, W)
         assert np.isfinite(rgb).all() and rgb.sum() > 0
+
+
+def _wall(z=-1.2, albedo=0.5):
+    sd = scene.SceneDesc()
+    m = sd.add_material(scene.lambertian((albedo, albedo, albedo)))
+    v, i = scene._quad((-1, -1, z), (1, -1, z), (1, 1, z), (-1, 1, z))  # Ng = +z, towards the camera
+    sd.add_mesh(v, i, m)
+    return sd
+
+
+@pytest.mark.parametrize("estimator", [abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED])
+def test_utility_passes_closed_forms(estimator):
+    """DepthPass / NormalPass / AOVPass (UtilityIntegrators.cpp:26-77) on a wall 5 units in front of
+    a narrow pinhole: depth = (distance / maxDepth)^power, the background where the wall is missed;
+    the wall's normal (0, 0, 1) maps to (0.5, 1, 1) as written (y offset by 1, A50) and (0.5, 0.5, 1)
+    corrected, misses to 0; the AOV pass is black."""
+    o, _ = helpers.oracle_scene(_wall(), abi.LIGHTS_POWER)
+    W = H = 8
+    cam = scene.camera_pinhole((0.0, 0.0, 3.8), W, H, math.pi, 0.0, 0.02)  # all rays hit the wall
+    wide = scene.camera_pinhole((0.0, 0.0, 3.8), W, H, math.pi, 0.0, 1.2)  # the corners miss it
+    for integ, want in [(abi.INTEGRATOR_DEPTH, None), (abi.INTEGRATOR_NORMAL, None), (abi.INTEGRATOR_AOV, 0.0)]:
+        d = helpers.desc(cam, 0, 2, estimator=estimator)
+        d.integrator = integ
+        d.pass_max_depth, d.pass_power = 10.0, 2.0
+        rgb, cnt = o.render(d, W, H)
+        img = rgb / cnt[..., None]
+        if integ == abi.INTEGRATOR_DEPTH:
+            np.testing.assert_allclose(img, (5.0 / 10.0) ** 2, rtol=1e-3)
+        elif integ == abi.INTEGRATOR_NORMAL:
+            y = 1.0 if estimator == abi.ESTIMATOR_FAITHFUL else 0.5
+            np.testing.assert_allclose(img.reshape(-1, 3), np.tile([0.5, y, 1.0], (W * H, 1)), atol=1e-6)
+        else:
+            assert (img == 0).all()
+    d = helpers.desc(wide, 0, 1, estimator=estimator)
+    d.integrator = abi.INTEGRATOR_DEPTH
+    d.pass_max_depth, d.pass_power = 10.0, 1.0
+    d.pass_background[:] = [0.25, 0.5, 0.75]
+    rgb, cnt = o.render(d, W, H)
+    px = (rgb / cnt[..., None]).reshape(-1, 3)
+    miss = np.all(px == np.float32([0.25, 0.5, 0.75]), axis=1)
+    assert 0 < miss.sum() < W * H
+    assert (px[~miss] >= 0.5 - 1e-6).all()
