@@ -283,6 +283,18 @@ lgpu_status lgpu_scene_add_medium(lgpu_scene scene, const lgpu_medium_desc *desc
 lgpu_status lgpu_scene_add_material(lgpu_scene scene, const lgpu_material_desc *desc, uint32_t *id);
 /* Scene::AddObject with TriangleMesh; an emissive material makes a MeshLight */
 lgpu_status lgpu_scene_add_mesh(lgpu_scene scene, const lgpu_mesh_desc *desc, uint32_t *object_id);
+/* InstanceProxy(object) + InstanceProxy::Commit (Instance.cpp:7-16): a mesh kept for
+ * instancing; it is not part of the scene by itself. desc->material is the default material of
+ * its instances (Instance::Commit uses the proxy object's material when it has none). */
+lgpu_status lgpu_scene_add_proxy(lgpu_scene scene, const lgpu_mesh_desc *desc, uint32_t *proxy_id);
+/* Instance(proxy) + Instance::Commit + Scene::AddObject (Instance.cpp:18-34, Scene.cpp:114-121):
+ * the proxy's mesh under the affine transform xfm, 3x4 column-major as affine3 stores it
+ * (affine3.h:18-33: columns x, y, z, then the translation). material = LGPU_NONE takes the
+ * proxy's. Instances are flattened into world space at this call (normals by the inverse
+ * transpose, tangents by the linear part); an emissive material makes the instance a mesh
+ * light. Returns the instance's object id (shared with lgpu_scene_add_mesh's ids). */
+lgpu_status lgpu_scene_add_instance(lgpu_scene scene, uint32_t proxy_id, const float xfm[12], uint32_t material,
+                                    uint32_t *object_id);
 /* Scene::AddLight for point / spot / environment lights */
 lgpu_status lgpu_scene_add_light(lgpu_scene scene, const lgpu_light_desc *desc, uint32_t *id);
 /* Scene::Commit + LightSampler::Commit: builds BVH and light sampler, uploads. */

@@ -130,15 +130,14 @@ lgpu_status lgpu_scene_add_material(lgpu_scene s, const lgpu_material_desc *d, u
   return LGPU_OK;
 }
 
-// Scene::AddObject(TriangleMesh) (Scene.cpp:114-121); lambdaTriangleMeshSetBuffer (Lambda.cpp:211-244)
-lgpu_status lgpu_scene_add_mesh(lgpu_scene s, const lgpu_mesh_desc *d, uint32_t *object_id) {
-  CHECK_ARG(s && d && d->positions && d->indices, "add_mesh: null argument");
-  CHECK_ARG(!s->committed, "add_mesh: scene already committed");
-  CHECK_ARG(d->material < s->materials.size(), "add_mesh: material id out of range");
-  CHECK_ARG(d->num_triangles > 0 && d->num_vertices > 0, "add_mesh: empty mesh");
+// Checks a mesh descriptor and copies it (TriangleMesh, lambdaTriangleMeshSetBuffer Lambda.cpp:211-244).
+static lgpu_status CopyMesh(lgpu_scene s, const lgpu_mesh_desc *d, const char *what, HostMesh &m) {
+  if (!(s && d && d->positions && d->indices)) return Fail(LGPU_ERR_INVALID_ARGUMENT, std::string(what) + ": null argument");
+  if (s->committed) return Fail(LGPU_ERR_INVALID_ARGUMENT, std::string(what) + ": scene already committed");
+  if (d->material >= s->materials.size()) return Fail(LGPU_ERR_INVALID_ARGUMENT, std::string(what) + ": material id out of range");
+  if (!(d->num_triangles > 0 && d->num_vertices > 0)) return Fail(LGPU_ERR_INVALID_ARGUMENT, std::string(what) + ": empty mesh");
   for (size_t i = 0; i < 3ull * d->num_triangles; ++i)
-    if (d->indices[i] >= d->num_vertices) return Fail(LGPU_ERR_INVALID_ARGUMENT, "add_mesh: index out of range");
-  HostMesh m;
+    if (d->indices[i] >= d->num_vertices) return Fail(LGPU_ERR_INVALID_ARGUMENT, std::string(what) + ": index out of range");
   m.pos.assign(d->positions, d->positions + 3ull * d->num_vertices);
   m.idx.assign(d->indices, d->indices + 3ull * d->num_triangles);
   if (d->normals && d->tangents) {
@@ -147,16 +146,90 @@ lgpu_status lgpu_scene_add_mesh(lgpu_scene s, const lgpu_mesh_desc *d, uint32_t 
   }
   if (d->uvs) m.uv.assign(d->uvs, d->uvs + 2ull * d->num_vertices);
   m.material = d->material;
+  return LGPU_OK;
+}
+
+// Scene::AddObject (Scene.cpp:114-121); an emissive material also adds its MeshLight.
+static uint32_t AddObject(lgpu_scene s, HostMesh &&m) {
+  const uint32_t material = m.material;
   const int mi = (int)s->meshes.size();
   s->meshes.push_back(std::move(m));
-  if (s->materials[d->material].desc.emission_root != LGPU_NONE) {  // MeshLight(mesh) + AddLight
+  if (s->materials[material].desc.emission_root != LGPU_NONE) {  // MeshLight(mesh) + AddLight
     HostLight L;
     std::memset(&L.desc, 0, sizeof(L.desc));
     L.mesh = mi;
-    L.desc.intensity = s->materials[d->material].desc.emission_intensity;
+    L.desc.intensity = s->materials[material].desc.emission_intensity;
     s->lights.push_back(std::move(L));
   }
-  if (object_id) *object_id = (uint32_t)mi;
+  return (uint32_t)mi;
+}
+
+// Scene::AddObject(TriangleMesh) (Scene.cpp:114-121); lambdaTriangleMeshSetBuffer (Lambda.cpp:211-244)
+lgpu_status lgpu_scene_add_mesh(lgpu_scene s, const lgpu_mesh_desc *d, uint32_t *object_id) {
+  HostMesh m;
+  const lgpu_status st = CopyMesh(s, d, "add_mesh", m);
+  if (st != LGPU_OK) return st;
+  const uint32_t id = AddObject(s, std::move(m));
+  if (object_id) *object_id = id;
+  return LGPU_OK;
+}
+
+// InstanceProxy(object) (Instance.cpp:7-16)
+lgpu_status lgpu_scene_add_proxy(lgpu_scene s, const lgpu_mesh_desc *d, uint32_t *proxy_id) {
+  HostMesh m;
+  const lgpu_status st = CopyMesh(s, d, "add_proxy", m);
+  if (st != LGPU_OK) return st;
+  s->proxies.push_back(std::move(m));
+  if (proxy_id) *proxy_id = (uint32_t)s->proxies.size() - 1;
+  return LGPU_OK;
+}
+
+// Instance::Commit (Instance.cpp:22-34) flattened: the proxy's vertices through the affine
+// transform (affine3::operator*, affine3.h:38-40), normals through the inverse transpose of its
+// linear part, tangents through the linear part (bitangent sign flipped by a mirroring transform).
+lgpu_status lgpu_scene_add_instance(lgpu_scene s, uint32_t proxy_id, const float x[12], uint32_t material,
+                                    uint32_t *object_id) {
+  CHECK_ARG(s && x, "add_instance: null argument");
+  CHECK_ARG(!s->committed, "add_instance: scene already committed");
+  CHECK_ARG(proxy_id < s->proxies.size(), "add_instance: proxy id out of range");
+  CHECK_ARG(material == LGPU_NONE || material < s->materials.size(), "add_instance: material id out of range");
+  for (int k = 0; k < 12; ++k) CHECK_ARG(std::isfinite(x[k]), "add_instance: transform is not finite");
+  // cofactor matrix of the linear part M (columns x[0..2], x[3..5], x[6..8]): M^-T = cof / det
+  const float c00 = x[4] * x[8] - x[5] * x[7], c01 = x[5] * x[6] - x[3] * x[8], c02 = x[3] * x[7] - x[4] * x[6];
+  const float c10 = x[7] * x[2] - x[8] * x[1], c11 = x[8] * x[0] - x[6] * x[2], c12 = x[6] * x[1] - x[7] * x[0];
+  const float c20 = x[1] * x[5] - x[2] * x[4], c21 = x[2] * x[3] - x[0] * x[5], c22 = x[0] * x[4] - x[1] * x[3];
+  const float det = x[0] * c00 + x[1] * c01 + x[2] * c02;
+  CHECK_ARG(det != 0.f && std::isfinite(det), "add_instance: singular transform");
+  const float sg = det < 0.f ? -1.f : 1.f;
+  const HostMesh &p = s->proxies[proxy_id];
+  HostMesh m = p;
+  if (material != LGPU_NONE) m.material = material;
+  const size_t nv = p.pos.size() / 3;
+  for (size_t v = 0; v < nv; ++v) {
+    const float px = p.pos[3 * v], py = p.pos[3 * v + 1], pz = p.pos[3 * v + 2];
+    m.pos[3 * v] = x[0] * px + x[3] * py + x[6] * pz + x[9];
+    m.pos[3 * v + 1] = x[1] * px + x[4] * py + x[7] * pz + x[10];
+    m.pos[3 * v + 2] = x[2] * px + x[5] * py + x[8] * pz + x[11];
+  }
+  if (!p.nrm.empty()) {
+    for (size_t v = 0; v < nv; ++v) {
+      const float nx = p.nrm[3 * v], ny = p.nrm[3 * v + 1], nz = p.nrm[3 * v + 2];
+      // column j of cof holds the cofactors of column j of M: n' = cof * n
+      float ox = (c00 * nx + c10 * ny + c20 * nz) * sg, oy = (c01 * nx + c11 * ny + c21 * nz) * sg,
+            oz = (c02 * nx + c12 * ny + c22 * nz) * sg;
+      float l = std::sqrt(ox * ox + oy * oy + oz * oz);
+      if (l > 0.f) { ox /= l; oy /= l; oz /= l; }
+      m.nrm[3 * v] = ox; m.nrm[3 * v + 1] = oy; m.nrm[3 * v + 2] = oz;
+      const float tx = p.tan[4 * v], ty = p.tan[4 * v + 1], tz = p.tan[4 * v + 2];
+      float ux = x[0] * tx + x[3] * ty + x[6] * tz, uy = x[1] * tx + x[4] * ty + x[7] * tz,
+            uz = x[2] * tx + x[5] * ty + x[8] * tz;
+      l = std::sqrt(ux * ux + uy * uy + uz * uz);
+      if (l > 0.f) { ux /= l; uy /= l; uz /= l; }
+      m.tan[4 * v] = ux; m.tan[4 * v + 1] = uy; m.tan[4 * v + 2] = uz; m.tan[4 * v + 3] = p.tan[4 * v + 3] * sg;
+    }
+  }
+  const uint32_t id = AddObject(s, std::move(m));
+  if (object_id) *object_id = id;
   return LGPU_OK;
 }
 

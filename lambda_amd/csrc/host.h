@@ -92,6 +92,7 @@ struct lgpu_scene_s {
   bool committed = false;
   std::vector<lgpu::HostMaterial> materials;
   std::vector<lgpu::HostMesh> meshes;
+  std::vector<lgpu::HostMesh> proxies;  // InstanceProxy objects: instanced, not in the scene themselves
   std::vector<lgpu::HostLight> lights;  // scene light order (mesh lights interleaved)
   std::vector<std::vector<float>> textures;
   std::vector<uint32_t> tex_dims;        // w, h, ch per texture

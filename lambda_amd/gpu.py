@@ -24,6 +24,7 @@ EXPORTS = [
     "lgpu_camera_init", "lgpu_camera_set_rotation", "lgpu_camera_set_forwards", "lgpu_camera_set_fov",
     "lgpu_film_create", "lgpu_film_destroy", "lgpu_film_clear", "lgpu_film_read", "lgpu_film_read_rgba",
     "lgpu_render", "lgpu_trace_paths", "lgpu_intersect", "lgpu_occluded", "lgpu_philox", "lgpu_copy_bench",
+    "lgpu_scene_add_proxy", "lgpu_scene_add_instance",
 ]
 
 
@@ -51,6 +52,7 @@ def lib():
             "lgpu_trace_paths": [vp, u32, vp, vp, vp], "lgpu_intersect": [vp, u32, vp, vp],
             "lgpu_occluded": [vp, u32, vp, vp], "lgpu_philox": [vp, u32, vp, vp, vp],
             "lgpu_copy_bench": [vp, C.c_uint64, u32, vp],
+            "lgpu_scene_add_proxy": [vp, vp, vp], "lgpu_scene_add_instance": [vp, u32, vp, u32, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -144,6 +146,15 @@ class Scene:
 
     def add_light(self, d):
         return self._add(lib().lgpu_scene_add_light, d)
+
+    def add_proxy(self, d):
+        return self._add(lib().lgpu_scene_add_proxy, d)
+
+    def add_instance(self, proxy, xfm, material=abi.NONE):
+        x = (C.c_float * 12)(*[float(v) for v in xfm])
+        out = C.c_uint32()
+        _check(lib().lgpu_scene_add_instance(self.h, proxy, x, material, C.byref(out)))
+        return out.value
 
     def commit(self, strategy=abi.LIGHTS_TREE):
         _check(lib().lgpu_scene_commit(self.h, strategy))

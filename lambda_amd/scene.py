@@ -92,7 +92,23 @@ class SceneDesc:
                                       tangents=None if tangents is None else _f32(tangents).reshape(-1, 4),
                                       uvs=None if uvs is None else _f32(uvs).reshape(-1, 2))))
         self.n_tris += len(indices)
-        return sum(1 for o in self.ops if o[0] == "mesh") - 1
+        return sum(1 for o in self.ops if o[0] in ("mesh", "instance")) - 1
+
+    def add_proxy(self, positions, indices, material, normals=None, tangents=None, uvs=None):
+        """InstanceProxy: a mesh kept for instancing (not rendered by itself)."""
+        self.add_mesh(positions, indices, material, normals, tangents, uvs)
+        op = self.ops[-1]
+        self.n_tris -= len(op[1]["indices"])
+        self.ops[-1] = ("proxy", op[1])
+        return sum(1 for o in self.ops if o[0] == "proxy") - 1
+
+    def add_instance(self, proxy, xfm, material=abi.NONE):
+        """Instance of a proxy under a 3x4 column-major affine transform (columns x, y, z, translation)."""
+        xfm = [float(v) for v in np.asarray(xfm, dtype=np.float32).reshape(12)]
+        ops = [o for o in self.ops if o[0] == "proxy"]
+        self.ops.append(("instance", dict(proxy=int(proxy), xfm=xfm, material=int(material))))
+        self.n_tris += len(ops[proxy][1]["indices"])
+        return sum(1 for o in self.ops if o[0] in ("mesh", "instance")) - 1
 
     def add_point_light(self, position, intensity, rgb=(1, 1, 1), temperature=0.0):
         self.ops.append(("light", dict(type=abi.LIGHT_POINT, position=position, intensity=intensity, rgb=rgb,
@@ -132,7 +148,9 @@ class SceneDesc:
                 d, arr = pl.to_c()
                 keep.append(arr)
                 backend.add_material(d)
-            elif kind == "mesh":
+            elif kind == "instance":
+                backend.add_instance(pl["proxy"], pl["xfm"], pl["material"])
+            elif kind in ("mesh", "proxy"):
                 d = abi.MeshDesc()
                 d.num_vertices = len(pl["positions"])
                 d.num_triangles = len(pl["indices"])
@@ -142,7 +160,7 @@ class SceneDesc:
                     if pl[k] is not None:
                         setattr(d, k, pl[k].ctypes.data_as(abi.FP))
                 d.material = pl["material"]
-                backend.add_mesh(d)
+                (backend.add_mesh if kind == "mesh" else backend.add_proxy)(d)
             else:
                 d = abi.LightDesc()
                 d.type = pl["type"]
@@ -529,6 +547,44 @@ def _tangents(n):
     t = np.cross(up, n)
     t /= np.linalg.norm(t, axis=1, keepdims=True)
     return np.concatenate([t, np.ones((len(n), 1))], axis=1)
+
+
+def affine(rot=(0.0, 0.0, 0.0), scale=(1.0, 1.0, 1.0), translate=(0.0, 0.0, 0.0)):
+    """3x4 column-major affine transform (affine3.h layout): R_x R_y R_z diag(scale), then translate."""
+    cx, cy, cz = np.cos(rot)
+    sx, sy, sz = np.sin(rot)
+    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    m = rx @ ry @ rz @ np.diag(scale)
+    return np.concatenate([m[:, 0], m[:, 1], m[:, 2], np.asarray(translate, dtype=np.float64)]).astype(np.float32)
+
+
+def instanced_cornell(n=24, seed=2, level=2):
+    """Cornell room holding `n` instances of one smooth icosphere proxy (uvs, tangents) under random
+    rotations, non-uniform scales and translations (every fourth one mirrored), with the proxy's
+    material, a Lambertian override or an emissive override, plus instances of a quad light proxy."""
+    sd = cornell_box()
+    rng = np.random.default_rng(seed)
+    v, f = icosphere(level)
+    nrm = _vertex_normals(v, f)
+    uv = np.stack([0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * np.pi), 0.5 + v[:, 1] * 0.5], axis=1)
+    glossy = sd.add_material(ggx_conductor(0.2))
+    blue = sd.add_material(lambertian((0.2, 0.3, 0.7)))
+    glow = sd.add_material(emitter_blackbody(3500.0, 3.0, rgb_albedo=(0.5, 0.5, 0.5)))
+    sphere = sd.add_proxy(v, f, glossy, normals=nrm, tangents=_tangents(nrm), uvs=uv)
+    for k in range(n):
+        sc = rng.uniform(0.06, 0.16, size=3)
+        if k % 4 == 3:
+            sc[0] = -sc[0]
+        x = affine(rng.uniform(0, 2 * np.pi, size=3), sc, rng.uniform([-0.8, -0.85, -0.8], [0.8, 0.6, 0.6]))
+        sd.add_instance(sphere, x, [abi.NONE, blue, glow][k % 3])
+    lq = sd.add_material(emitter_blackbody(6000.0, 6.0))
+    qv, qi = _quad((-0.05, 0, -0.05), (0.05, 0, -0.05), (0.05, 0, 0.05), (-0.05, 0, 0.05))  # Ng = -y
+    quad = sd.add_proxy(qv, qi, lq)
+    for k in range(4):
+        sd.add_instance(quad, affine((0.0, 0.5 * k, 0.0), (1, 1, 1), (-0.6 + 0.4 * k, 0.95, -0.5)))
+    return sd
 
 
 COPPER_ETA = (0.200438, 0.924033, 1.102212)  # RGB-fitted copper optical constants

@@ -67,6 +67,8 @@ def lib():
         L.oracle_hg_sample.argtypes = [f32, vp, f32, f32, C.c_int, vp]
         L.oracle_hg_sample.restype = f32
         L.oracle_eval_node.argtypes = [vp, u32, u32, vp, C.c_int, vp, vp]
+        L.oracle_scene_add_proxy.argtypes = [vp, vp]
+        L.oracle_scene_add_instance.argtypes = [vp, u32, vp, u32]
         L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
         L.oracle_tree_importance.restype = f32
         _LIB = L
@@ -105,6 +107,13 @@ class OracleScene:
 
     def add_mesh(self, d):
         return self._chk(self.L.oracle_scene_add_mesh(self.h, C.byref(d)), "add_mesh")
+
+    def add_proxy(self, d):
+        return self._chk(self.L.oracle_scene_add_proxy(self.h, C.byref(d)), "add_proxy")
+
+    def add_instance(self, proxy, xfm, material=0xFFFFFFFF):
+        x = (C.c_float * 12)(*[float(v) for v in xfm])
+        return self._chk(self.L.oracle_scene_add_instance(self.h, proxy, x, material), "add_instance")
 
     def add_light(self, d):
         return self._chk(self.L.oracle_scene_add_light(self.h, C.byref(d)), "add_light")

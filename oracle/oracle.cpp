@@ -376,6 +376,11 @@ struct Material {
   std::vector<char> bbConst;
 };
 
+struct Proxy {  // InstanceProxy's mesh, as given
+  uint32_t nv = 0, nt = 0, material = 0;
+  std::vector<float> pos, nrm, tan, uv;
+  std::vector<uint32_t> idx;
+};
 struct Mesh {
   std::vector<V3> p, n;
   std::vector<float> tan;  // 4 per vertex
@@ -466,6 +471,7 @@ struct oracle_scene_s {
   std::vector<MediumO> media;
   std::vector<Material> materials;
   std::vector<Mesh> meshes;
+  std::vector<Proxy> proxies;
   std::vector<uint32_t> triMesh;  // global tri -> mesh
   std::vector<Light> lights;      // scene lights in add order (Scene::lights)
   int envLight = -1;
@@ -2982,6 +2988,60 @@ int oracle_scene_add_mesh(oracle_scene s, const lgpu_mesh_desc *d) {
   }
   s->meshes.push_back(std::move(m));
   return mi;
+}
+// InstanceProxy (Instance.cpp:7-16): a mesh kept for instancing.
+int oracle_scene_add_proxy(oracle_scene s, const lgpu_mesh_desc *d) {
+  if (!s || !d || !d->positions || !d->indices || d->material >= s->materials.size()) return -1;
+  Proxy p;
+  p.nv = d->num_vertices; p.nt = d->num_triangles; p.material = d->material;
+  p.pos.assign(d->positions, d->positions + 3 * (size_t)p.nv);
+  p.idx.assign(d->indices, d->indices + 3 * (size_t)p.nt);
+  if (d->normals && d->tangents) {
+    p.nrm.assign(d->normals, d->normals + 3 * (size_t)p.nv);
+    p.tan.assign(d->tangents, d->tangents + 4 * (size_t)p.nv);
+  }
+  if (d->uvs) p.uv.assign(d->uvs, d->uvs + 2 * (size_t)p.nv);
+  s->proxies.push_back(std::move(p));
+  return (int)s->proxies.size() - 1;
+}
+// Instance::Commit (Instance.cpp:22-34) with a world-space result: points through affine3's
+// operator* (affine3.h:38-40), normals through the inverse transpose of the linear part (the
+// columns of its cofactor matrix are b x c, c x a, a x b), tangents through the linear part.
+int oracle_scene_add_instance(oracle_scene s, uint32_t proxy, const float x[12], uint32_t material) {
+  if (!s || proxy >= s->proxies.size() || (material != LGPU_NONE && material >= s->materials.size())) return -1;
+  const Proxy &p = s->proxies[proxy];
+  const V3 a(x[0], x[1], x[2]), b(x[3], x[4], x[5]), c(x[6], x[7], x[8]), t(x[9], x[10], x[11]);
+  const V3 k0 = Cross(b, c), k1 = Cross(c, a), k2 = Cross(a, b);
+  const float det = a.x * k0.x + a.y * k0.y + a.z * k0.z;
+  if (!(det != 0.f && std::isfinite(det))) return -1;
+  const float sg = det < 0 ? -1.f : 1.f;
+  std::vector<float> pos(3 * (size_t)p.nv), nrm(p.nrm.size()), tan(p.tan.size());
+  for (uint32_t v = 0; v < p.nv; ++v) {
+    const float px = p.pos[3 * v], py = p.pos[3 * v + 1], pz = p.pos[3 * v + 2];
+    pos[3 * v] = a.x * px + b.x * py + c.x * pz + t.x;
+    pos[3 * v + 1] = a.y * px + b.y * py + c.y * pz + t.y;
+    pos[3 * v + 2] = a.z * px + b.z * py + c.z * pz + t.z;
+    if (p.nrm.empty()) continue;
+    const float nx = p.nrm[3 * v], ny = p.nrm[3 * v + 1], nz = p.nrm[3 * v + 2];
+    float o[3] = {(k0.x * nx + k1.x * ny + k2.x * nz) * sg, (k0.y * nx + k1.y * ny + k2.y * nz) * sg,
+                  (k0.z * nx + k1.z * ny + k2.z * nz) * sg};
+    float l = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    for (int k = 0; k < 3; ++k) nrm[3 * v + k] = l > 0 ? o[k] / l : o[k];
+    const float tx = p.tan[4 * v], ty = p.tan[4 * v + 1], tz = p.tan[4 * v + 2];
+    float u[3] = {a.x * tx + b.x * ty + c.x * tz, a.y * tx + b.y * ty + c.y * tz, a.z * tx + b.z * ty + c.z * tz};
+    l = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int k = 0; k < 3; ++k) tan[4 * v + k] = l > 0 ? u[k] / l : u[k];
+    tan[4 * v + 3] = p.tan[4 * v + 3] * sg;
+  }
+  lgpu_mesh_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.num_vertices = p.nv; d.num_triangles = p.nt;
+  d.positions = pos.data(); d.indices = p.idx.data();
+  d.normals = p.nrm.empty() ? nullptr : nrm.data();
+  d.tangents = p.nrm.empty() ? nullptr : tan.data();
+  d.uvs = p.uv.empty() ? nullptr : p.uv.data();
+  d.material = material == LGPU_NONE ? p.material : material;
+  return oracle_scene_add_mesh(s, &d);
 }
 int oracle_scene_add_light(oracle_scene s, const lgpu_light_desc *d) {
   if (!s || !d) return -1;

@@ -26,6 +26,8 @@ int oracle_scene_add_texture(oracle_scene s, const lgpu_texture_desc *d);
 int oracle_scene_add_medium(oracle_scene s, const lgpu_medium_desc *d);
 int oracle_scene_add_material(oracle_scene s, const lgpu_material_desc *d);
 int oracle_scene_add_mesh(oracle_scene s, const lgpu_mesh_desc *d);
+int oracle_scene_add_proxy(oracle_scene s, const lgpu_mesh_desc *d);
+int oracle_scene_add_instance(oracle_scene s, uint32_t proxy, const float xfm[12], uint32_t material);
 int oracle_scene_add_light(oracle_scene s, const lgpu_light_desc *d);
 int oracle_scene_commit(oracle_scene s, uint32_t light_strategy);
 int oracle_light_tree(oracle_scene s, float *nodes, uint32_t max_nodes);

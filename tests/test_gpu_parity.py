@@ -334,3 +334,25 @@ def test_utility_pass_per_path_parity(dev, integrator, spectral, estimator):
         ok = _compare_paths(g, c)
         print(integrator, spectral, estimator, geometric, "matching", ok.mean())
         assert ok.mean() >= 0.999
+
+
+@pytest.mark.parametrize("estimator", [abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED])
+def test_instanced_scene_per_path_parity(dev, estimator):
+    """Instances (InstanceProxy/Instance) of a smooth sphere under rotations, non-uniform scales and
+    mirrors with proxy, override and emissive materials, plus instanced quad lights."""
+    sd = scene.instanced_cornell()
+    W = H = 48
+    s, k1 = gpu_scene(dev, sd)
+    o, k2 = helpers.oracle_scene(sd)
+    assert (s.light_tree() == o.light_tree()).all()
+    cam = scene.cornell_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    d = gpu.render_desc(s, film, cam, 0, 2, max_bounces=8, min_bounces=3, spectral=abi.SPECTRAL_HERO4,
+                        estimator=estimator)
+    g = gpu.trace_paths(d, pixels, samples)
+    c = o.trace_paths(helpers.desc(cam, 0, 2, 8, 3, abi.SPECTRAL_HERO4, estimator), W, H, pixels, samples)
+    ok = _compare_paths(g, c)
+    print("instanced", estimator, "matching", ok.mean())
+    assert ok.mean() >= 0.999

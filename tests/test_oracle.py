@@ -412,3 +412,79 @@ def test_utility_passes_closed_forms(estimator):
     miss = np.all(px == np.float32([0.25, 0.5, 0.75]), axis=1)
     assert 0 < miss.sum() < W * H
     assert (px[~miss] >= 0.5 - 1e-6).all()
+
+
+def _ellipsoid_scene(as_instance, xfm, level=2):
+    """One smooth icosphere under xfm: as an instance of a proxy, or as a mesh transformed here in
+    float64 (normals by the inverse transpose, tangents by the linear part)."""
+    sd = scene.SceneDesc()
+    m = sd.add_material(scene.lambertian((0.5, 0.5, 0.5)))
+    v, f = scene.icosphere(level)
+    n = scene._vertex_normals(v, f)
+    t = scene._tangents(n)
+    if as_instance:
+        p = sd.add_proxy(v, f, m, normals=n, tangents=t)
+        sd.add_instance(p, xfm)
+    else:
+        x = np.asarray(xfm, dtype=np.float64)
+        M = np.stack([x[0:3], x[3:6], x[6:9]], axis=1)
+        vv = v @ M.T + x[9:12]
+        nn = n @ np.linalg.inv(M)
+        nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+        tt = t[:, :3] @ M.T
+        tt /= np.linalg.norm(tt, axis=1, keepdims=True)
+        tt = np.concatenate([tt, t[:, 3:] * np.sign(np.linalg.det(M))], axis=1)
+        sd.add_mesh(vv, f, m, normals=nn, tangents=tt)
+    return sd
+
+
+@pytest.mark.parametrize("mirror", [False, True])
+def test_instance_matches_transformed_mesh(mirror):
+    """Instance::Commit flattened (Instance.cpp:22-43): an instance of a smooth sphere under rotation,
+    non-uniform scale (and a mirror) gives the same shading normals (NormalPass) and depths as the
+    same mesh transformed independently in float64 with inverse-transpose normals."""
+    sc = (-1.4, 0.7, 1.1) if mirror else (1.4, 0.7, 1.1)
+    xfm = scene.affine((0.3, -0.5, 0.9), sc, (0.1, -0.2, 0.0))
+    W = H = 24
+    cam = scene.cornell_camera(W, H)
+    imgs = {}
+    for inst in (True, False):
+        o, _ = helpers.oracle_scene(_ellipsoid_scene(inst, xfm), abi.LIGHTS_POWER)
+        for integ in (abi.INTEGRATOR_NORMAL, abi.INTEGRATOR_DEPTH):
+            d = helpers.desc(cam, 0, 1, estimator=abi.ESTIMATOR_CORRECTED)
+            d.integrator = integ
+            d.pass_max_depth = 10.0
+            rgb, cnt = o.render(d, W, H)
+            imgs[inst, integ] = rgb / cnt[..., None]
+    for integ in (abi.INTEGRATOR_NORMAL, abi.INTEGRATOR_DEPTH):
+        a, b = imgs[True, integ], imgs[False, integ]
+        hit = np.any(a != (0 if integ == abi.INTEGRATOR_NORMAL else np.float32([1, 0, 1])), axis=2)
+        assert 0.1 < hit.mean() < 0.9
+        np.testing.assert_allclose(a, b, atol=2e-4)
+
+
+def test_identity_instance_equals_mesh_and_rejects_singular():
+    sd_i, sd_m = _ellipsoid_scene(True, scene.affine()), _ellipsoid_scene(False, scene.affine())
+    W = H = 16
+    d = helpers.desc(scene.cornell_camera(W, H), 0, 2, estimator=abi.ESTIMATOR_CORRECTED)
+    d.integrator = abi.INTEGRATOR_NORMAL
+    oi, _ = helpers.oracle_scene(sd_i, abi.LIGHTS_POWER)
+    om, _ = helpers.oracle_scene(sd_m, abi.LIGHTS_POWER)
+    # the instance renormalises the normals in float32: equal up to rounding
+    np.testing.assert_allclose(oi.render(d, W, H)[0], om.render(d, W, H)[0], atol=1e-6)
+    bad = scene.SceneDesc()
+    m = bad.add_material(scene.lambertian((0.5, 0.5, 0.5)))
+    v, f = scene.icosphere(0)
+    p = bad.add_proxy(v, f, m)
+    bad.add_instance(p, [1, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0])
+    with pytest.raises(Exception):
+        helpers.oracle_scene(bad, abi.LIGHTS_POWER)
+
+
+def test_emissive_instances_are_lights():
+    """An instance with an emissive material is a mesh light (Scene::AddObject adds its light)."""
+    o, _ = helpers.oracle_scene(scene.instanced_cornell(n=6), abi.LIGHTS_TREE)
+    # 2 (room light) + 2 x 80 (two glowing spheres of 80 triangles) + 4 x 2 (quad instances)
+    pdfs, env = o.light_pdfs((0, 0, 0), (0, 1, 0))
+    assert len(pdfs) == 2 + 2 * 320 + 8
+    assert pdfs.sum() + env == pytest.approx(1.0, abs=1e-5)
