@@ -58,10 +58,10 @@ typedef enum lgpu_node_type {
   LGPU_NODE_IMAGE_COLOUR = 6,   /* out colour; p[0] = texture id */
   LGPU_NODE_IMAGE_SCALAR = 7,   /* out scalar; p[0] = texture id */
   LGPU_NODE_UV = 8,             /* out vec2 = surface uv (SurfaceInput) */
-  LGPU_NODE_FRESNEL = 9,        /* FresnelInput (GraphInputs.cpp:187-199): out scalar = Schlick
+  LGPU_NODE_FRESNEL = 9,        /* FresnelInput (GraphInputs.cpp:183-198): out scalar = Schlick
                                    with n1 = the path's eta, n2 = in[0] (p[0] when unset),
                                    cos = woL.y */
-  LGPU_NODE_SPECTRAL = 10,      /* SpectralInput (GraphInputs.cpp:52-66): out spectrum; samples
+  LGPU_NODE_SPECTRAL = 10,      /* SpectralInput (GraphInputs.cpp:55-71): out spectrum; samples
                                    in texture p[0] (1 channel, width x height values in order),
                                    evenly spaced over [p[1], p[2]] nm */
   LGPU_NODE_IMAGE_CHANNEL = 11, /* ImageTextureChannelInput: out scalar; p[0] texture, p[1]

@@ -429,7 +429,7 @@ struct Compiler {
       case LGPU_NODE_CHECKER: case LGPU_NODE_PERLIN: case LGPU_NODE_VALUE_NOISE: case LGPU_NODE_VORONOI:
         Emit(kOpNoise, dst, 0, 0, n.type, AddConst(make_float4(n.p[0], 1, 1, 0)));
         break;
-      case LGPU_NODE_OCTAVE_NOISE: {  // OctaveNoise::Get2D/3D (GraphTexture.cpp:156-174), unrolled
+      case LGPU_NODE_OCTAVE_NOISE: {  // OctaveNoise::Get2D/3D (GraphTexture.cpp:157-175), unrolled
         const uint32_t base = (uint32_t)n.p[2];
         if (base != LGPU_NODE_CHECKER && base != LGPU_NODE_PERLIN && base != LGPU_NODE_VALUE_NOISE && base != LGPU_NODE_VORONOI) {
           err = "octave noise needs a checker, Perlin, value-noise or Voronoi base"; return dst;

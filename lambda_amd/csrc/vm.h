@@ -228,21 +228,21 @@ static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmC
         o = make_float4(px[0], px[1], px[2], px[3]);
         break;
       }
-      case kOpTexChannel: {  // GraphInputs.cpp:150-160
+      case kOpTexChannel: {  // GraphInputs.cpp:148-151
         const uint4 t = c.textures[in.imm];
         float px[4];
         TexBilinear(c.texels + t.w, t.x, t.y, t.z, Fract(c.u), Fract(c.v), px);
         o.x = px[in.c];
         break;
       }
-      case kOpSpecTex: {  // GraphInputs.cpp:162-171
+      case kOpSpecTex: {  // GraphInputs.cpp:162-165
         const uint4 t = c.textures[in.imm];
         float px[4];
         TexBilinear(c.texels + t.w, t.x, t.y, t.z, Fract(c.u), Fract(c.v), px);
         o = UpliftV(px[0], px[1], px[2], c);
         break;
       }
-      case kOpSpectral: {  // GraphInputs.cpp:52-66
+      case kOpSpectral: {  // GraphInputs.cpp:55-71
         const float4 k = c.consts[in.imm];
         if (!c.mode) { o = c.consts[in.imm + 1]; break; }
         const uint4 t = c.textures[(uint32_t)k.z];
@@ -262,12 +262,12 @@ static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmC
         o = make_float4(lv[0], lv[1], lv[2], lv[3]);
         break;
       }
-      case kOpNoise: {  // GraphTexture.cpp:51-219
+      case kOpNoise: {  // GraphTexture.cpp:56-220
         const float4 k = c.consts[in.imm];
         o.x = NoiseAt(in.c, c, k.x, k.y, k.z);
         break;
       }
-      case kOpFresnel: {  // GraphInputs.cpp:190-199
+      case kOpFresnel: {  // GraphInputs.cpp:188-198
         const float n1 = c.eta, n2 = r[in.a].x;
         float R0 = (n1 - n2) / (n1 + n2);
         R0 *= R0;

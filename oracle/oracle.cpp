@@ -757,18 +757,18 @@ static V2 AsVec2(const Val &x) {  // GetAs<Vec2> (ShaderGraph.h:176-199)
 // lattice coordinate converts as x86-64 compiles float -> unsigned (through a 64-bit signed
 // integer, so negative values wrap modulo 2^32).
 static inline uint32_t LatticeU32(float f) { return (uint32_t)(int64_t)f; }
-static inline uint32_t Hash21(uint32_t x, uint32_t y) {  // :25-29
+static inline uint32_t Hash21(uint32_t x, uint32_t y) {  // :21-25
   x *= 1597334677u;
   y *= 3812015801u;
   return (x ^ y) * 1597334677u;
 }
 static inline float HashUnit(uint32_t h) { return (float)((double)h * (1. / 4294967296.)); }  // Hash * inv32
-static inline V2 Hash22(V2 v) {  // :31-34
+static inline V2 Hash22(V2 v) {  // :27-30
   const float a = v.x * 127.1f + v.y * 311.7f, b = v.x * 269.5f + v.y * 183.3f;
   const float sa = std::sin(a) * 18.5453f, sb = std::sin(b) * 18.5453f;
   return V2(sa - std::floor(sa), sb - std::floor(sb));
 }
-static inline V3 Hash33(V3 p) {  // :37-41
+static inline V3 Hash33(V3 p) {  // :33-37
   p = V3(p.x * .1031f, p.y * .1030f, p.z * .0973f);
   p = V3(p.x - std::floor(p.x), p.y - std::floor(p.y), p.z - std::floor(p.z));
   const float d = (p.x * (p.y + 33.33f) + p.y * (p.x + 33.33f)) + p.z * (p.z + 33.33f);
@@ -780,7 +780,7 @@ static inline float Quintic(float x) { return x * x * x * (x * (x * 6.f - 15.f) 
 static inline float CubicS(float x) { return x * x * (3 - 2 * x); }
 static inline float LerpF(float a, float b, float r) { return a + (b - a) * r; }
 
-static float Perlin2D(V2 t) {  // :61-79 (t already scaled)
+static float Perlin2D(V2 t) {  // :66-86 (t already scaled)
   const float gx = std::floor(t.x), gy = std::floor(t.y);
   auto grad = [&](float px, float py) {
     float u = HashUnit(Hash21(LatticeU32(px), LatticeU32(py)));
@@ -796,7 +796,7 @@ static float Perlin2D(V2 t) {  // :61-79 (t already scaled)
   const float vx1 = LerpF(dg01, dg11, Quintic(t.x - gx));
   return LerpF(vx0, vx1, Quintic(t.y - gy)) + .5f;
 }
-static float Value2D(V2 t) {  // :126-138
+static float Value2D(V2 t) {  // :127-138
   const float x0 = std::floor(t.x), y0 = std::floor(t.y);
   const V2 p(t.x - x0, t.y - y0);
   const float v00 = HashUnit(Hash21(LatticeU32(x0), LatticeU32(y0)));
@@ -815,7 +815,7 @@ static float Checker3D(V3 t) {  // :110-113: ((x > .5) == (y > .5)) == (z > .5)
   const float x = t.x - std::floor(t.x), y = t.y - std::floor(t.y), z = t.z - std::floor(t.z);
   return ((x > .5f) == (y > .5f)) == (z > .5f) ? 0.f : 1.f;
 }
-static float Voronoi2D(V2 t) {  // :187-201
+static float Voronoi2D(V2 t) {  // :189-202
   const float x0 = std::floor(t.x), y0 = std::floor(t.y);
   float m = INFINITY;
   for (int y = -1; y < 2; ++y)
@@ -828,7 +828,7 @@ static float Voronoi2D(V2 t) {  // :187-201
     }
   return std::sqrt(m);
 }
-static float Voronoi3D(V3 t) {  // :203-219
+static float Voronoi3D(V3 t) {  // :204-220
   const V3 p0(std::floor(t.x), std::floor(t.y), std::floor(t.z));
   float m = INFINITY;
   for (int z = -1; z < 2; ++z)
@@ -844,7 +844,7 @@ static float Voronoi3D(V3 t) {  // :203-219
 }
 // One noise lookup: texture type, coordinates scaled by s0 then s1 then s2 (the octave node's
 // scale, octave factor and the base node's scale). At a medium interaction the 3-D variants
-// are used on the event point: Perlin and value noise return 0 there as written (:81-84, :140-142).
+// are used on the event point: Perlin and value noise return 0 there as written (:88-91, :140-142).
 static float NoiseAt(uint32_t type, const Event &ev, float s0, float s1, float s2) {
   if (ev.mediumInteraction) {
     const V3 &p = ev.hit->point;
@@ -918,7 +918,7 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
       break;
     }
     case LGPU_NODE_UV: r.t = VT_VEC2; r.v[0] = ev.hit->uv.x; r.v[1] = ev.hit->uv.y; break;
-    case LGPU_NODE_FRESNEL: {  // FresnelInput::SchlickApprox (GraphInputs.cpp:187-199)
+    case LGPU_NODE_FRESNEL: {  // FresnelInput::SchlickApprox (GraphInputs.cpp:188-198)
       const float n1 = ev.eta;
       const float n2 = n.in[0] == LGPU_NONE ? n.p[0] : AsReal(Input(sc, m, i, 0, ev), ev.lanes);
       float R0 = (n1 - n2) / (n1 + n2);
@@ -927,7 +927,7 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
       r.t = VT_SCALAR; r.v[0] = R0 + (1 - R0) * cc * cc * cc * cc * cc;
       break;
     }
-    case LGPU_NODE_SPECTRAL: {  // SpectralInput (GraphInputs.cpp:52-66): a sampled SPD
+    case LGPU_NODE_SPECTRAL: {  // SpectralInput (GraphInputs.cpp:55-71): a sampled SPD
       const Texture &t = sc.textures[(uint32_t)n.p[0]];
       r.t = VT_SPECTRUM;
       if (ev.lanes.mode == HERO4) {
@@ -942,39 +942,39 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
       }
       break;
     }
-    case LGPU_NODE_IMAGE_CHANNEL: {  // ImageTextureChannelInput (GraphInputs.cpp:150-160)
+    case LGPU_NODE_IMAGE_CHANNEL: {  // ImageTextureChannelInput (GraphInputs.cpp:142-151)
       const Texture &t = sc.textures[(uint32_t)n.p[0]];
       float c[4];
       t.Bilinear4(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c);
       r.t = VT_SCALAR; r.v[0] = c[std::min(std::max((int)n.p[1], 0), 3)];
       break;
     }
-    case LGPU_NODE_SPECTRAL_TEXTURE: {  // SpectralTextureInput (GraphInputs.cpp:162-171): RGB spectra
+    case LGPU_NODE_SPECTRAL_TEXTURE: {  // SpectralTextureInput (GraphInputs.cpp:157-165): RGB spectra
       const Texture &t = sc.textures[(uint32_t)n.p[0]];
       float c[3];
       t.Bilinear(Fract(ev.hit->uv.x), Fract(ev.hit->uv.y), c);
       r.t = VT_SPECTRUM; r.s = FromRGB(c, ev.lanes);
       break;
     }
-    case LGPU_NODE_NORMAL:  // SurfaceInfoInput (GraphInputs.cpp:173-181): shading normal
+    case LGPU_NODE_NORMAL:  // SurfaceInfoInput (GraphInputs.cpp:171-177): shading normal
       r.t = VT_VEC3; r.v[0] = ev.hit->normalS.x; r.v[1] = ev.hit->normalS.y; r.v[2] = ev.hit->normalS.z;
       break;
-    case LGPU_NODE_CROSS: {  // CrossProductNode (GraphMaths.cpp:77-86)
+    case LGPU_NODE_CROSS: {  // CrossProductNode (GraphMaths.cpp:75-84)
       const V3 o = Cross(AsVec3(Input(sc, m, i, 0, ev)), AsVec3(Input(sc, m, i, 1, ev)));
       r.t = VT_VEC3; r.v[0] = o.x; r.v[1] = o.y; r.v[2] = o.z;
       break;
     }
-    case LGPU_NODE_SEPARATE_XYZ: {  // SeparateXYZ (GraphConverters.cpp:8-28)
+    case LGPU_NODE_SEPARATE_XYZ: {  // SeparateXYZ (GraphConverters.cpp:8-29)
       const V3 v = AsVec3(Input(sc, m, i, 0, ev));
       const int k = std::min(std::max((int)n.p[0], 0), 2);
       r.t = VT_SCALAR; r.v[0] = k == 0 ? v.x : (k == 1 ? v.y : v.z);
       break;
     }
-    case LGPU_NODE_MERGE_XYZ:  // MergeXYZ (GraphConverters.cpp:63-75)
+    case LGPU_NODE_MERGE_XYZ:  // MergeXYZ (GraphConverters.cpp:62-75)
       r.t = VT_VEC3;
       for (int k = 0; k < 3; ++k) r.v[k] = AsReal(Input(sc, m, i, k, ev), ev.lanes);
       break;
-    case LGPU_NODE_SCALAR_TO_COLOUR: {  // ScalarToColour (GraphConverters.cpp:95-103)
+    case LGPU_NODE_SCALAR_TO_COLOUR: {  // ScalarToColour (GraphConverters.cpp:98-106)
       const float x = AsReal(Input(sc, m, i, 0, ev), ev.lanes);
       r.t = VT_COLOUR; r.v[0] = r.v[1] = r.v[2] = x; r.v[3] = 1;
       break;
@@ -982,7 +982,7 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
     case LGPU_NODE_PERLIN: case LGPU_NODE_VALUE_NOISE: case LGPU_NODE_VORONOI:
       r.t = VT_SCALAR; r.v[0] = NoiseAt(n.type, ev, n.p[0], 1.f, 1.f);
       break;
-    case LGPU_NODE_OCTAVE_NOISE: {  // OctaveNoise (GraphTexture.cpp:156-174)
+    case LGPU_NODE_OCTAVE_NOISE: {  // OctaveNoise (GraphTexture.cpp:157-175)
       float out = 0, amp = .5f, scl = 1;
       const unsigned oct = (unsigned)n.p[1];
       for (unsigned k = 0; k < oct; ++k) {
@@ -1018,12 +1018,12 @@ static Val EvalNode(const Scene &sc, const Material &m, uint32_t i, const Event 
     }
     case LGPU_NODE_DOT: r.t = VT_SCALAR; r.v[0] = Dot(AsVec3(Input(sc, m, i, 0, ev)), AsVec3(Input(sc, m, i, 1, ev))); break;
     case LGPU_NODE_LENGTH: r.t = VT_SCALAR; r.v[0] = Mag(AsVec3(Input(sc, m, i, 0, ev))); break;
-    case LGPU_NODE_MERGE_RGB:  // MergeRGBA (GraphConverters.cpp:78-92); alpha 1 when unconnected
+    case LGPU_NODE_MERGE_RGB:  // MergeRGBA (GraphConverters.cpp:79-95); alpha 1 when unconnected
       r.t = VT_COLOUR;
       for (int k = 0; k < 3; ++k) r.v[k] = AsReal(Input(sc, m, i, k, ev), ev.lanes);
       r.v[3] = n.in[3] == LGPU_NONE ? 1.f : AsReal(Input(sc, m, i, 3, ev), ev.lanes);
       break;
-    case LGPU_NODE_SEPARATE_R: {  // SeparateRGBA (GraphConverters.cpp:31-60)
+    case LGPU_NODE_SEPARATE_R: {  // SeparateRGBA (GraphConverters.cpp:32-59)
       const Val x = Input(sc, m, i, 0, ev);
       float c[4];
       AsColour(x, c);
