@@ -325,6 +325,30 @@ lgpu_status lgpu_film_read_rgba(lgpu_film film, float *rgba);
  * lambdaStartProgressiveRenderer: Render.cpp:43-86, MosaicRenderer.cpp:22-80) */
 lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
 
+/* ---------------- progressive rendering (ProgressiveRender.cpp:13-62;
+ * lambdaCreate/Release/Start/Stop/SetCallback/GetData ProgressiveRenderer,
+ * Lambda.h:186-202, Lambda.cpp:431-458) ----------------------------------- */
+typedef struct lgpu_progressive_s *lgpu_progressive;
+typedef void (*lgpu_progressive_callback)(void *user);
+/* desc is copied; its spp range [b, e) sets the pass: pass k renders sample indices
+ * [b + k (e - b), b + (k + 1) (e - b)) of every pixel into desc->film. */
+lgpu_status lgpu_progressive_create(const lgpu_render_desc *desc, lgpu_progressive *out);
+/* Stops a running renderer (the pass in flight completes) and frees it. */
+lgpu_status lgpu_progressive_destroy(lgpu_progressive p);
+/* Called on the worker thread after each pass, once the new image can be read. */
+lgpu_status lgpu_progressive_set_callback(lgpu_progressive p, lgpu_progressive_callback cb, void *user);
+/* Starts the worker and returns at once; a second start while running does nothing. */
+lgpu_status lgpu_progressive_start(lgpu_progressive p);
+/* Returns after the pass in flight has completed; the film keeps what was rendered. */
+lgpu_status lgpu_progressive_stop(lgpu_progressive p);
+/* Film::Clear and restart at the first pass (between passes when running). */
+lgpu_status lgpu_progressive_clear(lgpu_progressive p);
+/* Copies the latest completed RGBA32F image (4 floats per pixel, film size) and the number of
+ * passes in it; either may be NULL. Returns the worker's error if a pass failed. */
+lgpu_status lgpu_progressive_read(lgpu_progressive p, float *rgba, uint32_t *passes);
+/* 1 while the worker runs. */
+int lgpu_progressive_running(lgpu_progressive p);
+
 /* Per-path radiance for an explicit list of (pixel, sample) pairs. out holds
  * 5 floats per path: radiance lanes 0..3 (RGB in lanes 0..2, or the four hero
  * wavelengths) and lambda0 (HERO4) or 0. desc->film may be NULL. */

@@ -346,6 +346,7 @@ lgpu_status lgpu_film_destroy(lgpu_film f) {
 }
 lgpu_status lgpu_film_clear(lgpu_film f) {
   CHECK_ARG(f, "film_clear: null film");
+  std::lock_guard<std::mutex> lk(f->dev->mu);
   LGPU_HIP(hipSetDevice(f->dev->device));
   LGPU_HIP(hipMemsetAsync(f->sum.ptr, 0, f->sum.bytes, f->dev->stream));
   LGPU_HIP(hipMemsetAsync(f->count.ptr, 0, f->count.bytes, f->dev->stream));
@@ -354,6 +355,7 @@ lgpu_status lgpu_film_clear(lgpu_film f) {
 }
 lgpu_status lgpu_film_read(lgpu_film f, double *rgb, uint32_t *counts) {
   CHECK_ARG(f, "film_read: null film");
+  std::lock_guard<std::mutex> lk(f->dev->mu);
   LGPU_HIP(hipSetDevice(f->dev->device));
   LGPU_HIP(hipStreamSynchronize(f->dev->stream));
   if (rgb) LGPU_HIP(hipMemcpy(rgb, f->sum.ptr, 24ull * f->width * f->height, hipMemcpyDeviceToHost));
@@ -363,23 +365,8 @@ lgpu_status lgpu_film_read(lgpu_film f, double *rgb, uint32_t *counts) {
 // Film::ToRGBTexture (Film.cpp:11-24): mean per pixel; pixels with no samples read 0.
 lgpu_status lgpu_film_read_rgba(lgpu_film f, float *rgba) {
   CHECK_ARG(f && rgba, "film_read_rgba: null argument");
-  const size_t n = (size_t)f->width * f->height;
-  std::string err;
-  double *sum = new double[3 * n];
-  uint32_t *cnt = new uint32_t[n];
-  lgpu_status st = lgpu_film_read(f, sum, cnt);
-  if (st == LGPU_OK) {
-    for (size_t i = 0; i < n; ++i) {
-      const double inv = cnt[i] ? 1.0 / cnt[i] : 0.0;
-      rgba[4 * i + 0] = (float)(sum[3 * i] * inv);
-      rgba[4 * i + 1] = (float)(sum[3 * i + 1] * inv);
-      rgba[4 * i + 2] = (float)(sum[3 * i + 2] * inv);
-      rgba[4 * i + 3] = 1.f;
-    }
-  }
-  delete[] sum;
-  delete[] cnt;
-  return st;
+  std::lock_guard<std::mutex> lk(f->dev->mu);
+  return FilmToRgba(f, rgba);
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,11 +71,15 @@ struct HostLight {
   int mesh = -1;  // mesh lights
 };
 
+// Film::ToRGBTexture into host memory (4 floats per pixel); the caller holds the device mutex.
+lgpu_status FilmToRgba(lgpu_film f, float *rgba);
+
 }  // namespace lgpu
 
 struct lgpu_device_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  std::mutex mu;  // one caller at a time on the stream and the wave pool (renders, film reads)
   hipEvent_t ev[8] = {};
   void *wave = nullptr;               // path-state pool of the render loop
   void (*wave_free)(void *) = nullptr;

@@ -590,6 +590,8 @@ extern "C" lgpu_status lgpu_render(const lgpu_render_desc *d, lgpu_render_stats 
   if (st != LGPU_OK) return st;
   if (!d->film) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: null film");
   if (d->spp_end <= d->spp_begin) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: empty spp range");
+  if (d->film->dev != d->scene->dev) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: film and scene on different devices");
+  std::lock_guard<std::mutex> lk(d->scene->dev->mu);
   lgpu_film f = d->film;
   const uint64_t total = (uint64_t)f->width * f->height * (d->spp_end - d->spp_begin);
   return RunWaves(d, f->width, f->height, total, nullptr, nullptr, f->sum.as<double>(), f->count.as<uint32_t>(), nullptr, stats);
@@ -603,6 +605,7 @@ extern "C" lgpu_status lgpu_trace_paths(const lgpu_render_desc *d, uint32_t num,
   if (!num) return LGPU_OK;
   if (!pixels || !samples || !out) return Fail(LGPU_ERR_INVALID_ARGUMENT, "trace_paths: null array");
   lgpu_film f = d->film;
+  std::lock_guard<std::mutex> lk(d->scene->dev->mu);
   const uint64_t npx = (uint64_t)f->width * f->height;
   for (uint32_t i = 0; i < num; ++i)
     if (pixels[i] >= npx) return Fail(LGPU_ERR_INVALID_ARGUMENT, "trace_paths: pixel index out of range");
@@ -622,6 +625,7 @@ static lgpu_status Query(lgpu_scene s, uint32_t n, const float *rays, uint32_t *
   if (!s || !rays || (!hits && !occ)) return Fail(LGPU_ERR_INVALID_ARGUMENT, "query: null argument");
   if (!s->committed) return Fail(LGPU_ERR_STATE, "query: scene not committed");
   if (!n) return LGPU_OK;
+  std::lock_guard<std::mutex> lk(s->dev->mu);
   LGPU_HIP(hipSetDevice(s->dev->device));
   (void)hipGetLastError();
   DevBuf dr, dh, dsp;

@@ -24,7 +24,9 @@ EXPORTS = [
     "lgpu_camera_init", "lgpu_camera_set_rotation", "lgpu_camera_set_forwards", "lgpu_camera_set_fov",
     "lgpu_film_create", "lgpu_film_destroy", "lgpu_film_clear", "lgpu_film_read", "lgpu_film_read_rgba",
     "lgpu_render", "lgpu_trace_paths", "lgpu_intersect", "lgpu_occluded", "lgpu_philox", "lgpu_copy_bench",
-    "lgpu_scene_add_proxy", "lgpu_scene_add_instance",
+    "lgpu_scene_add_proxy", "lgpu_scene_add_instance", "lgpu_progressive_create", "lgpu_progressive_destroy",
+    "lgpu_progressive_set_callback", "lgpu_progressive_start", "lgpu_progressive_stop", "lgpu_progressive_clear",
+    "lgpu_progressive_read", "lgpu_progressive_running",
 ]
 
 
@@ -53,6 +55,10 @@ def lib():
             "lgpu_occluded": [vp, u32, vp, vp], "lgpu_philox": [vp, u32, vp, vp, vp],
             "lgpu_copy_bench": [vp, C.c_uint64, u32, vp],
             "lgpu_scene_add_proxy": [vp, vp, vp], "lgpu_scene_add_instance": [vp, u32, vp, u32, vp],
+            "lgpu_progressive_create": [vp, vp], "lgpu_progressive_destroy": [vp],
+            "lgpu_progressive_set_callback": [vp, vp, vp], "lgpu_progressive_start": [vp],
+            "lgpu_progressive_stop": [vp], "lgpu_progressive_clear": [vp], "lgpu_progressive_read": [vp, vp, vp],
+            "lgpu_progressive_running": [vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -213,6 +219,53 @@ class Film:
         out = np.zeros((self.height, self.width, 4), dtype=np.float32)
         _check(lib().lgpu_film_read_rgba(self.h, _ptr(out)))
         return out
+
+
+_CALLBACK = C.CFUNCTYPE(None, C.c_void_p)
+
+
+class Progressive:
+    """Progressive renderer (ProgressiveRender.cpp:13-62): a native worker thread renders passes of
+    desc's spp-range size into desc's film; read() copies the latest completed RGBA32F image."""
+
+    def __init__(self, desc):
+        self.desc = desc  # keeps the film and scene objects referenced by the caller alive
+        self.width, self.height = None, None
+        h = C.c_void_p()
+        _check(lib().lgpu_progressive_create(C.byref(desc), C.byref(h)))
+        self.h = h
+        self._cb = None
+
+    def set_callback(self, fn):
+        """fn() runs on the worker thread after each pass."""
+        self._cb = _CALLBACK(lambda user: fn())
+        _check(lib().lgpu_progressive_set_callback(self.h, C.cast(self._cb, C.c_void_p), None))
+
+    def start(self):
+        _check(lib().lgpu_progressive_start(self.h))
+
+    def stop(self):
+        _check(lib().lgpu_progressive_stop(self.h))
+
+    def clear(self):
+        _check(lib().lgpu_progressive_clear(self.h))
+
+    def running(self):
+        return bool(lib().lgpu_progressive_running(self.h))
+
+    def read(self, width, height):
+        out = np.zeros((height, width, 4), dtype=np.float32)
+        n = C.c_uint32()
+        _check(lib().lgpu_progressive_read(self.h, _ptr(out), C.byref(n)))
+        return out, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgpu_progressive_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def render_desc(scene, film, camera, spp_begin=0, spp_end=1, max_bounces=8, min_bounces=3,

@@ -69,3 +69,14 @@ def test_device_create_without_gpu_reports_no_device():
         gpu.lib().lgpu_device_destroy(h)
         pytest.skip("a GPU is present")
     assert st == abi.ERR_NO_DEVICE
+
+
+def test_progressive_validates_arguments_without_a_device():
+    """Null descriptors and handles are rejected before any device call."""
+    L = gpu.lib()
+    out = C.c_void_p()
+    assert L.lgpu_progressive_create(None, C.byref(out)) == abi.ERR_INVALID_ARGUMENT
+    assert L.lgpu_progressive_start(None) == abi.ERR_INVALID_ARGUMENT
+    assert L.lgpu_progressive_read(None, None, None) == abi.ERR_INVALID_ARGUMENT
+    assert L.lgpu_progressive_destroy(None) == abi.OK
+    assert L.lgpu_progressive_running(None) == 0
