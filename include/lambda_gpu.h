@@ -325,6 +325,26 @@ lgpu_status lgpu_film_read_rgba(lgpu_film film, float *rgba);
  * lambdaStartProgressiveRenderer: Render.cpp:43-86, MosaicRenderer.cpp:22-80) */
 lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
 
+/* ---------------- OBJ ingestion (AssetImporter::Import with aiProcess_Triangulate |
+ * aiProcess_CalcTangentSpace, AssetImporter.cpp:5-15; MeshImport::LoadMeshVertexBuffers,
+ * MeshImport.cpp:11-92). Host only, no device needed. ------------------------------- */
+typedef struct lgpu_obj_s *lgpu_obj;
+typedef struct lgpu_obj_material {
+  const char *name;          /* usemtl / newmtl name ("" for faces without one) */
+  float kd[3], ke[3];        /* MTL diffuse and emissive colours, when given */
+  uint32_t has_kd, has_ke;
+} lgpu_obj_material;
+/* Parses an OBJ file (and the MTL files it names, for Kd / Ke): one mesh per run of faces of one
+ * object and material, unshared corners, triangulated, uvs flipped to (1 - u, 1 - v); normals and
+ * tangents (w = bitangent sign) only when every corner has a normal and a uv. */
+lgpu_status lgpu_obj_load(const char *path, lgpu_obj *out);
+lgpu_status lgpu_obj_counts(lgpu_obj obj, uint32_t *num_meshes, uint32_t *num_materials);
+/* Mesh i as a descriptor for lgpu_scene_add_mesh (arrays owned by obj until lgpu_obj_destroy);
+ * desc->material is the index of the OBJ material, to be mapped to a scene material id. */
+lgpu_status lgpu_obj_mesh(lgpu_obj obj, uint32_t i, lgpu_mesh_desc *desc, const char **name);
+lgpu_status lgpu_obj_material_info(lgpu_obj obj, uint32_t i, lgpu_obj_material *out);
+lgpu_status lgpu_obj_destroy(lgpu_obj obj);
+
 /* ---------------- progressive rendering (ProgressiveRender.cpp:13-62;
  * lambdaCreate/Release/Start/Stop/SetCallback/GetData ProgressiveRenderer,
  * Lambda.h:186-202, Lambda.cpp:431-458) ----------------------------------- */

@@ -91,6 +91,11 @@ class RenderDesc(C.Structure):
 RENDER_COUNT_TRAVERSAL = 1
 
 
+class ObjMaterial(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("kd", C.c_float * 3), ("ke", C.c_float * 3), ("has_kd", C.c_uint32),
+                ("has_ke", C.c_uint32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_camera", C.c_double), ("ms_extend", C.c_double),
                 ("ms_shade", C.c_double), ("ms_shadow", C.c_double), ("samples", C.c_uint64),

@@ -26,7 +26,8 @@ EXPORTS = [
     "lgpu_render", "lgpu_trace_paths", "lgpu_intersect", "lgpu_occluded", "lgpu_philox", "lgpu_copy_bench",
     "lgpu_scene_add_proxy", "lgpu_scene_add_instance", "lgpu_progressive_create", "lgpu_progressive_destroy",
     "lgpu_progressive_set_callback", "lgpu_progressive_start", "lgpu_progressive_stop", "lgpu_progressive_clear",
-    "lgpu_progressive_read", "lgpu_progressive_running",
+    "lgpu_progressive_read", "lgpu_progressive_running", "lgpu_obj_load", "lgpu_obj_counts", "lgpu_obj_mesh",
+    "lgpu_obj_material_info", "lgpu_obj_destroy",
 ]
 
 
@@ -59,6 +60,8 @@ def lib():
             "lgpu_progressive_set_callback": [vp, vp, vp], "lgpu_progressive_start": [vp],
             "lgpu_progressive_stop": [vp], "lgpu_progressive_clear": [vp], "lgpu_progressive_read": [vp, vp, vp],
             "lgpu_progressive_running": [vp],
+            "lgpu_obj_load": [C.c_char_p, vp], "lgpu_obj_counts": [vp, vp, vp], "lgpu_obj_mesh": [vp, u32, vp, vp],
+            "lgpu_obj_material_info": [vp, u32, vp], "lgpu_obj_destroy": [vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -219,6 +222,41 @@ class Film:
         out = np.zeros((self.height, self.width, 4), dtype=np.float32)
         _check(lib().lgpu_film_read_rgba(self.h, _ptr(out)))
         return out
+
+
+def load_obj(path):
+    """OBJ ingestion through the library (host only): ([mesh dict], [material dict]). Mesh arrays
+    are copied out; 'material' indexes the material list."""
+    L = lib()
+    h = C.c_void_p()
+    _check(L.lgpu_obj_load(str(path).encode(), C.byref(h)))
+    try:
+        nm, nmat = C.c_uint32(), C.c_uint32()
+        _check(L.lgpu_obj_counts(h, C.byref(nm), C.byref(nmat)))
+        meshes = []
+        for i in range(nm.value):
+            d = abi.MeshDesc()
+            name = C.c_char_p()
+            _check(L.lgpu_obj_mesh(h, i, C.byref(d), C.byref(name)))
+            nv, nt = d.num_vertices, d.num_triangles
+
+            def arr(p, n):
+                return None if not p else np.ctypeslib.as_array(p, shape=(n,)).copy()
+            m = dict(name=name.value.decode(), material=d.material,
+                     positions=arr(d.positions, 3 * nv).reshape(-1, 3), indices=arr(d.indices, 3 * nt).reshape(-1, 3))
+            for k, w in (("normals", 3), ("tangents", 4), ("uvs", 2)):
+                a = arr(getattr(d, k), w * nv)
+                m[k] = None if a is None else a.reshape(-1, w)
+            meshes.append(m)
+        mats = []
+        for i in range(nmat.value):
+            om = abi.ObjMaterial()
+            _check(L.lgpu_obj_material_info(h, i, C.byref(om)))
+            mats.append(dict(name=om.name.decode(), kd=tuple(om.kd) if om.has_kd else None,
+                             ke=tuple(om.ke) if om.has_ke else None))
+        return meshes, mats
+    finally:
+        L.lgpu_obj_destroy(h)
 
 
 _CALLBACK = C.CFUNCTYPE(None, C.c_void_p)

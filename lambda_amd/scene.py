@@ -110,6 +110,33 @@ class SceneDesc:
         self.n_tris += len(ops[proxy][1]["indices"])
         return sum(1 for o in self.ops if o[0] in ("mesh", "instance")) - 1
 
+    def add_obj(self, path, materials=None, transform=None):
+        """Meshes of an OBJ file (library loader). materials: {OBJ material name: material id};
+        others follow MaterialImport::BuildMaterial (MaterialImport.cpp:233-281): a Kd colour gives
+        Oren-Nayar (albedo Kd, sigma socket 2), anything else the magenta Lambertian of a missing
+        material. transform: optional 3x4 column-major affine (the meshes become instances).
+        Returns the object ids."""
+        from . import gpu
+        meshes, mats = gpu.load_obj(path)
+        ids = {}
+        for i, m in enumerate(mats):
+            if materials and m["name"] in materials:
+                ids[i] = materials[m["name"]]
+            elif m["kd"] is not None:
+                ids[i] = self.add_material(Material([node(abi.NODE_RGB, p=m["kd"]), node(abi.NODE_SCALAR, p=[2.0]),
+                                                     node(abi.NODE_OREN_NAYAR_BRDF, [0, 1])], bxdf=2))
+            else:
+                ids[i] = self.add_material(lambertian((1.0, 0.0, 1.0)))
+        out = []
+        for m in meshes:
+            args = (m["positions"], m["indices"], ids[m["material"]])
+            kw = dict(normals=m["normals"], tangents=m["tangents"], uvs=m["uvs"])
+            if transform is None:
+                out.append(self.add_mesh(*args, **kw))
+            else:
+                out.append(self.add_instance(self.add_proxy(*args, **kw), transform))
+        return out
+
     def add_point_light(self, position, intensity, rgb=(1, 1, 1), temperature=0.0):
         self.ops.append(("light", dict(type=abi.LIGHT_POINT, position=position, intensity=intensity, rgb=rgb,
                                        temperature=temperature)))

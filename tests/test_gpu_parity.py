@@ -356,3 +356,45 @@ def test_instanced_scene_per_path_parity(dev, estimator):
     ok = _compare_paths(g, c)
     print("instanced", estimator, "matching", ok.mean())
     assert ok.mean() >= 0.999
+
+
+def _write_obj(path, level=2):
+    """An OBJ with a smooth icosphere (vt, vn) as object 'Ball' and a flat quad without normals."""
+    v, f = scene.icosphere(level)
+    n = scene._vertex_normals(v, f)
+    uv = np.stack([0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * np.pi), 0.5 + v[:, 1] * 0.5], axis=1)
+    lines = ["mtllib ball.mtl", "o Ball"]
+    lines += ["v %.6f %.6f %.6f" % tuple(0.3 * p + np.array([0.2, -0.6, 0.1])) for p in v]
+    lines += ["vt %.6f %.6f" % tuple(t) for t in uv]
+    lines += ["vn %.6f %.6f %.6f" % tuple(x) for x in n]
+    lines += ["usemtl Red"] + ["f %d/%d/%d %d/%d/%d %d/%d/%d" % (a + 1, a + 1, a + 1, b + 1, b + 1, b + 1, c + 1, c + 1, c + 1)
+                               for a, b, c in f]
+    k = len(v)
+    lines += ["o Card", "v -0.8 -0.9 -0.6", "v -0.2 -0.9 -0.6", "v -0.2 -0.1 -0.7", "v -0.8 -0.1 -0.7",
+              "usemtl Blue", "f %d %d %d %d" % (k + 1, k + 2, k + 3, k + 4)]
+    path.with_name("ball.mtl").write_text("newmtl Red\nKd 0.7 0.2 0.2\nnewmtl Blue\nKd 0.2 0.3 0.8\n")
+    path.write_text("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("estimator", [abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED])
+def test_obj_scene_per_path_parity(dev, tmp_path, estimator):
+    """Meshes ingested by the library's OBJ loader (smooth sphere with tangents, flat quad) in the
+    Cornell room, once directly and once as instances."""
+    p = tmp_path / "ball.obj"
+    _write_obj(p)
+    sd = scene.cornell_box()
+    sd.add_obj(p)
+    sd.add_obj(p, transform=scene.affine((0, 0.7, 0), (0.8, 1.2, 0.8), (0.1, 0.5, -0.3)))
+    W = H = 48
+    s, k1 = gpu_scene(dev, sd)
+    o, k2 = helpers.oracle_scene(sd)
+    cam = scene.cornell_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    d = gpu.render_desc(s, film, cam, 0, 2, spectral=abi.SPECTRAL_HERO4, estimator=estimator)
+    g = gpu.trace_paths(d, pixels, samples)
+    c = o.trace_paths(helpers.desc(cam, 0, 2, 8, 3, abi.SPECTRAL_HERO4, estimator), W, H, pixels, samples)
+    ok = _compare_paths(g, c)
+    print("obj", estimator, "matching", ok.mean())
+    assert ok.mean() >= 0.999
