@@ -325,6 +325,25 @@ lgpu_status lgpu_film_read_rgba(lgpu_film film, float *rgba);
  * lambdaStartProgressiveRenderer: Render.cpp:43-86, MosaicRenderer.cpp:22-80) */
 lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
 
+/* ---------------- post-processing (ToneMap.cpp:5-73, PostProcessing.cpp:16-45,
+ * Texture::SaveToImageFile Texture.cpp:19-81) ------------------------------------- */
+typedef enum lgpu_tonemap { LGPU_TONEMAP_NONE = 0, LGPU_TONEMAP_ACES_FITTED = 1 } lgpu_tonemap;
+typedef struct lgpu_post_desc {
+  uint32_t tonemap;          /* ToneMap::method */
+  uint32_t clamp;            /* ToneMap::clamp: channels to [0, 1] after the tone map */
+  uint32_t gamma;            /* SaveToImageFile gamma correction c^(1/2.2), 8-bit output only */
+  uint32_t estimator_mode;   /* faithful: ACES matrices applied transposed and the convolution's
+                                edge wrap, as written; corrected: rows and clamp-to-edge */
+  const float *kernel;       /* ConvolutionFilter kernel, kernel_w x kernel_h weights, or NULL */
+  uint32_t kernel_w, kernel_h;
+  float kernel_intensity;    /* ConvolutionFilter::intensity */
+} lgpu_post_desc;
+/* Film mean -> convolution -> tone map -> clamp on the device; rgba receives 4 floats per pixel,
+ * rgba8 the gamma-corrected PackChannels bytes (alpha 255). Either may be NULL. */
+lgpu_status lgpu_film_post(lgpu_film film, const lgpu_post_desc *desc, float *rgba, uint8_t *rgba8);
+/* 8-bit RGBA PNG, rows in film order (stored deflate blocks). */
+lgpu_status lgpu_write_png(const char *path, const uint8_t *rgba8, uint32_t width, uint32_t height);
+
 /* ---------------- OBJ ingestion (AssetImporter::Import with aiProcess_Triangulate |
  * aiProcess_CalcTangentSpace, AssetImporter.cpp:5-15; MeshImport::LoadMeshVertexBuffers,
  * MeshImport.cpp:11-92). Host only, no device needed. ------------------------------- */

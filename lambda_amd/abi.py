@@ -91,6 +91,29 @@ class RenderDesc(C.Structure):
 RENDER_COUNT_TRAVERSAL = 1
 
 
+TONEMAP_NONE, TONEMAP_ACES_FITTED = 0, 1
+
+
+class PostDesc(C.Structure):
+    _fields_ = [("tonemap", C.c_uint32), ("clamp", C.c_uint32), ("gamma", C.c_uint32), ("estimator_mode", C.c_uint32),
+                ("kernel", FP), ("kernel_w", C.c_uint32), ("kernel_h", C.c_uint32), ("kernel_intensity", C.c_float)]
+
+
+def post_desc(tonemap=TONEMAP_ACES_FITTED, clamp=True, gamma=True, estimator=ESTIMATOR_CORRECTED, kernel=None,
+              intensity=1.0):
+    """lgpu_post_desc; returns (desc, keep-alive array)."""
+    import numpy as np
+    d = PostDesc()
+    d.tonemap, d.clamp, d.gamma, d.estimator_mode = tonemap, int(clamp), int(gamma), estimator
+    k = None
+    if kernel is not None:
+        k = np.ascontiguousarray(kernel, dtype=np.float32)
+        d.kernel = k.ctypes.data_as(FP)
+        d.kernel_h, d.kernel_w = k.shape
+        d.kernel_intensity = intensity
+    return d, k
+
+
 class ObjMaterial(C.Structure):
     _fields_ = [("name", C.c_char_p), ("kd", C.c_float * 3), ("ke", C.c_float * 3), ("has_kd", C.c_uint32),
                 ("has_ke", C.c_uint32)]

@@ -27,7 +27,7 @@ EXPORTS = [
     "lgpu_scene_add_proxy", "lgpu_scene_add_instance", "lgpu_progressive_create", "lgpu_progressive_destroy",
     "lgpu_progressive_set_callback", "lgpu_progressive_start", "lgpu_progressive_stop", "lgpu_progressive_clear",
     "lgpu_progressive_read", "lgpu_progressive_running", "lgpu_obj_load", "lgpu_obj_counts", "lgpu_obj_mesh",
-    "lgpu_obj_material_info", "lgpu_obj_destroy",
+    "lgpu_obj_material_info", "lgpu_obj_destroy", "lgpu_film_post", "lgpu_write_png",
 ]
 
 
@@ -62,6 +62,7 @@ def lib():
             "lgpu_progressive_running": [vp],
             "lgpu_obj_load": [C.c_char_p, vp], "lgpu_obj_counts": [vp, vp, vp], "lgpu_obj_mesh": [vp, u32, vp, vp],
             "lgpu_obj_material_info": [vp, u32, vp], "lgpu_obj_destroy": [vp],
+            "lgpu_film_post": [vp, vp, vp, vp], "lgpu_write_png": [C.c_char_p, vp, u32, u32],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -218,10 +219,23 @@ class Film:
         _check(lib().lgpu_film_read(self.h, _ptr(rgb), _ptr(cnt)))
         return rgb, cnt
 
+    def post(self, desc):
+        """Post-processed image: (RGBA32F, RGBA8) of the film (lgpu_film_post)."""
+        f32 = np.zeros((self.height, self.width, 4), dtype=np.float32)
+        u8 = np.zeros((self.height, self.width, 4), dtype=np.uint8)
+        _check(lib().lgpu_film_post(self.h, C.byref(desc), _ptr(f32), _ptr(u8)))
+        return f32, u8
+
     def read_rgba(self):
         out = np.zeros((self.height, self.width, 4), dtype=np.float32)
         _check(lib().lgpu_film_read_rgba(self.h, _ptr(out)))
         return out
+
+
+def write_png(path, rgba8):
+    rgba8 = np.ascontiguousarray(rgba8, dtype=np.uint8)
+    h, w = rgba8.shape[:2]
+    _check(lib().lgpu_write_png(str(path).encode(), _ptr(rgba8), w, h))
 
 
 def load_obj(path):

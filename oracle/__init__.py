@@ -67,6 +67,8 @@ def lib():
         L.oracle_hg_sample.argtypes = [f32, vp, f32, f32, C.c_int, vp]
         L.oracle_hg_sample.restype = f32
         L.oracle_eval_node.argtypes = [vp, u32, u32, vp, C.c_int, vp, vp]
+        L.oracle_post.argtypes = [vp, u32, u32, vp, vp, vp]
+        L.oracle_post.restype = None
         L.oracle_scene_add_proxy.argtypes = [vp, vp]
         L.oracle_scene_add_instance.argtypes = [vp, u32, vp, u32]
         L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
@@ -225,6 +227,16 @@ def hg_sample(g, wo, u0, u1, corrected=True):
     wo = _v3(wo)
     p = lib().oracle_hg_sample(g, _ptr(wo), u0, u1, int(corrected), _ptr(wi))
     return p, wi
+
+
+def post(rgba, desc):
+    """Post-processing of an RGBA32F image (h, w, 4): (RGBA32F, RGBA8)."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[:2]
+    o32 = np.zeros((h, w, 4), dtype=np.float32)
+    o8 = np.zeros((h, w, 4), dtype=np.uint8)
+    lib().oracle_post(_ptr(rgba), w, h, C.byref(desc), _ptr(o32), _ptr(o8))
+    return o32, o8
 
 
 def tree_importance(center, diag, axis, theta_o, theta_e, power, p, n, corrected=True):

@@ -3297,6 +3297,71 @@ static Lanes MkLanes(int mode, const float *lambda) {
   for (int i = 0; i < 4; ++i) l.lambda[i] = lambda ? lambda[i] : 0.f;
   return l;
 }
+// ---------------------------------------------------------------- post-processing
+// ConvolutionFilter (PostProcessing.cpp:16-45), AcesFitted + ClampTextureChannels (ToneMap.cpp:19-73),
+// Gamma + PackChannels (Texture.cpp:19-32). in: RGBA32F film means; out32 (RGBA) and out8 (RGBA8)
+// may be null. asWritten: ACES matrices by the transpose and the convolution's unsigned edge wrap.
+static const float ACES_IN[3][3] = {{0.59719f, 0.35458f, 0.04823f}, {0.07600f, 0.90834f, 0.01566f}, {0.02840f, 0.13383f, 0.83777f}};
+static const float ACES_OUT[3][3] = {{1.60475f, -0.53108f, -0.07367f}, {-0.10208f, 1.10813f, -0.00605f}, {-0.00327f, -0.07276f, 1.07602f}};
+static V3 Mat3(const float m[3][3], const V3 &v, bool asWritten) {  // MulVec3Mat3x3 (ToneMap.cpp:19-24)
+  if (asWritten) return V3(m[0][0] * v.x + m[1][0] * v.y + m[2][0] * v.z, m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z,
+                           m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z);
+  return V3(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+            m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z);
+}
+static float RrtFit(float v) {  // RRTAndOTDFit (ToneMap.cpp:29-33)
+  return (v * (v + 0.0245786f) - 0.000090537f) / (v * (v * 0.983729f + 0.4329510f) + 0.238081f);
+}
+static int PackChannel(float c) {  // out-of-range (int) conversion as x86 gives it: INT_MIN
+  const float s = c * 256.f;
+  const int i = (s > -2147483648.f && s < 2147483648.f) ? (int)s : INT32_MIN;
+  return std::max(0, std::min(255, i));
+}
+void oracle_post(const float *in, uint32_t w, uint32_t h, const lgpu_post_desc *d, float *out32, uint8_t *out8) {
+  const bool aw = d->estimator_mode == LGPU_ESTIMATOR_FAITHFUL;
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t x = 0; x < w; ++x) {
+      V3 c;
+      if (d->kernel) {
+        c = V3(0, 0, 0);
+        const uint32_t kw2 = d->kernel_w / 2, kh2 = d->kernel_h / 2;
+        for (uint32_t ky = 0; ky < d->kernel_h; ++ky)
+          for (uint32_t kx = 0; kx < d->kernel_w; ++kx) {
+            uint32_t sx, sy;
+            if (aw) {  // maths::Clamp(_x + x - kw2, 0u, w - 1) on unsigned values
+              sx = std::min(x + kx - kw2, w - 1);
+              sy = std::min(y + ky - kh2, h - 1);
+            } else {
+              sx = (uint32_t)std::max(0, std::min((int)w - 1, (int)x + (int)kx - (int)kw2));
+              sy = (uint32_t)std::max(0, std::min((int)h - 1, (int)y + (int)ky - (int)kh2));
+            }
+            const float k = d->kernel[ky * d->kernel_w + kx] * d->kernel_intensity;
+            const float *t = in + 4 * ((size_t)sy * w + sx);
+            c.x += k * t[0]; c.y += k * t[1]; c.z += k * t[2];
+          }
+      } else {
+        const float *t = in + 4 * ((size_t)y * w + x);
+        c = V3(t[0], t[1], t[2]);
+      }
+      if (d->tonemap == LGPU_TONEMAP_ACES_FITTED) {
+        c = Mat3(ACES_IN, c, aw);
+        c = V3(RrtFit(c.x), RrtFit(c.y), RrtFit(c.z));
+        c = Mat3(ACES_OUT, c, aw);
+      }
+      if (d->clamp) c = V3(Clampf(c.x, 0.f, 1.f), Clampf(c.y, 0.f, 1.f), Clampf(c.z, 0.f, 1.f));
+      const size_t i = (size_t)y * w + x;
+      if (out32) { out32[4 * i] = c.x; out32[4 * i + 1] = c.y; out32[4 * i + 2] = c.z; out32[4 * i + 3] = 1.f; }
+      if (out8) {
+        if (d->gamma) {
+          const float g = (float)(1.0 / 2.2);
+          c = V3(std::pow(c.x, g), std::pow(c.y, g), std::pow(c.z, g));
+        }
+        out8[4 * i] = (uint8_t)PackChannel(c.x); out8[4 * i + 1] = (uint8_t)PackChannel(c.y);
+        out8[4 * i + 2] = (uint8_t)PackChannel(c.z); out8[4 * i + 3] = 255;
+      }
+    }
+}
+
 // One shader-node evaluation (test hook). ev[11] = {u, v, point xyz, shading normal xyz, eta,
 // woL.y, medium interaction (0/1)}; out[4] receives the value (a spectrum's lanes, or the
 // components of a scalar / vec / colour). Returns the value type (VT_*) or -1.

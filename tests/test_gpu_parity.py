@@ -398,3 +398,26 @@ def test_obj_scene_per_path_parity(dev, tmp_path, estimator):
     ok = _compare_paths(g, c)
     print("obj", estimator, "matching", ok.mean())
     assert ok.mean() >= 0.999
+
+
+@pytest.mark.parametrize("estimator", [abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED])
+def test_film_post_matches_oracle(dev, tmp_path, estimator):
+    """Device post-processing (convolution, ACES fit, clamp, gamma, 8-bit packing) of a rendered film
+    against the CPU checker applied to the same film means; PNG round trip."""
+    import oracle
+    W, H = 48, 40
+    s, k1 = gpu_scene(dev, scene.cornell_box())
+    film = gpu.Film(dev, W, H)
+    gpu.render(gpu.render_desc(s, film, scene.cornell_camera(W, H), 0, 8, spectral=abi.SPECTRAL_HERO4))
+    means = film.read_rgba()
+    kern = np.array([[1, 2, 1], [2, 4, 2], [1, 2, 1]], dtype=np.float32)
+    for kernel in (None, kern):
+        d, keep = abi.post_desc(estimator=estimator, kernel=kernel, intensity=1.0 / 16)
+        g32, g8 = film.post(d)
+        c32, c8 = oracle.post(means, d)
+        np.testing.assert_allclose(g32, c32, rtol=1e-6, atol=1e-7)
+        diff = np.abs(g8.astype(int) - c8.astype(int))
+        assert diff.max() <= 1 and (diff == 0).mean() >= 0.999
+    p = tmp_path / "film.png"
+    gpu.write_png(p, g8)
+    assert p.stat().st_size > W * H * 4
