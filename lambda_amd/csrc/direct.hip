@@ -51,7 +51,7 @@ __device__ inline float PdfLiRay(const DevScene &sc, uint32_t pick, const Hit &h
   return 0.f;
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, 3) k_shade_direct(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                          uint32_t n, Queues q, uint32_t *spillBase) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
   __shared__ TreeImp treeLds[kTreeLds];

@@ -148,7 +148,7 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
   q.shadow[slot] = r;
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, 3) k_shade(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, Queues q) {
   __shared__ TreeImp treeLds[kTreeLds];
   LoadTreeLds(sc, treeLds);

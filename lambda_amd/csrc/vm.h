@@ -43,6 +43,7 @@ LGPU_HD float Perlin2D(float tx, float ty) {
   const float gx = floorf(tx), gy = floorf(ty);
   float gxs[4], gys[4];
   const float cx[4] = {gx, gx + 1, gx + 1, gx}, cy[4] = {gy, gy, gy + 1, gy + 1};
+#pragma unroll 1
   for (int k = 0; k < 4; ++k) {
     float u = HashUnit(Hash21(LatticeU32(cx[k]), LatticeU32(cy[k])));
     u *= 6.28318530717958647692f;
@@ -70,7 +71,9 @@ LGPU_HD float Value2D(float tx, float ty) {
 LGPU_HD float Voronoi2D(float tx, float ty) {
   const float x0 = floorf(tx), y0 = floorf(ty);
   float m = INFINITY;
+#pragma unroll 1
   for (int y = -1; y < 2; ++y)
+#pragma unroll 1
     for (int x = -1; x < 2; ++x) {
       const float ox = x0 + (float)x, oy = y0 + (float)y;
       const float a = sinf(ox * 127.1f + oy * 311.7f) * 18.5453f, b = sinf(ox * 269.5f + oy * 183.3f) * 18.5453f;
@@ -84,8 +87,11 @@ LGPU_HD float Voronoi2D(float tx, float ty) {
 LGPU_HD float Voronoi3D(float tx, float ty, float tz) {
   const float x0 = floorf(tx), y0 = floorf(ty), z0 = floorf(tz);
   float m = INFINITY;
+#pragma unroll 1
   for (int z = -1; z < 2; ++z)
+#pragma unroll 1
     for (int y = -1; y < 2; ++y)
+#pragma unroll 1
       for (int x = -1; x < 2; ++x) {
         const float ox = x0 + (float)x, oy = y0 + (float)y, oz = z0 + (float)z;
         float px = ox * .1031f, py = oy * .1030f, pz = oz * .0973f;  // Hash33
@@ -187,7 +193,10 @@ LGPU_HD float Comp(float4 r, uint32_t i) { return i == 0 ? r.x : (i == 1 ? r.y :
 // One procedural-texture lookup of node type `type`: the coordinates are multiplied by s0,
 // s1, s2 in turn (the octave node's scale, octave factor, base node scale). A medium
 // interaction takes the 3-D variant on the event point (0 for Perlin and value noise).
-LGPU_HD float NoiseAt(uint32_t type, const VmCtx &c, float s0, float s1, float s2) {
+// Out of line and with rolled loops: the bytecode interpreter's register budget is shared by every
+// shading kernel that may call it.
+static __host__ __device__ __attribute__((noinline)) float NoiseAt(uint32_t type, const VmCtx &c, float s0, float s1,
+                                                                   float s2) {
   if (c.medium) {
     const float tx = c.px * s0 * s1 * s2, ty = c.py * s0 * s1 * s2, tz = c.pz * s0 * s1 * s2;
     if (type == LGPU_NODE_CHECKER) {
@@ -209,9 +218,69 @@ LGPU_HD float NoiseAt(uint32_t type, const VmCtx &c, float s0, float s1, float s
   return Voronoi2D(tx, ty);
 }
 
-// Straight-line program; returns r[res]. Kept out of line: its register file
-// is an indexed array (scratch), which must not burden the shading kernels'
-// common path where every socket is a folded constant.
+// The ops that depend on the wavelengths or touch textures and tables, out of line: inlined into
+// the interpreter loop, their per-wavelength constants were hoisted and kept live across the whole
+// loop, which raised the register count of every shading kernel that calls the interpreter.
+static __host__ __device__ __attribute__((noinline)) float4 RunHeavyOp(const VmCtx &c, const Instr in, float4 a) {
+  float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+  switch (in.op) {
+    case kOpTex: case kOpTexChannel: case kOpSpecTex: {  // GraphInputs.cpp:128-165
+      const uint4 t = c.textures[in.imm];
+      float px[4];
+      TexBilinear(c.texels + t.w, t.x, t.y, t.z, Fract(c.u), Fract(c.v), px);
+      if (in.op == kOpTex) o = make_float4(px[0], px[1], px[2], px[3]);
+      else if (in.op == kOpTexChannel) o.x = px[in.c];
+      else o = UpliftV(px[0], px[1], px[2], c);
+      break;
+    }
+    case kOpSpectral: {  // GraphInputs.cpp:55-71
+      const float4 k = c.consts[in.imm];
+      if (!c.mode) { o = c.consts[in.imm + 1]; break; }
+      const uint4 t = c.textures[(uint32_t)k.z];
+      const float *d = c.texels + t.w;
+      const uint32_t n = t.x * t.y;
+      const float step = (k.y - k.x) / (float)(n - 1);
+      float lv[4];
+      for (int j = 0; j < 4; ++j) {
+        const float l = c.lam[j];
+        if (l <= k.x) { lv[j] = d[0]; continue; }
+        if (l >= k.y) { lv[j] = d[(n - 1) * t.z]; continue; }
+        uint32_t q = (uint32_t)((l - k.x) / step);
+        if (q > n - 2) q = n - 2;
+        const float f = (l - (k.x + step * (float)q)) / step;
+        lv[j] = d[q * t.z] + (d[(q + 1) * t.z] - d[q * t.z]) * f;
+      }
+      o = make_float4(lv[0], lv[1], lv[2], lv[3]);
+      break;
+    }
+    case kOpNoise: {  // GraphTexture.cpp:56-220
+      const float4 k = c.consts[in.imm];
+      o.x = NoiseAt(in.c, c, k.x, k.y, k.z);
+      break;
+    }
+    case kOpBlackbody: {
+      const float T = a.x;
+      if (c.mode) {
+        o = make_float4(BlackbodyNormalised(c.lam[0], T), BlackbodyNormalised(c.lam[1], T),
+                        BlackbodyNormalised(c.lam[2], T), BlackbodyNormalised(c.lam[3], T));
+      } else {
+        // 3-sample spectrum at 400/500/600 nm through the XYZ->RGB weights in consts[imm..imm+2]
+        const float v0 = BlackbodyNormalised(400.f, T), v1 = BlackbodyNormalised(500.f, T), v2 = BlackbodyNormalised(600.f, T);
+        const float4 w0 = c.consts[in.imm], w1 = c.consts[in.imm + 1], w2 = c.consts[in.imm + 2];
+        o = make_float4(v0 * w0.x + v1 * w0.y + v2 * w0.z, v0 * w1.x + v1 * w1.y + v2 * w1.z,
+                        v0 * w2.x + v1 * w2.y + v2 * w2.z, 0.f);
+      }
+      break;
+    }
+    case kOpToReal: o.x = AsRealV(a, in.c, c.mode); break;
+    case kOpToSpectrum: o = AsSpectrumV(a, in.c, c); break;
+  }
+  return o;
+}
+
+// Straight-line program; returns r[res]. Kept out of line: its register file is an indexed array
+// (scratch), which must not burden the shading kernels' common path where every socket is a
+// folded constant.
 static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmCtx &c, uint32_t off, uint32_t len,
                                                                         uint32_t res) {
   float4 r[kMaxRegs];
@@ -221,52 +290,10 @@ static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmC
     switch (in.op) {
       case kOpConst: o = c.consts[in.imm]; break;
       case kOpUV: o = make_float4(c.u, c.v, 0.f, 0.f); break;
-      case kOpTex: {
-        const uint4 t = c.textures[in.imm];
-        float px[4];
-        TexBilinear(c.texels + t.w, t.x, t.y, t.z, Fract(c.u), Fract(c.v), px);
-        o = make_float4(px[0], px[1], px[2], px[3]);
+      case kOpTex: case kOpTexChannel: case kOpSpecTex: case kOpSpectral: case kOpNoise: case kOpBlackbody:
+      case kOpToReal: case kOpToSpectrum:
+        o = RunHeavyOp(c, in, r[in.a & (kMaxRegs - 1)]);
         break;
-      }
-      case kOpTexChannel: {  // GraphInputs.cpp:148-151
-        const uint4 t = c.textures[in.imm];
-        float px[4];
-        TexBilinear(c.texels + t.w, t.x, t.y, t.z, Fract(c.u), Fract(c.v), px);
-        o.x = px[in.c];
-        break;
-      }
-      case kOpSpecTex: {  // GraphInputs.cpp:162-165
-        const uint4 t = c.textures[in.imm];
-        float px[4];
-        TexBilinear(c.texels + t.w, t.x, t.y, t.z, Fract(c.u), Fract(c.v), px);
-        o = UpliftV(px[0], px[1], px[2], c);
-        break;
-      }
-      case kOpSpectral: {  // GraphInputs.cpp:55-71
-        const float4 k = c.consts[in.imm];
-        if (!c.mode) { o = c.consts[in.imm + 1]; break; }
-        const uint4 t = c.textures[(uint32_t)k.z];
-        const float *d = c.texels + t.w;
-        const uint32_t n = t.x * t.y;
-        const float step = (k.y - k.x) / (float)(n - 1);
-        float lv[4];
-        for (int j = 0; j < 4; ++j) {
-          const float l = c.lam[j];
-          if (l <= k.x) { lv[j] = d[0]; continue; }
-          if (l >= k.y) { lv[j] = d[(n - 1) * t.z]; continue; }
-          uint32_t q = (uint32_t)((l - k.x) / step);
-          if (q > n - 2) q = n - 2;
-          const float f = (l - (k.x + step * (float)q)) / step;
-          lv[j] = d[q * t.z] + (d[(q + 1) * t.z] - d[q * t.z]) * f;
-        }
-        o = make_float4(lv[0], lv[1], lv[2], lv[3]);
-        break;
-      }
-      case kOpNoise: {  // GraphTexture.cpp:56-220
-        const float4 k = c.consts[in.imm];
-        o.x = NoiseAt(in.c, c, k.x, k.y, k.z);
-        break;
-      }
       case kOpFresnel: {  // GraphInputs.cpp:188-198
         const float n1 = c.eta, n2 = r[in.a].x;
         float R0 = (n1 - n2) / (n1 + n2);
@@ -299,24 +326,8 @@ static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmC
       case kOpLength: { const float4 a = r[in.a]; o.x = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); break; }
       case kOpMerge: o = make_float4(r[in.a].x, r[in.b].x, r[in.c].x, 1.f); break;  // w: alpha (MergeRGBA) or 0 (MergeXYZ)
       case kOpSeparate: o.x = Comp(r[in.a], in.c); break;
-      case kOpBlackbody: {
-        const float T = r[in.a].x;
-        if (c.mode) {
-          o = make_float4(BlackbodyNormalised(c.lam[0], T), BlackbodyNormalised(c.lam[1], T),
-                          BlackbodyNormalised(c.lam[2], T), BlackbodyNormalised(c.lam[3], T));
-        } else {
-          // 3-sample spectrum at 400/500/600 nm through the XYZ->RGB weights in consts[imm..imm+2]
-          const float v0 = BlackbodyNormalised(400.f, T), v1 = BlackbodyNormalised(500.f, T), v2 = BlackbodyNormalised(600.f, T);
-          const float4 w0 = c.consts[in.imm], w1 = c.consts[in.imm + 1], w2 = c.consts[in.imm + 2];
-          o = make_float4(v0 * w0.x + v1 * w0.y + v2 * w0.z, v0 * w1.x + v1 * w1.y + v2 * w1.z,
-                          v0 * w2.x + v1 * w2.y + v2 * w2.z, 0.f);
-        }
-        break;
-      }
-      case kOpToReal: o.x = AsRealV(r[in.a], in.c, c.mode); break;
       case kOpToColour: o = AsColourV(r[in.a], in.c); break;
       case kOpToVec3: o = AsVec3V(r[in.a], in.c); break;
-      case kOpToSpectrum: o = AsSpectrumV(r[in.a], in.c, c); break;
     }
     r[in.dst] = o;
   }

@@ -88,7 +88,7 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
   return Ld;
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, 3) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                       uint32_t n, Queues q, uint32_t *spillBase) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
   __shared__ TreeImp treeLds[kTreeLds];
