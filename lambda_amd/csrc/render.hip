@@ -148,8 +148,9 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
   q.shadow[slot] = r;
 }
 
-__global__ void __launch_bounds__(kBlock, 3) k_shade(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, 3) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, Queues q) {
+  const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
   LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_query(DevScene sc, const float4 *ray
 // ---------------------------------------------------------------- host side
 struct Wave {
   // two path-state buffers (ping-pong), per-sample radiance, shadow queue
-  DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount;
+  DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene;
   uint32_t cap = 0;
   PathState ps[2]{};
   SampleState ss{};
@@ -424,7 +425,7 @@ struct Wave {
     }
     LGPU_HIP(L.alloc(16ull * n)); LGPU_HIP(pix.alloc(4ull * n)); LGPU_HIP(lam.alloc(4ull * n));
     LGPU_HIP(shadow.alloc(sizeof(ShadowRay) * (size_t)n));
-    LGPU_HIP(counters.alloc(64)); LGPU_HIP(tcount.alloc(64));
+    LGPU_HIP(counters.alloc(64)); LGPU_HIP(tcount.alloc(64)); LGPU_HIP(scene.alloc(sizeof(DevScene)));
     LGPU_HIP(spill.alloc(4ull * kSpillWords * n));
     ss.L = L.as<float4>(); ss.pixel = pix.as<uint32_t>(); ss.lambda0 = lam.as<float>();
     cap = n;
@@ -494,6 +495,10 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   DevScene dsc = s->dsc;
   if (countTrav) dsc.voxel_count = W.tcount.as<unsigned long long>() + 2;
   hipStream_t stream = dev->stream;
+  // The shading kernels read the scene record through a device pointer: a by-value kernel argument
+  // whose address reaches out-of-line functions is copied into every lane's scratch.
+  LGPU_HIP(hipMemcpyAsync(W.scene.ptr, &dsc, sizeof(DevScene), hipMemcpyHostToDevice, stream));
+  const DevScene *dscp = W.scene.as<DevScene>();
   hipEvent_t *ev = dev->ev;
   lgpu_render_stats S;
   std::memset(&S, 0, sizeof(S));
@@ -526,13 +531,13 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       LGPU_HIP(hipEventRecord(ev[3], stream));
       Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
       if (rp.integrator == LGPU_INTEGRATOR_PATH)
-        hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, rp, W.ps[cur], W.ss, count, q);
+        hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, q);
       else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
-        LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        LaunchShadeDirect(Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
         LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
       else
-        LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        LaunchShadeVol(Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       LGPU_HIP(hipGetLastError());
       LGPU_HIP(hipEventRecord(ev[4], stream));
       uint32_t cnt[2];
