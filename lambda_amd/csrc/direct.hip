@@ -4,6 +4,7 @@
 // to the extend stage), a shading surface estimates direct light from one light chosen by the
 // light sampler (light sample + BSDF sample, power-heuristic MIS) and ends the path. The
 // visibility, PDF_Li and MIS rays are traced inline.
+#define LGPU_LOBE_INLINE
 #include "host.h"
 #include "lights.h"
 #include "media.h"
@@ -51,9 +52,8 @@ __device__ inline float PdfLiRay(const DevScene &sc, uint32_t pick, const Hit &h
   return 0.f;
 }
 
-__global__ void __launch_bounds__(kBlock, 3) k_shade_direct(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                          uint32_t n, Queues q, uint32_t *spillBase) {
-  const DevScene &sc = *scp;
   __shared__ uint32_t stk[kLdsStack][kBlock];
   __shared__ TreeImp treeLds[kTreeLds];
   LoadTreeLds(sc, treeLds);
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_shade_direct(const DevScene *__re
 
 }  // namespace
 
-void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene *sc, const RenderParams &rp, PathState ps,
+void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                        SampleState ss, uint32_t n, Queues q, uint32_t *spill) {
   hipLaunchKernelGGL(k_shade_direct, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
 }

@@ -334,13 +334,13 @@ struct RenderParams {
 };
 
 // Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator.
-void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene *sc, const RenderParams &rp, PathState ps,
+void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                     SampleState ss, uint32_t n, Queues q, uint32_t *spill);
 // Utility-pass shade stage (utility.hip): DepthPass, NormalPass, AOVPass.
 void LaunchShadeUtility(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                         SampleState ss, uint32_t n);
 // Direct-lighting shade stage (direct.hip): DirectLightingIntegrator.
-void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene *sc, const RenderParams &rp, PathState ps,
+void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                        SampleState ss, uint32_t n, Queues q, uint32_t *spill);
 
 }  // namespace lgpu

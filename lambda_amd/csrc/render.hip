@@ -533,11 +533,11 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       if (rp.integrator == LGPU_INTEGRATOR_PATH)
         hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, q);
       else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
-        LaunchShadeDirect(Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
         LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
       else
-        LaunchShadeVol(Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
       LGPU_HIP(hipGetLastError());
       LGPU_HIP(hipEventRecord(ev[4], stream));
       uint32_t cnt[2];

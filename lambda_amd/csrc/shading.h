@@ -208,12 +208,20 @@ __device__ inline void RoughDielectricEval(V3 wo, V3 wi, float alpha, float ior,
 }
 
 // ---------------------------------------------------------------- lobes
+// The lobe functions are inlined into the path-integrator shading kernel: out of line, their
+// reference arguments (event, hit, sampler state, the shader's VM context) live in per-lane scratch
+// and are written and re-read at every call. The volume and direct-lighting kernels define
+// LGPU_LOBE_INLINE empty before including this header and leave the choice to the compiler
+// (inlined into the volume kernel, the lobes spill several hundred registers).
+#ifndef LGPU_LOBE_INLINE
+#define LGPU_LOBE_INLINE __attribute__((always_inline))
+#endif
 struct Shader {
   const DevScene *sc;
   VmCtx vm;
   bool corrected;
   __device__ Spec SockS(uint32_t i) const { return i == kNone ? S(0.f) : SocketSpectrum(sc->sockets[i], vm); }
-  __device__ float SockR(uint32_t i) const { return i == kNone ? 0.f : SocketReal(sc->sockets[i], vm); }
+  __device__ LGPU_LOBE_INLINE float SockR(uint32_t i) const { return i == kNone ? 0.f : SocketReal(sc->sockets[i], vm); }
   __device__ void Alpha(const Lobe &L, float *ax, float *ay) const {
     if (L.sock[1] == kNone) { *ax = *ay = 0.f; }
     else SocketRoughness(sc->sockets[L.sock[1]], vm, ax, ay);
@@ -225,7 +233,7 @@ struct Shader {
   }
 
   // BxDF::f for a non-mix lobe
-  __device__ Spec LeafF(const Lobe &L, const Ev &ev) const {
+  __device__ LGPU_LOBE_INLINE Spec LeafF(const Lobe &L, const Ev &ev) const {
     switch (L.kind) {
       case kLambertian: return SockS(L.sock[0]) * kInvPi;  // BxDF.cpp:29-31
       case kOrenNayarR:
@@ -263,7 +271,7 @@ struct Shader {
     }
     return S(0.f);
   }
-  __device__ float LeafPdf(const Lobe &L, V3 wo, V3 wi) const {
+  __device__ LGPU_LOBE_INLINE float LeafPdf(const Lobe &L, V3 wo, V3 wi) const {
     switch (L.kind) {
       case kLambertian: case kOrenNayarR: case kOrenNayarT: return CosineHemispherePdf(wo, wi);
       case kMicrofacet: {
@@ -281,13 +289,13 @@ struct Shader {
     }
     return 0.f;
   }
-  __device__ Spec F(uint32_t li, const Ev &ev) const {
+  __device__ LGPU_LOBE_INLINE Spec F(uint32_t li, const Ev &ev) const {
     const Lobe &L = sc->lobes[li];
     if (L.kind != kMix) return LeafF(L, ev);
     const float r = SockR(L.sock[2]);  // BxDF.cpp:45-48
     return LeafF(sc->lobes[L.child[0]], ev) * (1.f - r) + LeafF(sc->lobes[L.child[1]], ev) * r;
   }
-  __device__ float Pdf(uint32_t li, V3 wo, V3 wi) const {
+  __device__ LGPU_LOBE_INLINE float Pdf(uint32_t li, V3 wo, V3 wi) const {
     const Lobe &L = sc->lobes[li];
     if (L.kind != kMix) return LeafPdf(L, wo, wi);
     const float r = SockR(L.sock[2]);  // BxDF.cpp:62-65
@@ -295,7 +303,7 @@ struct Shader {
   }
 
   // BxDF::Sample_f for a non-mix lobe; mutates hit point/normal as the reference does.
-  __device__ Spec LeafSample(const Lobe &L, Ev &ev, Hit &h, Rng &rng, float &pdf, bool &spec) const {
+  __device__ LGPU_LOBE_INLINE Spec LeafSample(const Lobe &L, Ev &ev, Hit &h, Rng &rng, float &pdf, bool &spec) const {
     spec = false;
     switch (L.kind) {
       case kLambertian:
@@ -426,7 +434,7 @@ struct Shader {
     pdf = 0.f;
     return S(0.f);
   }
-  __device__ Spec Sample(uint32_t li, Ev &ev, Hit &h, Rng &rng, float &pdf, bool &spec) const {
+  __device__ LGPU_LOBE_INLINE Spec Sample(uint32_t li, Ev &ev, Hit &h, Rng &rng, float &pdf, bool &spec) const {
     const Lobe &L = sc->lobes[li];
     if (L.kind != kMix) return LeafSample(L, ev, h, rng, pdf, spec);
     const float r = SockR(L.sock[2]);  // BxDF.cpp:50-60

@@ -13,6 +13,7 @@
 //   4. otherwise the surface vertex (NEE + BSDF sample) or a pass-through boundary;
 //   5. Russian roulette, then the continuation is appended to the next queue.
 // NEE visibility and the medium points' phase-sampled light lookups are traced inline.
+#define LGPU_LOBE_INLINE
 #include "host.h"
 #include "lights.h"
 #include "media.h"
@@ -88,9 +89,8 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
   return Ld;
 }
 
-__global__ void __launch_bounds__(kBlock, 3) k_shade_vol(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                       uint32_t n, Queues q, uint32_t *spillBase) {
-  const DevScene &sc = *scp;
   __shared__ uint32_t stk[kLdsStack][kBlock];
   __shared__ TreeImp treeLds[kTreeLds];
   LoadTreeLds(sc, treeLds);
@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_shade_vol(const DevScene *__restr
 
 }  // namespace
 
-void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene *sc, const RenderParams &rp, PathState ps,
+void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                     SampleState ss, uint32_t n, Queues q, uint32_t *spill) {
   hipLaunchKernelGGL(k_shade_vol, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
 }
