@@ -103,9 +103,12 @@ __global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, PathState ps
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, uint32_t n, uint32_t *spill,
-                                                   unsigned long long *counts) {
+// Kernels of the path loop take their item count either as a value or, when `np` is set, from a
+// device counter written by the previous stage (the grid then covers an upper bound of it).
+__global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, uint32_t n, const uint32_t *np,
+                                                   uint32_t *spill, unsigned long long *counts) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
+  if (np) n = *np;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const V3 o = ToV3(ps.ray_o[i]), d = ToV3(ps.ray_d[i]);
@@ -122,8 +125,9 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, ui
 }
 
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene sc, SampleState ss, const ShadowRay *rays, uint32_t n,
-                                                   uint32_t *spill) {
+                                                   const uint32_t *np, uint32_t *spill) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
+  if (np) n = *np;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const ShadowRay r = rays[i];
@@ -149,9 +153,11 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
 }
 
 __global__ void __launch_bounds__(kBlock, 3) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
-                                                  uint32_t n, Queues q) {
+                                                  uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
+  if (np) n = *np;
+  if (blockIdx.x * kBlock >= n) return;  // whole block idle: skip the LDS tree load
   LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -410,11 +416,27 @@ __global__ void __launch_bounds__(kBlock) k_query(DevScene sc, const float4 *ray
 // ---------------------------------------------------------------- host side
 struct Wave {
   // two path-state buffers (ping-pong), per-sample radiance, shadow queue
-  DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene;
+  DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene, ring;
   uint32_t cap = 0;
   PathState ps[2]{};
   SampleState ss{};
+  // pipelined path loop: device counter slots, their pinned host copies, per-slot stage events
+  static constexpr int kSlots = 4;
+  uint32_t *hring = nullptr;
+  hipEvent_t pev[kSlots][5] = {};
+  ~Wave() {
+    if (hring) (void)hipHostFree(hring);
+    for (auto &row : pev)
+      for (hipEvent_t &e : row)
+        if (e) (void)hipEventDestroy(e);
+  }
   lgpu_status alloc(uint32_t n) {
+    if (!hring) {
+      LGPU_HIP(hipHostMalloc((void **)&hring, 16 * kSlots, hipHostMallocDefault));
+      LGPU_HIP(ring.alloc(16 * kSlots));
+      for (auto &row : pev)
+        for (hipEvent_t &e : row) LGPU_HIP(hipEventCreate(&e));
+    }
     if (n <= cap) return LGPU_OK;
     for (int b = 0; b < 2; ++b) {
       for (int k = 0; k < 9; ++k) LGPU_HIP(st[b][k].alloc(16ull * n));
@@ -515,58 +537,127 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
     float ms;
     LGPU_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     S.ms_camera += ms;
-    uint32_t count = n;
-    uint32_t iter = 0;
-    while (count > 0 && iter < 4 * (d->max_bounces + 2) + 1100) {
-      ++iter;
-      LGPU_HIP(hipMemsetAsync(W.counters.ptr, 0, 8, stream));
-      LGPU_HIP(hipEventRecord(ev[2], stream));
-      if (countTrav)
-        hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count,
-                           W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
-      else
-        hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count,
-                           W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
-      LGPU_HIP(hipGetLastError());
-      LGPU_HIP(hipEventRecord(ev[3], stream));
-      Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
-      if (rp.integrator == LGPU_INTEGRATOR_PATH)
-        hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, q);
-      else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
-        LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
-      else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
-        LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
-      else
-        LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
-      LGPU_HIP(hipGetLastError());
-      LGPU_HIP(hipEventRecord(ev[4], stream));
-      uint32_t cnt[2];
-      LGPU_HIP(hipMemcpyAsync(cnt, W.counters.ptr, 8, hipMemcpyDeviceToHost, stream));
-      LGPU_HIP(hipStreamSynchronize(stream));
-      S.extension_rays += count;
-      S.launches_extend++;
-      S.launches_shade++;
-      LGPU_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-      S.ms_extend += ms;
-      LGPU_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
-      S.ms_shade += ms;
-      if (cnt[0] > count || cnt[1] > count) return Fail(LGPU_ERR_STATE, "render: queue overflow");
-      if (cnt[1]) {
-        LGPU_HIP(hipEventRecord(ev[5], stream));
-        hipLaunchKernelGGL(k_shadow, dim3(Blocks(cnt[1])), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(),
-                           cnt[1], W.spill.as<uint32_t>());
-        LGPU_HIP(hipGetLastError());
-        LGPU_HIP(hipEventRecord(ev[6], stream));
-        LGPU_HIP(hipEventSynchronize(ev[6]));
-        LGPU_HIP(hipEventElapsedTime(&ms, ev[5], ev[6]));
-        S.ms_shadow += ms;
-        S.shadow_rays += cnt[1];
-        S.launches_shadow++;
+    const uint32_t maxIter = 4 * (d->max_bounces + 2) + 1100;
+    if (rp.integrator == LGPU_INTEGRATOR_PATH) {
+      // Pipelined bounces: each stage reads its item count from the device counter the previous
+      // stage wrote, the grids cover the last count the host has seen (counts never grow), and the
+      // host stays up to kLag bounces ahead, retiring them through pinned copies and events. No
+      // host round trip sits between two kernels.
+      constexpr int kLag = 3;
+      static_assert(kLag < Wave::kSlots, "a counter slot is reused only after its bounce retired");
+      uint32_t *ring = W.ring.as<uint32_t>();
+      LGPU_HIP(hipMemsetD32Async((hipDeviceptr_t)(ring + 4 * (Wave::kSlots - 1)), (int)n, 1, stream));  // bounce 0's input count
+      uint32_t ub = n, live = n, issued = 0, retired = 0;
+      bool done = false;
+      while (!(done && retired == issued)) {
+        if (!done && issued < maxIter && issued - retired < (uint32_t)kLag) {
+          const int sl = (int)(issued % Wave::kSlots), prev = (int)((issued + Wave::kSlots - 1) % Wave::kSlots);
+          uint32_t *cin = ring + 4 * prev, *cout = ring + 4 * sl;
+          hipEvent_t *e = W.pev[sl];
+          LGPU_HIP(hipMemsetAsync(cout, 0, 8, stream));
+          LGPU_HIP(hipEventRecord(e[0], stream));
+          if (countTrav)
+            hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
+                               W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+          else
+            hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
+                               W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+          LGPU_HIP(hipGetLastError());
+          LGPU_HIP(hipEventRecord(e[1], stream));
+          Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), cout};
+          hipLaunchKernelGGL(k_shade, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
+          LGPU_HIP(hipGetLastError());
+          LGPU_HIP(hipEventRecord(e[2], stream));
+          hipLaunchKernelGGL(k_shadow, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(), ub,
+                             cout + 1, W.spill.as<uint32_t>());
+          LGPU_HIP(hipGetLastError());
+          LGPU_HIP(hipEventRecord(e[3], stream));
+          LGPU_HIP(hipMemcpyAsync(W.hring + 4 * sl, cout, 8, hipMemcpyDeviceToHost, stream));
+          LGPU_HIP(hipEventRecord(e[4], stream));
+          cur ^= 1;
+          ++issued;
+          continue;
+        }
+        if (retired == issued) break;  // bounce limit reached without termination
+        const int sl = (int)(retired % Wave::kSlots);
+        hipEvent_t *e = W.pev[sl];
+        LGPU_HIP(hipEventSynchronize(e[4]));
+        ++retired;
+        if (done) continue;  // issued after the wave ended: processed no paths
+        const uint32_t c0 = W.hring[4 * sl], c1 = W.hring[4 * sl + 1];
+        float ms;
+        S.extension_rays += live;
+        S.launches_extend++;
+        S.launches_shade++;
+        LGPU_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+        S.ms_extend += ms;
+        LGPU_HIP(hipEventElapsedTime(&ms, e[1], e[2]));
+        S.ms_shade += ms;
+        if (c1) {
+          LGPU_HIP(hipEventElapsedTime(&ms, e[2], e[3]));
+          S.ms_shadow += ms;
+          S.shadow_rays += c1;
+          S.launches_shadow++;
+        }
+        if (c0 > live || c1 > live) return Fail(LGPU_ERR_STATE, "render: queue overflow");
+        live = ub = c0;
+        if (c0 == 0) done = true;
       }
-      count = cnt[0];
-      cur ^= 1;
+      if (!done) return Fail(LGPU_ERR_STATE, "render: path wave did not terminate");
+    } else {
+      uint32_t count = n;
+      uint32_t iter = 0;
+      while (count > 0 && iter < maxIter) {
+        ++iter;
+        LGPU_HIP(hipMemsetAsync(W.counters.ptr, 0, 8, stream));
+        LGPU_HIP(hipEventRecord(ev[2], stream));
+        if (countTrav)
+          hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count, nullptr,
+                             W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+        else
+          hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count, nullptr,
+                             W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+        LGPU_HIP(hipGetLastError());
+        LGPU_HIP(hipEventRecord(ev[3], stream));
+        Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
+        if (rp.integrator == LGPU_INTEGRATOR_PATH)
+          hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
+        else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
+          LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
+          LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
+        else
+          LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+        LGPU_HIP(hipGetLastError());
+        LGPU_HIP(hipEventRecord(ev[4], stream));
+        uint32_t cnt[2];
+        LGPU_HIP(hipMemcpyAsync(cnt, W.counters.ptr, 8, hipMemcpyDeviceToHost, stream));
+        LGPU_HIP(hipStreamSynchronize(stream));
+        S.extension_rays += count;
+        S.launches_extend++;
+        S.launches_shade++;
+        LGPU_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        S.ms_extend += ms;
+        LGPU_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
+        S.ms_shade += ms;
+        if (cnt[0] > count || cnt[1] > count) return Fail(LGPU_ERR_STATE, "render: queue overflow");
+        if (cnt[1]) {
+          LGPU_HIP(hipEventRecord(ev[5], stream));
+          hipLaunchKernelGGL(k_shadow, dim3(Blocks(cnt[1])), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(),
+                             cnt[1], nullptr, W.spill.as<uint32_t>());
+          LGPU_HIP(hipGetLastError());
+          LGPU_HIP(hipEventRecord(ev[6], stream));
+          LGPU_HIP(hipEventSynchronize(ev[6]));
+          LGPU_HIP(hipEventElapsedTime(&ms, ev[5], ev[6]));
+          S.ms_shadow += ms;
+          S.shadow_rays += cnt[1];
+          S.launches_shadow++;
+        }
+        count = cnt[0];
+        cur ^= 1;
+      }
+      if (count > 0) return Fail(LGPU_ERR_STATE, "render: path wave did not terminate");
     }
-    if (count > 0) return Fail(LGPU_ERR_STATE, "render: path wave did not terminate");
     hipLaunchKernelGGL(k_accumulate, dim3(Blocks(n)), dim3(kBlock), 0, stream, rp, W.ss, n, film, counts,
                        out ? out + 5 * base : nullptr);
     LGPU_HIP(hipGetLastError());
