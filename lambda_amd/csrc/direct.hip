@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
   LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const TraceCtx tc{stk, spillBase + (size_t)i * kSpillWords};
+  const TraceCtx tc{stk, spillBase + (size_t)i * sc.spill_words};
   const bool corrected = rp.corrected != 0;
   const int mode = (int)rp.mode;
   const uint4 ids = ps.ids[i];

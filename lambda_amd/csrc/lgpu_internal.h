@@ -13,7 +13,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kLdsStack = 16;             // per-thread BVH stack entries held in LDS
 constexpr int kMaxBvhDepth = 64;          // binary-BVH depth bound (builder guarantees)
 constexpr int kMaxStack = 96;             // 4-wide traversal stack entries per thread (LDS + spill)
-constexpr int kSpillWords = kMaxStack - kLdsStack;  // per-thread global spill
+constexpr int kSpillWords = kMaxStack - kLdsStack;  // per-thread global spill, worst case over trees
 
 // ---------------------------------------------------------------- BVH2
 // Two child boxes per node (64 B, one node = 4 x 16-B loads):
@@ -45,7 +45,7 @@ struct Bvh4Node {
   uint32_t child[4];
 };
 
-// Triangles in BVH order: {v0.xyz, prim}, {e1.xyz, mesh}, {e2.xyz, 0} with
+// Triangles in BVH order: {v0.xyz, prim}, {e1.xyz, mesh}, {e2.xyz, shade key} with
 // e1 = v1 - v0, e2 = v2 - v0 (48 B). Ng = e1 x e2 is bit-identical to
 // (v1 - v0) x (v2 - v0).
 struct BvhTri {
@@ -53,8 +53,14 @@ struct BvhTri {
   uint32_t prim;
   float e1[3];
   uint32_t mesh;
-  float e2[4];
+  float e2[3];
+  uint32_t key;  // ShadeKey of the triangle's material
 };
+
+// Shading-code family of a path vertex; the path shade stage groups a block's paths by it so
+// that a wave runs one material family's code. 0 = ray left the scene, 1 = pass-through
+// surface, 2 + (root lobe kind - kLambertian) otherwise.
+constexpr uint32_t kShadeKeys = 16;
 // Shading attributes per triangle in BVH order (7 x float4 = 112 B), present
 // for meshes with smooth normals or uvs:
 //   {n0, uv0.x} {n1, uv0.y} {n2, uv1.x} {t0} {t1} {t2} {uv1.y, uv2.x, uv2.y, 0}
@@ -299,6 +305,7 @@ struct DevScene {
   const MediumRec *media;
   const float *density;      // grid densities of all media
   uint32_t num_media;
+  uint32_t spill_words;      // per-thread traversal-stack words beyond the LDS part this tree can need
   unsigned long long *voxel_count;  // non-null in counting renders: voxels visited by grid walks
 };
 

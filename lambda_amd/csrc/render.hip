@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, ui
   uint32_t prim, bi;
   TraceStats ts{0, 0};
   const bool hit = Trace<false, kCount>(sc, o, d, 0.f, INFINITY, &t, &u, &v, &prim, &bi, stk,
-                                        spill + (size_t)i * kSpillWords, &ts);
+                                        spill + (size_t)i * sc.spill_words, &ts);
   ps.hit[i] = hit ? make_float4(t, u, v, __uint_as_float(bi)) : make_float4(INFINITY, 0.f, 0.f, __uint_as_float(kNone));
   if (kCount) {
     atomicAdd(&counts[0], (unsigned long long)ts.nodes);
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene sc, SampleState ss, 
   float t, u, v;
   uint32_t prim, bi;
   const bool occ = Trace<true, false>(sc, ToV3(r.o), ToV3(r.d), 0.f, r.tmax, &t, &u, &v, &prim, &bi, stk,
-                                      spill + (size_t)i * kSpillWords, nullptr);
+                                      spill + (size_t)i * sc.spill_words, nullptr);
   if (!occ) {  // one shadow ray per path per bounce: no other writer of this slot runs concurrently
     float4 L = ss.L[r.sid];
     L.x += r.contrib[0]; L.y += r.contrib[1]; L.z += r.contrib[2]; L.w += r.contrib[3];
@@ -156,11 +156,37 @@ __global__ void __launch_bounds__(kBlock, 3) k_shade(const DevScene *__restrict_
                                                   uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
+  __shared__ uint32_t keyStart[kShadeKeys];
+  __shared__ uint16_t perm[kBlock];
   if (np) n = *np;
-  if (blockIdx.x * kBlock >= n) return;  // whole block idle: skip the LDS tree load
-  LoadTreeLds(sc, treeLds);
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t base = blockIdx.x * kBlock;
+  if (base >= n) return;  // whole block idle: skip the LDS tree load
+  const uint32_t m = min(n - base, (uint32_t)kBlock);
+  if (threadIdx.x < kShadeKeys) keyStart[threadIdx.x] = 0u;
+  LoadTreeLds(sc, treeLds);  // ends with a barrier
+  // Group the block's paths by shading family (counting sort in LDS): each thread then shades the
+  // path at its sorted position, so a wave mostly runs one family's BSDF and shader code. The
+  // gathers stay inside the block's own 256 consecutive paths.
+  uint32_t key = 0, rank = 0;
+  if (threadIdx.x < m) {
+    const uint32_t hb = __float_as_uint(ps.hit[base + threadIdx.x].w);
+    key = hb == kNone ? 0u : sc.tris[hb].key;
+    rank = atomicAdd(&keyStart[key], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < kShadeKeys; ++k) {
+      const uint32_t c = keyStart[k];
+      keyStart[k] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < m) perm[keyStart[key] + rank] = (uint16_t)threadIdx.x;
+  __syncthreads();
+  if (threadIdx.x >= m) return;
+  const uint32_t i = base + perm[threadIdx.x];
   const bool corrected = rp.corrected != 0;
   const int mode = (int)rp.mode;
   const uint4 ids = ps.ids[i];
@@ -403,7 +429,7 @@ __global__ void __launch_bounds__(kBlock) k_query(DevScene sc, const float4 *ray
   const float4 a = rays[2 * i], b = rays[2 * i + 1];
   float t, u, v;
   uint32_t prim, bi;
-  uint32_t *sp = spill + (size_t)i * kSpillWords;
+  uint32_t *sp = spill + (size_t)i * sc.spill_words;
   if (occ) {
     occ[i] = Trace<true, false>(sc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &t, &u, &v, &prim, &bi, stk, sp, nullptr) ? 1u : 0u;
   } else if (Trace<false, false>(sc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &t, &u, &v, &prim, &bi, stk, sp, nullptr)) {
@@ -430,7 +456,18 @@ struct Wave {
       for (hipEvent_t &e : row)
         if (e) (void)hipEventDestroy(e);
   }
-  lgpu_status alloc(uint32_t n) {
+  // Device bytes one path in flight holds (both state buffers, sample state, shadow slot, spill).
+  static uint64_t BytesPerPath(uint32_t spillWords) {
+    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords;
+  }
+  uint64_t bytes() const {
+    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes;
+    for (const auto &row : st)
+      for (const DevBuf &x : row) b += x.bytes;
+    return b;
+  }
+  lgpu_status alloc(uint32_t n, uint32_t spillWords) {
+    LGPU_HIP(spill.alloc(std::max<size_t>(256, 4ull * spillWords * n)));
     if (!hring) {
       LGPU_HIP(hipHostMalloc((void **)&hring, 16 * kSlots, hipHostMallocDefault));
       LGPU_HIP(ring.alloc(16 * kSlots));
@@ -448,7 +485,6 @@ struct Wave {
     LGPU_HIP(L.alloc(16ull * n)); LGPU_HIP(pix.alloc(4ull * n)); LGPU_HIP(lam.alloc(4ull * n));
     LGPU_HIP(shadow.alloc(sizeof(ShadowRay) * (size_t)n));
     LGPU_HIP(counters.alloc(64)); LGPU_HIP(tcount.alloc(64)); LGPU_HIP(scene.alloc(sizeof(DevScene)));
-    LGPU_HIP(spill.alloc(4ull * kSpillWords * n));
     ss.L = L.as<float4>(); ss.pixel = pix.as<uint32_t>(); ss.lambda0 = lam.as<float>();
     cap = n;
     return LGPU_OK;
@@ -502,15 +538,27 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   LGPU_HIP(hipSetDevice(dev->device));
   (void)hipGetLastError();  // clear a sticky error left by an earlier call
   const auto t0 = std::chrono::steady_clock::now();
-  uint64_t cap = d->path_capacity ? d->path_capacity : (1u << 22);
-  if (cap > total) cap = total;
-  if (cap == 0) cap = 1;
   if (!dev->wave) {  // path pool kept on the device object and reused across calls
     dev->wave = new Wave();
     dev->wave_free = [](void *p) { delete static_cast<Wave *>(p); };
   }
   Wave &W = *static_cast<Wave *>(dev->wave);
-  lgpu_status st = W.alloc((uint32_t)cap);
+  // Paths in flight: the caller's figure, else up to kDefaultPaths, held to a quarter of the device
+  // memory that is free or already in the pool. Fewer, larger waves spend less time in the
+  // launch-bound tails of their bounce loops.
+  uint64_t cap = d->path_capacity;
+  if (!cap) {
+    constexpr uint64_t kDefaultPaths = 1ull << 26;
+    cap = kDefaultPaths;
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
+      const uint64_t fit = (freeB + W.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words);
+      cap = std::min<uint64_t>(cap, std::max<uint64_t>(fit, 1ull << 16));
+    }
+  }
+  if (cap > total) cap = total;
+  if (cap == 0) cap = 1;
+  lgpu_status st = W.alloc((uint32_t)cap, s->dsc.spill_words);
   if (st != LGPU_OK) return st;
   const RenderParams rp = MakeParams(d, w, h);
   const bool countTrav = (d->flags & LGPU_RENDER_COUNT_TRAVERSAL) != 0;
@@ -727,7 +775,7 @@ static lgpu_status Query(lgpu_scene s, uint32_t n, const float *rays, uint32_t *
   DevBuf dr, dh, dsp;
   LGPU_HIP(dr.upload(rays, 32ull * n));
   LGPU_HIP(dh.alloc(occ ? 4ull * n : 16ull * n));
-  LGPU_HIP(dsp.alloc(4ull * kSpillWords * n));
+  LGPU_HIP(dsp.alloc(std::max<size_t>(256, 4ull * s->dsc.spill_words * n)));
   hipLaunchKernelGGL(k_query, dim3(Blocks(n)), dim3(kBlock), 0, s->dev->stream, s->dsc, dr.as<float4>(), n,
                      occ ? nullptr : dh.as<uint4>(), occ ? dh.as<uint32_t>() : nullptr, dsp.as<uint32_t>());
   const hipError_t le = hipGetLastError();

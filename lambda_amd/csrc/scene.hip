@@ -763,6 +763,10 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   }
   // each 4-wide level pushes at most three entries
   if (3 * (bb.depth4 + 1) + 1 > (uint32_t)kMaxStack) return Fail(LGPU_ERR_UNSUPPORTED, "4-wide BVH deeper than the traversal stack");
+  // Spill words a traversal can reach: the larger of the 4-wide bound and the binary bound
+  // (traversal-count renders walk the binary tree).
+  const uint32_t stackNeed = std::max(3 * (bb.depth4 + 1) + 1, bb.maxDepth + 1);
+  const uint32_t spillWords = stackNeed > (uint32_t)kLdsStack ? stackNeed - (uint32_t)kLdsStack : 0u;
   std::vector<BvhTri> btris(nt);
   bool anyAttr = false;
   for (const MeshRec &m : meshes) anyAttr |= (m.flags & (kMeshSmooth | kMeshUV)) != 0;
@@ -775,7 +779,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
     BvhTri &o = btris[i];
     o.v0[0] = a.x; o.v0[1] = a.y; o.v0[2] = a.z; o.prim = t;
     o.e1[0] = e1.x; o.e1[1] = e1.y; o.e1[2] = e1.z; o.mesh = tv.w;
-    o.e2[0] = e2.x; o.e2[1] = e2.y; o.e2[2] = e2.z; o.e2[3] = 0;
+    o.e2[0] = e2.x; o.e2[1] = e2.y; o.e2[2] = e2.z; o.key = 0;  // set once materials are compiled
     if (anyAttr) {
       float4 *r = &attr[(size_t)i * kTriAttrF4];
       const float2 u0 = uv[tv.x], u1 = uv[tv.y], u2 = uv[tv.z];
@@ -819,6 +823,11 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
       if (id != kNone && id >= s->media.size())
         return Fail(LGPU_ERR_INVALID_ARGUMENT, "material " + std::to_string(mats.size()) + ": unknown medium id");
     mats.push_back(r);
+  }
+  for (BvhTri &t : btris) {
+    const uint32_t root = mats[meshes[t.mesh].material].lobe;
+    t.key = root == kNone ? 1u : 2u + (lobes[root].kind - (uint32_t)kLambertian);
+    if (t.key >= kShadeKeys) t.key = kShadeKeys - 1;
   }
 
   // ------------------------------------------------ media
@@ -952,6 +961,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   d.strategy = strategy;
   d.num_lights = (uint32_t)lights.size();
   d.num_media = (uint32_t)media.size();
+  d.spill_words = spillWords;
   d.has_env = hasEnv;
   d.env_light = envLight;
   d.env = env;

@@ -47,7 +47,7 @@ __device__ inline float SafeDir(float c) { return fabsf(c) < 1e-20f ? copysignf(
 
 // Closest hit (kAny = false) or any hit (kAny = true) against the 4-wide quantized BVH.
 // Children are visited nearest first; the per-thread stack lives in LDS column `lane`
-// and entries beyond kLdsStack spill to `spill` (kSpillWords per thread).
+// and entries beyond kLdsStack spill to `spill` (DevScene::spill_words per thread).
 template <bool kAny>
 __device__ inline bool Trace4(const DevScene &sc, V3 o, V3 d, float tmin, float tmax, float *tOut, float *uOut,
                               float *vOut, uint32_t *primOut, uint32_t *biOut, uint32_t (*stk)[kBlock], uint32_t *spill) {

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
   LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const TraceCtx tc{stk, spillBase + (size_t)i * kSpillWords};
+  const TraceCtx tc{stk, spillBase + (size_t)i * sc.spill_words};
   const bool corrected = rp.corrected != 0;
   const bool volumes = corrected && sc.num_media > 0;  // Scene::hasVolumes: never set as written
   const bool misVol = rp.integrator == LGPU_INTEGRATOR_MIS_VOLPATH;
