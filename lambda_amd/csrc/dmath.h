@@ -160,7 +160,8 @@ LGPU_HD void HeroWavelengths(float u, float lam[4]) {
 }
 
 // ---------------------------------------------------------------- Piecewise1D in a flat pool
-// layout at off: [n (float bits), integral, pdf[n], cdf[n+1]]  (Piecewise.cpp:9-44)
+// layout at off: [n (float bits), integral, pdf[n], cdf[n+1], guide[n+1] (u32 bits)]
+// (Piecewise.cpp:9-44; the guide table is this library's search accelerator)
 LGPU_HD uint32_t AsU32(float f) { return __builtin_bit_cast(uint32_t, f); }
 LGPU_HD float AsF32(uint32_t u) { return __builtin_bit_cast(float, u); }
 LGPU_HD uint32_t PwN(const float *pool, uint32_t off) { return AsU32(pool[off]); }
@@ -177,12 +178,34 @@ LGPU_HD uint32_t FindInterval(int size, const P &pred) {  // maths.h:77-90
   if (r < 0) r = 0;
   return (uint32_t)r;
 }
+// FindInterval(n + 1, cdf[i] <= u) through the guide table: with c(u) = #{i : cdf[i] <= u} for a
+// non-decreasing cdf, c(u) lies in [guide[k], guide[k + 1]] for the bin k = floor(u n) holding u.
+// The bracket is checked against cdf itself (float binning may miss by one bin, u may be NaN) and
+// widened to [0, n + 1] when the check fails, so the result equals the full search's.
+LGPU_HD uint32_t PwFind(const float *cdf, uint32_t n, float u) {
+  if (n == 0) return 0;
+  const uint32_t *guide = reinterpret_cast<const uint32_t *>(cdf + n + 1);
+  const float x = u * (float)n;
+  const uint32_t k = x >= 0.f ? (x < (float)(n - 1) ? (uint32_t)x : n - 1) : 0u;
+  uint32_t a = guide[k], b = guide[k + 1];
+  if (!(a <= b && b <= n + 1 && (a == 0 || cdf[a - 1] <= u) && (b == n + 1 || cdf[b] > u))) { a = 0; b = n + 1; }
+  int first = (int)a, len = (int)(b - a);
+  while (len > 0) {
+    const int half = len >> 1, middle = first + half;
+    if (cdf[middle] <= u) { first = middle + 1; len -= half + 1; }
+    else len = half;
+  }
+  int r = first - 1;
+  if (r > (int)n - 1) r = (int)n - 1;
+  if (r < 0) r = 0;
+  return (uint32_t)r;
+}
 LGPU_HD uint32_t PwSampleDiscrete(const float *pool, uint32_t off, float u, float *pdf) {
   const uint32_t n = PwN(pool, off);
   const float integral = pool[off + 1];
   const float *pdfv = pool + off + 2;
   const float *cdf = pdfv + n;
-  const uint32_t o = FindInterval((int)n + 1, [&](int i) { return cdf[i] <= u; });
+  const uint32_t o = PwFind(cdf, n, u);
   if (pdf) *pdf = pdfv[o] / (integral * (float)n);
   return o;
 }
@@ -191,7 +214,7 @@ LGPU_HD float PwSampleContinuous(const float *pool, uint32_t off, float u, float
   const float integral = pool[off + 1];
   const float *pdfv = pool + off + 2;
   const float *cdf = pdfv + n;
-  const uint32_t o = FindInterval((int)n + 1, [&](int i) { return cdf[i] <= u; });
+  const uint32_t o = PwFind(cdf, n, u);
   if (offOut) *offOut = o;
   float du = u - cdf[o];
   if ((cdf[o + 1] - cdf[o]) > 0) du /= (cdf[o + 1] - cdf[o]);

@@ -152,7 +152,7 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
   q.shadow[slot] = r;
 }
 
-__global__ void __launch_bounds__(kBlock, 3) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, 4) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
@@ -372,21 +372,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_shade(const DevScene *__restrict_
 }
 
 // Film::AddSample (Film.h:30-33) in FP64; hero wavelengths go through the CIE fit to XYZ then sRGB.
-__global__ void __launch_bounds__(kBlock) k_accumulate(RenderParams rp, SampleState ss, uint32_t n, double *film,
-                                                       uint32_t *counts, float *out) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 Lf = ss.L[i];
-  const float l0 = ss.lambda0[i];
-  if (out) {
-    out[5 * (size_t)i + 0] = Lf.x;
-    out[5 * (size_t)i + 1] = Lf.y;
-    out[5 * (size_t)i + 2] = Lf.z;
-    out[5 * (size_t)i + 3] = rp.mode ? Lf.w : 0.f;
-    out[5 * (size_t)i + 4] = l0;
-    return;
-  }
-  double rgb[3];
+__device__ inline void SampleRGB(const RenderParams &rp, float4 Lf, float l0, double rgb[3]) {
   if (rp.mode) {
     const float Lc[4] = {Lf.x, Lf.y, Lf.z, Lf.w};
     double xyz[3] = {0, 0, 0};
@@ -405,11 +391,54 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(RenderParams rp, SampleSt
   } else {
     rgb[0] = Lf.x; rgb[1] = Lf.y; rgb[2] = Lf.z;
   }
+}
+
+// Per-sample film update for list-mode waves (any pixel may repeat): FP64 atomics, or the raw
+// per-sample record when `out` is set.
+__global__ void __launch_bounds__(kBlock) k_accumulate(RenderParams rp, SampleState ss, uint32_t n, double *film,
+                                                       uint32_t *counts, float *out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 Lf = ss.L[i];
+  const float l0 = ss.lambda0[i];
+  if (out) {
+    out[5 * (size_t)i + 0] = Lf.x;
+    out[5 * (size_t)i + 1] = Lf.y;
+    out[5 * (size_t)i + 2] = Lf.z;
+    out[5 * (size_t)i + 3] = rp.mode ? Lf.w : 0.f;
+    out[5 * (size_t)i + 4] = l0;
+    return;
+  }
+  double rgb[3];
+  SampleRGB(rp, Lf, l0, rgb);
   const uint32_t px = ss.pixel[i];
   unsafeAtomicAdd(&film[3 * (size_t)px + 0], rgb[0]);
   unsafeAtomicAdd(&film[3 * (size_t)px + 1], rgb[1]);
   unsafeAtomicAdd(&film[3 * (size_t)px + 2], rgb[2]);
   atomicAdd(&counts[px], 1u);
+}
+
+// Film update of an image-order wave: slot i holds global sample g = base + i, pixel g mod npx,
+// so a pixel's samples in the wave sit npx slots apart. Thread t owns slot t's pixel, sums that
+// pixel's samples of the wave in sample order and updates its film entry once: each pixel has
+// one owner per launch (t < min(n, npx)), so no atomics are needed and the sum order is fixed.
+__global__ void __launch_bounds__(kBlock) k_accumulate_px(RenderParams rp, SampleState ss, uint32_t n, uint32_t npx,
+                                                          double *film, uint32_t *counts) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n || t >= npx) return;
+  double acc[3] = {0, 0, 0};
+  uint32_t c = 0;
+  for (uint32_t i = t; i < n; i += npx) {
+    double rgb[3];
+    SampleRGB(rp, ss.L[i], ss.lambda0[i], rgb);
+    acc[0] += rgb[0]; acc[1] += rgb[1]; acc[2] += rgb[2];
+    ++c;
+    if (npx > n - i) break;  // next slot would pass n (also guards i + npx overflow)
+  }
+  const uint32_t px = ss.pixel[t];
+  double *f = film + 3 * (size_t)px;
+  f[0] += acc[0]; f[1] += acc[1]; f[2] += acc[2];
+  counts[px] += c;
 }
 
 __global__ void k_philox(uint32_t n, const uint4 *ctr, uint32_t k0, uint32_t k1, uint4 *out) {
@@ -706,8 +735,13 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       }
       if (count > 0) return Fail(LGPU_ERR_STATE, "render: path wave did not terminate");
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(Blocks(n)), dim3(kBlock), 0, stream, rp, W.ss, n, film, counts,
-                       out ? out + 5 * base : nullptr);
+    const uint32_t npx = w * h;
+    if (!listPix && !out && npx > 0)
+      hipLaunchKernelGGL(k_accumulate_px, dim3(Blocks(std::min(n, npx))), dim3(kBlock), 0, stream, rp, W.ss, n, npx, film,
+                         counts);
+    else
+      hipLaunchKernelGGL(k_accumulate, dim3(Blocks(n)), dim3(kBlock), 0, stream, rp, W.ss, n, film, counts,
+                         out ? out + 5 * base : nullptr);
     LGPU_HIP(hipGetLastError());
     S.waves++;
   }

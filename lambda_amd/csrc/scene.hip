@@ -37,6 +37,17 @@ struct Pool {
     }
     f[off + 1] = integral;
     for (uint32_t i = 0; i < n + 1; ++i) f.push_back(cdf[i]);
+    // Guide table: guide[k] = number of cdf entries <= k / n, k = 0..n (the search in PwFind
+    // starts from the bracket [guide[k], guide[k + 1]] of the bin k that holds u). A CDF that is
+    // not non-decreasing gets no guide (all ones), so its searches are the plain binary search.
+    bool monotone = true;
+    for (uint32_t i = 1; i < n + 1; ++i) monotone &= cdf[i] >= cdf[i - 1];
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < n + 1; ++k) {
+      const float t = (float)k / (float)n;
+      while (c < n + 1 && cdf[c] <= t) ++c;
+      f.push_back(AsF32(monotone ? c : 0xFFFFFFFFu));
+    }
     return off;
   }
 };
