@@ -152,6 +152,9 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
   q.shadow[slot] = r;
 }
 
+// kCorrected: the estimator mode (rp.corrected) as a template argument, so each instantiation
+// carries only its own mode's branches.
+template <bool kCorrected>
 __global__ void __launch_bounds__(kBlock, 4) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
@@ -187,7 +190,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_shade(const DevScene *__restrict_
   __syncthreads();
   if (threadIdx.x >= m) return;
   const uint32_t i = base + perm[threadIdx.x];
-  const bool corrected = rp.corrected != 0;
+  constexpr bool corrected = kCorrected;
   const int mode = (int)rp.mode;
   const uint4 ids = ps.ids[i];
   const uint4 aux = ps.aux[i];
@@ -342,15 +345,16 @@ __global__ void __launch_bounds__(kBlock, 4) k_shade(const DevScene *__restrict_
       hasNee = true;
     }
   }
+  // Corrected: the NEE estimate stands whatever the BSDF sample does, so its shadow ray leaves
+  // now (its state is then dead during the BSDF sample). Faithful: the reference's `break` on a
+  // failed BSDF sample drops it, so it waits for the sample.
+  if (corrected && hasNee) PushShadow(q, sh, sid, beta * cNee);
   float scatPdf = 0.f;
   bool spec = false;
   Spec f = shd.Sample(mat.lobe, ev, h, rng, scatPdf, spec);
   f = f * fabsf(ev.wiL.y);
-  if (!(scatPdf > 0 && !IsBlack(f, mode))) {
-    if (corrected && hasNee) PushShadow(q, sh, sid, beta * cNee);  // NEE is dropped by `break` in the reference
-    return;
-  }
-  if (hasNee) PushShadow(q, sh, sid, beta * cNee);
+  if (!(scatPdf > 0 && !IsBlack(f, mode))) return;
+  if (!corrected && hasNee) PushShadow(q, sh, sid, beta * cNee);
   beta = beta * (f / scatPdf);
   uint32_t nf = spec ? kFlagSpecular : 0u;
   float qrr = 0.f;
@@ -642,7 +646,10 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[1], stream));
           Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), cout};
-          hipLaunchKernelGGL(k_shade, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
+          if (rp.corrected)
+            hipLaunchKernelGGL(k_shade<true>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
+          else
+            hipLaunchKernelGGL(k_shade<false>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[2], stream));
           hipLaunchKernelGGL(k_shadow, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(), ub,
@@ -697,8 +704,10 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[3], stream));
         Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
-        if (rp.integrator == LGPU_INTEGRATOR_PATH)
-          hipLaunchKernelGGL(k_shade, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
+        if (rp.integrator == LGPU_INTEGRATOR_PATH && rp.corrected)
+          hipLaunchKernelGGL(k_shade<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
+        else if (rp.integrator == LGPU_INTEGRATOR_PATH)
+          hipLaunchKernelGGL(k_shade<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
         else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
           LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
         else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
