@@ -252,10 +252,11 @@ __device__ inline float HGSample(float g, V3 wo, V3 *wi, float u0, float u1, boo
 
 // ---------------------------------------------------------------- transmittance rays
 // Scene::Intersect + hit processing; h is left untouched on a miss (as the reference's RayHit).
+// The one-node-per-step loop (Trace4): these rays are traced inside divergent shading code.
 __device__ inline bool ClosestHit(const DevScene &sc, V3 o, V3 d, const TraceCtx &tc, bool corrected, Hit &h, uint32_t &bi) {
   float t, u, v;
   uint32_t prim, b;
-  if (!Trace<false, false>(sc, o, d, 0.f, INFINITY, &t, &u, &v, &prim, &b, tc.stk, tc.spill, nullptr)) return false;
+  if (!Trace4<false>(sc, o, d, 0.f, INFINITY, &t, &u, &v, &prim, &b, tc.stk, tc.spill)) return false;
   ProcessHit(sc, o, d, t, u, v, b, corrected, h);
   bi = b;
   return true;

@@ -45,9 +45,11 @@ __device__ inline float ByteF(uint32_t w, int k) { return (float)((w >> (8 * k))
 // tree. Such components are replaced by +-1e-20 for the box tests only (triangle tests use d).
 __device__ inline float SafeDir(float c) { return fabsf(c) < 1e-20f ? copysignf(1e-20f, c) : c; }
 
-// Closest hit (kAny = false) or any hit (kAny = true) against the 4-wide quantized BVH.
-// Children are visited nearest first; the per-thread stack lives in LDS column `lane`
-// and entries beyond kLdsStack spill to `spill` (DevScene::spill_words per thread).
+// Closest hit (kAny = false) or any hit (kAny = true) against the 4-wide quantized BVH, one
+// inner node or one leaf per step. Children are visited nearest first; the per-thread stack
+// lives in LDS column `lane` and entries beyond kLdsStack spill to `spill`
+// (DevScene::spill_words per thread). Shadow rays use this loop: a lane stops at its first
+// occluder, which parking leaves (Trace4Closest) would delay.
 template <bool kAny>
 __device__ inline bool Trace4(const DevScene &sc, V3 o, V3 d, float tmin, float tmax, float *tOut, float *uOut,
                               float *vOut, uint32_t *primOut, uint32_t *biOut, uint32_t (*stk)[kBlock], uint32_t *spill) {
@@ -123,6 +125,102 @@ __device__ inline bool Trace4(const DevScene &sc, V3 o, V3 d, float tmin, float 
     ref = sp < kLdsStack ? stk[sp][lane] : spill[sp - kLdsStack];
   }
   if (kAny) return false;
+  if (bestPrim == kNone) return false;
+  *tOut = best; *uOut = bu; *vOut = bv; *primOut = bestPrim; *biOut = bestBi;
+  return true;
+}
+
+// Closest hit against the 4-wide quantized BVH; same stack and child order as Trace4.
+// Loop shape (while-while with one postponed leaf, Aila and Laine 2009): a lane that reaches a
+// leaf while its wave is still walking inner nodes parks it and keeps walking; the inner-node
+// loop ends when every lane still in it holds a parked leaf, then the wave tests leaves
+// together. Inner-node and leaf work thus run in separate wave-wide phases instead of
+// alternating under divergent masks every step. The closest hit does not depend on the visit
+// order: ties at equal t go to the smaller primitive index.
+__device__ inline bool Trace4Closest(const DevScene &sc, V3 o, V3 d, float tmin, float tmax, float *tOut, float *uOut,
+                              float *vOut, uint32_t *primOut, uint32_t *biOut, uint32_t (*stk)[kBlock], uint32_t *spill) {
+  const int lane = threadIdx.x;
+  const float ix = 1.f / SafeDir(d.x), iy = 1.f / SafeDir(d.y), iz = 1.f / SafeDir(d.z);
+  const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+  const bool nx = ix < 0.f, ny = iy < 0.f, nz = iz < 0.f;  // near plane = hi bound on negative axes
+  float best = tmax;
+  uint32_t bestPrim = kNone, bestBi = kNone;
+  float bu = 0.f, bv = 0.f;
+  int sp = 0;
+  uint32_t ref = 0;        // next node or leaf to visit; kNone once the stack is empty
+  uint32_t parked = kNone; // postponed leaf
+#define LGPU_POP() (sp == 0 ? kNone : (--sp, sp < kLdsStack ? stk[sp][lane] : spill[sp - kLdsStack]))
+  for (;;) {
+    while (ref != kNone && !(ref & kLeafBit)) {
+      const float4 *np = reinterpret_cast<const float4 *>(sc.nodes4 + ref);
+      const float4 a = np[0], b = np[1];
+      const uint4 qc = reinterpret_cast<const uint4 *>(np)[2], ch = reinterpret_cast<const uint4 *>(np)[3];
+      // t = (p + q s) / d - o / d = q (s / d) + (p / d - o / d)
+      const float Ax = a.w * ix, Bx = a.x * ix - ox;
+      const float Ay = b.x * iy, By = a.y * iy - oy;
+      const float Az = b.y * iz, Bz = a.z * iz - oz;
+      const uint32_t qxl = __float_as_uint(b.z), qxh = __float_as_uint(b.w);
+      const uint32_t qxn = nx ? qxh : qxl, qxf = nx ? qxl : qxh;
+      const uint32_t qyn = ny ? qc.y : qc.x, qyf = ny ? qc.x : qc.y;
+      const uint32_t qzn = nz ? qc.w : qc.z, qzf = nz ? qc.z : qc.w;
+      float tn[4];
+      uint32_t cr[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float lo = fmaxf(fmaxf(fmaf(ByteF(qxn, k), Ax, Bx), fmaf(ByteF(qyn, k), Ay, By)),
+                               fmaxf(fmaf(ByteF(qzn, k), Az, Bz), tmin));
+        const float hi = fminf(fminf(fmaf(ByteF(qxf, k), Ax, Bx), fmaf(ByteF(qyf, k), Ay, By)),
+                               fminf(fmaf(ByteF(qzf, k), Az, Bz), best));
+        const uint32_t r = k == 0 ? ch.x : (k == 1 ? ch.y : (k == 2 ? ch.z : ch.w));
+        const bool h = lo <= hi && r != kNone;
+        tn[k] = h ? lo : INFINITY;
+        cr[k] = h ? r : kNone;
+      }
+      // sort the four (t, ref) pairs nearest first (5 compare-exchanges)
+#define LGPU_CX(i, j)                                                          \
+  if (tn[j] < tn[i]) {                                                         \
+    const float tt_ = tn[i]; tn[i] = tn[j]; tn[j] = tt_;                        \
+    const uint32_t rr_ = cr[i]; cr[i] = cr[j]; cr[j] = rr_;                     \
+  }
+      LGPU_CX(0, 1) LGPU_CX(2, 3) LGPU_CX(0, 2) LGPU_CX(1, 3) LGPU_CX(1, 2)
+#undef LGPU_CX
+      if (cr[0] != kNone) {
+#pragma unroll
+        for (int k = 3; k >= 1; --k) {
+          if (cr[k] != kNone) {
+            if (sp < kLdsStack) stk[sp][lane] = cr[k];
+            else spill[sp - kLdsStack] = cr[k];
+            ++sp;
+          }
+        }
+        ref = cr[0];
+      } else {
+        ref = LGPU_POP();
+      }
+      if (ref != kNone && (ref & kLeafBit) && parked == kNone) {  // park the leaf, keep walking
+        parked = ref;
+        ref = LGPU_POP();
+      }
+      if (__all(parked != kNone)) break;
+    }
+    // leaves: the parked one, then `ref` while it is a leaf
+    while (parked != kNone || (ref != kNone && (ref & kLeafBit))) {
+      uint32_t leaf;
+      if (parked != kNone) { leaf = parked; parked = kNone; }
+      else { leaf = ref; ref = LGPU_POP(); }
+      const uint32_t first = leaf & 0x03FFFFFFu, count = (leaf >> 26) & 31u;
+      for (uint32_t i = first; i < first + count; ++i) {
+        float tt, uu, vv;
+        uint32_t prim;
+        if (!HitTri(sc, i, o, d, tmin, tmax, &tt, &uu, &vv, &prim)) continue;
+        if (tt < best || (tt == best && prim < bestPrim)) {
+          best = tt; bestPrim = prim; bestBi = i; bu = uu; bv = vv;
+        }
+      }
+    }
+    if (ref == kNone) break;
+  }
+#undef LGPU_POP
   if (bestPrim == kNone) return false;
   *tOut = best; *uOut = bu; *vOut = bv; *primOut = bestPrim; *biOut = bestBi;
   return true;
@@ -215,7 +313,8 @@ __device__ inline bool Trace(const DevScene &sc, V3 o, V3 d, float tmin, float t
                              float *vOut, uint32_t *primOut, uint32_t *biOut, uint32_t (*stk)[kBlock], uint32_t *spill,
                              TraceStats *ts) {
   if (kCount) return Trace2<kAny, true>(sc, o, d, tmin, tmax, tOut, uOut, vOut, primOut, biOut, stk, spill, ts);
-  return Trace4<kAny>(sc, o, d, tmin, tmax, tOut, uOut, vOut, primOut, biOut, stk, spill);
+  if (kAny) return Trace4<true>(sc, o, d, tmin, tmax, tOut, uOut, vOut, primOut, biOut, stk, spill);
+  return Trace4Closest(sc, o, d, tmin, tmax, tOut, uOut, vOut, primOut, biOut, stk, spill);
 }
 
 }  // namespace lgpu
