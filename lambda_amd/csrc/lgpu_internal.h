@@ -283,6 +283,7 @@ struct DevScene {
   const uint4 *textures;     // {w, h, channels, offset}
   const LightRec *lights;    // scene lights (add order)
   const LightRec *tlights;   // tree lights
+  const float4 *tri_geo;     // 3 per tree light: triangle lights' {v0, emission socket}, {v1, intensity}, {v2, 0}
   const uint32_t *tree_order;// leaf position -> tree light index
   const TreeNode *tree;
   const TreeImp *timp;       // same nodes, pick-walk layout, breadth-first order

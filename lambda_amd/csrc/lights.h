@@ -142,22 +142,40 @@ struct ShadowSpec {
 // Spotlight.cpp:29-42; EnvironmentLight.cpp:24-45). kInline = false: visibility is deferred to
 // the shadow stage, `sh` receives the occlusion ray of Light::MutualVisibility / RayEscapes.
 // kInline = true: the ray is resolved here through `vc` (volumetric paths), Li carries Tr.
+// treeIdx (kInline = false only): the tree-light index of L when L is a tree light; its
+// triangle record in sc.tri_geo is then fetched together with L.
 template <bool kInline>
 __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit &h, Ev &ev, Rng &rng, const VmCtx &vm,
-                                bool corrected, float &pdf, ShadowSpec &sh, const VisCtx *vc) {
+                                bool corrected, float &pdf, ShadowSpec &sh, const VisCtx *vc, uint32_t treeIdx = kNone) {
   const bool noOff = kInline && vc->volumes && vc->atMedium;  // medium vertex: no surface offset
   Spec Tr = S(1.f);
+  const bool geo = !kInline && treeIdx != kNone;
+  float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0, g2 = g0;
+  if (geo) {
+    const float4 *gp = sc.tri_geo + 3 * (size_t)treeIdx;
+    g0 = gp[0]; g1 = gp[1]; g2 = gp[2];
+  }
   switch (L.kind) {
     case kLightMesh:
     case kLightTri: {
-      const MeshRec m = sc.meshes[L.mesh];
-      const LightRec &ML = L.kind == kLightTri ? sc.lights[L.parent] : L;
+      const bool fromGeo = geo && L.kind == kLightTri;
+      uint32_t firstTri = 0;
+      float mlIntensity = g1.w;
       uint32_t tri;
+      if (!fromGeo) {
+        firstTri = sc.meshes[L.mesh].first_tri;
+        mlIntensity = (L.kind == kLightTri ? sc.lights[L.parent] : L).intensity;
+      }
       if (L.kind == kLightMesh) tri = PwSampleDiscrete(sc.dist, L.cdf_off, rng.Get1D(), &pdf);
       else tri = L.tri;
       const float ux = rng.Get1D(), uy = rng.Get1D();
-      const uint4 tv = sc.tri_vtx[m.first_tri + tri];
-      const V3 v0 = ToV3(sc.vtx_pos[tv.x]), v1 = ToV3(sc.vtx_pos[tv.y]), v2 = ToV3(sc.vtx_pos[tv.z]);
+      V3 v0, v1, v2;
+      if (fromGeo) {
+        v0 = ToV3(g0); v1 = ToV3(g1); v2 = ToV3(g2);
+      } else {
+        const uint4 tv = sc.tri_vtx[firstTri + tri];
+        v0 = ToV3(sc.vtx_pos[tv.x]); v1 = ToV3(sc.vtx_pos[tv.y]); v2 = ToV3(sc.vtx_pos[tv.z]);
+      }
       const V3 pL = PointInTriangle(v0, v1, v2, ux, uy, corrected);
       const V3 pS = noOff ? h.point : h.point + h.normalG * kSurfaceEpsilon * (float)ev.sidedness;
       const V3 diff = pL - pS;  // Scene::MutualVisibility (Scene.cpp:83-95)
@@ -175,7 +193,14 @@ __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit
         if (L.kind == kLightMesh) pdf *= DistSq(h.point, pL) / (cosT * area);
         else if (corrected) pdf = DistSq(h.point, pL) / (cosT * area);
         else pdf = DistSq(h.point, pL) * (cosT * area);  // MeshLight.cpp:145 as written
-        return EmissionOf(sc, L.mesh, vm) * Tr * ML.intensity * kInvPi;
+        Spec e;
+        if (fromGeo) {
+          const uint32_t em = __float_as_uint(g0.w);
+          e = em == kNone ? S(0.f) : SocketSpectrum(sc.sockets[em], vm);
+        } else {
+          e = EmissionOf(sc, L.mesh, vm);
+        }
+        return e * Tr * mlIntensity * kInvPi;
       }
       pdf = 0.f;
       return S(0.f);

@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_shade(const DevScene *__restrict_
   if (pick != kNone) {
     const LightRec &LL = LightOf(sc, pick);
     delta = LL.kind == kLightPoint || LL.kind == kLightSpot;
-    Li = SampleLi<false>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, nullptr);
+    Li = SampleLi<false>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, nullptr, (pick & kTreeBit) ? (pick & ~kTreeBit) : kNone);
   }
   lightPDF *= lightDistPdf;
   bool hasNee = false;

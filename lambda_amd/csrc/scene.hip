@@ -1129,6 +1129,20 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   UP(texRecs, textures);
   UP(lights, lights);
   UP(tl, tlights);
+  // Triangle lights' vertices, emission socket and intensity by tree-light index, so light
+  // sampling reads them in one step instead of through mesh, vertex-index and material records.
+  std::vector<float4> triGeo(3 * tl.size(), make_float4(0.f, 0.f, 0.f, 0.f));
+  for (size_t i = 0; i < tl.size(); ++i) {
+    const LightRec &R = tl[i];
+    if (R.kind != kLightTri) continue;
+    const MeshRec &m = meshes[R.mesh];
+    const uint4 tv = triVtx[m.first_tri + R.tri];
+    const float4 p0 = pos[tv.x], p1 = pos[tv.y], p2 = pos[tv.z];
+    triGeo[3 * i + 0] = make_float4(p0.x, p0.y, p0.z, AsF32(mats[m.material].emission));
+    triGeo[3 * i + 1] = make_float4(p1.x, p1.y, p1.z, lights[R.parent].intensity);
+    triGeo[3 * i + 2] = make_float4(p2.x, p2.y, p2.z, 0.f);
+  }
+  UP(triGeo, tri_geo);
   UP(treeOrder, tree_order);
   UP(tree, tree);
   // pick-walk records in breadth-first order: the top levels form a prefix that
