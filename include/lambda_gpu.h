@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 3u
+#define LGPU_ABI_VERSION 4u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -321,6 +321,10 @@ lgpu_status lgpu_film_clear(lgpu_film film);
 lgpu_status lgpu_film_read(lgpu_film film, double *rgb_sum, uint32_t *counts);
 /* Film::ToRGBTexture: RGBA32F mean per pixel (alpha = 1). */
 lgpu_status lgpu_film_read_rgba(lgpu_film film, float *rgba);
+/* The inverse of lgpu_film_read (no reference counterpart: Denoise::SetData, Denoise.cpp:13-44,
+ * takes any image): restores sums and counts from host arrays, e.g. a saved film to resume or an
+ * image to post-process or denoise. counts may be NULL: every count is then 1. */
+lgpu_status lgpu_film_write(lgpu_film film, const double *rgb_sum, const uint32_t *counts);
 
 /* ---------------- render (MosaicRenderer::Render / TileRenderer /
  * lambdaStartProgressiveRenderer: Render.cpp:43-86, MosaicRenderer.cpp:22-80) */
@@ -342,6 +346,23 @@ typedef struct lgpu_post_desc {
 /* Film mean -> convolution -> tone map -> clamp on the device; rgba receives 4 floats per pixel,
  * rgba8 the gamma-corrected PackChannels bytes (alpha 255). Either may be NULL. */
 lgpu_status lgpu_film_post(lgpu_film film, const lgpu_post_desc *desc, float *rgba, uint8_t *rgba8);
+/* Denoise::SetData + Denoise::Process (Denoise.cpp:13-67; OIDN's filter with colour, albedo and
+ * normal images). OIDN is not available: the filter is the edge-avoiding A-Trous wavelet transform
+ * (Dammertz et al. 2010) over the film means, `iterations` passes of the 5x5 B3-spline kernel with
+ * taps 2^i apart, weighted by exp(-|dc|^2/sc_i^2 - |dn|^2/sn^2 - |da|^2/sa^2) against the centre
+ * pixel, sc_i = sigma_colour 2^-i; a sigma of 0 drops its term. albedo and normal may be NULL.
+ * With an albedo film and demodulate != 0 the filter runs on colour / albedo (channels with
+ * albedo > 1e-3) and multiplies the albedo back. The result goes to `out` as sums with counts of
+ * 1 (out may be the colour film), ready for lgpu_film_post. */
+typedef struct lgpu_denoise_desc {
+  uint32_t iterations;       /* 1..12, typically 5 */
+  float sigma_colour;        /* colour edge-stopping sigma (halved every pass) */
+  float sigma_normal;        /* normal-image edge-stopping sigma */
+  float sigma_albedo;        /* albedo-image edge-stopping sigma */
+  uint32_t demodulate;       /* filter colour / albedo (needs an albedo film) */
+} lgpu_denoise_desc;
+lgpu_status lgpu_film_denoise(lgpu_film colour, lgpu_film albedo, lgpu_film normal, const lgpu_denoise_desc *desc,
+                              lgpu_film out);
 /* 8-bit RGBA PNG, rows in film order (stored deflate blocks). */
 lgpu_status lgpu_write_png(const char *path, const uint8_t *rgba8, uint32_t width, uint32_t height);
 

@@ -5,7 +5,7 @@ binding, so both are driven from the same scene description.
 """
 import ctypes as C
 
-ABI_VERSION = 3  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
+ABI_VERSION = 4  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
 NONE = 0xFFFFFFFF
 
 # lgpu_status
@@ -97,6 +97,19 @@ TONEMAP_NONE, TONEMAP_ACES_FITTED = 0, 1
 class PostDesc(C.Structure):
     _fields_ = [("tonemap", C.c_uint32), ("clamp", C.c_uint32), ("gamma", C.c_uint32), ("estimator_mode", C.c_uint32),
                 ("kernel", FP), ("kernel_w", C.c_uint32), ("kernel_h", C.c_uint32), ("kernel_intensity", C.c_float)]
+
+
+class DenoiseDesc(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("sigma_colour", C.c_float), ("sigma_normal", C.c_float),
+                ("sigma_albedo", C.c_float), ("demodulate", C.c_uint32)]
+
+
+def denoise_desc(iterations=5, sigma_colour=1.0, sigma_normal=0.1, sigma_albedo=0.1, demodulate=True):
+    """lgpu_denoise_desc (edge-avoiding A-Trous filter parameters)."""
+    d = DenoiseDesc()
+    d.iterations, d.sigma_colour, d.sigma_normal, d.sigma_albedo = iterations, sigma_colour, sigma_normal, sigma_albedo
+    d.demodulate = int(demodulate)
+    return d
 
 
 def post_desc(tonemap=TONEMAP_ACES_FITTED, clamp=True, gamma=True, estimator=ESTIMATOR_CORRECTED, kernel=None,

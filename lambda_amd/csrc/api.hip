@@ -362,6 +362,23 @@ lgpu_status lgpu_film_read(lgpu_film f, double *rgb, uint32_t *counts) {
   if (counts) LGPU_HIP(hipMemcpy(counts, f->count.ptr, 4ull * f->width * f->height, hipMemcpyDeviceToHost));
   return LGPU_OK;
 }
+// The inverse of lgpu_film_read: the film's sums and counts from host arrays (a saved film to
+// resume, or an image to post-process or denoise, as sums with counts of 1). NULL counts set 1.
+lgpu_status lgpu_film_write(lgpu_film f, const double *rgb, const uint32_t *counts) {
+  CHECK_ARG(f && rgb, "film_write: null argument");
+  std::lock_guard<std::mutex> lk(f->dev->mu);
+  LGPU_HIP(hipSetDevice(f->dev->device));
+  LGPU_HIP(hipStreamSynchronize(f->dev->stream));
+  const size_t n = (size_t)f->width * f->height;
+  LGPU_HIP(hipMemcpy(f->sum.ptr, rgb, 24ull * n, hipMemcpyHostToDevice));
+  if (counts) {
+    LGPU_HIP(hipMemcpy(f->count.ptr, counts, 4ull * n, hipMemcpyHostToDevice));
+  } else {
+    std::vector<uint32_t> ones(n, 1u);
+    LGPU_HIP(hipMemcpy(f->count.ptr, ones.data(), 4ull * n, hipMemcpyHostToDevice));
+  }
+  return LGPU_OK;
+}
 // Film::ToRGBTexture (Film.cpp:11-24): mean per pixel; pixels with no samples read 0.
 lgpu_status lgpu_film_read_rgba(lgpu_film f, float *rgba) {
   CHECK_ARG(f && rgba, "film_read_rgba: null argument");

@@ -27,7 +27,7 @@ EXPORTS = [
     "lgpu_scene_add_proxy", "lgpu_scene_add_instance", "lgpu_progressive_create", "lgpu_progressive_destroy",
     "lgpu_progressive_set_callback", "lgpu_progressive_start", "lgpu_progressive_stop", "lgpu_progressive_clear",
     "lgpu_progressive_read", "lgpu_progressive_running", "lgpu_obj_load", "lgpu_obj_counts", "lgpu_obj_mesh",
-    "lgpu_obj_material_info", "lgpu_obj_destroy", "lgpu_film_post", "lgpu_write_png",
+    "lgpu_obj_material_info", "lgpu_obj_destroy", "lgpu_film_post", "lgpu_write_png", "lgpu_film_write", "lgpu_film_denoise",
 ]
 
 
@@ -63,6 +63,7 @@ def lib():
             "lgpu_obj_load": [C.c_char_p, vp], "lgpu_obj_counts": [vp, vp, vp], "lgpu_obj_mesh": [vp, u32, vp, vp],
             "lgpu_obj_material_info": [vp, u32, vp], "lgpu_obj_destroy": [vp],
             "lgpu_film_post": [vp, vp, vp, vp], "lgpu_write_png": [C.c_char_p, vp, u32, u32],
+            "lgpu_film_write": [vp, vp, vp], "lgpu_film_denoise": [vp, vp, vp, vp, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -218,6 +219,23 @@ class Film:
         cnt = np.zeros((self.height, self.width), dtype=np.uint32)
         _check(lib().lgpu_film_read(self.h, _ptr(rgb), _ptr(cnt)))
         return rgb, cnt
+
+    def write(self, rgb, counts=None):
+        """Sets the film's sums (H x W x 3 float64) and counts (H x W, default 1) (lgpu_film_write)."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float64)
+        if rgb.shape != (self.height, self.width, 3):
+            raise ValueError("film write: expected shape %s" % ((self.height, self.width, 3),))
+        cnt = None if counts is None else np.ascontiguousarray(counts, dtype=np.uint32)
+        if cnt is not None and cnt.shape != (self.height, self.width):
+            raise ValueError("film write: counts must be H x W")
+        _check(lib().lgpu_film_write(self.h, _ptr(rgb), _ptr(cnt) if cnt is not None else None))
+
+    def denoise(self, desc, albedo=None, normal=None, out=None):
+        """Edge-avoiding A-Trous denoise of this film into `out` (default: this film) (lgpu_film_denoise)."""
+        o = out if out is not None else self
+        _check(lib().lgpu_film_denoise(self.h, albedo.h if albedo is not None else None,
+                                       normal.h if normal is not None else None, C.byref(desc), o.h))
+        return o
 
     def post(self, desc):
         """Post-processed image: (RGBA32F, RGBA8) of the film (lgpu_film_post)."""

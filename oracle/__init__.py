@@ -69,6 +69,8 @@ def lib():
         L.oracle_eval_node.argtypes = [vp, u32, u32, vp, C.c_int, vp, vp]
         L.oracle_post.argtypes = [vp, u32, u32, vp, vp, vp]
         L.oracle_post.restype = None
+        L.oracle_denoise.argtypes = [u32, u32, vp, vp, vp, vp, vp]
+        L.oracle_denoise.restype = None
         L.oracle_scene_add_proxy.argtypes = [vp, vp]
         L.oracle_scene_add_instance.argtypes = [vp, u32, vp, u32]
         L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
@@ -237,6 +239,18 @@ def post(rgba, desc):
     o8 = np.zeros((h, w, 4), dtype=np.uint8)
     lib().oracle_post(_ptr(rgba), w, h, C.byref(desc), _ptr(o32), _ptr(o8))
     return o32, o8
+
+
+def denoise(colour, desc, albedo=None, normal=None):
+    """Edge-avoiding A-Trous filter of float32 (h, w, 3) film means; returns (h, w, 3)."""
+    c = np.ascontiguousarray(colour, dtype=np.float32)
+    h, w = c.shape[:2]
+    a = None if albedo is None else np.ascontiguousarray(albedo, dtype=np.float32)
+    n = None if normal is None else np.ascontiguousarray(normal, dtype=np.float32)
+    out = np.zeros((h, w, 3), dtype=np.float32)
+    lib().oracle_denoise(w, h, _ptr(c), _ptr(a) if a is not None else None, _ptr(n) if n is not None else None,
+                         C.byref(desc), _ptr(out))
+    return out
 
 
 def tree_importance(center, diag, axis, theta_o, theta_e, power, p, n, corrected=True):

@@ -3297,6 +3297,57 @@ static Lanes MkLanes(int mode, const float *lambda) {
   for (int i = 0; i < 4; ++i) l.lambda[i] = lambda ? lambda[i] : 0.f;
   return l;
 }
+// ---------------------------------------------------------------- denoiser
+// The library's replacement for OIDN's filter behind Denoise::SetData/Process (Denoise.cpp:13-67):
+// the edge-avoiding A-Trous wavelet transform (Dammertz et al. 2010), restated per pixel on the
+// film means (3 floats per pixel; albedo and normal may be null): `iterations` passes of the 5x5
+// B3-spline kernel with taps 2^i apart, weight h(dx) h(dy) exp(-(|dc|^2/sc_i^2 + |dn|^2/sn^2 +
+// |da|^2/sa^2)), sc_i = sigma_colour 2^-i, a zero sigma dropping its term; taps outside the image
+// are skipped. Demodulation divides colour by albedo channels above 1e-3 and multiplies back.
+void oracle_denoise(uint32_t w, uint32_t h, const float *colour, const float *albedo, const float *normal,
+                    const lgpu_denoise_desc *d, float *out) {
+  const size_t n = (size_t)w * h;
+  const float eps = 1e-3f;
+  const bool demod = albedo && d->demodulate;
+  std::vector<float> a(colour, colour + 3 * n), b(3 * n);
+  if (demod)
+    for (size_t i = 0; i < 3 * n; ++i) a[i] = albedo[i] > eps ? a[i] / albedo[i] : a[i];
+  const float kw[3] = {0.375f, 0.25f, 0.0625f};
+  auto dist2 = [](const float *p, const float *q) {
+    const float x = q[0] - p[0], y = q[1] - p[1], z = q[2] - p[2];
+    return x * x + y * y + z * z;
+  };
+  for (uint32_t it = 0; it < d->iterations; ++it) {
+    const int step = 1 << it;
+    const float sc = d->sigma_colour * std::ldexp(1.f, -(int)it);
+    const float isc = sc > 0.f ? 1.f / (sc * sc) : 0.f;
+    const float isn = d->sigma_normal > 0.f ? 1.f / (d->sigma_normal * d->sigma_normal) : 0.f;
+    const float isa = d->sigma_albedo > 0.f ? 1.f / (d->sigma_albedo * d->sigma_albedo) : 0.f;
+    for (uint32_t y = 0; y < h; ++y)
+      for (uint32_t x = 0; x < w; ++x) {
+        const size_t pi = (size_t)y * w + x;
+        float sx = 0.f, sy = 0.f, sz = 0.f, sw = 0.f;
+        for (int dy = -2; dy <= 2; ++dy) {
+          const int qy = (int)y + dy * step;
+          if (qy < 0 || qy >= (int)h) continue;
+          for (int dx = -2; dx <= 2; ++dx) {
+            const int qx = (int)x + dx * step;
+            if (qx < 0 || qx >= (int)w) continue;
+            const size_t qi = (size_t)qy * w + (size_t)qx;
+            float e = dist2(&a[3 * pi], &a[3 * qi]) * isc;
+            if (normal) e += dist2(normal + 3 * pi, normal + 3 * qi) * isn;
+            if (albedo) e += dist2(albedo + 3 * pi, albedo + 3 * qi) * isa;
+            const float wt = kw[std::abs(dx)] * kw[std::abs(dy)] * std::exp(-e);
+            sx += wt * a[3 * qi]; sy += wt * a[3 * qi + 1]; sz += wt * a[3 * qi + 2]; sw += wt;
+          }
+        }
+        b[3 * pi] = sx / sw; b[3 * pi + 1] = sy / sw; b[3 * pi + 2] = sz / sw;
+      }
+    a.swap(b);
+  }
+  for (size_t i = 0; i < 3 * n; ++i) out[i] = demod && albedo[i] > eps ? a[i] * albedo[i] : a[i];
+}
+
 // ---------------------------------------------------------------- post-processing
 // ConvolutionFilter (PostProcessing.cpp:16-45), AcesFitted + ClampTextureChannels (ToneMap.cpp:19-73),
 // Gamma + PackChannels (Texture.cpp:19-32). in: RGBA32F film means; out32 (RGBA) and out8 (RGBA8)

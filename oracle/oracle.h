@@ -79,6 +79,8 @@ int oracle_light_sample(oracle_scene s, const float p[3], const float n[3], int 
 int oracle_eval_node(oracle_scene s, uint32_t material, uint32_t node, const float ev[11], int mode, const float *lambda,
                      float out[4]);
 void oracle_post(const float *in, uint32_t w, uint32_t h, const lgpu_post_desc *d, float *out32, uint8_t *out8);
+void oracle_denoise(uint32_t w, uint32_t h, const float *colour, const float *albedo, const float *normal,
+                    const lgpu_denoise_desc *d, float *out);
 float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected);
 float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]);
 
