@@ -155,7 +155,7 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
 // kCorrected: the estimator mode (rp.corrected) as a template argument, so each instantiation
 // carries only its own mode's branches.
 template <bool kCorrected>
-__global__ void __launch_bounds__(kBlock, 4) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
