@@ -31,7 +31,7 @@ struct BvhBuilder {
   std::vector<std::array<Box, 2>> cbox;  // exact child boxes per node (for the 4-wide collapse)
   uint32_t maxDepth = 0;
   float pad = 0;
-  static constexpr int kBins = 16;
+  static constexpr int kBins = 32;
   static constexpr uint32_t kMaxLeaf = 31;
 
   void storeChild(BvhNode &n, int slot, const Box &b) {
@@ -98,7 +98,9 @@ struct BvhBuilder {
           la.grow(bb[b]);
           ln += bc[b];
           if (!ln || !rightCount[b + 1]) continue;
-          const float cost = 1.f + (la.area() * (float)ln + rightArea[b + 1] * (float)rightCount[b + 1]) / pa;
+          // SAH with a node-step cost of half a triangle test: the 4-wide collapse makes each
+          // binary level cheaper to traverse than a standalone node (measured best of 0.25..3)
+          const float cost = 0.5f + (la.area() * (float)ln + rightArea[b + 1] * (float)rightCount[b + 1]) / pa;
           if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = b; }
         }
       }
