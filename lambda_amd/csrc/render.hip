@@ -62,6 +62,24 @@ __device__ inline void GenerateRay(const CameraRec &cam, float u, float v, Rng &
   *d = Norm(fp - oo);
 }
 
+// Image-order slot -> pixel: a bijection on [0, w h) that walks the image in 8x8 tiles (row-major
+// tiles, row-major inside a tile) over the largest multiple-of-8 rectangle, then the rest of the
+// pixels in scanline order. A wave's 64 camera rays then cover one tile instead of a 64-pixel run of
+// a scanline, so they take similar paths through the BVH.
+__device__ inline uint32_t TiledPixel(uint32_t j, uint32_t w, uint32_t h) {
+  const uint32_t wt = w & ~7u, ht = h & ~7u, nt = wt * ht;
+  if (j < nt) {
+    const uint32_t tile = j >> 6, in = j & 63u, tilesX = wt >> 3;
+    const uint32_t x = (tile % tilesX) * 8 + (in & 7u), y = (tile / tilesX) * 8 + (in >> 3);
+    return y * w + x;
+  }
+  j -= nt;  // pixels right of the tiled rectangle (rows < ht), then the rows below it
+  const uint32_t rw = w - wt;
+  if (j < rw * ht) return (j / rw) * w + wt + j % rw;
+  j -= rw * ht;
+  return ht * w + j;
+}
+
 __global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, PathState ps, SampleState ss, uint32_t n,
                                                    uint64_t base, const uint32_t *listPix, const uint32_t *listSmp) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -73,7 +91,7 @@ __global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, PathState ps
   } else {
     const uint64_t g = base + i;
     const uint64_t npx = (uint64_t)rp.width * rp.height;
-    pixel = (uint32_t)(g % npx);
+    pixel = TiledPixel((uint32_t)(g % npx), rp.width, rp.height);
     sample = rp.spp_begin + (uint32_t)(g / npx);
   }
   Rng rng;
