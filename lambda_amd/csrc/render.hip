@@ -62,15 +62,17 @@ __device__ inline void GenerateRay(const CameraRec &cam, float u, float v, Rng &
   *d = Norm(fp - oo);
 }
 
-// Image-order slot -> pixel: a bijection on [0, w h) that walks the image in 8x8 tiles (row-major
-// tiles, row-major inside a tile) over the largest multiple-of-8 rectangle, then the rest of the
-// pixels in scanline order. A wave's 64 camera rays then cover one tile instead of a 64-pixel run of
-// a scanline, so they take similar paths through the BVH.
+// Image-order slot -> pixel: a bijection on [0, w h) that walks the image in 16x16 tiles (row-major
+// tiles, each four 8x8 quarters in row-major order, row-major inside a quarter) over the largest
+// multiple-of-16 rectangle, then the rest of the pixels in scanline order. A wave's 64 camera rays
+// then cover one 8x8 square and a 256-path block one 16x16 square instead of runs of a scanline,
+// so they take similar paths through the BVH.
 __device__ inline uint32_t TiledPixel(uint32_t j, uint32_t w, uint32_t h) {
-  const uint32_t wt = w & ~7u, ht = h & ~7u, nt = wt * ht;
+  const uint32_t wt = w & ~15u, ht = h & ~15u, nt = wt * ht;
   if (j < nt) {
-    const uint32_t tile = j >> 6, in = j & 63u, tilesX = wt >> 3;
-    const uint32_t x = (tile % tilesX) * 8 + (in & 7u), y = (tile / tilesX) * 8 + (in >> 3);
+    const uint32_t tile = j >> 8, quarter = (j >> 6) & 3u, in = j & 63u, tilesX = wt >> 4;
+    const uint32_t x = (tile % tilesX) * 16 + (quarter & 1u) * 8 + (in & 7u);
+    const uint32_t y = (tile / tilesX) * 16 + (quarter >> 1) * 8 + (in >> 3);
     return y * w + x;
   }
   j -= nt;  // pixels right of the tiled rectangle (rows < ht), then the rows below it
