@@ -1,5 +1,6 @@
-// Host-side BVH construction: binned-SAH binary BVH over triangle boxes, then a
-// greedy surface-area collapse into 4-wide nodes with 8-bit quantized child boxes.
+// Host-side BVH construction: binned-SAH binary BVH over triangle boxes, then an SAH-optimal
+// collapse (dynamic programming over the binary tree) into 4-wide nodes with 8-bit quantized
+// child boxes.
 #pragma once
 #include <algorithm>
 #include <array>
@@ -150,24 +151,61 @@ struct BvhBuilder {
       hi[a] = b.hi[a] + (std::fabs(b.hi[a]) * 8e-7f + 2.f * pad);
     }
   }
-  // Child boxes of one 4-wide node: pulls up the largest-area inner child until
-  // four children (SAH-built binary tree, greedy surface-area collapse).
+  // Which binary nodes become 4-wide nodes: with the binary tree's leaves fixed, the SAH cost
+  // of the 4-wide tree differs only by the summed area of its nodes. dpf[n][k] = least summed
+  // area of the 4-wide nodes representing n's subtree in at most k child slots of its parent;
+  // dps[n][k] = slots given to n's left child when n is opened into them (0: n is one slot, a
+  // 4-wide node of its own). Children follow their parent in `nodes` (pre-order build).
+  std::vector<std::array<float, 5>> dpf;
+  std::vector<std::array<uint8_t, 5>> dps;
+  float Fdp(uint32_t ref, int k) const { return (ref & kLeafBit) ? 0.f : dpf[ref][k]; }
+  int bestRootSplit(uint32_t n) const {
+    int ba = 1;
+    float bc = INFINITY;
+    for (int a = 1; a <= 3; ++a) {
+      const float c = Fdp(nodes[n].child[0], a) + Fdp(nodes[n].child[1], 4 - a);
+      if (c < bc) { bc = c; ba = a; }
+    }
+    return ba;
+  }
+  void solveCollapse() {
+    dpf.assign(nodes.size(), {});
+    dps.assign(nodes.size(), {});
+    for (size_t i = nodes.size(); i-- > 0;) {
+      const uint32_t l = nodes[i].child[0], r = nodes[i].child[1];
+      Box b = cbox[i][0];
+      b.grow(cbox[i][1]);
+      const int a4 = bestRootSplit((uint32_t)i);
+      dpf[i][1] = b.area() + Fdp(l, a4) + Fdp(r, 4 - a4);
+      dps[i][1] = 0;
+      for (int k = 2; k <= 4; ++k) {
+        float bo = INFINITY;
+        int bs = 0;
+        for (int a = 1; a < k; ++a) {
+          const float c = Fdp(l, a) + Fdp(r, k - a);
+          if (c < bo) { bo = c; bs = a; }
+        }
+        if (bo < dpf[i][1]) { dpf[i][k] = bo; dps[i][k] = (uint8_t)bs; }
+        else { dpf[i][k] = dpf[i][1]; dps[i][k] = 0; }
+      }
+    }
+  }
+  struct C { uint32_t ref; Box box; };
+  void expandSlots(uint32_t ref, const Box &box, int k, C *ch, int &nc) const {
+    if ((ref & kLeafBit) || k == 1 || dps[ref][k] == 0) { ch[nc++] = {ref, box}; return; }
+    const int a = dps[ref][k];
+    expandSlots(nodes[ref].child[0], cbox[ref][0], a, ch, nc);
+    expandSlots(nodes[ref].child[1], cbox[ref][1], k - a, ch, nc);
+  }
+  // Child boxes of one 4-wide node: the cut of n2's binary subtree chosen by solveCollapse.
   uint32_t collapse(uint32_t n2, uint32_t depth) {
     depth4 = std::max(depth4, depth);
-    struct C { uint32_t ref; Box box; };
+    if (dpf.size() != nodes.size()) solveCollapse();
     C ch[4];
     int nc = 0;
-    for (int k = 0; k < 2; ++k) ch[nc++] = {nodes[n2].child[k], cbox[n2][k]};
-    while (nc < 4) {
-      int bi = -1;
-      float ba = -1.f;
-      for (int k = 0; k < nc; ++k)
-        if (!(ch[k].ref & kLeafBit) && ch[k].box.area() > ba) { ba = ch[k].box.area(); bi = k; }
-      if (bi < 0) break;
-      const uint32_t n = ch[bi].ref;
-      ch[bi] = {nodes[n].child[0], cbox[n][0]};
-      ch[nc++] = {nodes[n].child[1], cbox[n][1]};
-    }
+    const int a = bestRootSplit(n2);
+    expandSlots(nodes[n2].child[0], cbox[n2][0], a, ch, nc);
+    expandSlots(nodes[n2].child[1], cbox[n2][1], 4 - a, ch, nc);
     const uint32_t me = (uint32_t)nodes4.size();
     nodes4.emplace_back();
     uint32_t refs[4];
