@@ -63,7 +63,7 @@ __device__ inline uint32_t SampleLight(const DevScene &sc, V3 point, V3 normalS,
 }
 
 // LightSampler::Pdf for the light hit by a BSDF sample (ManyLightSampler.cpp:29-35, 380-393; LightSampler.cpp:18-20)
-__device__ inline float LightPdf(const DevScene &sc, uint32_t pick, V3 point, V3 normalS, bool corrected) {
+__device__ LGPU_LOBE_INLINE inline float LightPdf(const DevScene &sc, uint32_t pick, V3 point, V3 normalS, bool corrected) {
   if (sc.strategy == LGPU_LIGHTS_POWER) return sc.lights[pick].power * sc.inv_total_power;
   if (!(pick & kTreeBit)) return sc.inf_power / (sc.inf_power + sc.tree_power);
   const uint32_t tl = pick & ~kTreeBit;
@@ -100,7 +100,7 @@ __device__ inline float Env2DPdf(const DevScene &sc, float u, float v) {  // Pie
   iv = iv < 0 ? 0 : (iv > nv - 1 ? nv - 1 : iv);
   return PwPdfRaw(sc.dist, sc.env.cond_off + (uint32_t)iv * sc.env.cond_stride, (uint32_t)iu) / PwIntegral(sc.dist, sc.env.marg_off);
 }
-__device__ inline float EnvPdfLi(const DevScene &sc, V3 wi) {  // EnvironmentLight.cpp:60-67
+__device__ LGPU_LOBE_INLINE inline float EnvPdfLi(const DevScene &sc, V3 wi) {  // EnvironmentLight.cpp:60-67
   const float theta = SphericalTheta(wi) - sc.env.offset[1];
   const float phi = SphericalPhi(wi) - sc.env.offset[0];
   const float sinT = sinf(theta);

@@ -6,6 +6,16 @@
 #include "lgpu_internal.h"
 #include "vm.h"
 
+// The lobe functions and the Fresnel and light-pdf helpers they call are inlined into the
+// path-integrator shading kernel: out of line, their reference arguments (event, hit, sampler
+// state, the shader's VM context) live in per-lane scratch and are written and re-read at every
+// call, and the caller saves its live registers around each call. The volume and direct-lighting
+// kernels define LGPU_LOBE_INLINE empty before including this header and leave the choice to the
+// compiler (inlined into the volume kernel, the lobes spill several hundred registers).
+#ifndef LGPU_LOBE_INLINE
+#define LGPU_LOBE_INLINE __attribute__((always_inline))
+#endif
+
 namespace lgpu {
 
 struct Hit {
@@ -98,7 +108,7 @@ __device__ inline V3 SampleCosineHemisphere(float ux, float uy) {
   return v3(dx, up, dy);
 }
 
-__device__ inline Spec FrConductor(float cosI, Spec etaI, Spec etaT, Spec k) {  // Fresnel.cpp:23-45
+__device__ LGPU_LOBE_INLINE inline Spec FrConductor(float cosI, Spec etaI, Spec etaT, Spec k) {  // Fresnel.cpp:23-45
   cosI = Clampf(cosI, -1.f, 1.f);
   const Spec eta = etaT / etaI;
   const Spec etak = k / etaI;
@@ -208,14 +218,6 @@ __device__ inline void RoughDielectricEval(V3 wo, V3 wi, float alpha, float ior,
 }
 
 // ---------------------------------------------------------------- lobes
-// The lobe functions are inlined into the path-integrator shading kernel: out of line, their
-// reference arguments (event, hit, sampler state, the shader's VM context) live in per-lane scratch
-// and are written and re-read at every call. The volume and direct-lighting kernels define
-// LGPU_LOBE_INLINE empty before including this header and leave the choice to the compiler
-// (inlined into the volume kernel, the lobes spill several hundred registers).
-#ifndef LGPU_LOBE_INLINE
-#define LGPU_LOBE_INLINE __attribute__((always_inline))
-#endif
 struct Shader {
   const DevScene *sc;
   VmCtx vm;
