@@ -307,6 +307,7 @@ struct DevScene {
   const float *density;      // grid densities of all media
   uint32_t num_media;
   uint32_t spill_words;      // per-thread traversal-stack words beyond the LDS part this tree can need
+  uint32_t has_programs;     // some socket runs VM code (shading kernels specialise on it)
   unsigned long long *voxel_count;  // non-null in counting renders: voxels visited by grid walks
 };
 

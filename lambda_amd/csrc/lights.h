@@ -116,10 +116,11 @@ __device__ inline void TriAreaNormal(V3 v0, V3 v1, V3 v2, float *area, V3 *norma
   *area = Mag(c) * .5f;
   if (normal) *normal = c / (*area * 2.f);
 }
+template <bool kPrograms = true>
 __device__ inline Spec EmissionOf(const DevScene &sc, uint32_t mesh, const VmCtx &vm) {
   const MaterialRec m = sc.materials[sc.meshes[mesh].material];
   if (m.emission == kNone) return S(0.f);
-  return SocketSpectrum(sc.sockets[m.emission], vm);
+  return SocketSpectrum<kPrograms>(sc.sockets[m.emission], vm);
 }
 __device__ inline Spec LightColour(const LightRec &L, const VmCtx &vm) {
   if (L.temperature > 0) {
@@ -144,7 +145,7 @@ struct ShadowSpec {
 // kInline = true: the ray is resolved here through `vc` (volumetric paths), Li carries Tr.
 // treeIdx (kInline = false only): the tree-light index of L when L is a tree light; its
 // triangle record in sc.tri_geo is then fetched together with L.
-template <bool kInline>
+template <bool kInline, bool kPrograms = true>
 __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit &h, Ev &ev, Rng &rng, const VmCtx &vm,
                                 bool corrected, float &pdf, ShadowSpec &sh, const VisCtx *vc, uint32_t treeIdx = kNone) {
   const bool noOff = kInline && vc->volumes && vc->atMedium;  // medium vertex: no surface offset
@@ -196,9 +197,9 @@ __device__ inline Spec SampleLi(const DevScene &sc, const LightRec &L, const Hit
         Spec e;
         if (fromGeo) {
           const uint32_t em = __float_as_uint(g0.w);
-          e = em == kNone ? S(0.f) : SocketSpectrum(sc.sockets[em], vm);
+          e = em == kNone ? S(0.f) : SocketSpectrum<kPrograms>(sc.sockets[em], vm);
         } else {
-          e = EmissionOf(sc, L.mesh, vm);
+          e = EmissionOf<kPrograms>(sc, L.mesh, vm);
         }
         return e * Tr * mlIntensity * kInvPi;
       }

@@ -174,7 +174,9 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
 
 // kCorrected: the estimator mode (rp.corrected) as a template argument, so each instantiation
 // carries only its own mode's branches.
-template <bool kCorrected>
+// kPrograms: the scene has program sockets (DevScene::has_programs); without them the kernel is
+// compiled without the shader VM.
+template <bool kCorrected, bool kPrograms>
 __global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
@@ -257,7 +259,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict_
     } else {
       const MeshRec m = sc.meshes[h.mesh];
       if (m.mesh_light != kNone && (!corrected || Dot(rd, h.normalG) < 0)) {  // :26
-        Ladd = EmissionOf(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
+        Ladd = EmissionOf<kPrograms>(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
         addL = true;
       }
     }
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict_
         const LightRec &NL = LightOf(sc, nl);
         if (!corrected && sc.strategy == LGPU_LIGHTS_TREE) pl = PdfLiHit(sc, sc.lights[NL.parent], rd, h, bi, false);
         else pl = PdfLiHit(sc, NL, rd, h, bi, corrected);
-        Le = EmissionOf(sc, h.mesh, vm) * sc.lights[sc.meshes[h.mesh].mesh_light].intensity * kInvPi;
+        Le = EmissionOf<kPrograms>(sc, h.mesh, vm) * sc.lights[sc.meshes[h.mesh].mesh_light].intensity * kInvPi;
         if (corrected && Dot(rd, h.normalG) > 0) Le = S(0.f);
       } else {
         pl = EnvPdfLi(sc, rd);
@@ -329,7 +331,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict_
   }
 
   // ---- shading vertex (PathIntegrator.cpp:28-80)
-  Shader shd;
+  ShaderT<kPrograms> shd;
   shd.sc = &sc;
   shd.corrected = corrected;
   Ev ev;
@@ -351,7 +353,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict_
   if (pick != kNone) {
     const LightRec &LL = LightOf(sc, pick);
     delta = LL.kind == kLightPoint || LL.kind == kLightSpot;
-    Li = SampleLi<false>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, nullptr, (pick & kTreeBit) ? (pick & ~kTreeBit) : kNone);
+    Li = SampleLi<false, kPrograms>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, nullptr, (pick & kTreeBit) ? (pick & ~kTreeBit) : kNone);
   }
   lightPDF *= lightDistPdf;
   bool hasNee = false;
@@ -493,6 +495,16 @@ __global__ void __launch_bounds__(kBlock) k_query(DevScene sc, const float4 *ray
 }
 
 // ---------------------------------------------------------------- host side
+static void LaunchShade(bool corrected, bool programs, uint32_t blocks, hipStream_t stream, const DevScene *scp,
+                        const RenderParams &rp, const PathState &ps, const SampleState &ss, uint32_t n, const uint32_t *np,
+                        const Queues &q) {
+  const dim3 g(blocks), b(kBlock);
+  if (corrected && programs) hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
+  else if (corrected) hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
+  else if (programs) hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
+  else hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
+}
+
 struct Wave {
   // two path-state buffers (ping-pong), per-sample radiance, shadow queue
   DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene, ring;
@@ -622,6 +634,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   // whose address reaches out-of-line functions is copied into every lane's scratch.
   LGPU_HIP(hipMemcpyAsync(W.scene.ptr, &dsc, sizeof(DevScene), hipMemcpyHostToDevice, stream));
   const DevScene *dscp = W.scene.as<DevScene>();
+  const bool progs = dsc.has_programs != 0;
   hipEvent_t *ev = dev->ev;
   lgpu_render_stats S;
   std::memset(&S, 0, sizeof(S));
@@ -666,10 +679,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[1], stream));
           Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), cout};
-          if (rp.corrected)
-            hipLaunchKernelGGL(k_shade<true>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
-          else
-            hipLaunchKernelGGL(k_shade<false>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
+          LaunchShade(rp.corrected != 0, progs, Blocks(ub), stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[2], stream));
           hipLaunchKernelGGL(k_shadow, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(), ub,
@@ -724,10 +734,8 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[3], stream));
         Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
-        if (rp.integrator == LGPU_INTEGRATOR_PATH && rp.corrected)
-          hipLaunchKernelGGL(k_shade<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
-        else if (rp.integrator == LGPU_INTEGRATOR_PATH)
-          hipLaunchKernelGGL(k_shade<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
+        if (rp.integrator == LGPU_INTEGRATOR_PATH)
+          LaunchShade(rp.corrected != 0, progs, Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
         else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
           LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
         else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)

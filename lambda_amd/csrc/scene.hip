@@ -973,6 +973,9 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   d.num_lights = (uint32_t)lights.size();
   d.num_media = (uint32_t)media.size();
   d.spill_words = spillWords;
+  d.has_programs = 0;
+  for (const Socket &k : sockets)
+    if (k.kind != kSockScalar && k.kind != kSockColour && k.kind != kSockBlackbody && k.kind != kSockVec2) d.has_programs = 1;
   d.has_env = hasEnv;
   d.env_light = envLight;
   d.env = env;

@@ -218,15 +218,19 @@ __device__ inline void RoughDielectricEval(V3 wo, V3 wi, float alpha, float ior,
 }
 
 // ---------------------------------------------------------------- lobes
-struct Shader {
+// kPrograms: as for the socket readers (false only on scenes without program sockets).
+template <bool kPrograms = true>
+struct ShaderT {
   const DevScene *sc;
   VmCtx vm;
   bool corrected;
-  __device__ Spec SockS(uint32_t i) const { return i == kNone ? S(0.f) : SocketSpectrum(sc->sockets[i], vm); }
-  __device__ LGPU_LOBE_INLINE float SockR(uint32_t i) const { return i == kNone ? 0.f : SocketReal(sc->sockets[i], vm); }
+  __device__ Spec SockS(uint32_t i) const { return i == kNone ? S(0.f) : SocketSpectrum<kPrograms>(sc->sockets[i], vm); }
+  __device__ LGPU_LOBE_INLINE float SockR(uint32_t i) const {
+    return i == kNone ? 0.f : SocketReal<kPrograms>(sc->sockets[i], vm);
+  }
   __device__ void Alpha(const Lobe &L, float *ax, float *ay) const {
     if (L.sock[1] == kNone) { *ax = *ay = 0.f; }
-    else SocketRoughness(sc->sockets[L.sock[1]], vm, ax, ay);
+    else SocketRoughness<kPrograms>(sc->sockets[L.sock[1]], vm, ax, ay);
     if (L.p[9] == 0.f) { *ax = ToAlpha(*ax); *ay = ToAlpha(*ay); }
   }
   __device__ Spec Conductor(const float *rgb) const {
@@ -453,6 +457,7 @@ struct Shader {
     return bf * r + LeafF(A, ev) * (1.f - r);
   }
 };
+using Shader = ShaderT<true>;
 
 // ---------------------------------------------------------------- light tree importance (ManyLightSampler.cpp:305-326)
 // ManyLightSampler::ImportanceMeasure (ManyLightSampler.cpp:305-326) without inverse

@@ -334,7 +334,11 @@ static __host__ __device__ __attribute__((noinline)) float4 RunProgram(const VmC
   return r[res];
 }
 
+// Socket readers. kPrograms = false compiles them without the VM: for shading kernels launched
+// only on scenes without program sockets (DevScene::has_programs == 0), where the out-of-line
+// RunProgram calls would otherwise shape the whole kernel's register allocation.
 // Socket read as a spectrum (GetAsSpectrum).
+template <bool kPrograms = true>
 LGPU_HD Spec SocketSpectrum(const Socket &s, const VmCtx &c) {
   switch (s.kind) {
     case kSockScalar: return S(s.value[0]);
@@ -351,26 +355,34 @@ LGPU_HD Spec SocketSpectrum(const Socket &s, const VmCtx &c) {
       return o;
     }
     case kSockVec2: return S(0.f);
-    default: return S4(RunProgram(c, s.prog_off, s.prog_len, s.res));
+    default:
+      if constexpr (kPrograms) return S4(RunProgram(c, s.prog_off, s.prog_len, s.res));
+      else return S(0.f);
   }
 }
 // Socket read as a real (GetAs<Real>).
+template <bool kPrograms = true>
 LGPU_HD float SocketReal(const Socket &s, const VmCtx &c) {
   switch (s.kind) {
     case kSockScalar: return s.value[0];
     case kSockVec2: return (s.value[0] + s.value[1]) * .5f;
-    case kSockProgram: return RunProgram(c, s.prog_off, s.prog_len, s.res).x;
+    case kSockProgram:
+      if constexpr (kPrograms) return RunProgram(c, s.prog_off, s.prog_len, s.res).x;
+      else return 0.f;
   }
   return s.value[0];
 }
 // Microfacet roughness: scalar (isotropic) or vec2 (anisotropic) (Microfacet.cpp:57-69).
+template <bool kPrograms = true>
 LGPU_HD void SocketRoughness(const Socket &s, const VmCtx &c, float *a, float *b) {
   if (s.kind == kSockVec2) { *a = s.value[0]; *b = s.value[1]; return; }
-  if (s.kind == kSockProgram) {
-    const float4 v = RunProgram(c, s.prog_off, s.prog_len, s.res);
-    if (s.res_type == kVec2) { *a = v.x; *b = v.y; }
-    else { *a = v.x; *b = v.x; }
-    return;
+  if constexpr (kPrograms) {
+    if (s.kind == kSockProgram) {
+      const float4 v = RunProgram(c, s.prog_off, s.prog_len, s.res);
+      if (s.res_type == kVec2) { *a = v.x; *b = v.y; }
+      else { *a = v.x; *b = v.x; }
+      return;
+    }
   }
   *a = *b = s.value[0];
 }
