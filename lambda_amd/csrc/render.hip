@@ -175,9 +175,10 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
 // kCorrected: the estimator mode (rp.corrected) as a template argument, so each instantiation
 // carries only its own mode's branches.
 // kPrograms: the scene has program sockets (DevScene::has_programs); without them the kernel is
-// compiled without the shader VM.
+// compiled without the shader VM, spills little, and runs best at four waves per SIMD (with the
+// VM's call frames and call-site spills, six).
 template <bool kCorrected, bool kPrograms>
-__global__ void __launch_bounds__(kBlock, 6) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
+__global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
                                                   uint32_t n, const uint32_t *np, Queues q) {
   const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
