@@ -277,6 +277,7 @@ __device__ inline float PdfLiHit(const DevScene &sc, const LightRec &L, V3 wi, c
 // ray's hit (si: it hit something); refP/refN is the corrected light-selection reference point.
 // Returns Le * w; the caller applies f or Tr and 1/pdf. envNoLightPdf: the light technique
 // cannot produce the environment (point sampling at medium vertices, corrected).
+template <bool kPrograms = true>
 __device__ inline Spec MisLight(const DevScene &sc, uint32_t l, float lightDistPdf, V3 refP, V3 refN, bool si, const Hit &h,
                                 uint32_t bi, V3 rd, float scatPdf, bool specular, bool envNoLightPdf, VmCtx vm,
                                 bool corrected) {
@@ -299,7 +300,7 @@ __device__ inline Spec MisLight(const DevScene &sc, uint32_t l, float lightDistP
     if (!corrected && sc.strategy == LGPU_LIGHTS_TREE) pl = PdfLiHit(sc, sc.lights[NL.parent], rd, h, bi, false);
     else pl = PdfLiHit(sc, NL, rd, h, bi, corrected);
     VmAtHit(vm, h.u, h.v, h.point, h.normalS);
-    Le = EmissionOf(sc, h.mesh, vm) * sc.lights[sc.meshes[h.mesh].mesh_light].intensity * kInvPi;
+    Le = EmissionOf<kPrograms>(sc, h.mesh, vm) * sc.lights[sc.meshes[h.mesh].mesh_light].intensity * kInvPi;
     if (corrected && Dot(rd, h.normalG) > 0) Le = S(0.f);
   } else {
     pl = envNoLightPdf ? 0.f : EnvPdfLi(sc, rd);
@@ -320,6 +321,7 @@ struct PartialLS {
 };
 // Light::SamplePoint (MeshLight.cpp:66-81,166-180; PointLight.cpp:40-44; Spotlight.cpp:50-54;
 // EnvironmentLight.cpp:69-74). Faithful writes the sampled uv into the event hit (h, vm) as written.
+template <bool kPrograms = true>
 __device__ inline Spec SamplePointL(const DevScene &sc, const LightRec &L, Hit &h, Rng &rng, VmCtx &vm, bool corrected,
                                     PartialLS &ls) {
   switch (L.kind) {
@@ -353,7 +355,7 @@ __device__ inline Spec SamplePointL(const DevScene &sc, const LightRec &L, Hit &
       ve.u = uu;
       ve.v = uv;
       if (!corrected) { h.u = uu; h.v = uv; vm.u = uu; vm.v = uv; }
-      return EmissionOf(sc, L.mesh, ve) * ML.intensity * kInvPi;
+      return EmissionOf<kPrograms>(sc, L.mesh, ve) * ML.intensity * kInvPi;
     }
     case kLightPoint:
     case kLightSpot:

@@ -48,6 +48,7 @@ __device__ inline void Push(const Queues &q, const uint4 &ids, uint32_t sid, uin
 
 // MISVolumetricPathIntegrator::LdMediumPoint (MISVolumetricPathIntegrator.cpp:23-69). `eh` is the
 // event's hit: the path's hit in faithful mode (overwritten here as written), a copy in corrected.
+template <bool kPrograms>
 __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uint32_t med, Ev &ev, Hit &eh, V3 p,
                                      PartialLS &ls, V3 refP, V3 refN, V3 *wiOut, Rng &rng, const VmCtx &vm,
                                      bool corrected, const VisCtx &vc) {
@@ -82,13 +83,16 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
   uint32_t tbi = kNone;
   const bool si = IntersectTr(sc, p, wi, med, -1.f, &Tr, vm, corrected, vc.tc, eh, tbi, nullptr, false);
   if (scatPdf > 0) {
-    const Spec C = MisLight(sc, ls.light, partialLightPDF, refP, refN, si, eh, tbi, wi, scatPdf, false, corrected, vmm,
+    const Spec C = MisLight<kPrograms>(sc, ls.light, partialLightPDF, refP, refN, si, eh, tbi, wi, scatPdf, false, corrected, vmm,
                             corrected);
     Ld = Ld + (corrected ? C * Tr : C * Tr / scatPdf);  // the phase value is missing as written
   }
   return Ld;
 }
 
+// kPrograms: the scene has program sockets (DevScene::has_programs); without them the kernel is
+// compiled without the shader VM.
+template <bool kPrograms>
 __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                       uint32_t n, Queues q, uint32_t *spillBase) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
@@ -150,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
     const float4 pn = ps.prev_n[i];
     VmCtx vmm = vm;
     vmm.medium = pn.w != 0.f;  // looked up for a medium interaction
-    const Spec C = MisLight(sc, aux.y, mis.y, ToV3(ps.prev_p[i]), ToV3(pn), si, th, tbi, rd, mis.x,
+    const Spec C = MisLight<kPrograms>(sc, aux.y, mis.y, ToV3(ps.prev_p[i]), ToV3(pn), si, th, tbi, rd, mis.x,
                             (flags & kFlagSpecular) != 0, false, vmm, corrected);
     Lacc = Lacc + beta * (C * Tr * mis.w);
     if ((flags & kFlagEmitOnly) || !hit) alive = false;
@@ -167,7 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
     } else {
       const MeshRec m = sc.meshes[h.mesh];
       if (m.mesh_light != kNone && (!corrected || Dot(rd, h.normalG) < 0)) {
-        Spec Le = EmissionOf(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
+        Spec Le = EmissionOf<kPrograms>(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
         if (med != kNone) Le = Le * MediumTr(sc, sc.media[med], ro, rd, h.tFar, vm);
         Lacc = Lacc + (corrected ? beta * Le : Le);  // beta taken as 1 as written
       }
@@ -210,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
             const LightRec &LL = LightOf(sc, pick);
             delta = LL.kind == kLightPoint || LL.kind == kLightSpot;
             ShadowSpec sh;
-            Li = SampleLi<true>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, &vc);
+            Li = SampleLi<true, kPrograms>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, &vc);
           }
           lightPDF *= lightDistPdf;
           Spec Ld = S(0.f);
@@ -247,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         ls.selPdf = ls.pdf;
         ls.point = v3(0, 0, 0);
         ls.normal = v3(0, 0, 0);
-        if (ls.light != kNone) ls.Le = SamplePointL(sc, LightOf(sc, ls.light), h, rng, vm, corrected, ls);
+        if (ls.light != kNone) ls.Le = SamplePointL<kPrograms>(sc, LightOf(sc, ls.light), h, rng, vm, corrected, ls);
         else { ls.pdf = 0.f; ls.Le = S(0.f); }
         const int choice = rng.Get1D() > .5f ? 0 : 1;  // 0 distance, 1 equiangular
         const float du0 = rng.Get1D(), du1 = rng.Get1D();
@@ -267,8 +271,8 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
             const float wD = PowerHeuristic(sD.pdf, pdfE2), wE = PowerHeuristic(sE.pdf, pdfD2);
             V3 wi0, wi1;
             Spec Ld = S(0.f);
-            Ld = Ld + LdMediumPoint(sc, M, med, ev, h, mp0, ls, refP, refN, &wi0, rng, vm, false, vc) * sD.w * wD;
-            Ld = Ld + LdMediumPoint(sc, M, med, ev, h, mp1, ls, refP, refN, &wi1, rng, vm, false, vc) * sE.w * wE;
+            Ld = Ld + LdMediumPoint<kPrograms>(sc, M, med, ev, h, mp0, ls, refP, refN, &wi0, rng, vm, false, vc) * sD.w * wD;
+            Ld = Ld + LdMediumPoint<kPrograms>(sc, M, med, ev, h, mp1, ls, refP, refN, &wi1, rng, vm, false, vc) * sE.w * wE;
             Lacc = Lacc + Ld;  // beta missing as written
             beta = beta * (choice == 0 ? sD.w : sE.w);
             out.o = choice == 0 ? mp0 : mp1;
@@ -284,13 +288,13 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
             PartialLS l0 = ls;
             Hit eh = h;
             const float wD = PowerHeuristic(sD.pdf, PdfEquiangular(ro, rd, ls.point, h.tFar, sD.t, true));
-            Ld = Ld + LdMediumPoint(sc, M, med, ev, eh, mp0, l0, refP, refN, &wi0, rng, vm, true, vc) * sD.w * wD;
+            Ld = Ld + LdMediumPoint<kPrograms>(sc, M, med, ev, eh, mp0, l0, refP, refN, &wi0, rng, vm, true, vc) * sD.w * wD;
           }
           if (sE.inter) {
             PartialLS l1 = ls;
             Hit eh = h;
             const float wE = PowerHeuristic(sE.pdf, PdfDistance(sc, M, ro, rd, sE.t, vm, true));
-            Ld = Ld + LdMediumPoint(sc, M, med, ev, eh, mp1, l1, refP, refN, &wi1, rng, vm, true, vc) * sE.w * wE;
+            Ld = Ld + LdMediumPoint<kPrograms>(sc, M, med, ev, eh, mp1, l1, refP, refN, &wi1, rng, vm, true, vc) * sE.w * wE;
           }
           Lacc = Lacc + beta * Ld;
           beta = beta * sD.w;
@@ -333,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         }
         push = false;
       } else {  // surface vertex (VolumetricPathIntegrator.cpp:45-90)
-        Shader shd;
+        ShaderT<kPrograms> shd;
         shd.sc = &sc;
         shd.corrected = corrected;
         Ev ev;
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
           const LightRec &LL = LightOf(sc, pick);
           delta = LL.kind == kLightPoint || LL.kind == kLightSpot;
           ShadowSpec sh;
-          Li = SampleLi<true>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, &vc);
+          Li = SampleLi<true, kPrograms>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, &vc);
         }
         lightPDF *= lightDistPdf;
         Spec cNee = S(0.f);
@@ -422,7 +426,10 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
 
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                     SampleState ss, uint32_t n, Queues q, uint32_t *spill) {
-  hipLaunchKernelGGL(k_shade_vol, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
+  if (sc.has_programs)
+    hipLaunchKernelGGL(k_shade_vol<true>, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
+  else
+    hipLaunchKernelGGL(k_shade_vol<false>, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
 }
 
 }  // namespace lgpu
