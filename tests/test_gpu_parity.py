@@ -421,3 +421,34 @@ def test_film_post_matches_oracle(dev, tmp_path, estimator):
     p = tmp_path / "film.png"
     gpu.write_png(p, g8)
     assert p.stat().st_size > W * H * 4
+
+
+def _program_material():
+    """A material whose albedo is a procedural texture: its sockets run shader-VM code."""
+    return scene.Material([scene.node(abi.NODE_PERLIN, p=[5]), scene.node(abi.NODE_PERLIN, p=[7]),
+                           scene.node(abi.NODE_PERLIN, p=[3]), scene.node(abi.NODE_MERGE_RGB, [0, 1, 2]),
+                           scene.node(abi.NODE_LAMBERTIAN, [3])], bxdf=4)
+
+
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_PATH, abi.INTEGRATOR_MIS_VOLPATH])
+def test_vm_free_kernels_match_vm_kernels(dev, integrator):
+    """Scenes whose sockets are all constants run shade kernels compiled without the shader VM.
+    Adding a material no mesh uses, with a procedural texture, makes the same scene run the
+    kernels with the VM: both must trace bit-identical paths."""
+    W = H = 32
+    make = scene.cornell_box if integrator == abi.INTEGRATOR_PATH else (lambda: scene.fog_cornell("grid"))
+    lean, full = make(), make()
+    full.add_material(_program_material())
+    s0, k0 = gpu_scene(dev, lean)
+    s1, k1 = gpu_scene(dev, full)
+    cam = scene.cornell_camera(W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    out = []
+    for s in (s0, s1):
+        film = gpu.Film(dev, W, H)
+        d = gpu.render_desc(s, film, cam, 0, 2, spectral=abi.SPECTRAL_HERO4, estimator=abi.ESTIMATOR_CORRECTED,
+                            integrator=integrator)
+        out.append(gpu.trace_paths(d, pixels, samples))
+    assert np.isfinite(out[0]).all()
+    np.testing.assert_array_equal(out[0], out[1])
