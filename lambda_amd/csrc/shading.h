@@ -7,11 +7,11 @@
 #include "vm.h"
 
 // The lobe functions and the Fresnel and light-pdf helpers they call are inlined into the
-// path-integrator shading kernel: out of line, their reference arguments (event, hit, sampler
-// state, the shader's VM context) live in per-lane scratch and are written and re-read at every
-// call, and the caller saves its live registers around each call. The volume and direct-lighting
-// kernels define LGPU_LOBE_INLINE empty before including this header and leave the choice to the
-// compiler (inlined into the volume kernel, the lobes spill several hundred registers).
+// shading kernels: out of line, their reference arguments (event, hit, sampler state, the
+// shader's VM context, the scene record the shader points at) live in per-lane scratch and are
+// written and re-read at every call, and the caller saves its live registers around each call.
+// The direct-lighting kernel defines LGPU_LOBE_INLINE empty before including this header and
+// leaves the choice to the compiler.
 #ifndef LGPU_LOBE_INLINE
 #define LGPU_LOBE_INLINE __attribute__((always_inline))
 #endif

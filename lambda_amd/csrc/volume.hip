@@ -13,7 +13,6 @@
 //   4. otherwise the surface vertex (NEE + BSDF sample) or a pass-through boundary;
 //   5. Russian roulette, then the continuation is appended to the next queue.
 // NEE visibility and the medium points' phase-sampled light lookups are traced inline.
-#define LGPU_LOBE_INLINE
 #include "host.h"
 #include "lights.h"
 #include "media.h"
