@@ -4,7 +4,6 @@
 // to the extend stage), a shading surface estimates direct light from one light chosen by the
 // light sampler (light sample + BSDF sample, power-heuristic MIS) and ends the path. The
 // visibility, PDF_Li and MIS rays are traced inline.
-#define LGPU_LOBE_INLINE
 #include "host.h"
 #include "lights.h"
 #include "media.h"
@@ -15,9 +14,10 @@ namespace lgpu {
 
 namespace {
 
+template <bool kPrograms>
 __device__ inline Spec EmissionAt(const DevScene &sc, const Hit &lh, VmCtx vm) {
   VmAtHit(vm, lh.u, lh.v, lh.point, lh.normalS);
-  return EmissionOf(sc, lh.mesh, vm) * sc.lights[sc.meshes[lh.mesh].mesh_light].intensity * kInvPi;
+  return EmissionOf<kPrograms>(sc, lh.mesh, vm) * sc.lights[sc.meshes[lh.mesh].mesh_light].intensity * kInvPi;
 }
 
 // Light::PDF_Li(event, sampler) as written (MeshLight.cpp:44-55, :153-155; EnvironmentLight.cpp:44-58;
@@ -52,6 +52,9 @@ __device__ inline float PdfLiRay(const DevScene &sc, uint32_t pick, const Hit &h
   return 0.f;
 }
 
+// kPrograms: the scene has program sockets (DevScene::has_programs); without them the kernel is
+// compiled without the shader VM.
+template <bool kPrograms>
 __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
                                                          uint32_t n, Queues q, uint32_t *spillBase) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
@@ -98,7 +101,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
     const MeshRec m = sc.meshes[h.mesh];
     const MaterialRec mat = sc.materials[m.material];
     if (corrected && m.mesh_light != kNone && Dot(rd, h.normalG) < 0)  // first-surface emission (corrected)
-      Lacc = EmissionOf(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
+      Lacc = EmissionOf<kPrograms>(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
     if (mat.lobe == kNone) {  // :24-26
       const uint32_t pass = aux.z + 1;
       if (pass <= 1024) {
@@ -110,7 +113,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
         q.next.aux[j] = make_uint4(sid, kNone, pass, kNone);
       }
     } else {
-      Shader shd;
+      ShaderT<kPrograms> shd;
       shd.sc = &sc;
       shd.corrected = corrected;
       Ev ev;
@@ -130,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
         const VisCtx vc{tc, false, false, kNone, &mat, h.normalG};
         float lightPDF = 0.f;
         ShadowSpec sh;
-        const Spec Li = SampleLi<true>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, &vc);
+        const Spec Li = SampleLi<true, kPrograms>(sc, LL, h, ev, rng, vm, corrected, lightPDF, sh, &vc);
         Spec Ld = S(0.f);
         if (lightPDF > 0 && !IsBlack(Li, mode)) {
           const Spec f = shd.F(mat.lobe, ev) * fabsf(ev.wiL.y);
@@ -159,7 +162,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
               if (ClosestHit(sc, h.point + h.normalG * .00001f, ev.wi, tc, false, lh, lbi)) {
                 // material->light == &_light: under the tree a TriangleLight never equals it
                 const uint32_t ml = sc.meshes[lh.mesh].mesh_light;
-                if (ml != kNone && !(pick & kTreeBit) && ml == pick) Lb = EmissionAt(sc, lh, vm);
+                if (ml != kNone && !(pick & kTreeBit) && ml == pick) Lb = EmissionAt<kPrograms>(sc, lh, vm);
               } else if (LL.kind == kLightEnv) {
                 Lb = EnvLe(sc, ev.wi, vm);
               }
@@ -174,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
               const bool same = ml != kNone && ((pick & kTreeBit) ? (kTreeBit | sc.tri_tlight[lh.prim]) == pick : ml == pick);
               if (same && Dot(ev.wi, lh.normalG) < 0) {
                 lp = PdfLiHit(sc, LL, ev.wi, lh, lbi, true);
-                Lb = EmissionAt(sc, lh, vm);
+                Lb = EmissionAt<kPrograms>(sc, lh, vm);
               }
             } else if (LL.kind == kLightEnv) {
               lp = EnvPdfLi(sc, ev.wi);
@@ -200,7 +203,10 @@ __global__ void __launch_bounds__(kBlock) k_shade_direct(DevScene sc, RenderPara
 
 void LaunchShadeDirect(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                        SampleState ss, uint32_t n, Queues q, uint32_t *spill) {
-  hipLaunchKernelGGL(k_shade_direct, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
+  if (sc.has_programs)
+    hipLaunchKernelGGL(k_shade_direct<true>, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
+  else
+    hipLaunchKernelGGL(k_shade_direct<false>, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
 }
 
 }  // namespace lgpu

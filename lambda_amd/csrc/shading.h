@@ -10,8 +10,8 @@
 // shading kernels: out of line, their reference arguments (event, hit, sampler state, the
 // shader's VM context, the scene record the shader points at) live in per-lane scratch and are
 // written and re-read at every call, and the caller saves its live registers around each call.
-// The direct-lighting kernel defines LGPU_LOBE_INLINE empty before including this header and
-// leaves the choice to the compiler.
+// A kernel that defines LGPU_LOBE_INLINE empty before including this header leaves the choice to
+// the compiler.
 #ifndef LGPU_LOBE_INLINE
 #define LGPU_LOBE_INLINE __attribute__((always_inline))
 #endif
