@@ -614,7 +614,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   // launch-bound tails of their bounce loops.
   uint64_t cap = d->path_capacity;
   if (!cap) {
-    constexpr uint64_t kDefaultPaths = 1ull << 26;
+    constexpr uint64_t kDefaultPaths = 1ull << 27;
     cap = kDefaultPaths;
     size_t freeB = 0, totalB = 0;
     if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
