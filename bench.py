@@ -67,7 +67,7 @@ def build_workload(name):
 def integrator_id(cfg):
     from lambda_amd import abi
     return {"path": abi.INTEGRATOR_PATH, "volpath": abi.INTEGRATOR_VOLPATH,
-            "mis_volpath": abi.INTEGRATOR_MIS_VOLPATH}[cfg["integrator"]]
+            "mis_volpath": abi.INTEGRATOR_MIS_VOLPATH, "direct": abi.INTEGRATOR_DIRECT}[cfg["integrator"]]
 
 
 def cpu_baseline(sd, cam, cfg, spectral, estimator, budget_s):
@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="spp timed (default: 256 many-light, 128 media, 64 cornell)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="many-light", choices=["many-light", "media", "cornell"])
+    ap.add_argument("--integrator", default="", choices=["", "path", "direct", "volpath", "mis_volpath"],
+                    help="render the workload with another integrator (default: the workload's own)")
     ap.add_argument("--spectral", default="hero4", choices=["hero4", "rgb"])
     ap.add_argument("--estimator", default="corrected", choices=["corrected", "faithful"])
     ap.add_argument("--capacity", type=int, default=0)
@@ -138,6 +140,9 @@ def main():
 
     t0 = time.perf_counter()
     sd, cam, cfg = build_workload(args.workload)
+    if args.integrator:
+        cfg["integrator"] = args.integrator
+        cfg["note"] = "workload rendered with the %s integrator" % args.integrator
     log("workload generated in %.1f s (%d triangles)" % (time.perf_counter() - t0, sd.n_tris))
     dev = gpu.Device(0)
     s = gpu.Scene(dev)
@@ -192,7 +197,7 @@ def main():
     #   plus 4 B per density-grid voxel visited by the vertex's medium walks.
     bytes_per_ray = 32.0 * 2.0 * fetch_per_ray + 36.0 * tris_per_ray + 32.0 + 20.0
     bytes_per_vertex = 192.0 + 4.0 * voxels_per_vertex
-    shade_name = "k_shade" if cfg["integrator"] == "path" else "k_shade_vol"
+    shade_name = {"path": "k_shade", "direct": "k_shade_direct"}.get(cfg["integrator"], "k_shade_vol")
     ext_s = st.ms_extend / 1000.0
     sh_s = st.ms_shade / 1000.0
     ext = {"bound": "hbm", "achieved": round(bytes_per_ray * st.extension_rays / ext_s / 1e9, 1),
