@@ -1,11 +1,13 @@
 // Wavefront path tracer kernels and the render loop.
 //
-// One wave = up to `capacity` paths (pixel, sample) in flight. Per bounce:
-//   extend  - closest-hit BVH2 traversal of the active queue (LDS stack)
-//   shade   - emission + MIS of the previous BSDF sample, NEE light sample,
-//             BSDF sample, Russian roulette; appends the next queue and shadow rays
+// One wave = up to `capacity` paths (pixel, sample) in flight; the camera stage lays the image
+// out in 16x16 pixel tiles. Per bounce:
+//   extend  - closest-hit traversal of the 4-wide BVH for the active queue (LDS stack)
+//   shade   - paths grouped by shading family within the block, then emission + MIS of the
+//             previous BSDF sample, NEE light sample, BSDF sample, Russian roulette; appends
+//             the next queue and the shadow rays
 //   shadow  - any-hit traversal; unoccluded rays add their contribution
-// then accumulate adds each finished path to the FP64 film.
+// then accumulate adds each pixel's samples of the wave to the FP64 film.
 // Reference: TileRenderers::UniformSpp (Render.cpp:43-64) and
 // PathIntegrator::Li (PathIntegrator.cpp:15-88).
 #include <chrono>
