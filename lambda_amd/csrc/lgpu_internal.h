@@ -1,6 +1,7 @@
 // Internal data layouts shared by the host scene builder (C++) and the HIP
 // kernels. Everything here lives in HBM as flat arrays; the kernels read them
-// through the DevScene pointer bundle passed by value.
+// through the DevScene pointer bundle, passed by value (the path shade kernel
+// reads a device copy of it through a pointer).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
