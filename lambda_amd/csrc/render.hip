@@ -409,7 +409,7 @@ __device__ inline void SampleRGB(const RenderParams &rp, float4 Lf, float l0, do
       float l = l0 + 117.5f * (float)k;
       if (l >= 830.f) l -= 470.f;
       float cm[3];
-      CMF(l, cm);
+      CMF<true>(l, cm);
       for (int c = 0; c < 3; ++c) xyz[c] += (double)Lc[k] * (double)cm[c];
     }
     const double s = 470.0 / 4.0 / kCieYIntegral;
