@@ -201,7 +201,8 @@ __device__ inline void RoughDielectricEval(V3 wo, V3 wi, float alpha, float ior,
   wh = wh / l;
   if (wh.y < 0) wh = -wh;
   const float dwo = Dot(wo, wh), dwi = Dot(wi, wh);
-  if (!reflect && dwo * dwi >= 0) return;
+  // refraction: wo and wi on opposite sides of the microfacet, wo in front of it
+  if (!reflect && (dwo * dwi >= 0 || dwo * wo.y <= 0)) return;
   const float F = FrDielectric(dwo, 1.f, ior);
   const float D = MfD(true, wh, alpha, alpha);
   const float G = MfG(true, wo, wi, alpha, alpha);
@@ -354,21 +355,24 @@ struct ShaderT {
         const float cosT = 1 / sqrtf(1 + tan2);
         const float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT));
         V3 wh = SphericalDirection(sinT, cosT, phi);
-        if (ev.woL.y < 0) wh = -wh;
+        // Fresnel of wo's side, as RoughDielectricEval takes it (the half vector on the +y side:
+        // a negative cosine swaps the indices, so a wo inside gets the glass-to-air reflectance)
         const float Fr = FrDielectric(Dot(ev.woL, wh), 1.f, ior);
+        if (ev.woL.y < 0) wh = -wh;
         if (uc < Fr) {
           ev.wiL = Reflect(-ev.woL, wh);
           if (!SameHemisphere(ev.woL, ev.wiL)) { pdf = 0.f; return S(0.f); }
         } else {
           const float etaI = ev.woL.y > 0 ? 1.f : ior, etaT = ev.woL.y > 0 ? ior : 1.f;
+          if (!(Dot(ev.woL, wh) > 0)) { pdf = 0.f; return S(0.f); }  // microfacet faces away from wo
           if (!Refract(ev.woL, wh, etaI / etaT, &ev.wiL)) { pdf = 0.f; return S(0.f); }
           if (SameHemisphere(ev.woL, ev.wiL) || ev.wiL.y == 0) { pdf = 0.f; return S(0.f); }
         }
         float f;
         RoughDielectricEval(ev.woL, ev.wiL, alpha, ior, &f, &pdf);
         ev.wi = ToWorld(h, ev.wiL);
-        const float side = ev.wiL.y > 0 ? 1.f : -1.f;
-        h.point = h.point + h.normalG * (kSurfaceEpsilon * side * (float)ev.sidedness);
+        // spawn on the side wi leaves into: reflections stay on wo's side, refractions cross
+        h.point = h.point + h.normalG * (ev.wiL.y > 0 ? kSurfaceEpsilon : -kSurfaceEpsilon);
         return SockS(L.sock[0]) * f;
       }
       case kFresnelBsdf: {  // Specular.cpp:18-45

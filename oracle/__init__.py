@@ -73,6 +73,11 @@ def lib():
         L.oracle_denoise.restype = None
         L.oracle_scene_add_proxy.argtypes = [vp, vp]
         L.oracle_scene_add_instance.argtypes = [vp, u32, vp, u32]
+        L.oracle_bxdf_sample_n.argtypes = [vp, u32, vp, C.c_int, vp, C.c_int, u32, u32, f32, vp]
+        L.oracle_bxdf_eval_n.argtypes = [vp, u32, u32, vp, vp, C.c_int, vp, C.c_int, vp]
+        L.oracle_generate_ray_n.argtypes = [vp, u32, vp, u32, C.c_int, vp]
+        L.oracle_light_sample_li_n.argtypes = [vp, u32, vp, vp, u32, C.c_int, vp, C.c_int, u32, vp]
+        L.oracle_env_pdf_n.argtypes = [vp, u32, u32, vp, vp]
         L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
         L.oracle_tree_importance.restype = f32
         _LIB = L
@@ -175,6 +180,42 @@ class OracleScene:
             raise ValueError("eval_node: bad material or node")
         return t, out
 
+    def bxdf_sample(self, material, wo, n, pixel=0, mode=0, corrected=True, lambdas=None, eta=1.001):
+        """n samples of a material's BxDF in the world frame (normal +y) for local direction wo:
+        dict of wi (n, 3), pdf, f (n, 4), spec, side (spawn offset sign along the normal), eta."""
+        out = np.zeros((n, 11), dtype=np.float32)
+        w = _v3(wo)
+        lam = np.asarray(lambdas if lambdas is not None else (0, 0, 0, 0), dtype=np.float32)
+        self._chk(self.L.oracle_bxdf_sample_n(self.h, material, _ptr(w), mode, _ptr(lam), int(corrected), pixel, n,
+                                              eta, _ptr(out)), "bxdf_sample")
+        return dict(wi=out[:, 0:3], pdf=out[:, 3], f=out[:, 4:8], spec=out[:, 8] != 0, side=out[:, 9], eta=out[:, 10])
+
+    def bxdf_eval(self, material, wo, wi, mode=0, corrected=True, lambdas=None):
+        """f (n, 4) and pdf (n,) of a material's BxDF for pairs of local directions."""
+        wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, dtype=np.float32), np.shape(wi)))
+        wi = np.ascontiguousarray(np.asarray(wi, dtype=np.float32))
+        n = len(wi)
+        out = np.zeros((n, 5), dtype=np.float32)
+        lam = np.asarray(lambdas if lambdas is not None else (0, 0, 0, 0), dtype=np.float32)
+        self._chk(self.L.oracle_bxdf_eval_n(self.h, material, n, _ptr(wo), _ptr(wi), mode, _ptr(lam), int(corrected),
+                                            _ptr(out)), "bxdf_eval")
+        return out[:, :4], out[:, 4]
+
+    def light_sample_li(self, light, p, n, count, pixel=0, mode=0, corrected=True, lambdas=None):
+        """count draws of Light::Sample_Li of scene light `light` at (p, n): wi (count, 3), pdf, Li (count, 4)."""
+        out = np.zeros((count, 8), dtype=np.float32)
+        lam = np.asarray(lambdas if lambdas is not None else (0, 0, 0, 0), dtype=np.float32)
+        p, n = _v3(p), _v3(n)
+        self._chk(self.L.oracle_light_sample_li_n(self.h, light, _ptr(p), _ptr(n), count, mode, _ptr(lam),
+                                                  int(corrected), pixel, _ptr(out)), "light_sample_li")
+        return out[:, 0:3], out[:, 3], out[:, 4:8]
+
+    def env_pdf(self, light, wi):
+        wi = np.ascontiguousarray(np.asarray(wi, dtype=np.float32).reshape(-1, 3))
+        out = np.zeros(len(wi), dtype=np.float32)
+        self._chk(self.L.oracle_env_pdf_n(self.h, light, len(wi), _ptr(wi), _ptr(out)), "env_pdf")
+        return out
+
     def occluded(self, rays):
         rays = np.ascontiguousarray(rays, dtype=np.float32)
         occ = np.zeros(len(rays), dtype=np.uint32)
@@ -257,6 +298,16 @@ def tree_importance(center, diag, axis, theta_o, theta_e, power, p, n, corrected
     """Light-tree node importance at point p with shading normal n (zero normal: medium point)."""
     node = np.array([*center, diag, *axis, theta_o, theta_e, power], dtype=np.float32)
     return float(lib().oracle_tree_importance(_ptr(node), _ptr(_v3(p)), _ptr(_v3(n)), int(corrected)))
+
+
+def generate_rays(cam, uv, pixel=0, corrected=True):
+    """Camera::GenerateRay at image points uv (n, 2): origins (n, 3), directions (n, 3)."""
+    uv = np.ascontiguousarray(np.asarray(uv, dtype=np.float32).reshape(-1, 2))
+    out = np.zeros((len(uv), 6), dtype=np.float32)
+    r = lib().oracle_generate_ray_n(C.byref(cam), len(uv), _ptr(uv), pixel, int(corrected), _ptr(out))
+    if r < 0:
+        raise RuntimeError("generate_rays failed")
+    return out[:, :3], out[:, 3:]
 
 
 def philox(ctr, key):

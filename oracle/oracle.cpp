@@ -1208,7 +1208,9 @@ static void RoughDielectricEval(const V3 &wo, const V3 &wi, float alpha, float i
   wh = wh / l;
   if (wh.y < 0) wh = -wh;
   const float dwo = Dot(wo, wh), dwi = Dot(wi, wh);
-  if (!reflect && dwo * dwi >= 0) return;
+  // refraction needs wo and wi on opposite sides of the microfacet and wo in front of it (the
+  // sampler reaches no refraction through a microfacet that faces away from wo)
+  if (!reflect && (dwo * dwi >= 0 || dwo * wo.y <= 0)) return;
   const float F = FrDielectric(dwo, 1.f, ior);
   const float D = MfD(1, wh, alpha, alpha);
   const float G = MfG(1, wo, wi, alpha, alpha);
@@ -1349,21 +1351,25 @@ static Spec BxDF_Sample(const Ctx &c, const Material &m, uint32_t node, Event &e
       const float cosT = 1 / std::sqrt(1 + tan2);
       const float sinT = std::sqrt(std::max(0.f, 1.f - cosT * cosT));
       V3 wh = SphericalDirection(sinT, cosT, phi);
-      if (ev.woL.y < 0) wh = -wh;
+      // reflect/refract chosen with the Fresnel of wo's side, the one RoughDielectricEval's pdf
+      // uses: the half vector is still on the +y side here, so a wo inside the glass (negative
+      // cosine) gets the glass-to-air reflectance (total internal reflection included)
       const float F = FrDielectric(Dot(ev.woL, wh), 1.f, ior);
+      if (ev.woL.y < 0) wh = -wh;
       if (uc < F) {
         ev.wiL = Reflect(-ev.woL, wh);
         if (!SameHemisphere(ev.woL, ev.wiL)) { pdf = 0; return Spec(0); }
       } else {
         const float etaI = ev.woL.y > 0 ? 1.f : ior, etaT = ev.woL.y > 0 ? ior : 1.f;
+        if (!(Dot(ev.woL, wh) > 0)) { pdf = 0; return Spec(0); }  // microfacet faces away from wo
         if (!Refract(ev.woL, wh, etaI / etaT, &ev.wiL)) { pdf = 0; return Spec(0); }
         if (SameHemisphere(ev.woL, ev.wiL) || ev.wiL.y == 0) { pdf = 0; return Spec(0); }
       }
       float f;
       RoughDielectricEval(ev.woL, ev.wiL, alpha, ior, &f, &pdf);
       ev.wi = ev.ToWorld(ev.wiL);
-      const float side = ev.wiL.y > 0 ? 1.f : -1.f;
-      h.point += h.normalG * (SURFACE_EPSILON * side * (float)ev.sidedness);
+      // the next ray starts on the side wi points into (reflection: wo's side; refraction: across)
+      h.point += h.normalG * (ev.wiL.y > 0 ? SURFACE_EPSILON : -SURFACE_EPSILON);
       return SockSpec(c, m, node, 0, ev) * f;
     }
     case LGPU_NODE_FRESNEL_BSDF: {  // Specular.cpp:18-45
@@ -2069,7 +2075,9 @@ static float TreeImportance(const LNode *nd, const V3 &point, const V3 &normalS,
     const float E = nd->power;
     // a zero normal marks a medium vertex (corrected mode): no surface-cosine bound
     if (normalS.x == 0.f && normalS.y == 0.f && normalS.z == 0.f) return E / d2 * std::cos(thetaDash);
-    const float thetaI = std::acos(corrected ? std::abs(Dot(delta, normalS)) : Dot(delta, normalS));
+    // a cosine that rounding pushed past 1 (normal pointing straight at the node centre) is clamped:
+    // acos would give NaN and a NaN pick probability (the device clamps the same way)
+    const float thetaI = std::acos(Clampf(corrected ? std::abs(Dot(delta, normalS)) : Dot(delta, normalS), -1.f, 1.f));
     return (std::cos(std::max(thetaI - thetaU, 0.f)) * E) / d2 * std::cos(thetaDash);
   }
   return 0.f;
@@ -3433,6 +3441,70 @@ int oracle_eval_node(oracle_scene s, uint32_t material, uint32_t node, const flo
   for (int i = 0; i < 4; ++i) out[i] = v.t == VT_SPECTRUM ? v.s.c[i] : v.v[i];
   return v.t;
 }
+// One BxDF of a material at a shading frame equal to the world frame (tangent x, normal y,
+// bitangent z; normalG = normalS = +y), hit point at the origin. Test access for the closed-form
+// pins of the lobes (sampling against pdf, energy, reciprocity). out per sample i (Philox pixel
+// `pixel`, sample i): wiL[3], pdf, f[4] (f cos|wiL.y| is NOT applied), spec flag, offset side of
+// the spawn point along normalG (+1, -1 or 0), eta after the sample.
+static void LocalFrameHit(Hit &h) {
+  h.point = V3(0, 0, 0);
+  h.normalG = V3(0, 1, 0);
+  h.normalS = V3(0, 1, 0);
+  h.tangent = V3(1, 0, 0);
+  h.bitangent = V3(0, 0, 1);
+}
+int oracle_bxdf_sample_n(oracle_scene s, uint32_t material, const float woL[3], int mode, const float *lambda,
+                         int corrected, uint32_t pixel, uint32_t n, float eta, float *out) {
+  if (!s || material >= s->materials.size() || s->materials[material].bxdf == LGPU_NONE) return -1;
+  const Material &m = s->materials[material];
+  Ctx c{s, corrected != 0, mode};
+  for (uint32_t i = 0; i < n; ++i) {
+    Hit h;
+    LocalFrameHit(h);
+    Sampler smp;
+    smp.pixel = pixel; smp.sample = i; smp.dim = 0; smp.seed = 0;
+    Event e;
+    e.hit = &h;
+    e.sampler = &smp;
+    e.eta = eta;
+    e.lanes = MkLanes(mode, lambda);
+    e.wo = V3(woL[0], woL[1], woL[2]);
+    e.wi = -e.wo;
+    e.SurfaceLocalise();
+    float pdf = 0;
+    bool sp = false;
+    const Spec f = BxDF_Sample(c, m, m.bxdf, e, smp, pdf, &sp);
+    float *o = out + 11 * (size_t)i;
+    o[0] = e.wiL.x; o[1] = e.wiL.y; o[2] = e.wiL.z; o[3] = pdf;
+    for (int k = 0; k < 4; ++k) o[4 + k] = f.c[k];
+    o[8] = sp ? 1.f : 0.f;
+    o[9] = h.point.y > 0 ? 1.f : (h.point.y < 0 ? -1.f : 0.f);
+    o[10] = e.eta;
+  }
+  return 0;
+}
+// f (4 lanes) and pdf of a material's BxDF for n (woL, wiL) pairs in the same frame: out 5 per pair.
+int oracle_bxdf_eval_n(oracle_scene s, uint32_t material, uint32_t n, const float *woL, const float *wiL, int mode,
+                       const float *lambda, int corrected, float *out) {
+  if (!s || material >= s->materials.size() || s->materials[material].bxdf == LGPU_NONE) return -1;
+  const Material &m = s->materials[material];
+  Ctx c{s, corrected != 0, mode};
+  for (uint32_t i = 0; i < n; ++i) {
+    Hit h;
+    LocalFrameHit(h);
+    Event e;
+    e.hit = &h;
+    e.lanes = MkLanes(mode, lambda);
+    e.wo = V3(woL + 3 * (size_t)i);
+    e.wi = V3(wiL + 3 * (size_t)i);
+    e.SurfaceLocalise();
+    const Spec f = BxDF_f(c, m, m.bxdf, e);
+    float *o = out + 5 * (size_t)i;
+    for (int k = 0; k < 4; ++k) o[k] = f.c[k];
+    o[4] = BxDF_Pdf(c, m, m.bxdf, e.woL, e.wiL, e);
+  }
+  return 0;
+}
 int oracle_medium_tr(oracle_scene s, uint32_t med, const float o[3], const float d[3], float tFar, int mode,
                      const float *lambda, float out[4]) {
   if (!s || med >= s->media.size()) return -1;
@@ -3496,6 +3568,57 @@ int oracle_light_sample(oracle_scene s, const float p[3], const float n[3], int 
   *pdf = pd;
   if (lp.kind == 1) return lp.index;
   return lp.kind == 0 ? -1 : -2;
+}
+
+// Camera::GenerateRay for n image points (u, v) (test hook): sample i draws its aperture
+// dimensions from Philox (pixel, sample i). out: o[3], d[3] per point.
+int oracle_generate_ray_n(const lgpu_camera *cam, uint32_t n, const float *uv, uint32_t pixel, int corrected, float *out) {
+  if (!cam) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    Sampler smp;
+    smp.pixel = pixel; smp.sample = i; smp.dim = 0; smp.seed = 0;
+    V3 o, d;
+    GenerateRay(*cam, uv[2 * (size_t)i], uv[2 * (size_t)i + 1], smp, &o, &d, corrected != 0);
+    float *r = out + 6 * (size_t)i;
+    r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = d.x; r[4] = d.y; r[5] = d.z;
+  }
+  return 0;
+}
+// Light::Sample_Li of scene light `light` at point p with normal n (world frame = shading frame
+// with normal n), n_samples draws from Philox (pixel, sample i): out wi[3], pdf, Li[4] per draw.
+int oracle_light_sample_li_n(oracle_scene s, uint32_t light, const float p[3], const float nrm[3], uint32_t count,
+                             int mode, const float *lambda, int corrected, uint32_t pixel, float *out) {
+  if (!s || !s->committed || light >= s->lights.size()) return -1;
+  const Ctx c{s, corrected != 0, mode};
+  const V3 N = Norm(V3(nrm[0], nrm[1], nrm[2]));
+  const V3 a = std::abs(N.x) > .5f ? V3(0, 1, 0) : V3(1, 0, 0);
+  const V3 T = Norm(Cross(a, N)), B = Cross(T, N);
+  for (uint32_t i = 0; i < count; ++i) {
+    Hit h;
+    h.point = V3(p[0], p[1], p[2]);
+    h.normalG = N; h.normalS = N; h.tangent = T; h.bitangent = B;
+    Sampler smp;
+    smp.pixel = pixel; smp.sample = i; smp.dim = 0; smp.seed = 0;
+    Event e;
+    e.hit = &h;
+    e.sampler = &smp;
+    e.lanes = MkLanes(mode, lambda);
+    e.wo = N;
+    e.wi = -N;
+    e.SurfaceLocalise();
+    float pdf = 0;
+    const Spec L = SampleLi(c, s->lights[light], e, smp, pdf);
+    float *r = out + 8 * (size_t)i;
+    r[0] = e.wi.x; r[1] = e.wi.y; r[2] = e.wi.z; r[3] = pdf;
+    for (int k = 0; k < 4; ++k) r[4 + k] = L.c[k];
+  }
+  return 0;
+}
+// EnvironmentLight::PDF_Li for n directions (test hook).
+int oracle_env_pdf_n(oracle_scene s, uint32_t light, uint32_t n, const float *wi, float *out) {
+  if (!s || !s->committed || light >= s->lights.size() || s->lights[light].kind != L_ENV) return -1;
+  for (uint32_t i = 0; i < n; ++i) out[i] = EnvPdfLi(s->lights[light], V3(wi + 3 * (size_t)i));
+  return 0;
 }
 
 // TreeImportance of one node (test hook): node[] = center xyz, diagonal, axis xyz, thetaO, thetaE,

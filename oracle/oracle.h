@@ -81,6 +81,19 @@ int oracle_eval_node(oracle_scene s, uint32_t material, uint32_t node, const flo
 void oracle_post(const float *in, uint32_t w, uint32_t h, const lgpu_post_desc *d, float *out32, uint8_t *out8);
 void oracle_denoise(uint32_t w, uint32_t h, const float *colour, const float *albedo, const float *normal,
                     const lgpu_denoise_desc *d, float *out);
+/* A material's BxDF in the world frame (normal +y): n samples (Philox pixel, sample i), 11 floats
+ * each: wiL[3], pdf, f[4], delta flag, spawn side along the normal, eta; and f[4] + pdf of n pairs. */
+int oracle_bxdf_sample_n(oracle_scene s, uint32_t material, const float woL[3], int mode, const float *lambda,
+                         int corrected, uint32_t pixel, uint32_t n, float eta, float *out);
+int oracle_bxdf_eval_n(oracle_scene s, uint32_t material, uint32_t n, const float *woL, const float *wiL, int mode,
+                       const float *lambda, int corrected, float *out);
+/* Camera::GenerateRay at n image points uv[2n] (aperture draws from Philox (pixel, i)): o[3], d[3]. */
+int oracle_generate_ray_n(const lgpu_camera *cam, uint32_t n, const float *uv, uint32_t pixel, int corrected, float *out);
+/* Light::Sample_Li of a scene light at p with normal n: count draws of wi[3], pdf, Li[4]. */
+int oracle_light_sample_li_n(oracle_scene s, uint32_t light, const float p[3], const float n[3], uint32_t count,
+                             int mode, const float *lambda, int corrected, uint32_t pixel, float *out);
+/* EnvironmentLight::PDF_Li for n world directions. */
+int oracle_env_pdf_n(oracle_scene s, uint32_t light, uint32_t n, const float *wi, float *out);
 float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected);
 float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]);
 
