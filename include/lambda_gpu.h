@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 4u
+#define LGPU_ABI_VERSION 5u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -266,6 +266,8 @@ typedef struct lgpu_render_stats {
   uint32_t waves;           /* path-pool refills */
   uint32_t launches_extend, launches_shade, launches_shadow;
   uint64_t voxel_steps;     /* with LGPU_RENDER_COUNT_TRAVERSAL: density-grid voxels visited by medium walks */
+  uint64_t nodes4_fetched;  /* with LGPU_RENDER_COUNT_TRAVERSAL: 64-B 4-wide node records the render's walk fetches */
+  uint64_t tris4_fetched;   /* with LGPU_RENDER_COUNT_TRAVERSAL: 48-B triangle records the render's walk fetches */
 } lgpu_render_stats;
 
 /* ---------------- device (Lambda.h:37-40 lambdaCreateDevice/Release) ---- */

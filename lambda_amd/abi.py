@@ -5,7 +5,7 @@ binding, so both are driven from the same scene description.
 """
 import ctypes as C
 
-ABI_VERSION = 4  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
+ABI_VERSION = 5  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
 NONE = 0xFFFFFFFF
 
 # lgpu_status
@@ -137,4 +137,5 @@ class RenderStats(C.Structure):
                 ("ms_shade", C.c_double), ("ms_shadow", C.c_double), ("samples", C.c_uint64),
                 ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64),
                 ("tris_tested", C.c_uint64), ("waves", C.c_uint32), ("launches_extend", C.c_uint32),
-                ("launches_shade", C.c_uint32), ("launches_shadow", C.c_uint32), ("voxel_steps", C.c_uint64)]
+                ("launches_shade", C.c_uint32), ("launches_shadow", C.c_uint32), ("voxel_steps", C.c_uint64),
+                ("nodes4_fetched", C.c_uint64), ("tris4_fetched", C.c_uint64)]

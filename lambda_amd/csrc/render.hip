@@ -143,6 +143,13 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, ui
   if (kCount) {
     atomicAdd(&counts[0], (unsigned long long)ts.nodes);
     atomicAdd(&counts[1], (unsigned long long)ts.tris);
+    // the same ray through the 4-wide tree the render walks: node and triangle records fetched
+    TraceStats t4{0, 0};
+    float t2, u2, v2;
+    uint32_t p2, b2;
+    Trace4Closest<true>(sc, o, d, 0.f, INFINITY, &t2, &u2, &v2, &p2, &b2, stk, spill + (size_t)i * sc.spill_words, &t4);
+    atomicAdd(&counts[3], (unsigned long long)t4.nodes);
+    atomicAdd(&counts[4], (unsigned long long)t4.tris);
   }
 }
 
@@ -641,7 +648,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   hipEvent_t *ev = dev->ev;
   lgpu_render_stats S;
   std::memset(&S, 0, sizeof(S));
-  if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount.ptr, 0, 24, stream));
+  if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount.ptr, 0, 40, stream));
   for (uint64_t base = 0; base < total; base += cap) {
     const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - base);
     int cur = 0;
@@ -787,11 +794,13 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   }
   LGPU_HIP(hipStreamSynchronize(stream));
   if (countTrav) {
-    unsigned long long tc[3];
-    LGPU_HIP(hipMemcpy(tc, W.tcount.ptr, 24, hipMemcpyDeviceToHost));
+    unsigned long long tc[5];
+    LGPU_HIP(hipMemcpy(tc, W.tcount.ptr, 40, hipMemcpyDeviceToHost));
     S.nodes_visited = tc[0];
     S.tris_tested = tc[1];
     S.voxel_steps = tc[2];
+    S.nodes4_fetched = tc[3];
+    S.tris4_fetched = tc[4];
   }
   S.samples = total;
   S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

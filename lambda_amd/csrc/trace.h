@@ -137,8 +137,11 @@ __device__ inline bool Trace4(const DevScene &sc, V3 o, V3 d, float tmin, float 
 // together. Inner-node and leaf work thus run in separate wave-wide phases instead of
 // alternating under divergent masks every step. The closest hit does not depend on the visit
 // order: ties at equal t go to the smaller primitive index.
+// kCount: count the 64-B node records and 48-B triangle records this walk fetches (into ts).
+template <bool kCount = false>
 __device__ inline bool Trace4Closest(const DevScene &sc, V3 o, V3 d, float tmin, float tmax, float *tOut, float *uOut,
-                              float *vOut, uint32_t *primOut, uint32_t *biOut, uint32_t (*stk)[kBlock], uint32_t *spill) {
+                              float *vOut, uint32_t *primOut, uint32_t *biOut, uint32_t (*stk)[kBlock], uint32_t *spill,
+                              TraceStats *ts = nullptr) {
   const int lane = threadIdx.x;
   const float ix = 1.f / SafeDir(d.x), iy = 1.f / SafeDir(d.y), iz = 1.f / SafeDir(d.z);
   const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
@@ -152,6 +155,7 @@ __device__ inline bool Trace4Closest(const DevScene &sc, V3 o, V3 d, float tmin,
 #define LGPU_POP() (sp == 0 ? kNone : (--sp, sp < kLdsStack ? stk[sp][lane] : spill[sp - kLdsStack]))
   for (;;) {
     while (ref != kNone && !(ref & kLeafBit)) {
+      if (kCount) ts->nodes++;
       const float4 *np = reinterpret_cast<const float4 *>(sc.nodes4 + ref);
       const float4 a = np[0], b = np[1];
       const uint4 qc = reinterpret_cast<const uint4 *>(np)[2], ch = reinterpret_cast<const uint4 *>(np)[3];
@@ -209,6 +213,7 @@ __device__ inline bool Trace4Closest(const DevScene &sc, V3 o, V3 d, float tmin,
       if (parked != kNone) { leaf = parked; parked = kNone; }
       else { leaf = ref; ref = LGPU_POP(); }
       const uint32_t first = leaf & 0x03FFFFFFu, count = (leaf >> 26) & 31u;
+      if (kCount) ts->tris += count;
       for (uint32_t i = first; i < first + count; ++i) {
         float tt, uu, vv;
         uint32_t prim;

@@ -531,7 +531,11 @@ static inline bool RayTri(const V3 &o, const V3 &d, const V3 &v0, const V3 &v1, 
   return true;
 }
 
-static void BuildBVH(Scene &s) {  // median split on the longest centroid axis
+// Binned SAH build (16 bins on each axis, traversal cost 1 per node against 1 per triangle, leaves
+// of at most 8 triangles); the hits it returns do not depend on the tree (closest t, ties to the
+// smaller triangle index), only the time to find them. A node whose centroids coincide on every
+// axis, or a best split that leaves one side empty, falls back to the median of the longest axis.
+static void BuildBVH(Scene &s) {
   const uint32_t nt = (uint32_t)s.triMesh.size();
   s.bvh.clear();
   s.bvhTris.resize(nt);
@@ -545,9 +549,10 @@ static void BuildBVH(Scene &s) {  // median split on the longest centroid axis
     cen[i] = (a + b + c) * (1.f / 3.f);
   }
   if (nt == 0) return;
-  struct Item { uint32_t node, first, count; };
+  constexpr int kBins = 16;
+  struct Item { uint32_t node, first, count, depth; };
   s.bvh.push_back(BVHNode());
-  std::vector<Item> stack{{0, 0, nt}};
+  std::vector<Item> stack{{0, 0, nt, 0}};
   while (!stack.empty()) {
     Item it = stack.back();
     stack.pop_back();
@@ -557,24 +562,77 @@ static void BuildBVH(Scene &s) {  // median split on the longest centroid axis
       cb = Union(cb, cen[s.bvhTris[i]]);
     }
     s.bvh[it.node].b = b;
-    if (it.count <= 4) {
+    if (it.count <= 2) {
       s.bvh[it.node].first = it.first;
       s.bvh[it.node].count = it.count;
       continue;
     }
-    const V3 ext = cb.mx - cb.mn;
-    const int ax = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
-    const uint32_t mid = it.first + it.count / 2;
-    std::nth_element(s.bvhTris.begin() + it.first, s.bvhTris.begin() + mid, s.bvhTris.begin() + it.first + it.count,
-                     [&](uint32_t a, uint32_t c) { return cen[a][ax] < cen[c][ax] || (cen[a][ax] == cen[c][ax] && a < c); });
+    // SAH over the bins of each axis
+    const float leafCost = (float)it.count;
+    float bestCost = INFINITY;
+    int bestAx = -1, bestSplit = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+      const float lo = cb.mn[ax], ext = cb.mx[ax] - lo;
+      if (!(ext > 0)) continue;
+      Bounds bb[kBins];
+      uint32_t bn[kBins] = {0};
+      const float scale = (float)kBins / ext;
+      for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+        const uint32_t t = s.bvhTris[i];
+        int k = (int)((cen[t][ax] - lo) * scale);
+        k = std::min(std::max(k, 0), kBins - 1);
+        bb[k] = bn[k] ? Union(bb[k], tb[t]) : tb[t];
+        bn[k]++;
+      }
+      float rightA[kBins];
+      uint32_t rightN[kBins];
+      Bounds acc;
+      uint32_t n = 0;
+      for (int k = kBins - 1; k >= 1; --k) {
+        if (bn[k]) { acc = n ? Union(acc, bb[k]) : bb[k]; n += bn[k]; }
+        rightA[k] = n ? acc.SurfaceArea() : 0.f;
+        rightN[k] = n;
+      }
+      n = 0;
+      for (int k = 0; k < kBins - 1; ++k) {
+        if (bn[k]) { acc = n ? Union(acc, bb[k]) : bb[k]; n += bn[k]; }
+        if (!n || !rightN[k + 1]) continue;
+        const float c = 1.f + (acc.SurfaceArea() * (float)n + rightA[k + 1] * (float)rightN[k + 1]) / b.SurfaceArea();
+        if (c < bestCost) { bestCost = c; bestAx = ax; bestSplit = k; }
+      }
+    }
+    if (it.count <= 8 && !(bestCost < leafCost)) {
+      s.bvh[it.node].first = it.first;
+      s.bvh[it.node].count = it.count;
+      continue;
+    }
+    uint32_t mid;
+    if (bestAx >= 0 && it.depth < 96) {
+      const float lo = cb.mn[bestAx], scale = (float)kBins / (cb.mx[bestAx] - lo);
+      auto *first = s.bvhTris.data() + it.first;
+      auto *split = std::stable_partition(first, first + it.count, [&](uint32_t t) {
+        int k = (int)((cen[t][bestAx] - lo) * scale);
+        return std::min(std::max(k, 0), kBins - 1) <= bestSplit;
+      });
+      mid = it.first + (uint32_t)(split - first);
+    } else {
+      mid = it.first;  // no usable split: median below
+    }
+    if (mid == it.first || mid == it.first + it.count) {
+      const V3 ext = cb.mx - cb.mn;
+      const int ax = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
+      mid = it.first + it.count / 2;
+      std::nth_element(s.bvhTris.begin() + it.first, s.bvhTris.begin() + mid, s.bvhTris.begin() + it.first + it.count,
+                       [&](uint32_t a, uint32_t c) { return cen[a][ax] < cen[c][ax] || (cen[a][ax] == cen[c][ax] && a < c); });
+    }
     const uint32_t l = (uint32_t)s.bvh.size();
     s.bvh.push_back(BVHNode());
     s.bvh.push_back(BVHNode());
     s.bvh[it.node].left = l;
     s.bvh[it.node].right = l + 1;
     s.bvh[it.node].count = 0;
-    stack.push_back({l, it.first, mid - it.first});
-    stack.push_back({l + 1, mid, it.first + it.count - mid});
+    stack.push_back({l, it.first, mid - it.first, it.depth + 1});
+    stack.push_back({l + 1, mid, it.first + it.count - mid, it.depth + 1});
   }
   s.sceneBounds = s.bvh[0].b;
 }
@@ -593,19 +651,40 @@ static inline bool BoxHit(const Bounds &b, const V3 &o, const V3 &invd, float tm
   return true;
 }
 
-// Closest hit; ties in t resolve to the lower triangle index.
+// Entry distance of a ray into a box within [tmin, tmax] (same slab arithmetic as BoxHit), or
+// INFINITY when it misses.
+static inline float BoxEntry(const Bounds &b, const V3 &o, const V3 &invd, float tmin, float tmax) {
+  float t0 = tmin, t1 = tmax;
+  for (int a = 0; a < 3; ++a) {
+    float tn = (b.mn[a] - o[a]) * invd[a], tf = (b.mx[a] - o[a]) * invd[a];
+    if (tn > tf) std::swap(tn, tf);
+    if (std::isnan(tn)) tn = -INFINITY;
+    if (std::isnan(tf)) tf = INFINITY;
+    t0 = std::max(t0, tn);
+    t1 = std::min(t1, tf * 1.0000004f);
+    if (t0 > t1) return INFINITY;
+  }
+  return t0;
+}
+
+// Closest hit; ties in t resolve to the lower triangle index. Children are visited nearest
+// first (the result does not depend on the order).
 static bool ClosestHit(const Scene &s, const V3 &o, const V3 &d, float tmin, float tmax, float *t, float *u, float *v,
                        uint32_t *prim) {
   if (s.bvh.empty()) return false;
   const V3 invd(1.f / d.x, 1.f / d.y, 1.f / d.z);
   bool found = false;
   float bt = tmax;
-  uint32_t stack[128];
+  uint32_t stack[256];
+  float stackT[256];
   int sp = 0;
-  stack[sp++] = 0;
+  if (!BoxHit(s.bvh[0].b, o, invd, tmin, bt)) return false;
+  stack[sp] = 0;
+  stackT[sp++] = tmin;
   while (sp) {
-    const BVHNode &n = s.bvh[stack[--sp]];
-    if (!BoxHit(n.b, o, invd, tmin, bt)) continue;
+    --sp;
+    if (stackT[sp] > bt) continue;
+    const BVHNode &n = s.bvh[stack[sp]];
     if (n.count) {
       for (uint32_t i = n.first; i < n.first + n.count; ++i) {
         const uint32_t tri = s.bvhTris[i];
@@ -619,8 +698,13 @@ static bool ClosestHit(const Scene &s, const V3 &o, const V3 &d, float tmin, flo
         }
       }
     } else {
-      stack[sp++] = n.left;
-      stack[sp++] = n.right;
+      const float tl = BoxEntry(s.bvh[n.left].b, o, invd, tmin, bt);
+      const float tr = BoxEntry(s.bvh[n.right].b, o, invd, tmin, bt);
+      const bool leftFirst = tl <= tr;
+      const uint32_t nearC = leftFirst ? n.left : n.right, farC = leftFirst ? n.right : n.left;
+      const float nearT = leftFirst ? tl : tr, farT = leftFirst ? tr : tl;
+      if (farT < INFINITY) { stack[sp] = farC; stackT[sp++] = farT; }
+      if (nearT < INFINITY) { stack[sp] = nearC; stackT[sp++] = nearT; }
     }
   }
   return found;
@@ -628,7 +712,7 @@ static bool ClosestHit(const Scene &s, const V3 &o, const V3 &d, float tmin, flo
 static bool AnyHit(const Scene &s, const V3 &o, const V3 &d, float tmin, float tmax) {
   if (s.bvh.empty()) return false;
   const V3 invd(1.f / d.x, 1.f / d.y, 1.f / d.z);
-  uint32_t stack[128];
+  uint32_t stack[256];
   int sp = 0;
   stack[sp++] = 0;
   while (sp) {

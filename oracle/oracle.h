@@ -7,7 +7,7 @@
  * (include/lambda_gpu.h) so that both are driven from one scene builder, and
  * re-implements the reference's algorithm independently: per-path recursive
  * PathIntegrator::Li, pull-evaluated shader graphs, pointer-based light tree,
- * median-split BVH.
+ * binned-SAH BVH.
  */
 #ifndef LAMBDA_ORACLE_H
 #define LAMBDA_ORACLE_H
