@@ -331,6 +331,13 @@ lgpu_status lgpu_film_write(lgpu_film film, const double *rgb_sum, const uint32_
 /* ---------------- render (MosaicRenderer::Render / TileRenderer /
  * lambdaStartProgressiveRenderer: Render.cpp:43-86, MosaicRenderer.cpp:22-80) */
 lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
+/* lgpu_render without any allocation (no counterpart in the reference): for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The path pool lives in a fixed 64 MiB device array per GPU, held until the process ends, so a
+ * frame renders in more, smaller waves; desc->path_capacity (0 = as many paths as the array holds
+ * for this scene) beyond that is rejected with LGPU_ERR_INVALID_ARGUMENT. Every call on one GPU
+ * shares its array: calls that overlap on the same GPU (from different device objects) race. */
+lgpu_status lgpu_render_noalloc(const lgpu_render_desc *desc, lgpu_render_stats *stats);
 
 /* ---------------- post-processing (ToneMap.cpp:5-73, PostProcessing.cpp:16-45,
  * Texture::SaveToImageFile Texture.cpp:19-81) ------------------------------------- */

@@ -46,6 +46,13 @@ lgpu_status lgpu_device_create(int hip_device, lgpu_device *out) {
       return Fail(LGPU_ERR_HIP, "device_create: event creation failed");
     }
   }
+  if (ResolveScratch(d) != LGPU_OK) {
+    const std::string msg = lgpu_last_error();
+    for (auto &e : d->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return Fail(LGPU_ERR_HIP, msg);
+  }
   *out = d;
   return LGPU_OK;
 }

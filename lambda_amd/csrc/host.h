@@ -83,6 +83,7 @@ struct lgpu_device_s {
   hipEvent_t ev[8] = {};
   void *wave = nullptr;               // path-state pool of the render loop
   void (*wave_free)(void *) = nullptr;
+  void *scratch = nullptr;            // device address of this GPU's no-allocation scratch array
 };
 
 struct lgpu_film_s {
@@ -117,4 +118,5 @@ struct lgpu_scene_s {
 
 namespace lgpu {
 lgpu_status CommitScene(lgpu_scene s, uint32_t strategy);
+lgpu_status ResolveScratch(lgpu_device dev);  // render.hip: the no-allocation scratch array's address
 }

@@ -521,6 +521,12 @@ struct Wave {
   uint32_t cap = 0;
   PathState ps[2]{};
   SampleState ss{};
+  // the loop's views of the buffers: into the DevBufs above (alloc) or into a caller-provided
+  // region (bind, the no-allocation entry point)
+  ShadowRay *shadow_p = nullptr;
+  uint32_t *counters_p = nullptr, *spill_p = nullptr;
+  unsigned long long *tcount_p = nullptr;
+  DevScene *scene_p = nullptr;
   // pipelined path loop: device counter slots, their pinned host copies, per-slot stage events
   static constexpr int kSlots = 4;
   uint32_t *hring = nullptr;
@@ -561,8 +567,37 @@ struct Wave {
     LGPU_HIP(shadow.alloc(sizeof(ShadowRay) * (size_t)n));
     LGPU_HIP(counters.alloc(64)); LGPU_HIP(tcount.alloc(64)); LGPU_HIP(scene.alloc(sizeof(DevScene)));
     ss.L = L.as<float4>(); ss.pixel = pix.as<uint32_t>(); ss.lambda0 = lam.as<float>();
+    shadow_p = shadow.as<ShadowRay>(); counters_p = counters.as<uint32_t>(); spill_p = spill.as<uint32_t>();
+    tcount_p = tcount.as<unsigned long long>(); scene_p = scene.as<DevScene>();
     cap = n;
     return LGPU_OK;
+  }
+  // Bytes a wave of n paths needs in one region (bind): 256-B aligned parts.
+  static uint64_t RegionBytes(uint64_t n, uint32_t spillWords) {
+    auto up = [](uint64_t b) { return (b + 255) & ~255ull; };
+    return 18 * up(16 * n) + up(16 * n) + 2 * up(4 * n) + up(sizeof(ShadowRay) * n) + up(4ull * spillWords * n) +
+           3 * 256 + up(sizeof(DevScene));
+  }
+  // Lays a wave of n paths out in [base, base + RegionBytes(n)) without allocating; base is a
+  // device address (256-B aligned).
+  void bind(uint8_t *base, uint32_t n, uint32_t spillWords) {
+    auto up = [](uint64_t b) { return (b + 255) & ~255ull; };
+    uint8_t *p = base;
+    auto take = [&](uint64_t b) { uint8_t *r = p; p += up(b); return r; };
+    for (int b = 0; b < 2; ++b) {
+      PathState &q = ps[b];
+      q.ray_o = (float4 *)take(16ull * n); q.ray_d = (float4 *)take(16ull * n); q.beta = (float4 *)take(16ull * n);
+      q.hit = (float4 *)take(16ull * n); q.ids = (uint4 *)take(16ull * n); q.aux = (uint4 *)take(16ull * n);
+      q.mis = (float4 *)take(16ull * n); q.prev_p = (float4 *)take(16ull * n); q.prev_n = (float4 *)take(16ull * n);
+    }
+    ss.L = (float4 *)take(16ull * n); ss.pixel = (uint32_t *)take(4ull * n); ss.lambda0 = (float *)take(4ull * n);
+    shadow_p = (ShadowRay *)take(sizeof(ShadowRay) * (uint64_t)n);
+    spill_p = (uint32_t *)take(4ull * spillWords * n);
+    counters_p = (uint32_t *)take(256);
+    tcount_p = (unsigned long long *)take(256);
+    (void)take(256);  // spare
+    scene_p = (DevScene *)take(sizeof(DevScene));
+    cap = n;
   }
 };
 
@@ -605,50 +640,59 @@ static RenderParams MakeParams(const lgpu_render_desc *d, uint32_t w, uint32_t h
   return rp;
 }
 
-// Renders `total` paths; list mode when listPix != nullptr (trace_paths).
+// Renders `total` paths; list mode when listPix != nullptr (trace_paths). `bound`: a wave already
+// laid out in caller memory (no-allocation entry point): no device or pinned allocation happens and
+// the bounce loop synchronises per bounce instead of running ahead through pinned counter copies.
 static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, uint64_t total, const uint32_t *listPix,
-                            const uint32_t *listSmp, double *film, uint32_t *counts, float *out, lgpu_render_stats *stats) {
+                            const uint32_t *listSmp, double *film, uint32_t *counts, float *out, lgpu_render_stats *stats,
+                            Wave *bound = nullptr) {
   lgpu_scene s = d->scene;
   lgpu_device dev = s->dev;
   LGPU_HIP(hipSetDevice(dev->device));
   (void)hipGetLastError();  // clear a sticky error left by an earlier call
   const auto t0 = std::chrono::steady_clock::now();
-  if (!dev->wave) {  // path pool kept on the device object and reused across calls
-    dev->wave = new Wave();
-    dev->wave_free = [](void *p) { delete static_cast<Wave *>(p); };
-  }
-  Wave &W = *static_cast<Wave *>(dev->wave);
-  // Paths in flight: the caller's figure, else up to kDefaultPaths, held to a quarter of the device
-  // memory that is free or already in the pool. Fewer, larger waves spend less time in the
-  // launch-bound tails of their bounce loops.
-  uint64_t cap = d->path_capacity;
-  if (!cap) {
-    constexpr uint64_t kDefaultPaths = 1ull << 27;
-    cap = kDefaultPaths;
-    size_t freeB = 0, totalB = 0;
-    if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
-      const uint64_t fit = (freeB + W.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words);
-      cap = std::min<uint64_t>(cap, std::max<uint64_t>(fit, 1ull << 16));
+  uint64_t cap;
+  if (bound) {
+    cap = bound->cap;
+  } else {
+    if (!dev->wave) {  // path pool kept on the device object and reused across calls
+      dev->wave = new Wave();
+      dev->wave_free = [](void *p) { delete static_cast<Wave *>(p); };
     }
+    Wave &W0 = *static_cast<Wave *>(dev->wave);
+    // Paths in flight: the caller's figure, else up to kDefaultPaths, held to a quarter of the device
+    // memory that is free or already in the pool. Fewer, larger waves spend less time in the
+    // launch-bound tails of their bounce loops.
+    cap = d->path_capacity;
+    if (!cap) {
+      constexpr uint64_t kDefaultPaths = 1ull << 27;
+      cap = kDefaultPaths;
+      size_t freeB = 0, totalB = 0;
+      if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
+        const uint64_t fit = (freeB + W0.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words);
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(fit, 1ull << 16));
+      }
+    }
+    if (cap > total) cap = total;
+    if (cap == 0) cap = 1;
+    lgpu_status st = W0.alloc((uint32_t)cap, s->dsc.spill_words);
+    if (st != LGPU_OK) return st;
   }
-  if (cap > total) cap = total;
-  if (cap == 0) cap = 1;
-  lgpu_status st = W.alloc((uint32_t)cap, s->dsc.spill_words);
-  if (st != LGPU_OK) return st;
+  Wave &W = bound ? *bound : *static_cast<Wave *>(dev->wave);
   const RenderParams rp = MakeParams(d, w, h);
   const bool countTrav = (d->flags & LGPU_RENDER_COUNT_TRAVERSAL) != 0;
   DevScene dsc = s->dsc;
-  if (countTrav) dsc.voxel_count = W.tcount.as<unsigned long long>() + 2;
+  if (countTrav) dsc.voxel_count = W.tcount_p + 2;
   hipStream_t stream = dev->stream;
   // The shading kernels read the scene record through a device pointer: a by-value kernel argument
   // whose address reaches out-of-line functions is copied into every lane's scratch.
-  LGPU_HIP(hipMemcpyAsync(W.scene.ptr, &dsc, sizeof(DevScene), hipMemcpyHostToDevice, stream));
-  const DevScene *dscp = W.scene.as<DevScene>();
+  LGPU_HIP(hipMemcpyAsync(W.scene_p, &dsc, sizeof(DevScene), hipMemcpyHostToDevice, stream));
+  const DevScene *dscp = W.scene_p;
   const bool progs = dsc.has_programs != 0;
   hipEvent_t *ev = dev->ev;
   lgpu_render_stats S;
   std::memset(&S, 0, sizeof(S));
-  if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount.ptr, 0, 40, stream));
+  if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount_p, 0, 40, stream));
   for (uint64_t base = 0; base < total; base += cap) {
     const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - base);
     int cur = 0;
@@ -662,7 +706,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
     LGPU_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     S.ms_camera += ms;
     const uint32_t maxIter = 4 * (d->max_bounces + 2) + 1100;
-    if (rp.integrator == LGPU_INTEGRATOR_PATH) {
+    if (rp.integrator == LGPU_INTEGRATOR_PATH && !bound) {
       // Pipelined bounces: each stage reads its item count from the device counter the previous
       // stage wrote, the grids cover the last count the host has seen (counts never grow), and the
       // host stays up to kLag bounces ahead, retiring them through pinned copies and events. No
@@ -682,18 +726,18 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
           LGPU_HIP(hipEventRecord(e[0], stream));
           if (countTrav)
             hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
-                               W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+                               W.spill_p, W.tcount_p);
           else
             hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
-                               W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+                               W.spill_p, W.tcount_p);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[1], stream));
-          Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), cout};
+          Queues q{W.ps[cur ^ 1], W.shadow_p, cout};
           LaunchShade(rp.corrected != 0, progs, Blocks(ub), stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[2], stream));
-          hipLaunchKernelGGL(k_shadow, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(), ub,
-                             cout + 1, W.spill.as<uint32_t>());
+          hipLaunchKernelGGL(k_shadow, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow_p, ub,
+                             cout + 1, W.spill_p);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[3], stream));
           LGPU_HIP(hipMemcpyAsync(W.hring + 4 * sl, cout, 8, hipMemcpyDeviceToHost, stream));
@@ -733,29 +777,29 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       uint32_t iter = 0;
       while (count > 0 && iter < maxIter) {
         ++iter;
-        LGPU_HIP(hipMemsetAsync(W.counters.ptr, 0, 8, stream));
+        LGPU_HIP(hipMemsetAsync(W.counters_p, 0, 8, stream));
         LGPU_HIP(hipEventRecord(ev[2], stream));
         if (countTrav)
           hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count, nullptr,
-                             W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+                             W.spill_p, W.tcount_p);
         else
           hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count, nullptr,
-                             W.spill.as<uint32_t>(), W.tcount.as<unsigned long long>());
+                             W.spill_p, W.tcount_p);
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[3], stream));
-        Queues q{W.ps[cur ^ 1], W.shadow.as<ShadowRay>(), W.counters.as<uint32_t>()};
+        Queues q{W.ps[cur ^ 1], W.shadow_p, W.counters_p};
         if (rp.integrator == LGPU_INTEGRATOR_PATH)
           LaunchShade(rp.corrected != 0, progs, Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
         else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
-          LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+          LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill_p);
         else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
           LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
         else
-          LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill.as<uint32_t>());
+          LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill_p);
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[4], stream));
         uint32_t cnt[2];
-        LGPU_HIP(hipMemcpyAsync(cnt, W.counters.ptr, 8, hipMemcpyDeviceToHost, stream));
+        LGPU_HIP(hipMemcpyAsync(cnt, W.counters_p, 8, hipMemcpyDeviceToHost, stream));
         LGPU_HIP(hipStreamSynchronize(stream));
         S.extension_rays += count;
         S.launches_extend++;
@@ -767,8 +811,8 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         if (cnt[0] > count || cnt[1] > count) return Fail(LGPU_ERR_STATE, "render: queue overflow");
         if (cnt[1]) {
           LGPU_HIP(hipEventRecord(ev[5], stream));
-          hipLaunchKernelGGL(k_shadow, dim3(Blocks(cnt[1])), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow.as<ShadowRay>(),
-                             cnt[1], nullptr, W.spill.as<uint32_t>());
+          hipLaunchKernelGGL(k_shadow, dim3(Blocks(cnt[1])), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow_p,
+                             cnt[1], nullptr, W.spill_p);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(ev[6], stream));
           LGPU_HIP(hipEventSynchronize(ev[6]));
@@ -795,7 +839,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   LGPU_HIP(hipStreamSynchronize(stream));
   if (countTrav) {
     unsigned long long tc[5];
-    LGPU_HIP(hipMemcpy(tc, W.tcount.ptr, 40, hipMemcpyDeviceToHost));
+    LGPU_HIP(hipMemcpy(tc, W.tcount_p, 40, hipMemcpyDeviceToHost));
     S.nodes_visited = tc[0];
     S.tris_tested = tc[1];
     S.voxel_steps = tc[2];
@@ -822,6 +866,58 @@ extern "C" lgpu_status lgpu_render(const lgpu_render_desc *d, lgpu_render_stats 
   lgpu_film f = d->film;
   const uint64_t total = (uint64_t)f->width * f->height * (d->spp_end - d->spp_begin);
   return RunWaves(d, f->width, f->height, total, nullptr, nullptr, f->sum.as<double>(), f->count.as<uint32_t>(), nullptr, stats);
+}
+
+namespace lgpu {
+// No-allocation render: the path pool lives in this fixed device array, one copy per GPU, held
+// until the process ends. Its host-side name is a stand-in, not a device address: host code uses
+// only the address hipGetSymbolAddress gives for the current device (kept on the device object),
+// for kernel arguments and memsets alike.
+constexpr uint64_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+// Called once per device object by lgpu_device_create, with that device current.
+lgpu_status ResolveScratch(lgpu_device dev) {
+  void *p = nullptr;
+  LGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p || (reinterpret_cast<uintptr_t>(p) & 255u)) return Fail(LGPU_ERR_HIP, "device_create: scratch array unaligned");
+  dev->scratch = p;
+  return LGPU_OK;
+}
+
+// Most paths a wave may hold in the scratch array for a scene with `spillWords` spill words.
+static uint64_t ScratchPaths(uint32_t spillWords) {
+  uint64_t n = kScratchBytes / Wave::BytesPerPath(spillWords);
+  n &= ~255ull;
+  while (n && Wave::RegionBytes(n, spillWords) > kScratchBytes) n -= 256;
+  return n;
+}
+}  // namespace lgpu
+
+extern "C" lgpu_status lgpu_render_noalloc(const lgpu_render_desc *d, lgpu_render_stats *stats) {
+  lgpu_status st = ValidateRender(d);
+  if (st != LGPU_OK) return st;
+  if (!d->film) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: null film");
+  if (d->spp_end <= d->spp_begin) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: empty spp range");
+  if (d->film->dev != d->scene->dev) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: film and scene on different devices");
+  lgpu_device dev = d->scene->dev;
+  std::lock_guard<std::mutex> lk(dev->mu);
+  if (!dev->scratch) return Fail(LGPU_ERR_STATE, "render_noalloc: device has no scratch array");
+  const uint64_t maxPaths = ScratchPaths(d->scene->dsc.spill_words);
+  if (maxPaths == 0)
+    return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: this scene's per-path state does not fit the 64 MiB scratch array");
+  if (d->path_capacity > maxPaths)
+    return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: path_capacity " + std::to_string(d->path_capacity) +
+                                               " exceeds the " + std::to_string(maxPaths) +
+                                               " paths the 64 MiB scratch array holds for this scene");
+  lgpu_film f = d->film;
+  const uint64_t total = (uint64_t)f->width * f->height * (d->spp_end - d->spp_begin);
+  uint64_t n = d->path_capacity ? d->path_capacity : maxPaths;
+  if (n > total) n = total;
+  Wave W;
+  W.bind(static_cast<uint8_t *>(dev->scratch), (uint32_t)n, d->scene->dsc.spill_words);
+  return RunWaves(d, f->width, f->height, total, nullptr, nullptr, f->sum.as<double>(), f->count.as<uint32_t>(), nullptr,
+                  stats, &W);
 }
 
 extern "C" lgpu_status lgpu_trace_paths(const lgpu_render_desc *d, uint32_t num, const uint32_t *pixels,

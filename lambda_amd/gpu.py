@@ -28,6 +28,7 @@ EXPORTS = [
     "lgpu_progressive_set_callback", "lgpu_progressive_start", "lgpu_progressive_stop", "lgpu_progressive_clear",
     "lgpu_progressive_read", "lgpu_progressive_running", "lgpu_obj_load", "lgpu_obj_counts", "lgpu_obj_mesh",
     "lgpu_obj_material_info", "lgpu_obj_destroy", "lgpu_film_post", "lgpu_write_png", "lgpu_film_write", "lgpu_film_denoise",
+    "lgpu_render_noalloc",
 ]
 
 
@@ -52,6 +53,7 @@ def lib():
             "lgpu_scene_info": [vp, vp, vp, vp, vp], "lgpu_scene_light_tree": [vp, vp, u32],
             "lgpu_film_create": [vp, u32, u32, vp], "lgpu_film_destroy": [vp], "lgpu_film_clear": [vp],
             "lgpu_film_read": [vp, vp, vp], "lgpu_film_read_rgba": [vp, vp], "lgpu_render": [vp, vp],
+            "lgpu_render_noalloc": [vp, vp],
             "lgpu_trace_paths": [vp, u32, vp, vp, vp], "lgpu_intersect": [vp, u32, vp, vp],
             "lgpu_occluded": [vp, u32, vp, vp], "lgpu_philox": [vp, u32, vp, vp, vp],
             "lgpu_copy_bench": [vp, C.c_uint64, u32, vp],
