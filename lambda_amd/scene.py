@@ -479,6 +479,68 @@ def procedural_cornell():
     return sd
 
 
+def lobe_zoo(level=2):
+    """Cornell room holding one object per reference lobe that the other workloads leave out:
+    FresnelBSDF and SpecularBTDF spheres, a SpecularBRDF box, a GhostBTDF card, an Oren-Nayar BTDF
+    sheet (mixed with Lambertian), Beckmann and Trowbridge-Reitz microfacet BRDFs with the
+    dielectric Fresnel (one through RoughnessToAlpha), a Beckmann conductor and a rough dielectric
+    (Specular.cpp:18-95, Ghost.cpp:13-28, OrenNayar.cpp:52-59, Microfacet.cpp:12-69,
+    MicrofacetDistribution.cpp:11-71)."""
+    sd = cornell_box()
+    N = abi
+    white = (1.0, 1.0, 1.0)
+
+    def mat(nodes, root):
+        return sd.add_material(Material(nodes, bxdf=root))
+
+    fresnel = mat([node(N.NODE_RGB, p=white), node(N.NODE_SCALAR, p=[1.5]), node(N.NODE_FRESNEL_BSDF, [0, 1])], 2)
+    spec_t = mat([node(N.NODE_RGB, p=(0.9, 0.95, 1.0)), node(N.NODE_SPECULAR_BTDF, [0], p=[1.33])], 1)
+    mirror = mat([node(N.NODE_RGB, p=(0.8, 0.75, 0.7)), node(N.NODE_SPECULAR_BRDF, [0])], 1)
+    ghost = mat([node(N.NODE_SCALAR, p=[0.6]), node(N.NODE_GHOST_BTDF, [0])], 1)
+    on_t = mat([node(N.NODE_RGB, p=(0.7, 0.5, 0.3)), node(N.NODE_SCALAR, p=[0.4]),
+                node(N.NODE_OREN_NAYAR_BTDF, [0, 1]), node(N.NODE_LAMBERTIAN, [0]),
+                node(N.NODE_SCALAR, p=[0.35]), node(N.NODE_MIX_BSDF, [2, 3, 4])], 5)
+
+    def micro(ggx, alpha, conductor, direct_alpha=True):
+        p = [1.0 if ggx else 0.0, 1.0 if conductor else 0.0, 1.45, *COPPER_ETA, *COPPER_K, 1.0 if direct_alpha else 0.0]
+        return mat([node(N.NODE_RGB, p=(0.9, 0.8, 0.7)), node(N.NODE_SCALAR, p=[alpha]),
+                     node(N.NODE_MICROFACET_BRDF, [0, 1], p=p)], 2)
+
+    beck_d = micro(False, 0.25, False)
+    ggx_d = micro(True, 0.4, False, direct_alpha=False)  # roughness 0.4 through RoughnessToAlpha
+    beck_c = micro(False, 0.15, True)
+    rough = sd.add_material(rough_dielectric(0.2, 1.5))
+    v, f = icosphere(level)
+    for c, r, m in [((-0.55, -0.72, 0.35), 0.27, fresnel), ((0.0, -0.75, 0.55), 0.24, spec_t),
+                    ((0.55, -0.72, 0.3), 0.27, beck_d), ((-0.5, 0.05, -0.45), 0.25, ggx_d),
+                    ((0.5, 0.0, -0.5), 0.28, beck_c), ((0.05, -0.35, -0.25), 0.3, rough)]:
+        p = v * r + np.array(c)
+        n = _vertex_normals(p, f)
+        sd.add_mesh(p, f, m, normals=n, tangents=_tangents(n))
+    bv, bi = _box((0.1, 0.55, -0.7), (0.5, 0.3, 0.2), 25.0)
+    sd.add_mesh(bv, bi, mirror)
+    gv, gi = _quad((-0.35, -0.95, 0.75), (0.35, -0.95, 0.75), (0.35, -0.3, 0.75), (-0.35, -0.3, 0.75))
+    sd.add_mesh(gv, gi, ghost)
+    tv, ti = _quad((0.7, -0.2, -0.9), (0.7, -0.2, 0.1), (0.7, 0.5, 0.1), (0.7, 0.5, -0.9))
+    sd.add_mesh(tv, ti, on_t)
+    return sd
+
+
+def camera_spherical(origin, width, height, phi=math.pi, theta=0.0, offset_phi=0.0, offset_theta=0.0):
+    """SphericalCamera (Camera.cpp:66-71): u -> phi = 2 pi u + offset, v -> theta = pi v + offset."""
+    cam = _camera(abi.CAMERA_SPHERICAL, origin, width, height, phi, theta, 1.0)
+    cam.offset_phi, cam.offset_theta = offset_phi, offset_theta
+    return cam
+
+
+def camera_blades(origin, width, height, phi, theta, fov, focal_length, aperture_size, blades):
+    """Thin lens with a BladeAperture of `blades` blades (Aperture.cpp:10-19)."""
+    cam = camera_thin_lens(origin, width, height, phi, theta, fov, focal_length, aperture_size)
+    cam.aperture = abi.APERTURE_BLADE
+    cam.blades = blades
+    return cam
+
+
 def cornell_camera(width=256, height=256):
     """Pinhole at (0,0,3.8), SetRotation(pi, 0) (looking down -z), SetFov(0.6911)."""
     return camera_pinhole((0.0, 0.0, 3.8), width, height, math.pi, 0.0, 0.6911)

@@ -8,6 +8,8 @@ Bars:
     transcendental ulp between the device math library and glibc);
   - converged images: per-channel mean within 1e-3 relative.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -452,3 +454,38 @@ def test_vm_free_kernels_match_vm_kernels(dev, integrator):
         out.append(gpu.trace_paths(d, pixels, samples))
     assert np.isfinite(out[0]).all()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+LOBE_CAMERAS = ["pinhole", "spherical", "blades"]
+
+
+def _zoo_camera(kind, W, H):
+    if kind == "spherical":
+        return scene.camera_spherical((0.0, -0.1, 0.2), W, H, offset_phi=0.3, offset_theta=0.05)
+    if kind == "blades":
+        return scene.camera_blades((0.0, 0.0, 3.8), W, H, math.pi, 0.0, 0.6911, 3.9, 0.12, 6)
+    return scene.cornell_camera(W, H)
+
+
+@pytest.mark.parametrize("spectral,estimator", [(sp, e) for sp in (abi.SPECTRAL_RGB, abi.SPECTRAL_HERO4)
+                                                for e in (abi.ESTIMATOR_FAITHFUL, abi.ESTIMATOR_CORRECTED)])
+@pytest.mark.parametrize("cam_kind", LOBE_CAMERAS)
+def test_lobe_zoo_per_path_parity(dev, cam_kind, spectral, estimator):
+    """Every remaining reference lobe (FresnelBSDF, SpecularBRDF/BTDF, GhostBTDF, Oren-Nayar BTDF,
+    Beckmann and dielectric-Fresnel microfacets, rough dielectric) seen through the pinhole, the
+    SphericalCamera and a six-blade BladeAperture thin lens."""
+    W, H = 64, 48
+    sd = scene.lobe_zoo()
+    s, k1 = gpu_scene(dev, sd)
+    o, k2 = helpers.oracle_scene(sd)
+    cam = _zoo_camera(cam_kind, W, H)
+    film = gpu.Film(dev, W, H)
+    pixels = np.tile(np.arange(W * H, dtype=np.uint32), 2)
+    samples = np.repeat(np.arange(2, dtype=np.uint32), W * H)
+    d = gpu.render_desc(s, film, cam, 0, 2, max_bounces=12, min_bounces=3, spectral=spectral, estimator=estimator)
+    g = gpu.trace_paths(d, pixels, samples)
+    c = o.trace_paths(helpers.desc(cam, 0, 2, 12, 3, spectral, estimator), W, H, pixels, samples)
+    ok = _compare_paths(g, c)
+    print("lobe zoo", cam_kind, spectral, estimator, "matching", ok.mean(), "nonzero", (np.abs(c[:, 0]) > 0).mean())
+    assert np.isfinite(g[:, :4]).all()
+    assert ok.mean() >= 0.999
