@@ -119,6 +119,9 @@ def main():
     ap.add_argument("--save-image", default="")
     ap.add_argument("--relmse", action="store_true", help="measure time-to-relMSE <= 1e-3 instead")
     ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--guided", action="store_true",
+                    help="with --relmse: also measure the guided estimator (online-trained guiding network)")
+    ap.add_argument("--guide-steps", type=int, default=16, help="Adam steps after each guided wave")
     args = ap.parse_args()
     if not args.steps:
         args.steps = DEFAULT_STEPS[args.workload]
@@ -154,10 +157,11 @@ def main():
     W, H = cfg["width"], cfg["height"]
     film = gpu.Film(dev, W, H)
 
-    def desc(b, e, flags=0, f=None, seed=0):
+    def desc(b, e, flags=0, f=None, seed=0, guide=None, steps=0, capacity=None):
         return gpu.render_desc(s, f or film, cam, b, e, max_bounces=cfg["max_bounces"], min_bounces=3,
-                               spectral=spectral, estimator=estimator, capacity=args.capacity, flags=flags,
-                               integrator=integrator_id(cfg), seed=seed)
+                               spectral=spectral, estimator=estimator,
+                               capacity=args.capacity if capacity is None else capacity, flags=flags,
+                               integrator=integrator_id(cfg), seed=seed, guide=guide, guide_train_steps=steps)
 
     if args.relmse:
         relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev)
@@ -262,25 +266,45 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     ref_s = time.perf_counter() - t0
     rsum, rcnt = ref.read()
     r = rsum / rcnt[..., None]
-    film.clear()
-    checkpoints = []
-    spent, done, k = 0.0, 0, 0
-    hit = None
-    while (1 << k) <= args.ref_spp // 2:
-        target = 1 << k
-        t = time.perf_counter()
-        gpu.render(desc(done, target))
-        spent += time.perf_counter() - t
-        done = target
-        xs, xc = film.read()
-        x = xs / xc[..., None]
-        rel = float(np.mean((x - r) ** 2 / (r * r + 0.01)))
-        checkpoints.append({"spp": target, "seconds": round(spent, 4), "relmse": rel})
-        log("spp %d: %.3f s, relMSE %.3e" % (target, spent, rel))
-        if rel <= 1e-3:
-            hit = checkpoints[-1]
-            break
-        k += 1
+    def progressive(guide=None):
+        """2^k-spp checkpoints of one progressive render (cumulative wall time of the render calls,
+        training included for the guided estimator); stops at relMSE <= 1e-3."""
+        film.clear()
+        checkpoints, hit = [], None
+        spent, done, k = 0.0, 0, 0
+        while (1 << k) <= args.ref_spp // 2:
+            target = 1 << k
+            t = time.perf_counter()
+            if guide is None:
+                gpu.render(desc(done, target))
+            else:
+                # waves of at most 8 spp so the guide trains between them (online)
+                per = min(target - done, 8)
+                cap = W * H * per
+                for b in range(done, target, per):
+                    gpu.render(desc(b, min(target, b + per), guide=guide, steps=args.guide_steps, capacity=cap))
+            spent += time.perf_counter() - t
+            done = target
+            xs, xc = film.read()
+            x = xs / xc[..., None]
+            rel = float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+            checkpoints.append({"spp": target, "seconds": round(spent, 4), "relmse": rel})
+            log("%s spp %d: %.3f s, relMSE %.3e" % ("guided" if guide else "plain", target, spent, rel))
+            if rel <= 1e-3:
+                hit = checkpoints[-1]
+                break
+            k += 1
+        return checkpoints, hit
+
+    checkpoints, hit = progressive()
+    guided = None
+    if args.guided:
+        g = gpu.Guide(dev, seed=7)
+        gck, ghit = progressive(g)
+        guided = {"value": ghit["seconds"] if ghit else None, "spp_at_target": ghit["spp"] if ghit else None,
+                  "checkpoints": gck, "guide": g.stats(), "guide_train_steps_per_wave": args.guide_steps,
+                  "spp_per_wave": 8}
+        g.close()
     # CPU validation of the reference on a 64x64 centre crop
     valid = None
     if not args.no_cpu:
@@ -310,7 +334,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             "spp_at_target": hit["spp"] if hit else None, "dtype": "f32", "data": "synthetic (procedural scene)",
             "config": dict(cfg, spectral=args.spectral, estimator=args.estimator, reference_spp=args.ref_spp,
                            reference_seed=1, reference_s=round(ref_s, 1)),
-            "checkpoints": checkpoints, "reference_check": valid}
+            "checkpoints": checkpoints, "reference_check": valid, "guided": guided}
     print(json.dumps(line), flush=True)
     ref.close()
 

@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 5u
+#define LGPU_ABI_VERSION 6u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -230,6 +230,8 @@ typedef enum lgpu_estimator_mode {
   LGPU_ESTIMATOR_CORRECTED = 1  /* defects fixed (unbiased estimator) */
 } lgpu_estimator_mode;
 
+typedef struct lgpu_guide_s *lgpu_guide;
+
 typedef struct lgpu_render_desc {
   lgpu_scene scene;
   lgpu_film film;
@@ -251,6 +253,12 @@ typedef struct lgpu_render_desc {
   float pass_power;         /* DepthPass::power */
   float pass_background[3]; /* DepthPass::background, returned on a miss */
   uint32_t pass_geometric_normal; /* NormalPass::normalG */
+  /* learned path guiding (no reference counterpart: an extension of PathIntegrator::Li's BSDF
+   * sampling step, PathIntegrator.cpp:46-67). NULL = plain BSDF sampling. With a guide, guidable
+   * vertices (no delta lobe) sample one-sample MIS of the guide's distribution and the BSDF;
+   * path integrator and corrected estimator only. */
+  lgpu_guide guide;
+  uint32_t guide_train_steps; /* Adam steps on each wave's records once the wave is done (0: none) */
 } lgpu_render_desc;
 
 #define LGPU_RENDER_COUNT_TRAVERSAL 1u
@@ -268,6 +276,9 @@ typedef struct lgpu_render_stats {
   uint64_t voxel_steps;     /* with LGPU_RENDER_COUNT_TRAVERSAL: density-grid voxels visited by medium walks */
   uint64_t nodes4_fetched;  /* with LGPU_RENDER_COUNT_TRAVERSAL: 64-B 4-wide node records the render's walk fetches */
   uint64_t tris4_fetched;   /* with LGPU_RENDER_COUNT_TRAVERSAL: 48-B triangle records the render's walk fetches */
+  double ms_guide_infer;    /* guided renders: summed inference kernel time */
+  double ms_guide_train;    /* guided renders: host wall time of record finalisation and training */
+  uint64_t guide_records;   /* guided renders: training records written */
 } lgpu_render_stats;
 
 /* ---------------- device (Lambda.h:37-40 lambdaCreateDevice/Release) ---- */
@@ -338,6 +349,39 @@ lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
  * for this scene) beyond that is rejected with LGPU_ERR_INVALID_ARGUMENT. Every call on one GPU
  * shares its array: calls that overlap on the same GPU (from different device objects) race. */
 lgpu_status lgpu_render_noalloc(const lgpu_render_desc *desc, lgpu_render_stats *stats);
+
+/* ---------------- learned path guiding (no reference counterpart; the north-star extension of
+ * the BSDF sampling step, PathIntegrator.cpp:46-67). A 6-layer, width-64 MLP (BF16 MFMA, FP32
+ * accumulation) maps a shading point and its outgoing direction (5 inputs, 8-octave frequency
+ * encoding) to a distribution over 128 world-space direction bins and the probability c of
+ * sampling it instead of the BSDF; trained online with Adam on the renderer's own radiance
+ * records (the objective -w log(c q + (1 - c) p_bsdf), w the radiance along the sampled
+ * direction over its pdf times f cos). */
+typedef struct lgpu_guide_desc {
+  uint32_t seed;            /* weight initialisation and batch selection */
+  uint32_t record_capacity; /* training records kept per wave (0: 4194304) */
+  uint32_t batch;           /* records per Adam step (0: 16384) */
+  float learning_rate;      /* Adam step size (0: 0.002) */
+} lgpu_guide_desc;
+lgpu_status lgpu_guide_create(lgpu_device dev, const lgpu_guide_desc *desc, lgpu_guide *out);
+lgpu_status lgpu_guide_destroy(lgpu_guide guide);
+/* Number of FP32 parameters: W1 (64x80), b1, W2..W5 (64x64), b2..b5, W6 (144x64: 128 bin
+ * logits, the mixing logit, 15 unused rows), b6; weights of every layer first (row-major, layer
+ * order), then the biases. */
+uint32_t lgpu_guide_num_params(void);
+lgpu_status lgpu_guide_get_params(lgpu_guide guide, float *params);
+/* Replaces the parameters (and resets Adam's moments). */
+lgpu_status lgpu_guide_set_params(lgpu_guide guide, const float *params);
+/* Network output for n queries of 5 inputs in [0, 1] (position in the scene box, theta/pi, phi/2pi
+ * of wo): 129 floats each, the 128 bin probabilities of the quantised table the renderer samples
+ * and the mixing probability c. */
+lgpu_status lgpu_guide_forward(lgpu_guide guide, uint32_t n, const float *inputs, float *out);
+/* Gradient of the guiding objective over n records of 8 floats (5 inputs, bin, BSDF pdf, weight)
+ * into grad (lgpu_guide_num_params floats) and the loss; parameters unchanged. */
+lgpu_status lgpu_guide_gradient(lgpu_guide guide, uint32_t n, const float *records, float *grad, float *loss);
+/* Adam steps on the records of the last guided wave; mean loss per step. */
+lgpu_status lgpu_guide_train(lgpu_guide guide, uint32_t steps, float *loss);
+lgpu_status lgpu_guide_stats(lgpu_guide guide, uint32_t *records, uint32_t *steps, float *last_loss);
 
 /* ---------------- post-processing (ToneMap.cpp:5-73, PostProcessing.cpp:16-45,
  * Texture::SaveToImageFile Texture.cpp:19-81) ------------------------------------- */

@@ -5,7 +5,7 @@ binding, so both are driven from the same scene description.
 """
 import ctypes as C
 
-ABI_VERSION = 5  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
+ABI_VERSION = 6  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
 NONE = 0xFFFFFFFF
 
 # lgpu_status
@@ -86,7 +86,12 @@ class RenderDesc(C.Structure):
                 ("min_bounces", C.c_uint32), ("spectral_mode", C.c_uint32), ("estimator_mode", C.c_uint32),
                 ("seed", C.c_uint32), ("path_capacity", C.c_uint32), ("flags", C.c_uint32),
                 ("pass_max_depth", C.c_float), ("pass_power", C.c_float), ("pass_background", C.c_float * 3),
-                ("pass_geometric_normal", C.c_uint32)]
+                ("pass_geometric_normal", C.c_uint32), ("guide", C.c_void_p), ("guide_train_steps", C.c_uint32)]
+
+
+class GuideDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("record_capacity", C.c_uint32), ("batch", C.c_uint32),
+                ("learning_rate", C.c_float)]
 
 RENDER_COUNT_TRAVERSAL = 1
 
@@ -138,4 +143,5 @@ class RenderStats(C.Structure):
                 ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64),
                 ("tris_tested", C.c_uint64), ("waves", C.c_uint32), ("launches_extend", C.c_uint32),
                 ("launches_shade", C.c_uint32), ("launches_shadow", C.c_uint32), ("voxel_steps", C.c_uint64),
-                ("nodes4_fetched", C.c_uint64), ("tris4_fetched", C.c_uint64)]
+                ("nodes4_fetched", C.c_uint64), ("tris4_fetched", C.c_uint64), ("ms_guide_infer", C.c_double),
+                ("ms_guide_train", C.c_double), ("guide_records", C.c_uint64)]

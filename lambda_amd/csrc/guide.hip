@@ -1,0 +1,736 @@
+// Learned path guiding: the network's MFMA forward (inference for every shaded path of a bounce)
+// and forward + backward (training on the renderer's own radiance records), Adam, and the C ABI.
+//
+// MFMA layout (v_mfma_f32_16x16x16_bf16, 64 lanes, g = lane / 16): A (16 x 16, M x K) lane holds
+// A[lane % 16][4 g .. 4 g + 3]; B (K x N) lane holds B[4 g .. 4 g + 3][lane % 16]; D (M x N) lane
+// holds D[4 g .. 4 g + 3][lane % 16]. Every layer is computed transposed, Y = W X with the
+// queries as columns: then a layer's D tiles (neurons 4 g + r of query lane % 16) are exactly
+// the B operand of the next layer, and activations never leave registers in the forward pass.
+// The backward pass reuses that for dX = W^T dZ and transposes through LDS for dW = dZ X^T.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "guide.h"
+#include "host.h"
+#include "lgpu_internal.h"
+
+namespace lgpu {
+
+typedef short Bf4 __attribute__((ext_vector_type(4)));
+typedef float F4v __attribute__((ext_vector_type(4)));
+
+__device__ inline F4v Mfma(Bf4 a, Bf4 b, F4v c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ inline Bf4 Pack4(float a, float b, float c, float d) {
+  Bf4 r;
+  r[0] = (short)ToBf16(a); r[1] = (short)ToBf16(b); r[2] = (short)ToBf16(c); r[3] = (short)ToBf16(d);
+  return r;
+}
+__device__ inline Bf4 LoadBf4(const uint16_t *p) {  // 8-byte aligned
+  const uint2 u = *reinterpret_cast<const uint2 *>(p);
+  Bf4 r;
+  r[0] = (short)(u.x & 0xFFFFu); r[1] = (short)(u.x >> 16); r[2] = (short)(u.y & 0xFFFFu); r[3] = (short)(u.y >> 16);
+  return r;
+}
+__device__ inline uint2 AsU2(Bf4 v) {
+  return make_uint2(((uint32_t)(uint16_t)v[0]) | ((uint32_t)(uint16_t)v[1] << 16),
+                    ((uint32_t)(uint16_t)v[2]) | ((uint32_t)(uint16_t)v[3] << 16));
+}
+__device__ inline Bf4 FromU2(uint2 u) {
+  Bf4 r;
+  r[0] = (short)(u.x & 0xFFFFu); r[1] = (short)(u.x >> 16); r[2] = (short)(u.y & 0xFFFFu); r[3] = (short)(u.y >> 16);
+  return r;
+}
+__device__ inline float Lane4Sum(float v) { v += __shfl_xor(v, 16); v += __shfl_xor(v, 32); return v; }
+__device__ inline float Lane4Max(float v) { v = fmaxf(v, __shfl_xor(v, 16)); v = fmaxf(v, __shfl_xor(v, 32)); return v; }
+
+// Encoded inputs of one query as this lane's B operand of layer 0: features 16 kt + 4 g + r.
+__device__ inline void EncodeB(const float s[kGuideScalars], int g, Bf4 x[5]) {
+#pragma unroll
+  for (int kt = 0; kt < 5; ++kt)
+    x[kt] = Pack4(GuideFeature(s, 16 * kt + 4 * g), GuideFeature(s, 16 * kt + 4 * g + 1),
+                  GuideFeature(s, 16 * kt + 4 * g + 2), GuideFeature(s, 16 * kt + 4 * g + 3));
+}
+
+// Hidden layers 0..4 of one wave's 64 queries (4 column tiles): x[kt][pt] in, result in x (the
+// next layer's B operands). W: BF16 weights (LDS or global), B: FP32 biases. `store`: per-lane
+// store of each layer's input (training), layout [slot][pt][lane] with 5 slots for layer 0
+// and 4 per hidden layer.
+template <bool kStore>
+__device__ inline void HiddenLayers(const uint16_t *W, const float *B, Bf4 x[5][4], uint2 *store) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  if (kStore) {
+#pragma unroll
+    for (int kt = 0; kt < 5; ++kt)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) store[(kt * 4 + pt) * 64 + lane] = AsU2(x[kt][pt]);
+  }
+#pragma unroll 1
+  for (int l = 0; l < kGuideLayers - 1; ++l) {
+    const int C = GuideCols(l), nk = C / 16;
+    const uint16_t *Wl = W + GuideWOff(l);
+    const float *Bl = B + GuideBOff(l);
+    F4v acc[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) acc[m][pt] = F4v{0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < nk; ++kt) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const Bf4 a = LoadBf4(Wl + (16 * m + c16) * C + 16 * kt + 4 * g);
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) acc[m][pt] = Mfma(a, x[kt][pt], acc[m][pt]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const float b0 = Bl[16 * m + 4 * g], b1 = Bl[16 * m + 4 * g + 1], b2 = Bl[16 * m + 4 * g + 2],
+                  b3 = Bl[16 * m + 4 * g + 3];
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt)
+        x[m][pt] = Pack4(fmaxf(acc[m][pt][0] + b0, 0.f), fmaxf(acc[m][pt][1] + b1, 0.f), fmaxf(acc[m][pt][2] + b2, 0.f),
+                         fmaxf(acc[m][pt][3] + b3, 0.f));
+    }
+    if (kStore) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) store[((5 + 4 * l + m) * 4 + pt) * 64 + lane] = AsU2(x[m][pt]);
+    }
+  }
+}
+
+// Output layer for column tile pt: z[m] (m < 9) D tiles = logits 16 m + 4 g + r of query 16 pt + lane % 16.
+__device__ inline void OutputLayer(const uint16_t *W, const float *B, const Bf4 x[5][4], int pt, F4v z[9]) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  const uint16_t *Wl = W + GuideWOff(kGuideLayers - 1);
+  const float *Bl = B + GuideBOff(kGuideLayers - 1);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = Mfma(LoadBf4(Wl + (16 * m + c16) * kGuideHidden + 16 * kt + 4 * g), x[kt][pt], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] += Bl[16 * m + 4 * g + r];
+    z[m] = acc;
+  }
+}
+
+// Query inputs: from path state (render) or from an array of scalars (test hook).
+struct GuideQuerySrc {
+  const float4 *ray_o, *ray_d, *hit;  // path state of the current queue
+  const float *scalars;               // or 5 scalars per query
+  float bmin[3], binv[3];
+};
+__device__ inline void QueryScalars(const GuideQuerySrc &src, uint32_t q, float s[kGuideScalars]) {
+  if (src.scalars) {
+#pragma unroll
+    for (int k = 0; k < kGuideScalars; ++k) s[k] = src.scalars[kGuideScalars * (size_t)q + k];
+    return;
+  }
+  const float4 h = src.hit[q];
+  if (__float_as_uint(h.w) == kNone) {
+#pragma unroll
+    for (int k = 0; k < kGuideScalars; ++k) s[k] = 0.f;
+    return;
+  }
+  const float4 o = src.ray_o[q], d = src.ray_d[q];
+  const V3 p = v3(o.x + d.x * h.x, o.y + d.y * h.x, o.z + d.z * h.x);
+  GuideInputs(p, v3(-d.x, -d.y, -d.z), src.bmin, src.binv, s);
+}
+
+constexpr int kInferBlock = 256;
+
+// Inference: per query, the quantised bin table (bytes max(1, round(255 exp(z - z_max))), the
+// distribution is byte / sum) and the mixing probability. Persistent blocks; weights in LDS.
+__global__ void __launch_bounds__(kInferBlock) k_guide_infer(const uint16_t *__restrict__ Wg, const float *__restrict__ Bg,
+                                                            GuideQuerySrc src, uint32_t n, const uint32_t *np,
+                                                            uint32_t *__restrict__ bins, float2 *__restrict__ hdr) {
+  __shared__ __align__(16) uint16_t W[kGuideWeights];
+  __shared__ float B[kGuideBiases];
+  if (np) n = *np;
+  if ((uint64_t)blockIdx.x * kInferBlock >= n) return;
+  for (int i = threadIdx.x; i < kGuideWeights / 8; i += kInferBlock)
+    reinterpret_cast<uint4 *>(W)[i] = reinterpret_cast<const uint4 *>(Wg)[i];
+  for (int i = threadIdx.x; i < kGuideBiases; i += kInferBlock) B[i] = Bg[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15, wave = threadIdx.x >> 6;
+  for (uint64_t base = (uint64_t)blockIdx.x * kInferBlock; base < n; base += (uint64_t)gridDim.x * kInferBlock) {
+    const uint64_t wbase = base + 64 * wave;
+    Bf4 x[5][4];
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt) {
+      const uint64_t q = wbase + 16 * pt + c16;
+      float s[kGuideScalars];
+      QueryScalars(src, (uint32_t)(q < n ? q : n - 1), s);
+      Bf4 e[5];
+      EncodeB(s, g, e);
+#pragma unroll
+      for (int kt = 0; kt < 5; ++kt) x[kt][pt] = e[kt];
+    }
+    HiddenLayers<false>(W, B, x, nullptr);
+#pragma unroll 1
+    for (int pt = 0; pt < 4; ++pt) {
+      F4v z[9];
+      OutputLayer(W, B, x, pt, z);
+      float zmax = -INFINITY;
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zmax = fmaxf(zmax, z[m][r]);
+      zmax = Lane4Max(zmax);
+      float tot = 0.f;
+      uint32_t words[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float e = __expf(z[m][r] - zmax);
+          uint32_t b = (uint32_t)(255.f * e + .5f);
+          b = b < 1u ? 1u : (b > 255u ? 255u : b);
+          tot += (float)b;
+          w |= b << (8 * r);
+        }
+        words[m] = w;
+      }
+      tot = Lane4Sum(tot);
+      const float zc = Lane4Sum(g == 0 ? z[8][0] : 0.f);
+      const uint64_t q = wbase + 16 * pt + c16;
+      if (q < n) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) bins[q * 32 + 4 * m + g] = words[m];
+        if (g == 0) hdr[q] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), tot);
+      }
+    }
+  }
+}
+
+// Training records: finalise the weights once the wave's paths are done.
+__global__ void k_guide_finalize(GuideRecord *rec, const uint32_t *count, uint32_t cap, const float4 *L, int mode) {
+  const uint32_t n = min(*count, cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    GuideRecord r = rec[i];
+    const uint32_t sid = __float_as_uint(r.b.w);
+    const float lf = Luminance(S4(L[sid]), mode);
+    float w = 0.f;
+    if (!isnan(r.c.x) && r.c.y > 0.f) w = fmaxf(0.f, (lf - r.c.x) / r.c.y);
+    if (!isfinite(w)) w = 0.f;
+    rec[i].c.z = w;
+  }
+}
+
+// A training batch: random record indices (with replacement) and the sum of their weights.
+__global__ void k_guide_batch(const GuideRecord *rec, const uint32_t *count, uint32_t cap, uint32_t batch, uint32_t step,
+                              uint32_t seed, uint32_t *idx, float *sumW) {
+  const uint32_t n = min(*count, cap);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch) return;
+  uint32_t c[4] = {i, step, 0x47554944u, 0u};
+  Philox(c, seed, 7u);
+  const uint32_t k = n ? (uint32_t)(((uint64_t)c[0] * n) >> 32) : 0u;
+  idx[i] = k;
+  if (n) atomicAdd(sumW, rec[k].c.z);
+}
+
+constexpr int kTrainBlock = 64;  // one wave: the LDS transposes are per wave
+
+// Forward and backward of the guiding objective on 64 records per block:
+//   loss = -sum_i (w_i / sum w) log(c q(wi_i) + (1 - c) p_bsdf,i), q = softmax_bin / (4 pi / 128).
+// Gradients (FP32) are added to grad with atomics. `act`: per-block store of the layer inputs.
+__global__ void __launch_bounds__(kTrainBlock) k_guide_grad(const uint16_t *__restrict__ W, const float *__restrict__ B,
+                                                           const GuideRecord *__restrict__ rec, const uint32_t *__restrict__ idx,
+                                                           uint32_t batch, const float *sumW, float *grad, float *loss,
+                                                           uint2 *act) {
+  __shared__ __align__(16) uint16_t dzt[kGuideOut * 64];  // dZ^T: [neuron][query]
+  __shared__ __align__(16) uint16_t xt[kGuideIn * 64];    // X: [feature][query]
+  const int lane = threadIdx.x, g = lane >> 4, c16 = lane & 15;
+  const uint32_t base = blockIdx.x * 64;
+  uint2 *st = act + (size_t)blockIdx.x * 25 * 4 * 64;
+  const float invW = *sumW > 0.f ? 1.f / *sumW : 0.f;
+  // ---- forward
+  Bf4 x[5][4];
+  float pb[4], wn[4];
+  uint32_t kb[4];
+#pragma unroll
+  for (int pt = 0; pt < 4; ++pt) {
+    const uint32_t j = base + 16 * pt + c16;
+    float s[kGuideScalars] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    pb[pt] = 1.f; wn[pt] = 0.f; kb[pt] = 0u;
+    if (j < batch) {
+      const GuideRecord r = rec[idx[j]];
+      s[0] = r.a.x; s[1] = r.a.y; s[2] = r.a.z; s[3] = r.a.w; s[4] = r.b.x;
+      kb[pt] = __float_as_uint(r.b.y);
+      pb[pt] = r.b.z;
+      wn[pt] = r.c.z * invW;
+    }
+    Bf4 e[5];
+    EncodeB(s, g, e);
+#pragma unroll
+    for (int kt = 0; kt < 5; ++kt) x[kt][pt] = e[kt];
+  }
+  HiddenLayers<true>(W, B, x, st);
+  // ---- output layer and dZ (bf16, D layout) of the objective
+  Bf4 dz[9][4];
+  float lsum = 0.f;
+#pragma unroll 1
+  for (int pt = 0; pt < 4; ++pt) {
+    F4v z[9];
+    OutputLayer(W, B, x, pt, z);
+    float zmax = -INFINITY;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) zmax = fmaxf(zmax, z[m][r]);
+    zmax = Lane4Max(zmax);
+    float e[8][4], sum = 0.f, ek = 0.f;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        e[m][r] = expf(z[m][r] - zmax);
+        sum += e[m][r];
+        if ((uint32_t)(16 * m + 4 * g + r) == kb[pt]) ek = e[m][r];
+      }
+    sum = Lane4Sum(sum);
+    ek = Lane4Sum(ek);
+    const float zc = Lane4Sum(g == 0 ? z[8][0] : 0.f);
+    const float sg = 1.f / (1.f + expf(-zc));
+    const float c = kGuideCMin + kGuideCSpan * sg;
+    const float q = ek / sum / kGuideBinSr;
+    const float pmix = c * q + (1.f - c) * pb[pt];
+    const float coef = wn[pt] > 0.f ? -wn[pt] / pmix : 0.f;
+    if (g == 0 && wn[pt] > 0.f) lsum += -wn[pt] * logf(pmix);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      float d[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float sj = e[m][r] / sum;
+        const float delta = (uint32_t)(16 * m + 4 * g + r) == kb[pt] ? 1.f : 0.f;
+        d[r] = coef * c * q * (delta - sj);
+      }
+      dz[m][pt] = Pack4(d[0], d[1], d[2], d[3]);
+    }
+    const float dc = coef * (q - pb[pt]) * kGuideCSpan * sg * (1.f - sg);
+    dz[8][pt] = Pack4(g == 0 ? dc : 0.f, 0.f, 0.f, 0.f);
+  }
+  if (lsum != 0.f) atomicAdd(loss, lsum);
+  // ---- backward, layer by layer
+#pragma unroll 1
+  for (int l = kGuideLayers - 1; l >= 0; --l) {
+    const int R = GuideRows(l), C = GuideCols(l), nm = R / 16, nk = C / 16;
+    // layer input X (features x queries) and dZ into LDS, transposed for the dW products
+    const int slot0 = l == 0 ? 0 : 5 + 4 * (l - 1);
+    for (int kt = 0; kt < nk; ++kt)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) {
+        const Bf4 v = FromU2(st[((slot0 + kt) * 4 + pt) * 64 + lane]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xt[(16 * kt + 4 * g + r) * 64 + 16 * pt + c16] = (uint16_t)v[r];
+      }
+    for (int m = 0; m < nm; ++m)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dzt[(16 * m + 4 * g + r) * 64 + 16 * pt + c16] = (uint16_t)dz[m][pt][r];
+    __syncthreads();
+    // dW = dZ X^T (inner dimension: the 64 queries)
+    float *gW = grad + GuideWOff(l);
+    for (int m = 0; m < nm; ++m)
+      for (int nt = 0; nt < nk; ++nt) {
+        F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kp = 0; kp < 4; ++kp)
+          acc = Mfma(LoadBf4(dzt + (16 * m + c16) * 64 + 16 * kp + 4 * g), LoadBf4(xt + (16 * nt + c16) * 64 + 16 * kp + 4 * g),
+                     acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) unsafeAtomicAdd(&gW[(16 * m + 4 * g + r) * C + 16 * nt + c16], acc[r]);
+      }
+    // db = sum over queries of dZ
+    float *gB = grad + kGuideWeights + GuideBOff(l);
+    for (int row = lane; row < R; row += 64) {
+      float s = 0.f;
+      for (int k = 0; k < 64; ++k) s += FromBf16(dzt[row * 64 + k]);
+      unsafeAtomicAdd(&gB[row], s);
+    }
+    __syncthreads();
+    if (l == 0) break;
+    // dX = W^T dZ, then through the ReLU of the previous layer (its output is this layer's input)
+    const uint16_t *Wl = W + GuideWOff(l);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      F4v acc[4];
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) acc[pt] = F4v{0.f, 0.f, 0.f, 0.f};
+      for (int kt = 0; kt < nm; ++kt) {
+        Bf4 a;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = (short)Wl[(16 * kt + 4 * g + r) * C + 16 * m + c16];
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) acc[pt] = Mfma(a, dz[kt][pt], acc[pt]);
+      }
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) {
+        const Bf4 xin = FromU2(st[((slot0 + m) * 4 + pt) * 64 + lane]);
+        float d[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d[r] = FromBf16((uint16_t)xin[r]) > 0.f ? acc[pt][r] : 0.f;
+        x[m][pt] = Pack4(d[0], d[1], d[2], d[3]);  // holds dZ of layer l - 1 until copied below
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) dz[m][pt] = x[m][pt];
+  }
+}
+
+// Adam (Kingma and Ba 2015) on the FP32 master parameters; writes the BF16 weights and FP32
+// biases the kernels read, and clears the gradient.
+__global__ void k_guide_adam(float *p, float *m, float *v, float *grad, uint16_t *wbf, float *bias, float lr, float b1t,
+                             float b2t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kGuideParams) return;
+  const float b1 = .9f, b2 = .999f, eps = 1e-8f;
+  const float gr = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gr;
+  const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi / (1.f - b1t)) / (sqrtf(vi / (1.f - b2t)) + eps);
+  p[i] = pi;
+  grad[i] = 0.f;
+  if (i < kGuideWeights) wbf[i] = ToBf16(pi);
+  else bias[i - kGuideWeights] = pi;
+}
+
+__global__ void k_guide_publish(const float *p, uint16_t *wbf, float *bias) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kGuideParams) return;
+  if (i < kGuideWeights) wbf[i] = ToBf16(p[i]);
+  else bias[i - kGuideWeights] = p[i];
+}
+
+}  // namespace lgpu
+
+using namespace lgpu;
+
+struct lgpu_guide_s {
+  lgpu_device dev = nullptr;  // null once the device was destroyed first
+  lgpu_guide_desc desc{};
+  DevBuf params, m, v, grad, wbf, bias, loss, sumw, idx, act;
+  DevBuf rec, rec_count;
+  uint32_t steps = 0;
+  uint32_t last_records = 0;
+  float last_loss = 0.f;
+  uint64_t expected_vertices = 0;  // guided vertices of the last wave (recording rate)
+};
+
+namespace lgpu {
+
+static lgpu_status GuideEnsureScratch(lgpu_guide g, uint32_t batch) {
+  const uint32_t blocks = (batch + 63) / 64;
+  LGPU_HIP(g->idx.alloc(4ull * std::max<uint32_t>(batch, 1)));
+  LGPU_HIP(g->act.alloc((size_t)blocks * 25 * 4 * 64 * sizeof(uint2) + 256));
+  return LGPU_OK;
+}
+
+// One gradient over `batch` records (indices already in g->idx, weights summed in g->sumw).
+static lgpu_status GuideGradient(lgpu_guide g, uint32_t batch, const GuideRecord *rec, hipStream_t s) {
+  if (!batch) return LGPU_OK;
+  const uint32_t blocks = (batch + 63) / 64;
+  hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(kTrainBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), rec,
+                     g->idx.as<uint32_t>(), batch, g->sumw.as<float>(), g->grad.as<float>(), g->loss.as<float>(),
+                     g->act.as<uint2>());
+  LGPU_HIP(hipGetLastError());
+  return LGPU_OK;
+}
+
+static lgpu_status GuideAdam(lgpu_guide g, hipStream_t s) {
+  g->steps++;
+  const float b1t = powf(.9f, (float)g->steps), b2t = powf(.999f, (float)g->steps);
+  hipLaunchKernelGGL(k_guide_adam, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g->params.as<float>(), g->m.as<float>(),
+                     g->v.as<float>(), g->grad.as<float>(), g->wbf.as<uint16_t>(), g->bias.as<float>(), g->desc.learning_rate,
+                     b1t, b2t);
+  LGPU_HIP(hipGetLastError());
+  return LGPU_OK;
+}
+
+// Training steps on the records of the last wave (the caller holds the device mutex).
+lgpu_status GuideTrainSteps(lgpu_guide g, uint32_t steps, hipStream_t s) {
+  if (!steps) return LGPU_OK;
+  const uint32_t batch = g->desc.batch;
+  lgpu_status st = GuideEnsureScratch(g, batch);
+  if (st != LGPU_OK) return st;
+  LGPU_HIP(hipMemsetAsync(g->loss.ptr, 0, 4, s));
+  for (uint32_t k = 0; k < steps; ++k) {
+    LGPU_HIP(hipMemsetAsync(g->sumw.ptr, 0, 4, s));
+    hipLaunchKernelGGL(k_guide_batch, dim3((batch + 255) / 256), dim3(256), 0, s, g->rec.as<GuideRecord>(),
+                       g->rec_count.as<uint32_t>(), g->desc.record_capacity, batch, g->steps, g->desc.seed,
+                       g->idx.as<uint32_t>(), g->sumw.as<float>());
+    LGPU_HIP(hipGetLastError());
+    st = GuideGradient(g, batch, g->rec.as<GuideRecord>(), s);
+    if (st != LGPU_OK) return st;
+    st = GuideAdam(g, s);
+    if (st != LGPU_OK) return st;
+  }
+  float l = 0.f;
+  LGPU_HIP(hipMemcpyAsync(&l, g->loss.ptr, 4, hipMemcpyDeviceToHost, s));
+  LGPU_HIP(hipStreamSynchronize(s));
+  g->last_loss = l / (float)steps;
+  return LGPU_OK;
+}
+
+// Render-loop hooks (render.hip).
+lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expectedVertices, GuideRefs *refs,
+                             const uint4 *bins, const float2 *hdr, hipStream_t s) {
+  refs->bins = bins;
+  refs->hdr = hdr;
+  refs->rec = g->rec.as<GuideRecord>();
+  refs->rec_count = g->rec_count.as<uint32_t>();
+  refs->rec_cap = g->desc.record_capacity;
+  const uint64_t ev = expectedVertices ? expectedVertices : 1;
+  const double frac = std::min(1.0, (double)g->desc.record_capacity / (1.25 * (double)ev));
+  refs->rec_threshold = frac >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(frac * 4294967295.0);
+  for (int k = 0; k < 3; ++k) {
+    refs->bmin[k] = bounds[k];
+    const float ext = bounds[3 + k] - bounds[k];
+    refs->binv[k] = ext > 0.f ? 1.f / ext : 0.f;
+  }
+  LGPU_HIP(hipMemsetAsync(g->rec_count.ptr, 0, 4, s));
+  return LGPU_OK;
+}
+
+lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps, uint32_t ub, const uint32_t *np,
+                       uint4 *bins, float2 *hdr, hipStream_t s) {
+  GuideQuerySrc src{};
+  src.ray_o = ps.ray_o; src.ray_d = ps.ray_d; src.hit = ps.hit; src.scalars = nullptr;
+  for (int k = 0; k < 3; ++k) { src.bmin[k] = refs.bmin[k]; src.binv[k] = refs.binv[k]; }
+  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((ub + kInferBlock - 1) / kInferBlock, 2048));
+  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, ub,
+                     np, reinterpret_cast<uint32_t *>(bins), hdr);
+  LGPU_HIP(hipGetLastError());
+  return LGPU_OK;
+}
+
+lgpu_status GuideFinishWave(lgpu_guide g, const SampleState &ss, int mode, uint32_t trainSteps, uint64_t vertices,
+                            hipStream_t s, uint32_t *records) {
+  hipLaunchKernelGGL(k_guide_finalize, dim3(1024), dim3(256), 0, s, g->rec.as<GuideRecord>(), g->rec_count.as<uint32_t>(),
+                     g->desc.record_capacity, ss.L, mode);
+  LGPU_HIP(hipGetLastError());
+  uint32_t cnt = 0;
+  LGPU_HIP(hipMemcpyAsync(&cnt, g->rec_count.ptr, 4, hipMemcpyDeviceToHost, s));
+  LGPU_HIP(hipStreamSynchronize(s));
+  g->last_records = std::min(cnt, g->desc.record_capacity);
+  if (records) *records = g->last_records;
+  g->expected_vertices = vertices;
+  if (g->last_records && trainSteps) return GuideTrainSteps(g, trainSteps, s);
+  return LGPU_OK;
+}
+
+uint64_t GuideExpectedVertices(lgpu_guide g) { return g->expected_vertices; }
+bool GuideOnDevice(lgpu_guide g, lgpu_device dev) { return g->dev == dev; }
+
+}  // namespace lgpu
+
+// ---------------------------------------------------------------- C ABI
+extern "C" uint32_t lgpu_guide_num_params(void) { return (uint32_t)kGuideParams; }
+
+extern "C" lgpu_status lgpu_guide_create(lgpu_device dev, const lgpu_guide_desc *desc, lgpu_guide *out) {
+  if (!dev || !out) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_create: null argument");
+  *out = nullptr;
+  lgpu_guide_desc d{};
+  if (desc) d = *desc;
+  if (!d.record_capacity) d.record_capacity = 1u << 22;
+  if (!d.batch) d.batch = 16384;
+  if (!(d.learning_rate > 0.f)) d.learning_rate = 2e-3f;
+  if (d.record_capacity > (1u << 28)) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_create: record_capacity above 2^28");
+  if (d.batch > (1u << 22)) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_create: batch above 2^22");
+  std::lock_guard<std::mutex> lk(dev->mu);
+  LGPU_HIP(hipSetDevice(dev->device));
+  lgpu_guide g = new lgpu_guide_s();
+  g->dev = dev;
+  g->desc = d;
+  auto fail = [&](hipError_t e) {
+    delete g;
+    return Fail(LGPU_ERR_HIP, std::string("guide_create: ") + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = g->params.alloc(4ull * kGuideParams)) != hipSuccess) return fail(e);
+  if ((e = g->m.alloc(4ull * kGuideParams)) != hipSuccess) return fail(e);
+  if ((e = g->v.alloc(4ull * kGuideParams)) != hipSuccess) return fail(e);
+  if ((e = g->grad.alloc(4ull * kGuideParams)) != hipSuccess) return fail(e);
+  if ((e = g->wbf.alloc(2ull * kGuideWeights)) != hipSuccess) return fail(e);
+  if ((e = g->bias.alloc(4ull * kGuideBiases)) != hipSuccess) return fail(e);
+  if ((e = g->loss.alloc(256)) != hipSuccess) return fail(e);
+  if ((e = g->sumw.alloc(256)) != hipSuccess) return fail(e);
+  if ((e = g->rec.alloc(sizeof(GuideRecord) * (size_t)d.record_capacity)) != hipSuccess) return fail(e);
+  if ((e = g->rec_count.alloc(256)) != hipSuccess) return fail(e);
+  // He-uniform initial weights from the seed; zero biases except the mixing logit (c = 0.3 at start)
+  std::vector<float> p(kGuideParams, 0.f);
+  std::mt19937 rng(d.seed ^ 0x9E3779B9u);
+  for (int l = 0; l < kGuideLayers; ++l) {
+    const int R = GuideRows(l), C = GuideCols(l);
+    const float lim = l == kGuideLayers - 1 ? 0.f : std::sqrt(6.f / (float)C);
+    std::uniform_real_distribution<float> U(-lim, lim);
+    for (int i = 0; i < R * C; ++i) p[GuideWOff(l) + i] = lim > 0.f ? U(rng) : 0.f;
+  }
+  // sigmoid(z) = (0.3 - cmin) / span
+  const float sg = (0.3f - kGuideCMin) / kGuideCSpan;
+  p[kGuideWeights + GuideBOff(kGuideLayers - 1) + kGuideBins] = std::log(sg / (1.f - sg));
+  hipStream_t s = dev->stream;
+  if ((e = hipMemcpyAsync(g->params.ptr, p.data(), 4ull * kGuideParams, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e);
+  (void)hipMemsetAsync(g->m.ptr, 0, 4ull * kGuideParams, s);
+  (void)hipMemsetAsync(g->v.ptr, 0, 4ull * kGuideParams, s);
+  (void)hipMemsetAsync(g->grad.ptr, 0, 4ull * kGuideParams, s);
+  (void)hipMemsetAsync(g->rec_count.ptr, 0, 4, s);
+  hipLaunchKernelGGL(k_guide_publish, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g->params.as<float>(),
+                     g->wbf.as<uint16_t>(), g->bias.as<float>());
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
+  dev->guides.push_back(g);
+  dev->guide_orphan = [](void *p) {  // device destroyed first: free the device memory, keep the handle
+    lgpu_guide gg = static_cast<lgpu_guide>(p);
+    gg->params.release(); gg->m.release(); gg->v.release(); gg->grad.release(); gg->wbf.release();
+    gg->bias.release(); gg->loss.release(); gg->sumw.release(); gg->idx.release(); gg->act.release();
+    gg->rec.release(); gg->rec_count.release();
+    gg->dev = nullptr;
+  };
+  *out = g;
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_destroy(lgpu_guide g) {
+  if (!g) return LGPU_OK;
+  if (lgpu_device dev = g->dev) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    (void)hipSetDevice(dev->device);
+    (void)hipStreamSynchronize(dev->stream);
+    dev->guides.erase(std::remove(dev->guides.begin(), dev->guides.end(), static_cast<void *>(g)), dev->guides.end());
+  }
+  delete g;
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_get_params(lgpu_guide g, float *params) {
+  if (!g || !params) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_get_params: null argument");
+  if (!g->dev) return Fail(LGPU_ERR_STATE, "guide_get_params: the guide's device was destroyed");
+  std::lock_guard<std::mutex> lk(g->dev->mu);
+  LGPU_HIP(hipSetDevice(g->dev->device));
+  LGPU_HIP(hipStreamSynchronize(g->dev->stream));
+  LGPU_HIP(hipMemcpy(params, g->params.ptr, 4ull * kGuideParams, hipMemcpyDeviceToHost));
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_set_params(lgpu_guide g, const float *params) {
+  if (!g || !params) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_set_params: null argument");
+  if (!g->dev) return Fail(LGPU_ERR_STATE, "guide_set_params: the guide's device was destroyed");
+  for (int i = 0; i < kGuideParams; ++i)
+    if (!std::isfinite(params[i])) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_set_params: non-finite parameter");
+  std::lock_guard<std::mutex> lk(g->dev->mu);
+  LGPU_HIP(hipSetDevice(g->dev->device));
+  hipStream_t s = g->dev->stream;
+  LGPU_HIP(hipMemcpyAsync(g->params.ptr, params, 4ull * kGuideParams, hipMemcpyHostToDevice, s));
+  LGPU_HIP(hipMemsetAsync(g->m.ptr, 0, 4ull * kGuideParams, s));
+  LGPU_HIP(hipMemsetAsync(g->v.ptr, 0, 4ull * kGuideParams, s));
+  LGPU_HIP(hipMemsetAsync(g->grad.ptr, 0, 4ull * kGuideParams, s));
+  hipLaunchKernelGGL(k_guide_publish, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g->params.as<float>(),
+                     g->wbf.as<uint16_t>(), g->bias.as<float>());
+  LGPU_HIP(hipGetLastError());
+  LGPU_HIP(hipStreamSynchronize(s));
+  g->steps = 0;
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_forward(lgpu_guide g, uint32_t n, const float *inputs, float *out) {
+  if (!g || (n && (!inputs || !out))) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_forward: null argument");
+  if (!g->dev) return Fail(LGPU_ERR_STATE, "guide_forward: the guide's device was destroyed");
+  if (!n) return LGPU_OK;
+  std::lock_guard<std::mutex> lk(g->dev->mu);
+  LGPU_HIP(hipSetDevice(g->dev->device));
+  hipStream_t s = g->dev->stream;
+  DevBuf din, dbins, dhdr;
+  LGPU_HIP(din.upload(inputs, 4ull * kGuideScalars * n));
+  LGPU_HIP(dbins.alloc(128ull * n));
+  LGPU_HIP(dhdr.alloc(8ull * n));
+  GuideQuerySrc src{};
+  src.scalars = din.as<float>();
+  const uint32_t blocks = std::min<uint32_t>((n + kInferBlock - 1) / kInferBlock, 2048);
+  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, n,
+                     nullptr, dbins.as<uint32_t>(), dhdr.as<float2>());
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(s);
+  LGPU_HIP(le);
+  LGPU_HIP(se);
+  std::vector<uint8_t> b(128ull * n);
+  std::vector<float> h(2ull * n);
+  LGPU_HIP(hipMemcpy(b.data(), dbins.ptr, b.size(), hipMemcpyDeviceToHost));
+  LGPU_HIP(hipMemcpy(h.data(), dhdr.ptr, 4 * h.size(), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i) {
+    for (int k = 0; k < kGuideBins; ++k) out[(size_t)i * (kGuideBins + 1) + k] = (float)b[(size_t)i * 128 + k] / h[2 * i + 1];
+    out[(size_t)i * (kGuideBins + 1) + kGuideBins] = h[2 * i];
+  }
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_gradient(lgpu_guide g, uint32_t n, const float *records, float *grad, float *loss) {
+  if (!g || !grad || (n && !records)) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_gradient: null argument");
+  if (!g->dev) return Fail(LGPU_ERR_STATE, "guide_gradient: the guide's device was destroyed");
+  std::lock_guard<std::mutex> lk(g->dev->mu);
+  LGPU_HIP(hipSetDevice(g->dev->device));
+  hipStream_t s = g->dev->stream;
+  std::vector<GuideRecord> rv(std::max<uint32_t>(n, 1));
+  std::vector<uint32_t> iv(std::max<uint32_t>(n, 1));
+  float sw = 0.f;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *r = records + 8ull * i;
+    const uint32_t bin = (uint32_t)r[5];
+    if (bin >= (uint32_t)kGuideBins || !(r[6] >= 0.f) || !(r[7] >= 0.f))
+      return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_gradient: record with a bad bin, pdf or weight");
+    rv[i].a = make_float4(r[0], r[1], r[2], r[3]);
+    rv[i].b = make_float4(r[4], AsF32(bin), r[6], 0.f);
+    rv[i].c = make_float4(0.f, 1.f, r[7], 0.f);
+    iv[i] = i;
+    sw += r[7];
+  }
+  lgpu_status st = GuideEnsureScratch(g, n);
+  if (st != LGPU_OK) return st;
+  DevBuf drec;
+  LGPU_HIP(drec.upload(rv.data(), sizeof(GuideRecord) * rv.size()));
+  LGPU_HIP(hipMemcpyAsync(g->idx.ptr, iv.data(), 4ull * iv.size(), hipMemcpyHostToDevice, s));
+  LGPU_HIP(hipMemcpyAsync(g->sumw.ptr, &sw, 4, hipMemcpyHostToDevice, s));
+  LGPU_HIP(hipMemsetAsync(g->grad.ptr, 0, 4ull * kGuideParams, s));
+  LGPU_HIP(hipMemsetAsync(g->loss.ptr, 0, 4, s));
+  st = GuideGradient(g, n, drec.as<GuideRecord>(), s);
+  const hipError_t se = hipStreamSynchronize(s);
+  if (st != LGPU_OK) return st;
+  LGPU_HIP(se);
+  LGPU_HIP(hipMemcpy(grad, g->grad.ptr, 4ull * kGuideParams, hipMemcpyDeviceToHost));
+  if (loss) LGPU_HIP(hipMemcpy(loss, g->loss.ptr, 4, hipMemcpyDeviceToHost));
+  LGPU_HIP(hipMemsetAsync(g->grad.ptr, 0, 4ull * kGuideParams, s));
+  LGPU_HIP(hipStreamSynchronize(s));
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_train(lgpu_guide g, uint32_t steps, float *loss) {
+  if (!g) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_train: null guide");
+  if (!g->dev) return Fail(LGPU_ERR_STATE, "guide_train: the guide's device was destroyed");
+  std::lock_guard<std::mutex> lk(g->dev->mu);
+  LGPU_HIP(hipSetDevice(g->dev->device));
+  if (!g->last_records) return Fail(LGPU_ERR_STATE, "guide_train: no training records (render with the guide first)");
+  lgpu_status st = GuideTrainSteps(g, steps, g->dev->stream);
+  if (st != LGPU_OK) return st;
+  if (loss) *loss = g->last_loss;
+  return LGPU_OK;
+}
+
+extern "C" lgpu_status lgpu_guide_stats(lgpu_guide g, uint32_t *records, uint32_t *steps, float *last_loss) {
+  if (!g) return Fail(LGPU_ERR_INVALID_ARGUMENT, "guide_stats: null guide");
+  if (records) *records = g->last_records;
+  if (steps) *steps = g->steps;
+  if (last_loss) *last_loss = g->last_loss;
+  return LGPU_OK;
+}

@@ -84,6 +84,8 @@ struct lgpu_device_s {
   void *wave = nullptr;               // path-state pool of the render loop
   void (*wave_free)(void *) = nullptr;
   void *scratch = nullptr;            // device address of this GPU's no-allocation scratch array
+  std::vector<void *> guides;         // live guiding networks (released with the device)
+  void (*guide_orphan)(void *) = nullptr;
 };
 
 struct lgpu_film_s {
@@ -112,6 +114,7 @@ struct lgpu_scene_s {
   uint32_t num_tris = 0;
   uint32_t num_tlights = 0;
 
+  float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene box (min xyz, max xyz) of all triangles
   lgpu::DevScene dsc{};
   std::deque<lgpu::DevBuf> bufs;  // owns every array referenced by dsc (stable addresses)
 };

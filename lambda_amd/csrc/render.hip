@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstring>
 
+#include "guide.h"
 #include "host.h"
 #include "lights.h"
 #include "shading.h"
@@ -186,9 +187,38 @@ __device__ inline void PushShadow(const Queues &q, const ShadowSpec &sh, uint32_
 // kPrograms: the scene has program sockets (DevScene::has_programs); without them the kernel is
 // compiled without the shader VM, spills little, and runs best at four waves per SIMD (with the
 // VM's call frames and call-site spills, six).
-template <bool kCorrected, bool kPrograms>
+// Guide table of one path slot: GuideBin / sampling over the 128 bin weights (sum `tot`).
+__device__ inline float GuideBinPdf(const uint8_t *tab, float tot, uint32_t bin) {
+  return (float)tab[bin] / (tot * kGuideBinSr);
+}
+__device__ inline uint32_t GuideSampleBin(const uint32_t *tab, float target) {  // target in [0, tot)
+  float acc = 0.f;
+  for (uint32_t k = 0; k < kGuideBins / 4; ++k) {
+    const uint32_t w = tab[k];
+    const float b0 = (float)(w & 255u), b1 = (float)((w >> 8) & 255u), b2 = (float)((w >> 16) & 255u), b3 = (float)(w >> 24);
+    const float s = b0 + b1 + b2 + b3;
+    if (acc + s > target) {
+      if (acc + b0 > target) return 4 * k;
+      if (acc + b0 + b1 > target) return 4 * k + 1;
+      if (acc + b0 + b1 + b2 > target) return 4 * k + 2;
+      return 4 * k + 3;
+    }
+    acc += s;
+  }
+  return kGuideBins - 1;  // target rounded up to tot
+}
+__device__ inline uint32_t Hash3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (c + 0x165667B1u) * 0xC2B2AE3Du;
+  h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
+  return h;
+}
+
+// kGuided (corrected estimator only): guidable vertices sample one-sample MIS of the learned
+// guide (the path slot's bin table, inferred after extension) and the BSDF, evaluate both
+// strategies' pdfs for MIS with next-event estimation, and append training records.
+template <bool kCorrected, bool kPrograms, bool kGuided>
 __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevScene *__restrict__ scp, RenderParams rp, PathState ps, SampleState ss,
-                                                  uint32_t n, const uint32_t *np, Queues q) {
+                                                  uint32_t n, const uint32_t *np, Queues q, GuideRefs gr) {
   const DevScene &sc = *scp;
   __shared__ TreeImp treeLds[kTreeLds];
   __shared__ uint32_t keyStart[kShadeKeys];
@@ -314,6 +344,11 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   } else if (!hit) {
     alive = false;
   }
+  if (kGuided && !(flags & kFlagCamera)) {
+    // the previous guided vertex's record: radiance so far, before anything along its wi lands
+    const uint32_t r = __float_as_uint(ps.mis[i].w);
+    if (r < gr.rec_cap) gr.rec[r].c.x = Luminance(S4(ss.L[sid]), mode);
+  }
   if (addL) {
     float4 L = ss.L[sid];
     L.x += Ladd.c[0]; L.y += Ladd.c[1]; L.z += Ladd.c[2]; L.w += Ladd.c[3];
@@ -334,7 +369,9 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
     q.next.beta[j] = F4(beta);
     q.next.ids[j] = make_uint4(ids.x, ids.y, rng.dim, (depth << 8) | flags);
     q.next.aux[j] = make_uint4(sid, aux.y, pass, aux.w);
-    q.next.mis[j] = ps.mis[i];
+    float4 pm = ps.mis[i];
+    if (kGuided) pm.w = __uint_as_float(kNone);  // record closed above
+    q.next.mis[j] = pm;
     q.next.prev_p[j] = ps.prev_p[i];
     q.next.prev_n[j] = ps.prev_n[i];
     return;
@@ -354,6 +391,15 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   vm.woLy = ev.woL.y;
   shd.vm = vm;
   const V3 shadeP = h.point, shadeN = h.normalS;
+  const bool guided = kGuided && (mat.pad[0] & 1u);
+  float gc = 0.f, gtot = 1.f;
+  const uint8_t *gtab = nullptr;
+  if (guided) {
+    const float2 gh = gr.hdr[i];
+    gc = gh.x;
+    gtot = gh.y;
+    gtab = reinterpret_cast<const uint8_t *>(gr.bins + (size_t)i * 8);
+  }
   float lightDistPdf = 1.f;
   const uint32_t pick = SampleLight(sc, h.point, h.normalS, rng, &lightDistPdf, treeLds, corrected);
   float lightPDF = 0.f;
@@ -372,7 +418,9 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
     const Spec f = shd.F(mat.lobe, ev) * fabsf(ev.wiL.y);
     const float sp = shd.Pdf(mat.lobe, ev.woL, ev.wiL);
     if (!IsBlack(f, mode) && sp > 0) {
-      const float w = (corrected && delta) ? 1.f : PowerHeuristic(lightPDF, sp);
+      // with a guide the BSDF-side strategy is the guide/BSDF mixture
+      const float spm = guided ? gc * GuideBinPdf(gtab, gtot, GuideBin(ev.wi)) + (1.f - gc) * sp : sp;
+      const float w = (corrected && delta) ? 1.f : PowerHeuristic(lightPDF, spm);
       cNee = Li * f * w / lightPDF;
       hasNee = true;
     }
@@ -383,10 +431,46 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   if (corrected && hasNee) PushShadow(q, sh, sid, beta * cNee);
   float scatPdf = 0.f;
   bool spec = false;
-  Spec f = shd.Sample(mat.lobe, ev, h, rng, scatPdf, spec);
+  Spec f;
+  uint32_t gbin = kNone;
+  float pbsdf = 0.f;
+  if (guided) {
+    // one-sample MIS: the guide with probability c, else the BSDF; pdf c q + (1 - c) p_bsdf.
+    // Directions the BSDF cannot sample (p_bsdf = 0) are rejected, as BSDF sampling never
+    // reaches them either.
+    if (rng.Get1D() < gc) {
+      gbin = GuideSampleBin(reinterpret_cast<const uint32_t *>(gtab), rng.Get1D() * gtot);
+      const float u0 = rng.Get1D(), u1 = rng.Get1D();
+      ev.wi = GuideBinDirection(gbin, u0, u1);
+      ev.wiL = ToLocal(h, ev.wi);
+      pbsdf = shd.Pdf(mat.lobe, ev.woL, ev.wiL);
+      f = pbsdf > 0 ? shd.F(mat.lobe, ev) : S(0.f);
+      h.point = h.point + h.normalG * (Dot(ev.wi, h.normalG) > 0 ? kSurfaceEpsilon : -kSurfaceEpsilon);
+    } else {
+      f = shd.Sample(mat.lobe, ev, h, rng, pbsdf, spec);
+      gbin = GuideBin(ev.wi);
+    }
+    scatPdf = pbsdf > 0 ? gc * GuideBinPdf(gtab, gtot, gbin) + (1.f - gc) * pbsdf : 0.f;
+  } else {
+    f = shd.Sample(mat.lobe, ev, h, rng, scatPdf, spec);
+  }
   f = f * fabsf(ev.wiL.y);
   if (!(scatPdf > 0 && !IsBlack(f, mode))) return;
   if (!corrected && hasNee) PushShadow(q, sh, sid, beta * cNee);
+  uint32_t recId = kNone;
+  if (guided && Hash3(ids.x, ids.y, depth) <= gr.rec_threshold) {
+    const uint32_t slot = atomicAdd(gr.rec_count, 1u);
+    if (slot < gr.rec_cap) {
+      float s5[kGuideScalars];
+      GuideInputs(shadeP, -rd, gr.bmin, gr.binv, s5);
+      GuideRecord R;
+      R.a = make_float4(s5[0], s5[1], s5[2], s5[3]);
+      R.b = make_float4(s5[4], __uint_as_float(gbin), pbsdf, __uint_as_float(sid));
+      R.c = make_float4(__uint_as_float(0x7FC00000u), Luminance(beta, mode), 0.f, 0.f);
+      gr.rec[slot] = R;
+      recId = slot;
+    }
+  }
   beta = beta * (f / scatPdf);
   uint32_t nf = spec ? kFlagSpecular : 0u;
   float qrr = 0.f;
@@ -402,7 +486,7 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   q.next.beta[j] = F4(beta);
   q.next.ids[j] = make_uint4(ids.x, ids.y, rng.dim, ((depth + 1) << 8) | nf);
   q.next.aux[j] = make_uint4(sid, pick, aux.z, aux.w);
-  q.next.mis[j] = make_float4(scatPdf, lightDistPdf, qrr, 0.f);
+  q.next.mis[j] = make_float4(scatPdf, lightDistPdf, qrr, kGuided ? __uint_as_float(recId) : 0.f);
   q.next.prev_p[j] = ToF4(shadeP, 0.f);
   q.next.prev_n[j] = ToF4(shadeN, 0.f);
 }
@@ -505,19 +589,33 @@ __global__ void __launch_bounds__(kBlock) k_query(DevScene sc, const float4 *ray
 }
 
 // ---------------------------------------------------------------- host side
-static void LaunchShade(bool corrected, bool programs, uint32_t blocks, hipStream_t stream, const DevScene *scp,
-                        const RenderParams &rp, const PathState &ps, const SampleState &ss, uint32_t n, const uint32_t *np,
-                        const Queues &q) {
+static void LaunchShade(bool corrected, bool programs, const GuideRefs *gr, uint32_t blocks, hipStream_t stream,
+                        const DevScene *scp, const RenderParams &rp, const PathState &ps, const SampleState &ss, uint32_t n,
+                        const uint32_t *np, const Queues &q) {
   const dim3 g(blocks), b(kBlock);
-  if (corrected && programs) hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
-  else if (corrected) hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
-  else if (programs) hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
-  else hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q);
+  const GuideRefs none{};
+  if (gr && programs) hipLaunchKernelGGL((k_shade<true, true, true>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, *gr);
+  else if (gr) hipLaunchKernelGGL((k_shade<true, false, true>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, *gr);
+  else if (corrected && programs) hipLaunchKernelGGL((k_shade<true, true, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, none);
+  else if (corrected) hipLaunchKernelGGL((k_shade<true, false, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, none);
+  else if (programs) hipLaunchKernelGGL((k_shade<false, true, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, none);
+  else hipLaunchKernelGGL((k_shade<false, false, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, none);
 }
+
+// Guiding hooks (guide.hip).
+lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expectedVertices, GuideRefs *refs,
+                             const uint4 *bins, const float2 *hdr, hipStream_t s);
+lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps, uint32_t ub, const uint32_t *np,
+                       uint4 *bins, float2 *hdr, hipStream_t s);
+lgpu_status GuideFinishWave(lgpu_guide g, const SampleState &ss, int mode, uint32_t trainSteps, uint64_t vertices,
+                            hipStream_t s, uint32_t *records);
+uint64_t GuideExpectedVertices(lgpu_guide g);
+bool GuideOnDevice(lgpu_guide g, lgpu_device dev);
 
 struct Wave {
   // two path-state buffers (ping-pong), per-sample radiance, shadow queue
   DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene, ring;
+  DevBuf gbins, ghdr;  // guided renders: per path slot, the guide's bin table and {c, sum}
   uint32_t cap = 0;
   PathState ps[2]{};
   SampleState ss{};
@@ -530,25 +628,30 @@ struct Wave {
   // pipelined path loop: device counter slots, their pinned host copies, per-slot stage events
   static constexpr int kSlots = 4;
   uint32_t *hring = nullptr;
-  hipEvent_t pev[kSlots][5] = {};
+  hipEvent_t pev[kSlots][6] = {};
   ~Wave() {
     if (hring) (void)hipHostFree(hring);
     for (auto &row : pev)
       for (hipEvent_t &e : row)
         if (e) (void)hipEventDestroy(e);
   }
-  // Device bytes one path in flight holds (both state buffers, sample state, shadow slot, spill).
-  static uint64_t BytesPerPath(uint32_t spillWords) {
-    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords;
+  // Device bytes one path in flight holds (both state buffers, sample state, shadow slot, spill;
+  // guided: the bin table and header).
+  static uint64_t BytesPerPath(uint32_t spillWords, bool guided = false) {
+    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords + (guided ? 136ull : 0ull);
   }
   uint64_t bytes() const {
-    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes;
+    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes + gbins.bytes + ghdr.bytes;
     for (const auto &row : st)
       for (const DevBuf &x : row) b += x.bytes;
     return b;
   }
-  lgpu_status alloc(uint32_t n, uint32_t spillWords) {
+  lgpu_status alloc(uint32_t n, uint32_t spillWords, bool guided = false) {
     LGPU_HIP(spill.alloc(std::max<size_t>(256, 4ull * spillWords * n)));
+    if (guided) {
+      LGPU_HIP(gbins.alloc(128ull * n));
+      LGPU_HIP(ghdr.alloc(8ull * n));
+    }
     if (!hring) {
       LGPU_HIP(hipHostMalloc((void **)&hring, 16 * kSlots, hipHostMallocDefault));
       LGPU_HIP(ring.alloc(16 * kSlots));
@@ -612,6 +715,10 @@ static lgpu_status ValidateRender(const lgpu_render_desc *d) {
   if (d->spectral_mode > LGPU_SPECTRAL_HERO4 || d->estimator_mode > LGPU_ESTIMATOR_CORRECTED)
     return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: bad spectral or estimator mode");
   if (d->max_bounces == 0) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: max_bounces must be > 0");
+  if (d->guide && (d->integrator != LGPU_INTEGRATOR_PATH || d->estimator_mode != LGPU_ESTIMATOR_CORRECTED))
+    return Fail(LGPU_ERR_UNSUPPORTED, "render: guiding needs the path integrator and the corrected estimator");
+  if (d->guide && !GuideOnDevice(d->guide, d->scene->dev))
+    return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: the guide belongs to another (or a destroyed) device");
   if (d->camera.type > LGPU_CAMERA_SPHERICAL) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: bad camera type");
   if (d->camera.aperture == LGPU_APERTURE_BLADE && d->camera.blades == 0)
     return Fail(LGPU_ERR_INVALID_ARGUMENT, "render: blade aperture needs blades > 0");
@@ -669,13 +776,13 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       cap = kDefaultPaths;
       size_t freeB = 0, totalB = 0;
       if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
-        const uint64_t fit = (freeB + W0.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words);
+        const uint64_t fit = (freeB + W0.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words, d->guide != nullptr);
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(fit, 1ull << 16));
       }
     }
     if (cap > total) cap = total;
     if (cap == 0) cap = 1;
-    lgpu_status st = W0.alloc((uint32_t)cap, s->dsc.spill_words);
+    lgpu_status st = W0.alloc((uint32_t)cap, s->dsc.spill_words, d->guide != nullptr);
     if (st != LGPU_OK) return st;
   }
   Wave &W = bound ? *bound : *static_cast<Wave *>(dev->wave);
@@ -693,9 +800,18 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
   lgpu_render_stats S;
   std::memset(&S, 0, sizeof(S));
   if (countTrav) LGPU_HIP(hipMemsetAsync(W.tcount_p, 0, 40, stream));
+  lgpu_guide G = d->guide;
+  GuideRefs gref{};
   for (uint64_t base = 0; base < total; base += cap) {
     const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - base);
     int cur = 0;
+    uint64_t waveRays = 0;
+    if (G) {
+      const uint64_t expect = GuideExpectedVertices(G) ? GuideExpectedVertices(G) * n / std::max<uint64_t>(1, cap)
+                                                         : 2ull * n;
+      lgpu_status gs = GuidePrepareWave(G, s->bounds, expect, &gref, W.gbins.as<uint4>(), W.ghdr.as<float2>(), stream);
+      if (gs != LGPU_OK) return gs;
+    }
     LGPU_HIP(hipEventRecord(ev[0], stream));
     hipLaunchKernelGGL(k_camera, dim3(Blocks(n)), dim3(kBlock), 0, stream, rp, W.ps[0], W.ss, n, base,
                        listPix ? listPix + base : nullptr, listSmp ? listSmp + base : nullptr);
@@ -732,8 +848,13 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
                                W.spill_p, W.tcount_p);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[1], stream));
+          if (G) {  // the guide's distribution for every path of the queue (before shading reads it)
+            lgpu_status gs = GuideInfer(G, gref, W.ps[cur], ub, cin, W.gbins.as<uint4>(), W.ghdr.as<float2>(), stream);
+            if (gs != LGPU_OK) return gs;
+          }
+          LGPU_HIP(hipEventRecord(e[5], stream));
           Queues q{W.ps[cur ^ 1], W.shadow_p, cout};
-          LaunchShade(rp.corrected != 0, progs, Blocks(ub), stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
+          LaunchShade(rp.corrected != 0, progs, G ? &gref : nullptr, Blocks(ub), stream, dscp, rp, W.ps[cur], W.ss, ub, cin, q);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[2], stream));
           hipLaunchKernelGGL(k_shadow, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ss, W.shadow_p, ub,
@@ -755,11 +876,14 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         const uint32_t c0 = W.hring[4 * sl], c1 = W.hring[4 * sl + 1];
         float ms;
         S.extension_rays += live;
+        waveRays += live;
         S.launches_extend++;
         S.launches_shade++;
         LGPU_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
         S.ms_extend += ms;
-        LGPU_HIP(hipEventElapsedTime(&ms, e[1], e[2]));
+        LGPU_HIP(hipEventElapsedTime(&ms, e[1], e[5]));
+        S.ms_guide_infer += ms;
+        LGPU_HIP(hipEventElapsedTime(&ms, e[5], e[2]));
         S.ms_shade += ms;
         if (c1) {
           LGPU_HIP(hipEventElapsedTime(&ms, e[2], e[3]));
@@ -772,6 +896,14 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         if (c0 == 0) done = true;
       }
       if (!done) return Fail(LGPU_ERR_STATE, "render: path wave did not terminate");
+      if (G) {  // close the wave's records, then train on them before the next wave infers
+        const auto tg = std::chrono::steady_clock::now();
+        uint32_t recs = 0;
+        lgpu_status gs = GuideFinishWave(G, W.ss, (int)rp.mode, d->guide_train_steps, waveRays, stream, &recs);
+        if (gs != LGPU_OK) return gs;
+        S.guide_records += recs;
+        S.ms_guide_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
+      }
     } else {
       uint32_t count = n;
       uint32_t iter = 0;
@@ -789,7 +921,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         LGPU_HIP(hipEventRecord(ev[3], stream));
         Queues q{W.ps[cur ^ 1], W.shadow_p, W.counters_p};
         if (rp.integrator == LGPU_INTEGRATOR_PATH)
-          LaunchShade(rp.corrected != 0, progs, Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
+          LaunchShade(rp.corrected != 0, progs, nullptr, Blocks(count), stream, dscp, rp, W.ps[cur], W.ss, count, nullptr, q);
         else if (rp.integrator == LGPU_INTEGRATOR_DIRECT)
           LaunchShadeDirect(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill_p);
         else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
@@ -900,6 +1032,7 @@ extern "C" lgpu_status lgpu_render_noalloc(const lgpu_render_desc *d, lgpu_rende
   if (!d->film) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: null film");
   if (d->spp_end <= d->spp_begin) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: empty spp range");
   if (d->film->dev != d->scene->dev) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_noalloc: film and scene on different devices");
+  if (d->guide) return Fail(LGPU_ERR_UNSUPPORTED, "render_noalloc: guiding allocates its tables; use lgpu_render");
   lgpu_device dev = d->scene->dev;
   std::lock_guard<std::mutex> lk(dev->mu);
   if (!dev->scratch) return Fail(LGPU_ERR_STATE, "render_noalloc: device has no scratch array");

@@ -835,10 +835,38 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
         return Fail(LGPU_ERR_INVALID_ARGUMENT, "material " + std::to_string(mats.size()) + ": unknown medium id");
     mats.push_back(r);
   }
+  // Guidable: no delta lobe and a pdf that matches the sampler (Lambertian, Oren-Nayar BRDF,
+  // microfacet BRDF, rough dielectric, mixes of those): the guided shade stage may then sample
+  // the lobe's directions from the guide as well (MaterialRec::pad[0] bit 0).
+  {
+    std::function<bool(uint32_t)> guidable = [&](uint32_t li) -> bool {
+      const Lobe &L = lobes[li];
+      switch (L.kind) {
+        case kLambertian: case kOrenNayarR: case kMicrofacet: case kRoughDielectric: return true;
+        case kMix: return guidable(L.child[0]) && guidable(L.child[1]);
+        default: return false;
+      }
+    };
+    for (MaterialRec &r : mats)
+      if (r.lobe != kNone && guidable(r.lobe)) r.pad[0] |= 1u;
+  }
   for (BvhTri &t : btris) {
     const uint32_t root = mats[meshes[t.mesh].material].lobe;
     t.key = root == kNone ? 1u : 2u + (lobes[root].kind - (uint32_t)kLambertian);
     if (t.key >= kShadeKeys) t.key = kShadeKeys - 1;
+  }
+  // scene bounds (guiding inputs are positions normalised to them)
+  {
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (const BvhTri &t : btris) {
+      const float px[3][3] = {{t.v0[0], t.v0[1], t.v0[2]},
+                              {t.v0[0] + t.e1[0], t.v0[1] + t.e1[1], t.v0[2] + t.e1[2]},
+                              {t.v0[0] + t.e2[0], t.v0[1] + t.e2[1], t.v0[2] + t.e2[2]}};
+      for (const auto &q : px)
+        for (int a = 0; a < 3; ++a) { b[a] = std::min(b[a], q[a]); b[3 + a] = std::max(b[3 + a], q[a]); }
+    }
+    if (btris.empty()) { for (int a = 0; a < 3; ++a) { b[a] = -1.f; b[3 + a] = 1.f; } }
+    for (int a = 0; a < 6; ++a) s->bounds[a] = b[a];
   }
 
   // ------------------------------------------------ media

@@ -28,7 +28,9 @@ EXPORTS = [
     "lgpu_progressive_set_callback", "lgpu_progressive_start", "lgpu_progressive_stop", "lgpu_progressive_clear",
     "lgpu_progressive_read", "lgpu_progressive_running", "lgpu_obj_load", "lgpu_obj_counts", "lgpu_obj_mesh",
     "lgpu_obj_material_info", "lgpu_obj_destroy", "lgpu_film_post", "lgpu_write_png", "lgpu_film_write", "lgpu_film_denoise",
-    "lgpu_render_noalloc",
+    "lgpu_render_noalloc", "lgpu_guide_create", "lgpu_guide_destroy", "lgpu_guide_num_params",
+    "lgpu_guide_get_params", "lgpu_guide_set_params", "lgpu_guide_forward", "lgpu_guide_gradient",
+    "lgpu_guide_train", "lgpu_guide_stats",
 ]
 
 
@@ -54,6 +56,10 @@ def lib():
             "lgpu_film_create": [vp, u32, u32, vp], "lgpu_film_destroy": [vp], "lgpu_film_clear": [vp],
             "lgpu_film_read": [vp, vp, vp], "lgpu_film_read_rgba": [vp, vp], "lgpu_render": [vp, vp],
             "lgpu_render_noalloc": [vp, vp],
+            "lgpu_guide_create": [vp, vp, vp], "lgpu_guide_destroy": [vp], "lgpu_guide_get_params": [vp, vp],
+            "lgpu_guide_set_params": [vp, vp], "lgpu_guide_forward": [vp, u32, vp, vp],
+            "lgpu_guide_gradient": [vp, u32, vp, vp, vp], "lgpu_guide_train": [vp, u32, vp],
+            "lgpu_guide_stats": [vp, vp, vp, vp],
             "lgpu_trace_paths": [vp, u32, vp, vp, vp], "lgpu_intersect": [vp, u32, vp, vp],
             "lgpu_occluded": [vp, u32, vp, vp], "lgpu_philox": [vp, u32, vp, vp, vp],
             "lgpu_copy_bench": [vp, C.c_uint64, u32, vp],
@@ -72,6 +78,8 @@ def lib():
             fn.argtypes = args
             fn.restype = C.c_int
         L.lgpu_last_error.restype = C.c_char_p
+        L.lgpu_guide_num_params.restype = u32
+        L.lgpu_guide_num_params.argtypes = []
         L.lgpu_abi_version.restype = u32
         L.lgpu_camera_init.argtypes = [vp, u32, vp, f32, f32]
         L.lgpu_camera_set_rotation.argtypes = [vp, f32, f32]
@@ -340,10 +348,68 @@ class Progressive:
         self.close()
 
 
+class Guide:
+    """Learned path guiding network (lgpu_guide_*): created on a device, passed to render_desc."""
+
+    def __init__(self, dev, seed=0, record_capacity=0, batch=0, learning_rate=0.0):
+        d = abi.GuideDesc(seed, record_capacity, batch, learning_rate)
+        h = C.c_void_p()
+        _check(lib().lgpu_guide_create(dev.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.dev = dev
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgpu_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def num_params():
+        return int(lib().lgpu_guide_num_params())
+
+    def params(self):
+        p = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().lgpu_guide_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params()
+        _check(lib().lgpu_guide_set_params(self.h, _ptr(p)))
+
+    def forward(self, inputs):
+        """(n, 5) inputs in [0, 1] -> (n, 129): bin probabilities and the mixing probability."""
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 5)
+        out = np.zeros((len(x), 129), dtype=np.float32)
+        _check(lib().lgpu_guide_forward(self.h, len(x), _ptr(x), _ptr(out)))
+        return out
+
+    def gradient(self, records):
+        """(n, 8) records (5 inputs, bin, BSDF pdf, weight) -> (gradient, loss)."""
+        r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 8)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().lgpu_guide_gradient(self.h, len(r), _ptr(r), _ptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, steps):
+        loss = C.c_float()
+        _check(lib().lgpu_guide_train(self.h, steps, C.byref(loss)))
+        return loss.value
+
+    def stats(self):
+        rec, steps, loss = C.c_uint32(), C.c_uint32(), C.c_float()
+        _check(lib().lgpu_guide_stats(self.h, C.byref(rec), C.byref(steps), C.byref(loss)))
+        return {"records": rec.value, "steps": steps.value, "loss": loss.value}
+
+
 def render_desc(scene, film, camera, spp_begin=0, spp_end=1, max_bounces=8, min_bounces=3,
                 spectral=abi.SPECTRAL_RGB, estimator=abi.ESTIMATOR_CORRECTED, seed=0, capacity=0, flags=0,
                 integrator=abi.INTEGRATOR_PATH, max_depth=1.0, depth_power=1.0, background=(1.0, 0.0, 1.0),
-                geometric_normal=False):
+                geometric_normal=False, guide=None, guide_train_steps=0):
     """lgpu_render_desc. The utility-pass fields default to the reference's DepthPass/NormalPass
     defaults (UtilityIntegrators.h:18-30)."""
     d = abi.RenderDesc()
@@ -358,6 +424,8 @@ def render_desc(scene, film, camera, spp_begin=0, spp_end=1, max_bounces=8, min_
     d.pass_max_depth, d.pass_power = max_depth, depth_power
     d.pass_background[:] = [float(x) for x in background]
     d.pass_geometric_normal = 1 if geometric_normal else 0
+    d.guide = guide.h.value if guide is not None else None
+    d.guide_train_steps = guide_train_steps
     return d
 
 

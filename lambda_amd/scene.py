@@ -305,6 +305,19 @@ def cornell_box(temperature=6500.0, intensity=10.0):
     return sd
 
 
+def cornell_indirect(temperature=6500.0, intensity=30.0):
+    """The Cornell room lit only indirectly: the emitter is a 0.4 x 0.4 quad at y = 0.3 above the
+    tall box facing up (+y, one-sided in the corrected estimator), so most of the room sees light
+    only after a bounce off the ceiling and walls (a case for guided BSDF sampling)."""
+    sd = cornell_box(temperature, intensity)
+    sd.ops = [op for op in sd.ops if not (op[0] == "mesh" and op[1]["material"] == 3)]
+    sd.n_tris -= 2
+    y = 0.3
+    v, i = _quad((-0.55, y, -0.5), (-0.55, y, -0.1), (-0.15, y, -0.1), (-0.15, y, -0.5))
+    sd.add_mesh(v, i, 3)
+    return sd
+
+
 def passthrough(interior=abi.NONE, exterior=abi.NONE):
     """Material with neither BxDF nor emission: a media boundary that rays cross
     (MediaBoundary, Media.h:31-57; skipped by Scene::IntersectTr, Scene.cpp:45-59)."""

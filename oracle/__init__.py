@@ -78,6 +78,13 @@ def lib():
         L.oracle_generate_ray_n.argtypes = [vp, u32, vp, u32, C.c_int, vp]
         L.oracle_light_sample_li_n.argtypes = [vp, u32, vp, vp, u32, C.c_int, vp, C.c_int, u32, vp]
         L.oracle_env_pdf_n.argtypes = [vp, u32, u32, vp, vp]
+        L.oracle_guide_num_params.restype = C.c_int
+        L.oracle_guide_forward.argtypes = [vp, u32, vp, vp]
+        L.oracle_guide_forward.restype = None
+        L.oracle_guide_gradient.argtypes = [vp, u32, vp, C.c_int, vp]
+        L.oracle_guide_gradient.restype = f32
+        L.oracle_guide_loss64.argtypes = [vp, u32, vp]
+        L.oracle_guide_loss64.restype = C.c_double
         L.oracle_tree_importance.argtypes = [vp, vp, vp, C.c_int]
         L.oracle_tree_importance.restype = f32
         _LIB = L
@@ -308,6 +315,34 @@ def generate_rays(cam, uv, pixel=0, corrected=True):
     if r < 0:
         raise RuntimeError("generate_rays failed")
     return out[:, :3], out[:, 3:]
+
+
+def guide_forward(params, inputs):
+    """Guiding network (BF16 steps as the device): (n, 5) inputs -> (n, 129) bin probabilities + c."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 5)
+    out = np.zeros((len(x), 129), dtype=np.float32)
+    lib().oracle_guide_forward(_ptr(p), len(x), _ptr(x), _ptr(out))
+    return out
+
+
+def guide_gradient(params, records, rounded=True):
+    """Gradient of the guiding objective over (n, 8) records and the loss."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 8)
+    g = np.zeros(lib().oracle_guide_num_params(), dtype=np.float32)
+    loss = lib().oracle_guide_gradient(_ptr(p), len(r), _ptr(r), int(rounded), _ptr(g))
+    return g, float(loss)
+
+
+def guide_loss64(params, records):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 8)
+    return float(lib().oracle_guide_loss64(_ptr(p), len(r), _ptr(r)))
+
+
+def guide_num_params():
+    return int(lib().oracle_guide_num_params())
 
 
 def philox(ctr, key):

@@ -3705,6 +3705,189 @@ int oracle_env_pdf_n(oracle_scene s, uint32_t light, uint32_t n, const float *wi
   return 0;
 }
 
+// ---------------------------------------------------------------- learned guiding (checker)
+// The guiding network restated from its documented layout: 5 inputs in [0, 1] -> 80 frequency
+// features (feature f: input f / 16, octave (f % 16) / 2, sin(2^octave pi x) for even f, cos for
+// odd) -> 5 layers of 64 ReLU units -> 144 outputs (128 bin logits, the mixing logit at 128).
+// Parameters: W1 (64 x 80), W2..W5 (64 x 64), W6 (144 x 64) row-major in layer order, then the
+// biases. The objective for a record (inputs, bin k, BSDF pdf p_b, weight w) with s = softmax of
+// the 128 logits, q = s_k / (4 pi / 128), c = 0.05 + 0.85 sigmoid(z_128):
+//   loss = -sum (w / sum w) log(c q + (1 - c) p_b).
+// Two restatements: `bf16` rounds what the device rounds (features, weights, activations and the
+// output gradients to BF16, round to nearest even; products accumulated in double and rounded to
+// float, where the device accumulates in float); and an exact double-precision one whose
+// gradient is checked against finite differences of its loss.
+namespace {
+constexpr int GIn = 80, GHid = 64, GOut = 144, GBins = 128, GLayers = 6;
+constexpr double GBinSr = 4.0 * 3.14159265358979323846 / GBins;
+int GRows(int l) { return l == GLayers - 1 ? GOut : GHid; }
+int GCols(int l) { return l == 0 ? GIn : GHid; }
+int GWOff(int l) { int o = 0; for (int k = 0; k < l; ++k) o += GRows(k) * GCols(k); return o; }
+int GBOff(int l) { int o = 0; for (int k = 0; k < l; ++k) o += GRows(k); return o + GWOff(GLayers); }
+int GParams() { return GBOff(GLayers); }
+float Bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+double GFeature(const float *s, int f) {
+  const int sc = f >> 4, w = f & 15, oct = w >> 1;
+  const float t = Fract(s[sc] * (float)(1 << oct) * .5f);
+  const float a = t * PI2;
+  return (w & 1) ? std::cos(a) : std::sin(a);
+}
+// Forward of one query. x[l] = input of layer l (x[0]: features). rounded: the device's BF16 steps.
+void GForward(const float *P, const float *in, bool rounded, std::vector<double> x[GLayers], std::vector<double> &z) {
+  x[0].assign(GIn, 0.0);
+  for (int f = 0; f < GIn; ++f) {
+    const double v = GFeature(in, f);
+    x[0][f] = rounded ? Bf16((float)v) : v;
+  }
+  for (int l = 0; l < GLayers; ++l) {
+    const int R = GRows(l), Cc = GCols(l);
+    std::vector<double> y(R);
+    for (int i = 0; i < R; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < Cc; ++k) {
+        const double w = rounded ? Bf16(P[GWOff(l) + i * Cc + k]) : (double)P[GWOff(l) + i * Cc + k];
+        acc += w * x[l][k];
+      }
+      if (rounded) acc = (double)((float)acc + P[GBOff(l) + i]);
+      else acc += (double)P[GBOff(l) + i];
+      y[i] = acc;
+    }
+    if (l == GLayers - 1) { z = y; break; }
+    x[l + 1].assign(R, 0.0);
+    for (int i = 0; i < R; ++i) {
+      const double h = y[i] > 0 ? y[i] : 0.0;
+      x[l + 1][i] = rounded ? Bf16((float)h) : h;
+    }
+  }
+}
+// Gradient of the objective for n records (8 floats each) into g (GParams doubles); returns the loss.
+double GGradient(const float *P, uint32_t n, const float *rec, bool rounded, std::vector<double> &g) {
+  g.assign(GParams(), 0.0);
+  double sw = 0.0;
+  for (uint32_t r = 0; r < n; ++r) sw += rec[8 * r + 7];
+  const double invW = sw > 0 ? 1.0 / sw : 0.0;
+  double loss = 0.0;
+  std::vector<double> x[GLayers], z;
+  for (uint32_t r = 0; r < n; ++r) {
+    const float *R = rec + 8 * (size_t)r;
+    const double wn = R[7] * invW;
+    if (!(wn > 0)) continue;
+    const int kb = (int)R[5];
+    const double pb = R[6];
+    GForward(P, R, rounded, x, z);
+    double zmax = -INFINITY;
+    for (int j = 0; j < GBins; ++j) zmax = std::max(zmax, z[j]);
+    std::vector<double> e(GBins);
+    double sum = 0.0;
+    for (int j = 0; j < GBins; ++j) { e[j] = std::exp(z[j] - zmax); sum += e[j]; }
+    const double q = e[kb] / sum / GBinSr;
+    const double sg = 1.0 / (1.0 + std::exp(-z[GBins]));
+    const double c = 0.05 + 0.85 * sg;
+    const double pmix = c * q + (1.0 - c) * pb;
+    loss += -wn * std::log(pmix);
+    const double coef = -wn / pmix;
+    std::vector<double> dz(GOut, 0.0);
+    for (int j = 0; j < GBins; ++j) dz[j] = coef * c * q * ((j == kb ? 1.0 : 0.0) - e[j] / sum);
+    dz[GBins] = coef * (q - pb) * 0.85 * sg * (1.0 - sg);
+    if (rounded)
+      for (double &v : dz) v = Bf16((float)v);
+    for (int l = GLayers - 1; l >= 0; --l) {
+      const int Rr = GRows(l), Cc = GCols(l);
+      for (int i = 0; i < Rr; ++i) {
+        if (dz[i] == 0.0) continue;
+        for (int k = 0; k < Cc; ++k) g[GWOff(l) + i * Cc + k] += dz[i] * x[l][k];
+        g[GBOff(l) + i] += dz[i];
+      }
+      if (l == 0) break;
+      std::vector<double> dn(Cc, 0.0);
+      for (int k = 0; k < Cc; ++k) {
+        double acc = 0.0;
+        for (int i = 0; i < Rr; ++i) {
+          const double w = rounded ? Bf16(P[GWOff(l) + i * Cc + k]) : (double)P[GWOff(l) + i * Cc + k];
+          acc += w * dz[i];
+        }
+        const double v = x[l][k] > 0 ? acc : 0.0;
+        dn[k] = rounded ? Bf16((float)v) : v;
+      }
+      dz = dn;
+    }
+  }
+  return loss;
+}
+}  // namespace
+
+int oracle_guide_num_params(void) { return GParams(); }
+// Bin probabilities of the quantised table (bytes max(1, round(255 exp(z - z_max))), over their
+// sum) and the mixing probability, per query: 129 floats.
+void oracle_guide_forward(const float *params, uint32_t n, const float *inputs, float *out) {
+  std::vector<double> x[GLayers], z;
+  for (uint32_t i = 0; i < n; ++i) {
+    GForward(params, inputs + 5 * (size_t)i, true, x, z);
+    double zmax = -INFINITY;
+    for (int j = 0; j < GBins; ++j) zmax = std::max(zmax, z[j]);
+    float b[GBins], tot = 0.f;
+    for (int j = 0; j < GBins; ++j) {
+      const float e = (float)std::exp(z[j] - zmax);
+      uint32_t v = (uint32_t)(255.f * e + .5f);
+      v = v < 1u ? 1u : (v > 255u ? 255u : v);
+      b[j] = (float)v;
+      tot += b[j];
+    }
+    float *o = out + (size_t)(GBins + 1) * i;
+    for (int j = 0; j < GBins; ++j) o[j] = b[j] / tot;
+    o[GBins] = (float)(0.05 + 0.85 / (1.0 + std::exp(-z[GBins])));
+  }
+}
+// rounded = 1: the device's BF16 arithmetic; 0: exact (double). grad: GParams floats.
+float oracle_guide_gradient(const float *params, uint32_t n, const float *records, int rounded, float *grad) {
+  std::vector<double> g;
+  const double l = GGradient(params, n, records, rounded != 0, g);
+  for (int i = 0; i < GParams(); ++i) grad[i] = (float)g[i];
+  return (float)l;
+}
+// Loss of the exact restatement in double (for finite differences), parameters in double.
+double oracle_guide_loss64(const double *params, uint32_t n, const float *records) {
+  std::vector<float> P(GParams());
+  // exact evaluation needs double parameters: run the forward on a double copy
+  double sw = 0.0;
+  for (uint32_t r = 0; r < n; ++r) sw += records[8 * r + 7];
+  const double invW = sw > 0 ? 1.0 / sw : 0.0;
+  double loss = 0.0;
+  for (uint32_t r = 0; r < n; ++r) {
+    const float *R = records + 8 * (size_t)r;
+    const double wn = R[7] * invW;
+    if (!(wn > 0)) continue;
+    std::vector<double> x(GIn);
+    for (int f = 0; f < GIn; ++f) x[f] = GFeature(R, f);
+    for (int l = 0; l < GLayers; ++l) {
+      const int Rr = GRows(l), Cc = GCols(l);
+      std::vector<double> y(Rr);
+      for (int i = 0; i < Rr; ++i) {
+        double acc = params[GBOff(l) + i];
+        for (int k = 0; k < Cc; ++k) acc += params[GWOff(l) + i * Cc + k] * x[k];
+        y[i] = (l == GLayers - 1) ? acc : (acc > 0 ? acc : 0.0);
+      }
+      x = y;
+    }
+    double zmax = -INFINITY;
+    for (int j = 0; j < GBins; ++j) zmax = std::max(zmax, x[j]);
+    double sum = 0.0;
+    for (int j = 0; j < GBins; ++j) sum += std::exp(x[j] - zmax);
+    const double q = std::exp(x[(int)R[5]] - zmax) / sum / GBinSr;
+    const double c = 0.05 + 0.85 / (1.0 + std::exp(-x[GBins]));
+    loss += -wn * std::log(c * q + (1.0 - c) * R[6]);
+  }
+  return loss;
+}
+
 // TreeImportance of one node (test hook): node[] = center xyz, diagonal, axis xyz, thetaO, thetaE,
 // power. The node's box is the cube of that diagonal around the center.
 float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected) {

@@ -94,6 +94,13 @@ int oracle_light_sample_li_n(oracle_scene s, uint32_t light, const float p[3], c
                              int mode, const float *lambda, int corrected, uint32_t pixel, float *out);
 /* EnvironmentLight::PDF_Li for n world directions. */
 int oracle_env_pdf_n(oracle_scene s, uint32_t light, uint32_t n, const float *wi, float *out);
+/* Learned guiding network (restated from its documented layout): parameter count, quantised
+ * forward (129 floats per query), objective gradient (rounded = 1: the device's BF16 steps;
+ * 0: exact) returning the loss, and the exact loss with double parameters. */
+int oracle_guide_num_params(void);
+void oracle_guide_forward(const float *params, uint32_t n, const float *inputs, float *out);
+float oracle_guide_gradient(const float *params, uint32_t n, const float *records, int rounded, float *grad);
+double oracle_guide_loss64(const double *params, uint32_t n, const float *records);
 float oracle_tree_importance(const float node[10], const float p[3], const float n[3], int corrected);
 float oracle_hg_sample(float g, const float wo[3], float u0, float u1, int corrected, float wi[3]);
 

@@ -33,10 +33,11 @@ def test_abi_version():
 def test_struct_sizes_match_header_layout():
     # descriptors are passed by pointer; their sizes must match the C layout
     assert C.sizeof(abi.Node) == 4 + 16 + 48
-    assert C.sizeof(abi.RenderStats) == 5 * 8 + 5 * 8 + 4 * 4 + 3 * 8
+    assert C.sizeof(abi.RenderStats) == 5 * 8 + 5 * 8 + 4 * 4 + 3 * 8 + 3 * 8
     assert C.sizeof(abi.Camera) == 4 + 12 * 4 + 3 * 4 + 4 + 4 + 4 + 8
-    # scene, film, camera, 10 words, utility-pass fields (6 words), padded to 8
-    assert C.sizeof(abi.RenderDesc) == 8 + 8 + C.sizeof(abi.Camera) + 10 * 4 + 6 * 4 + 4
+    # scene, film, camera, 10 words, utility-pass fields (6 words),
+    # ... padding to 8, the guide handle, the training-step count, padded to 8
+    assert C.sizeof(abi.RenderDesc) == 8 + 8 + C.sizeof(abi.Camera) + 10 * 4 + 6 * 4 + 4 + 8 + 4 + 4
 
 
 def test_camera_helpers_match_python_builder():
