@@ -119,9 +119,20 @@ def main():
     ap.add_argument("--save-image", default="")
     ap.add_argument("--relmse", action="store_true", help="measure time-to-relMSE <= 1e-3 instead")
     ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--ref-guided", action="store_true",
+                    help="with --relmse: render the reference with the guided estimator (its own guide, trained "
+                         "online in the reference's first 256 spp): lower variance than a plain reference")
     ap.add_argument("--guided", action="store_true",
                     help="with --relmse: also measure the guided estimator (online-trained guiding network)")
-    ap.add_argument("--guide-steps", type=int, default=16, help="Adam steps after each guided wave")
+    ap.add_argument("--guide-steps", type=int, default=16,
+                    help="Adam steps after each guided wave of the warm-up (waves of 8 spp)")
+    ap.add_argument("--guide-warm-spp", type=int, default=256,
+                    help="guided samples per pixel rendered in 8-spp waves with --guide-steps of training each")
+    ap.add_argument("--guide-late-steps", type=int, default=4,
+                    help="Adam steps after each later guided wave (waves of --guide-late-wave spp)")
+    ap.add_argument("--guide-late-wave", type=int, default=32)
+    ap.add_argument("--guide-depth", default="0",
+                    help="guide vertices below this path depth only (0: all); a comma-separated list measures each")
     args = ap.parse_args()
     if not args.steps:
         args.steps = DEFAULT_STEPS[args.workload]
@@ -260,9 +271,21 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     ref = gpu.Film(dev, W, H)
     chunk = 256
     t0 = time.perf_counter()
-    for b in range(0, args.ref_spp, chunk):
-        gpu.render(desc(b, min(args.ref_spp, b + chunk), f=ref, seed=1))
-        log("reference %d / %d spp (%.0f s)" % (min(args.ref_spp, b + chunk), args.ref_spp, time.perf_counter() - t0))
+    rguide = gpu.Guide(dev, seed=101) if args.ref_guided else None
+    b = 0
+    while b < args.ref_spp:
+        if rguide is not None:
+            per = 8 if b < 256 else 64
+            e = min(args.ref_spp, b + per)
+            gpu.render(desc(b, e, f=ref, seed=1, guide=rguide, steps=16 if b < 256 else 2, capacity=W * H * per))
+        else:
+            e = min(args.ref_spp, b + chunk)
+            gpu.render(desc(b, e, f=ref, seed=1))
+        b = e
+        if b % 256 == 0 or b == args.ref_spp:
+            log("reference %d / %d spp (%.0f s)" % (b, args.ref_spp, time.perf_counter() - t0))
+    if rguide is not None:
+        rguide.close()
     ref_s = time.perf_counter() - t0
     rsum, rcnt = ref.read()
     r = rsum / rcnt[..., None]
@@ -272,24 +295,36 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
         film.clear()
         checkpoints, hit = [], None
         spent, done, k = 0.0, 0, 0
+        kern = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_guide_infer": 0.0, "ms_guide_train": 0.0}
+
+        def tally(st):
+            for key in kern:
+                kern[key] += getattr(st, key)
+
         while (1 << k) <= args.ref_spp // 2:
             target = 1 << k
             t = time.perf_counter()
             if guide is None:
-                gpu.render(desc(done, target))
+                tally(gpu.render(desc(done, target)))
             else:
-                # waves of at most 8 spp so the guide trains between them (online)
-                per = min(target - done, 8)
-                cap = W * H * per
-                for b in range(done, target, per):
-                    gpu.render(desc(b, min(target, b + per), guide=guide, steps=args.guide_steps, capacity=cap))
+                # online training between waves: 8-spp waves with --guide-steps each during the
+                # warm-up, then larger waves with --guide-late-steps
+                b = done
+                while b < target:
+                    warm = b < args.guide_warm_spp
+                    per = min(target - b, 8 if warm else args.guide_late_wave)
+                    steps = args.guide_steps if warm else args.guide_late_steps
+                    tally(gpu.render(desc(b, b + per, guide=guide, steps=steps, capacity=W * H * per)))
+                    b += per
             spent += time.perf_counter() - t
             done = target
             xs, xc = film.read()
             x = xs / xc[..., None]
             rel = float(np.mean((x - r) ** 2 / (r * r + 0.01)))
-            checkpoints.append({"spp": target, "seconds": round(spent, 4), "relmse": rel})
-            log("%s spp %d: %.3f s, relMSE %.3e" % ("guided" if guide else "plain", target, spent, rel))
+            checkpoints.append({"spp": target, "seconds": round(spent, 4), "relmse": rel,
+                                "kernel_ms": {key: round(v, 1) for key, v in kern.items()}})
+            log("%s spp %d: %.3f s, relMSE %.3e; ms %s" % ("guided" if guide else "plain", target, spent, rel,
+                                                            {key: round(v, 1) for key, v in kern.items()}))
             if rel <= 1e-3:
                 hit = checkpoints[-1]
                 break
@@ -299,12 +334,16 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     checkpoints, hit = progressive()
     guided = None
     if args.guided:
-        g = gpu.Guide(dev, seed=7)
-        gck, ghit = progressive(g)
-        guided = {"value": ghit["seconds"] if ghit else None, "spp_at_target": ghit["spp"] if ghit else None,
-                  "checkpoints": gck, "guide": g.stats(), "guide_train_steps_per_wave": args.guide_steps,
-                  "spp_per_wave": 8}
-        g.close()
+        guided = []
+        for depth in [int(x) for x in args.guide_depth.split(",")]:
+            g = gpu.Guide(dev, seed=7, max_depth=depth)
+            gck, ghit = progressive(g)
+            guided.append({"value": ghit["seconds"] if ghit else None, "spp_at_target": ghit["spp"] if ghit else None,
+                           "checkpoints": gck, "guide": g.stats(),
+                           "schedule": {"warm_spp": args.guide_warm_spp, "warm_wave_spp": 8,
+                                        "warm_steps": args.guide_steps, "late_wave_spp": args.guide_late_wave,
+                                        "late_steps": args.guide_late_steps, "guided_depth": depth or None}})
+            g.close()
     # CPU validation of the reference on a 64x64 centre crop
     valid = None
     if not args.no_cpu:
@@ -333,7 +372,8 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             "value": hit["seconds"] if hit else None, "unit": "s", "n_gpus": 1, "higher_is_better": False,
             "spp_at_target": hit["spp"] if hit else None, "dtype": "f32", "data": "synthetic (procedural scene)",
             "config": dict(cfg, spectral=args.spectral, estimator=args.estimator, reference_spp=args.ref_spp,
-                           reference_seed=1, reference_s=round(ref_s, 1)),
+                           reference_seed=1, reference_s=round(ref_s, 1),
+                           reference_estimator="guided" if args.ref_guided else "plain"),
             "checkpoints": checkpoints, "reference_check": valid, "guided": guided}
     print(json.dumps(line), flush=True)
     ref.close()

@@ -362,6 +362,7 @@ typedef struct lgpu_guide_desc {
   uint32_t record_capacity; /* training records kept per wave (0: 4194304) */
   uint32_t batch;           /* records per Adam step (0: 16384) */
   float learning_rate;      /* Adam step size (0: 0.002) */
+  uint32_t max_depth;       /* guide vertices of path depth below this only (0: every depth) */
 } lgpu_guide_desc;
 lgpu_status lgpu_guide_create(lgpu_device dev, const lgpu_guide_desc *desc, lgpu_guide *out);
 lgpu_status lgpu_guide_destroy(lgpu_guide guide);

@@ -91,7 +91,7 @@ class RenderDesc(C.Structure):
 
 class GuideDesc(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("record_capacity", C.c_uint32), ("batch", C.c_uint32),
-                ("learning_rate", C.c_float)]
+                ("learning_rate", C.c_float), ("max_depth", C.c_uint32)]
 
 RENDER_COUNT_TRAVERSAL = 1
 

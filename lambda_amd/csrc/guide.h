@@ -55,14 +55,22 @@ LGPU_HD uint16_t ToBf16(float f) {
 }
 LGPU_HD float FromBf16(uint16_t h) { return AsF32((uint32_t)h << 16); }
 
-// Feature f of the encoding (f < 80): scalar f / 16, octave (f % 16) / 2, sin for even f.
-// sin(2 pi t) with t = fract(2^(octave - 1) s): the argument is reduced exactly.
-LGPU_HD float GuideFeature(const float s[kGuideScalars], int f) {
-  const int sc = f >> 4, w = f & 15, oct = w >> 1;
-  const float t = Fract(s[sc] * (float)(1 << oct) * .5f);
+// Feature w (< 16) of one input x: octave w / 2, sin for even w, cos for odd: sin(2 pi t) with
+// t = fract(2^(octave - 1) x), the argument reduced exactly. Feature f (< 80) of the encoding is
+// feature f % 16 of input f / 16.
+LGPU_HD float GuideFeatureOf(float x, int w) {
+  const int oct = w >> 1;
+  const float t = Fract(x * (float)(1 << oct) * .5f);
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the hardware sine and cosine take revolutions (sin(2 pi t)), one instruction each; their
+  // error (about 1e-6) is far below the BF16 rounding the features go through
+  return (w & 1) ? __builtin_amdgcn_cosf(t) : __builtin_amdgcn_sinf(t);
+#else
   const float a = t * kPi2;
   return (w & 1) ? cosf(a) : sinf(a);
+#endif
 }
+LGPU_HD float GuideFeature(const float s[kGuideScalars], int f) { return GuideFeatureOf(s[f >> 4], f & 15); }
 
 // Network inputs of a shading point: position in the scene bounds, wo = -ray direction.
 LGPU_HD void GuideInputs(V3 p, V3 wo, const float bmin[3], const float binv[3], float s[kGuideScalars]) {
@@ -111,6 +119,7 @@ struct GuideRefs {
   uint32_t *rec_count;    // records appended (may pass rec_cap; only the first rec_cap are written)
   uint32_t rec_cap;
   uint32_t rec_threshold; // record a vertex when its hash is below this (uint32 scale)
+  uint32_t max_depth;     // guided vertices have path depth below this
   float bmin[3], binv[3];
 };
 

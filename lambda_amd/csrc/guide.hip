@@ -23,6 +23,17 @@ typedef short Bf4 __attribute__((ext_vector_type(4)));
 typedef float F4v __attribute__((ext_vector_type(4)));
 
 __device__ inline F4v Mfma(Bf4 a, Bf4 b, F4v c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// gfx950's K = 32 form (twice the work per instruction). Lane group g supplies K positions
+// 8 g .. 8 g + 7; the kernels below map them to 4 g .. 4 g + 3 of two 16-wide K tiles, the same
+// mapping for A and B, so the product is unchanged and each layer's D tiles still feed the next
+// layer as they are.
+typedef __bf16 Bf8v __attribute__((ext_vector_type(8)));
+typedef short S8 __attribute__((ext_vector_type(8)));
+__device__ inline F4v Mfma32(Bf4 a0, Bf4 a1, Bf4 b0, Bf4 b1, F4v c) {
+  const S8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  const S8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(Bf8v, a), __builtin_bit_cast(Bf8v, b), c, 0, 0, 0);
+}
 __device__ inline Bf4 Pack4(float a, float b, float c, float d) {
   Bf4 r;
   r[0] = (short)ToBf16(a); r[1] = (short)ToBf16(b); r[2] = (short)ToBf16(c); r[3] = (short)ToBf16(d);
@@ -43,63 +54,150 @@ __device__ inline Bf4 FromU2(uint2 u) {
   r[0] = (short)(u.x & 0xFFFFu); r[1] = (short)(u.x >> 16); r[2] = (short)(u.y & 0xFFFFu); r[3] = (short)(u.y >> 16);
   return r;
 }
+// Register-resident access to tile pt of a 4-tile row with a run-time pt (selects, no indexing).
+__device__ inline Bf4 Get4(const Bf4 v[4], int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3])); }
+__device__ inline void Put4(Bf4 v[4], int i, Bf4 a) {
+  v[0] = i == 0 ? a : v[0]; v[1] = i == 1 ? a : v[1]; v[2] = i == 2 ? a : v[2]; v[3] = i == 3 ? a : v[3];
+}
 __device__ inline float Lane4Sum(float v) { v += __shfl_xor(v, 16); v += __shfl_xor(v, 32); return v; }
 __device__ inline float Lane4Max(float v) { v = fmaxf(v, __shfl_xor(v, 16)); v = fmaxf(v, __shfl_xor(v, 32)); return v; }
 
 // Encoded inputs of one query as this lane's B operand of layer 0: features 16 kt + 4 g + r.
+// (K tile kt holds the 16 features of input kt: octaves 2 g and 2 g + 1, sin and cos.)
 __device__ inline void EncodeB(const float s[kGuideScalars], int g, Bf4 x[5]) {
 #pragma unroll
   for (int kt = 0; kt < 5; ++kt)
-    x[kt] = Pack4(GuideFeature(s, 16 * kt + 4 * g), GuideFeature(s, 16 * kt + 4 * g + 1),
-                  GuideFeature(s, 16 * kt + 4 * g + 2), GuideFeature(s, 16 * kt + 4 * g + 3));
+    x[kt] = Pack4(GuideFeatureOf(s[kt], 4 * g), GuideFeatureOf(s[kt], 4 * g + 1), GuideFeatureOf(s[kt], 4 * g + 2),
+                  GuideFeatureOf(s[kt], 4 * g + 3));
 }
 
-// Hidden layers 0..4 of one wave's 64 queries (4 column tiles): x[kt][pt] in, result in x (the
-// next layer's B operands). W: BF16 weights (LDS or global), B: FP32 biases. `store`: per-lane
-// store of each layer's input (training), layout [slot][pt][lane] with 5 slots for layer 0
-// and 4 per hidden layer.
+// Hidden layer L of one wave's 64 queries (4 column tiles): x[kt][pt] in, result in x (the next
+// layer's B operands). W: BF16 weights (LDS or global), B: FP32 biases. `store`: per-lane store
+// of each layer's input (training), layout [slot][pt][lane] with 5 slots for layer 0 and 4 per
+// hidden layer. Compile-time layer shapes keep every operand array in registers.
+template <int L, bool kStore>
+__device__ inline void HiddenLayer(const uint16_t *W, const float *B, Bf4 x[5][4], uint2 *store) {
+  constexpr int C = GuideCols(L), nk = C / 16;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  const uint16_t *Wl = W + GuideWOff(L);
+  const float *Bl = B + GuideBOff(L);
+  F4v acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt) acc[m][pt] = F4v{0.f, 0.f, 0.f, 0.f};
+  // K tiles in a rolled loop (operands picked by selects): the scheduler then keeps one K tile's
+  // weights in flight instead of hoisting the whole layer's loads into registers
+#pragma unroll 1
+  for (int kt = 0; kt < nk; ++kt) {
+    Bf4 xb[4];
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt)
+      xb[pt] = kt == 0 ? x[0][pt] : (kt == 1 ? x[1][pt] : (kt == 2 ? x[2][pt] : (kt == 3 ? x[3][pt] : x[4][pt])));
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const Bf4 a = LoadBf4(Wl + (16 * m + c16) * C + 16 * kt + 4 * g);
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) acc[m][pt] = Mfma(a, xb[pt], acc[m][pt]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const float b0 = Bl[16 * m + 4 * g], b1 = Bl[16 * m + 4 * g + 1], b2 = Bl[16 * m + 4 * g + 2],
+                b3 = Bl[16 * m + 4 * g + 3];
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt)
+      x[m][pt] = Pack4(fmaxf(acc[m][pt][0] + b0, 0.f), fmaxf(acc[m][pt][1] + b1, 0.f), fmaxf(acc[m][pt][2] + b2, 0.f),
+                       fmaxf(acc[m][pt][3] + b3, 0.f));
+  }
+  if (kStore) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) store[((5 + 4 * L + m) * 4 + pt) * 64 + lane] = AsU2(x[m][pt]);
+  }
+}
 template <bool kStore>
 __device__ inline void HiddenLayers(const uint16_t *W, const float *B, Bf4 x[5][4], uint2 *store) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
   if (kStore) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int kt = 0; kt < 5; ++kt)
 #pragma unroll
       for (int pt = 0; pt < 4; ++pt) store[(kt * 4 + pt) * 64 + lane] = AsU2(x[kt][pt]);
   }
+  HiddenLayer<0, kStore>(W, B, x, store);
+  HiddenLayer<1, kStore>(W, B, x, store);
+  HiddenLayer<2, kStore>(W, B, x, store);
+  HiddenLayer<3, kStore>(W, B, x, store);
+  HiddenLayer<4, kStore>(W, B, x, store);
+}
+
+// Inference: a wave handles kInferPT column tiles (16 kInferPT queries), a block 4 waves.
+constexpr int kInferPT = 4;
+
+// Inference form of a hidden layer: K tiles in pairs through the K = 32 MFMA (layer 0's fifth
+// tile through the K = 16 one); same result layout as HiddenLayer.
+template <int L>
+__device__ inline void HiddenLayer32(const uint16_t *W, const float *B, Bf4 x[5][kInferPT]) {
+  constexpr int C = GuideCols(L), nk = C / 16;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  const uint16_t *Wl = W + GuideWOff(L);
+  const float *Bl = B + GuideBOff(L);
+  F4v acc[4][kInferPT];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = F4v{0.f, 0.f, 0.f, 0.f};
+  // K pairs in a rolled loop (operands picked by selects): one pair's weights in flight at a time
 #pragma unroll 1
-  for (int l = 0; l < kGuideLayers - 1; ++l) {
-    const int C = GuideCols(l), nk = C / 16;
-    const uint16_t *Wl = W + GuideWOff(l);
-    const float *Bl = B + GuideBOff(l);
-    F4v acc[4][4];
+  for (int j = 0; j < nk / 2; ++j) {
+    Bf4 x0[kInferPT], x1[kInferPT];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int pt = 0; pt < 4; ++pt) acc[m][pt] = F4v{0.f, 0.f, 0.f, 0.f};
-    for (int kt = 0; kt < nk; ++kt) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const Bf4 a = LoadBf4(Wl + (16 * m + c16) * C + 16 * kt + 4 * g);
-#pragma unroll
-        for (int pt = 0; pt < 4; ++pt) acc[m][pt] = Mfma(a, x[kt][pt], acc[m][pt]);
-      }
+    for (int pt = 0; pt < kInferPT; ++pt) {
+      x0[pt] = j == 0 ? x[0][pt] : x[2][pt];
+      x1[pt] = j == 0 ? x[1][pt] : x[3][pt];
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const float b0 = Bl[16 * m + 4 * g], b1 = Bl[16 * m + 4 * g + 1], b2 = Bl[16 * m + 4 * g + 2],
-                  b3 = Bl[16 * m + 4 * g + 3];
+      const uint16_t *row = Wl + (16 * m + c16) * C + 32 * j + 4 * g;
+      const Bf4 a0 = LoadBf4(row), a1 = LoadBf4(row + 16);
 #pragma unroll
-      for (int pt = 0; pt < 4; ++pt)
-        x[m][pt] = Pack4(fmaxf(acc[m][pt][0] + b0, 0.f), fmaxf(acc[m][pt][1] + b1, 0.f), fmaxf(acc[m][pt][2] + b2, 0.f),
-                         fmaxf(acc[m][pt][3] + b3, 0.f));
+      for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = Mfma32(a0, a1, x0[pt], x1[pt], acc[m][pt]);
     }
-    if (kStore) {
+  }
+  if constexpr (nk & 1) {
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+      const Bf4 a = LoadBf4(Wl + (16 * m + c16) * C + 16 * (nk - 1) + 4 * g);
 #pragma unroll
-        for (int pt = 0; pt < 4; ++pt) store[((5 + 4 * l + m) * 4 + pt) * 64 + lane] = AsU2(x[m][pt]);
+      for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = Mfma(a, x[nk - 1][pt], acc[m][pt]);
     }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const float b0 = Bl[16 * m + 4 * g], b1 = Bl[16 * m + 4 * g + 1], b2 = Bl[16 * m + 4 * g + 2],
+                b3 = Bl[16 * m + 4 * g + 3];
+#pragma unroll
+    for (int pt = 0; pt < kInferPT; ++pt)
+      x[m][pt] = Pack4(fmaxf(acc[m][pt][0] + b0, 0.f), fmaxf(acc[m][pt][1] + b1, 0.f), fmaxf(acc[m][pt][2] + b2, 0.f),
+                       fmaxf(acc[m][pt][3] + b3, 0.f));
+  }
+}
+__device__ inline void OutputLayer32(const uint16_t *W, const float *B, const Bf4 b0, const Bf4 b1, const Bf4 b2,
+                                     const Bf4 b3, F4v z[9]) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  const uint16_t *Wl = W + GuideWOff(kGuideLayers - 1);
+  const float *Bl = B + GuideBOff(kGuideLayers - 1);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const uint16_t *row = Wl + (16 * m + c16) * kGuideHidden + 4 * g;
+    F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
+    acc = Mfma32(LoadBf4(row), LoadBf4(row + 16), b0, b1, acc);
+    acc = Mfma32(LoadBf4(row + 32), LoadBf4(row + 48), b2, b3, acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] += Bl[16 * m + 4 * g + r];
+    z[m] = acc;
   }
 }
 
@@ -112,16 +210,18 @@ __device__ inline void OutputLayer(const uint16_t *W, const float *B, const Bf4 
   for (int m = 0; m < 9; ++m) {
     F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = Mfma(LoadBf4(Wl + (16 * m + c16) * kGuideHidden + 16 * kt + 4 * g), x[kt][pt], acc);
+    for (int kt = 0; kt < 4; ++kt) acc = Mfma(LoadBf4(Wl + (16 * m + c16) * kGuideHidden + 16 * kt + 4 * g), Get4(x[kt], pt), acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] += Bl[16 * m + 4 * g + r];
     z[m] = acc;
   }
 }
 
-// Query inputs: from path state (render) or from an array of scalars (test hook).
+// Query inputs: from path state (render: query q is queue slot list[q]) or from an array of
+// scalars (test hook: query q is slot q).
 struct GuideQuerySrc {
   const float4 *ray_o, *ray_d, *hit;  // path state of the current queue
+  const uint32_t *list;               // queue slots to infer (render)
   const float *scalars;               // or 5 scalars per query
   float bmin[3], binv[3];
 };
@@ -131,6 +231,7 @@ __device__ inline void QueryScalars(const GuideQuerySrc &src, uint32_t q, float 
     for (int k = 0; k < kGuideScalars; ++k) s[k] = src.scalars[kGuideScalars * (size_t)q + k];
     return;
   }
+  q = src.list[q];
   const float4 h = src.hit[q];
   if (__float_as_uint(h.w) == kNone) {
 #pragma unroll
@@ -139,43 +240,80 @@ __device__ inline void QueryScalars(const GuideQuerySrc &src, uint32_t q, float 
   }
   const float4 o = src.ray_o[q], d = src.ray_d[q];
   const V3 p = v3(o.x + d.x * h.x, o.y + d.y * h.x, o.z + d.z * h.x);
-  GuideInputs(p, v3(-d.x, -d.y, -d.z), src.bmin, src.binv, s);
+  // local copies: the kernel argument's address would otherwise reach the callee (scratch copy)
+  const float bmin[3] = {src.bmin[0], src.bmin[1], src.bmin[2]}, binv[3] = {src.binv[0], src.binv[1], src.binv[2]};
+  GuideInputs(p, v3(-d.x, -d.y, -d.z), bmin, binv, s);
 }
 
-constexpr int kInferBlock = 256;
+constexpr int kInferBlock = 256;  // 4 waves of 16 kInferPT queries
+
+// The queue slots whose next shading is a guided scattering event: a hit on a guidable material
+// by a path that is not terminating (kFlagEmitOnly gathers emission only). One atomic per wave.
+__global__ void k_guide_list(const BvhTri *__restrict__ tris, const float4 *__restrict__ hit, const uint4 *__restrict__ ids,
+                             uint32_t n, const uint32_t *np, uint32_t maxDepth, uint32_t *list, uint32_t *count) {
+  if (np) n = *np;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  if (i < n) {
+    const uint32_t bi = __float_as_uint(hit[i].w);
+    const uint32_t w = ids[i].w;
+    take = bi != kNone && (tris[bi].key & kKeyGuidable) && !(w & kFlagEmitOnly) && (w >> 8) < maxDepth;
+  }
+  const uint64_t mask = __ballot(take);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  uint32_t base = 0;
+  if (lane == __ffsll((unsigned long long)mask) - 1) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, __ffsll((unsigned long long)mask) - 1);
+  if (take) list[base + before] = i;
+}
 
 // Inference: per query, the quantised bin table (bytes max(1, round(255 exp(z - z_max))), the
 // distribution is byte / sum) and the mixing probability. Persistent blocks; weights in LDS.
 __global__ void __launch_bounds__(kInferBlock) k_guide_infer(const uint16_t *__restrict__ Wg, const float *__restrict__ Bg,
-                                                            GuideQuerySrc src, uint32_t n, const uint32_t *np,
-                                                            uint32_t *__restrict__ bins, float2 *__restrict__ hdr) {
+                                                               GuideQuerySrc src, uint32_t n, const uint32_t *np,
+                                                               uint32_t *__restrict__ bins, float2 *__restrict__ hdr) {
   __shared__ __align__(16) uint16_t W[kGuideWeights];
   __shared__ float B[kGuideBiases];
+  constexpr int kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
   if (np) n = *np;
-  if ((uint64_t)blockIdx.x * kInferBlock >= n) return;
+  if ((uint64_t)blockIdx.x * kPerBlock >= n) return;
   for (int i = threadIdx.x; i < kGuideWeights / 8; i += kInferBlock)
     reinterpret_cast<uint4 *>(W)[i] = reinterpret_cast<const uint4 *>(Wg)[i];
   for (int i = threadIdx.x; i < kGuideBiases; i += kInferBlock) B[i] = Bg[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15, wave = threadIdx.x >> 6;
-  for (uint64_t base = (uint64_t)blockIdx.x * kInferBlock; base < n; base += (uint64_t)gridDim.x * kInferBlock) {
-    const uint64_t wbase = base + 64 * wave;
-    Bf4 x[5][4];
+  for (uint64_t base = (uint64_t)blockIdx.x * kPerBlock; base < n; base += (uint64_t)gridDim.x * kPerBlock) {
+    const uint64_t wbase = base + 16 * kInferPT * wave;
+    // gather every tile's inputs first, so the dependent loads of the tiles overlap
+    float s[kInferPT][kGuideScalars];
 #pragma unroll
-    for (int pt = 0; pt < 4; ++pt) {
+    for (int pt = 0; pt < kInferPT; ++pt) {
       const uint64_t q = wbase + 16 * pt + c16;
-      float s[kGuideScalars];
-      QueryScalars(src, (uint32_t)(q < n ? q : n - 1), s);
-      Bf4 e[5];
-      EncodeB(s, g, e);
-#pragma unroll
-      for (int kt = 0; kt < 5; ++kt) x[kt][pt] = e[kt];
+      QueryScalars(src, (uint32_t)(q < n ? q : n - 1), s[pt]);
     }
-    HiddenLayers<false>(W, B, x, nullptr);
+    Bf4 x[5][kInferPT];
 #pragma unroll 1
-    for (int pt = 0; pt < 4; ++pt) {
+    for (int pt = 0; pt < kInferPT; ++pt) {
+      float sp[kGuideScalars];
+#pragma unroll
+      for (int k = 0; k < kGuideScalars; ++k)
+        sp[k] = pt == 0 ? s[0][k] : (pt == 1 ? s[1][k] : (pt == 2 ? s[2][k] : s[3][k]));
+      Bf4 e[5];
+      EncodeB(sp, g, e);
+#pragma unroll
+      for (int kt = 0; kt < 5; ++kt) Put4(x[kt], pt, e[kt]);
+    }
+    HiddenLayer32<0>(W, B, x);
+    HiddenLayer32<1>(W, B, x);
+    HiddenLayer32<2>(W, B, x);
+    HiddenLayer32<3>(W, B, x);
+    HiddenLayer32<4>(W, B, x);
+#pragma unroll 1
+    for (int pt = 0; pt < kInferPT; ++pt) {
       F4v z[9];
-      OutputLayer(W, B, x, pt, z);
+      OutputLayer32(W, B, Get4(x[0], pt), Get4(x[1], pt), Get4(x[2], pt), Get4(x[3], pt), z);
       float zmax = -INFINITY;
 #pragma unroll
       for (int m = 0; m < 8; ++m)
@@ -201,9 +339,11 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer(const uint16_t *__r
       const float zc = Lane4Sum(g == 0 ? z[8][0] : 0.f);
       const uint64_t q = wbase + 16 * pt + c16;
       if (q < n) {
+        const uint64_t slot = src.list ? src.list[q] : q;
+        // lane group g holds bytes 16 m + 4 g .. + 3: words 4 m + g
 #pragma unroll
-        for (int m = 0; m < 8; ++m) bins[q * 32 + 4 * m + g] = words[m];
-        if (g == 0) hdr[q] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), tot);
+        for (int m = 0; m < 8; ++m) bins[slot * 32 + 4 * m + g] = words[m];
+        if (g == 0) hdr[slot] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), tot);
       }
     }
   }
@@ -236,6 +376,82 @@ __global__ void k_guide_batch(const GuideRecord *rec, const uint32_t *count, uin
   if (n) atomicAdd(sumW, rec[k].c.z);
 }
 
+// Backward step of layer L for one wave: dW = dZ X^T and db = sum dZ added to grad (the inner
+// dimension, the 64 queries, transposed through LDS), then dZ of layer L - 1 = (W^T dZ) masked by
+// the ReLU of layer L's input, in place in dz (D layout, BF16).
+template <int L>
+__device__ inline void BackLayer(const uint16_t *W, const uint2 *st, uint16_t *dzt, uint16_t *xt, Bf4 dz[9][4],
+                                 float *grad) {
+  constexpr int R = GuideRows(L), C = GuideCols(L), nm = R / 16, nk = C / 16;
+  constexpr int slot0 = L == 0 ? 0 : 5 + 4 * (L - 1);
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+#pragma unroll
+  for (int kt = 0; kt < nk; ++kt)
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt) {
+      const Bf4 v = FromU2(st[((slot0 + kt) * 4 + pt) * 64 + lane]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xt[(16 * kt + 4 * g + r) * 64 + 16 * pt + c16] = (uint16_t)v[r];
+    }
+#pragma unroll
+  for (int m = 0; m < nm; ++m)
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dzt[(16 * m + 4 * g + r) * 64 + 16 * pt + c16] = (uint16_t)dz[m][pt][r];
+  __syncthreads();
+  float *gW = grad + GuideWOff(L);
+#pragma unroll 1
+  for (int m = 0; m < nm; ++m)
+#pragma unroll
+    for (int nt = 0; nt < nk; ++nt) {
+      F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kp = 0; kp < 4; ++kp)
+        acc = Mfma(LoadBf4(dzt + (16 * m + c16) * 64 + 16 * kp + 4 * g), LoadBf4(xt + (16 * nt + c16) * 64 + 16 * kp + 4 * g),
+                   acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) unsafeAtomicAdd(&gW[(16 * m + 4 * g + r) * C + 16 * nt + c16], acc[r]);
+    }
+  float *gB = grad + kGuideWeights + GuideBOff(L);
+  for (int row = lane; row < R; row += 64) {
+    float s = 0.f;
+    for (int k = 0; k < 64; ++k) s += FromBf16(dzt[row * 64 + k]);
+    unsafeAtomicAdd(&gB[row], s);
+  }
+  __syncthreads();
+  if constexpr (L > 0) {
+    const uint16_t *Wl = W + GuideWOff(L);
+    Bf4 nd[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      F4v acc[4];
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) acc[pt] = F4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < nm; ++kt) {
+        Bf4 a;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = (short)Wl[(16 * kt + 4 * g + r) * C + 16 * m + c16];
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) acc[pt] = Mfma(a, dz[kt][pt], acc[pt]);
+      }
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) {
+        const Bf4 xin = FromU2(st[((slot0 + m) * 4 + pt) * 64 + lane]);
+        float d[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d[r] = FromBf16((uint16_t)xin[r]) > 0.f ? acc[pt][r] : 0.f;
+        nd[m][pt] = Pack4(d[0], d[1], d[2], d[3]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) dz[m][pt] = nd[m][pt];
+  }
+}
+
 constexpr int kTrainBlock = 64;  // one wave: the LDS transposes are per wave
 
 // Forward and backward of the guiding objective on 64 records per block:
@@ -253,24 +469,18 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_grad(const uint16_t *__re
   const float invW = *sumW > 0.f ? 1.f / *sumW : 0.f;
   // ---- forward
   Bf4 x[5][4];
-  float pb[4], wn[4];
-  uint32_t kb[4];
-#pragma unroll
+#pragma unroll 1
   for (int pt = 0; pt < 4; ++pt) {
     const uint32_t j = base + 16 * pt + c16;
     float s[kGuideScalars] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    pb[pt] = 1.f; wn[pt] = 0.f; kb[pt] = 0u;
     if (j < batch) {
       const GuideRecord r = rec[idx[j]];
       s[0] = r.a.x; s[1] = r.a.y; s[2] = r.a.z; s[3] = r.a.w; s[4] = r.b.x;
-      kb[pt] = __float_as_uint(r.b.y);
-      pb[pt] = r.b.z;
-      wn[pt] = r.c.z * invW;
     }
     Bf4 e[5];
     EncodeB(s, g, e);
 #pragma unroll
-    for (int kt = 0; kt < 5; ++kt) x[kt][pt] = e[kt];
+    for (int kt = 0; kt < 5; ++kt) Put4(x[kt], pt, e[kt]);
   }
   HiddenLayers<true>(W, B, x, st);
   // ---- output layer and dZ (bf16, D layout) of the objective
@@ -278,6 +488,15 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_grad(const uint16_t *__re
   float lsum = 0.f;
 #pragma unroll 1
   for (int pt = 0; pt < 4; ++pt) {
+    const uint32_t j = base + 16 * pt + c16;
+    float pbv = 1.f, wnv = 0.f;
+    uint32_t kbv = 0u;
+    if (j < batch) {
+      const GuideRecord r = rec[idx[j]];
+      kbv = __float_as_uint(r.b.y);
+      pbv = r.b.z;
+      wnv = r.c.z * invW;
+    }
     F4v z[9];
     OutputLayer(W, B, x, pt, z);
     float zmax = -INFINITY;
@@ -293,7 +512,7 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_grad(const uint16_t *__re
       for (int r = 0; r < 4; ++r) {
         e[m][r] = expf(z[m][r] - zmax);
         sum += e[m][r];
-        if ((uint32_t)(16 * m + 4 * g + r) == kb[pt]) ek = e[m][r];
+        if ((uint32_t)(16 * m + 4 * g + r) == kbv) ek = e[m][r];
       }
     sum = Lane4Sum(sum);
     ek = Lane4Sum(ek);
@@ -301,92 +520,31 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_grad(const uint16_t *__re
     const float sg = 1.f / (1.f + expf(-zc));
     const float c = kGuideCMin + kGuideCSpan * sg;
     const float q = ek / sum / kGuideBinSr;
-    const float pmix = c * q + (1.f - c) * pb[pt];
-    const float coef = wn[pt] > 0.f ? -wn[pt] / pmix : 0.f;
-    if (g == 0 && wn[pt] > 0.f) lsum += -wn[pt] * logf(pmix);
+    const float pmix = c * q + (1.f - c) * pbv;
+    const float coef = wnv > 0.f ? -wnv / pmix : 0.f;
+    if (g == 0 && wnv > 0.f) lsum += -wnv * logf(pmix);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       float d[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float sj = e[m][r] / sum;
-        const float delta = (uint32_t)(16 * m + 4 * g + r) == kb[pt] ? 1.f : 0.f;
+        const float delta = (uint32_t)(16 * m + 4 * g + r) == kbv ? 1.f : 0.f;
         d[r] = coef * c * q * (delta - sj);
       }
-      dz[m][pt] = Pack4(d[0], d[1], d[2], d[3]);
+      Put4(dz[m], pt, Pack4(d[0], d[1], d[2], d[3]));
     }
-    const float dc = coef * (q - pb[pt]) * kGuideCSpan * sg * (1.f - sg);
-    dz[8][pt] = Pack4(g == 0 ? dc : 0.f, 0.f, 0.f, 0.f);
+    const float dc = coef * (q - pbv) * kGuideCSpan * sg * (1.f - sg);
+    Put4(dz[8], pt, Pack4(g == 0 ? dc : 0.f, 0.f, 0.f, 0.f));
   }
   if (lsum != 0.f) atomicAdd(loss, lsum);
   // ---- backward, layer by layer
-#pragma unroll 1
-  for (int l = kGuideLayers - 1; l >= 0; --l) {
-    const int R = GuideRows(l), C = GuideCols(l), nm = R / 16, nk = C / 16;
-    // layer input X (features x queries) and dZ into LDS, transposed for the dW products
-    const int slot0 = l == 0 ? 0 : 5 + 4 * (l - 1);
-    for (int kt = 0; kt < nk; ++kt)
-#pragma unroll
-      for (int pt = 0; pt < 4; ++pt) {
-        const Bf4 v = FromU2(st[((slot0 + kt) * 4 + pt) * 64 + lane]);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) xt[(16 * kt + 4 * g + r) * 64 + 16 * pt + c16] = (uint16_t)v[r];
-      }
-    for (int m = 0; m < nm; ++m)
-#pragma unroll
-      for (int pt = 0; pt < 4; ++pt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) dzt[(16 * m + 4 * g + r) * 64 + 16 * pt + c16] = (uint16_t)dz[m][pt][r];
-    __syncthreads();
-    // dW = dZ X^T (inner dimension: the 64 queries)
-    float *gW = grad + GuideWOff(l);
-    for (int m = 0; m < nm; ++m)
-      for (int nt = 0; nt < nk; ++nt) {
-        F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kp = 0; kp < 4; ++kp)
-          acc = Mfma(LoadBf4(dzt + (16 * m + c16) * 64 + 16 * kp + 4 * g), LoadBf4(xt + (16 * nt + c16) * 64 + 16 * kp + 4 * g),
-                     acc);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) unsafeAtomicAdd(&gW[(16 * m + 4 * g + r) * C + 16 * nt + c16], acc[r]);
-      }
-    // db = sum over queries of dZ
-    float *gB = grad + kGuideWeights + GuideBOff(l);
-    for (int row = lane; row < R; row += 64) {
-      float s = 0.f;
-      for (int k = 0; k < 64; ++k) s += FromBf16(dzt[row * 64 + k]);
-      unsafeAtomicAdd(&gB[row], s);
-    }
-    __syncthreads();
-    if (l == 0) break;
-    // dX = W^T dZ, then through the ReLU of the previous layer (its output is this layer's input)
-    const uint16_t *Wl = W + GuideWOff(l);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      F4v acc[4];
-#pragma unroll
-      for (int pt = 0; pt < 4; ++pt) acc[pt] = F4v{0.f, 0.f, 0.f, 0.f};
-      for (int kt = 0; kt < nm; ++kt) {
-        Bf4 a;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) a[r] = (short)Wl[(16 * kt + 4 * g + r) * C + 16 * m + c16];
-#pragma unroll
-        for (int pt = 0; pt < 4; ++pt) acc[pt] = Mfma(a, dz[kt][pt], acc[pt]);
-      }
-#pragma unroll
-      for (int pt = 0; pt < 4; ++pt) {
-        const Bf4 xin = FromU2(st[((slot0 + m) * 4 + pt) * 64 + lane]);
-        float d[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) d[r] = FromBf16((uint16_t)xin[r]) > 0.f ? acc[pt][r] : 0.f;
-        x[m][pt] = Pack4(d[0], d[1], d[2], d[3]);  // holds dZ of layer l - 1 until copied below
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int pt = 0; pt < 4; ++pt) dz[m][pt] = x[m][pt];
-  }
+  BackLayer<5>(W, st, dzt, xt, dz, grad);
+  BackLayer<4>(W, st, dzt, xt, dz, grad);
+  BackLayer<3>(W, st, dzt, xt, dz, grad);
+  BackLayer<2>(W, st, dzt, xt, dz, grad);
+  BackLayer<1>(W, st, dzt, xt, dz, grad);
+  BackLayer<0>(W, st, dzt, xt, dz, grad);
 }
 
 // Adam (Kingma and Ba 2015) on the FP32 master parameters; writes the BF16 weights and FP32
@@ -493,6 +651,7 @@ lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expec
   refs->rec = g->rec.as<GuideRecord>();
   refs->rec_count = g->rec_count.as<uint32_t>();
   refs->rec_cap = g->desc.record_capacity;
+  refs->max_depth = g->desc.max_depth ? g->desc.max_depth : 0xFFFFFFFFu;
   const uint64_t ev = expectedVertices ? expectedVertices : 1;
   const double frac = std::min(1.0, (double)g->desc.record_capacity / (1.25 * (double)ev));
   refs->rec_threshold = frac >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(frac * 4294967295.0);
@@ -505,14 +664,19 @@ lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expec
   return LGPU_OK;
 }
 
-lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps, uint32_t ub, const uint32_t *np,
-                       uint4 *bins, float2 *hdr, hipStream_t s) {
+lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const DevScene &sc, const PathState &ps, uint32_t ub,
+                       const uint32_t *np, uint32_t *list, uint32_t *count, uint4 *bins, float2 *hdr, hipStream_t s) {
+  LGPU_HIP(hipMemsetAsync(count, 0, 4, s));
+  hipLaunchKernelGGL(k_guide_list, dim3((ub + 255) / 256), dim3(256), 0, s, sc.tris, ps.hit, ps.ids, ub, np, refs.max_depth,
+                     list, count);
+  LGPU_HIP(hipGetLastError());
   GuideQuerySrc src{};
-  src.ray_o = ps.ray_o; src.ray_d = ps.ray_d; src.hit = ps.hit; src.scalars = nullptr;
+  src.ray_o = ps.ray_o; src.ray_d = ps.ray_d; src.hit = ps.hit; src.list = list; src.scalars = nullptr;
   for (int k = 0; k < 3; ++k) { src.bmin[k] = refs.bmin[k]; src.binv[k] = refs.binv[k]; }
-  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((ub + kInferBlock - 1) / kInferBlock, 2048));
+  constexpr uint32_t kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
+  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((ub + kPerBlock - 1) / kPerBlock, 2048));
   hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, ub,
-                     np, reinterpret_cast<uint32_t *>(bins), hdr);
+                     count, reinterpret_cast<uint32_t *>(bins), hdr);
   LGPU_HIP(hipGetLastError());
   return LGPU_OK;
 }
@@ -657,8 +821,10 @@ extern "C" lgpu_status lgpu_guide_forward(lgpu_guide g, uint32_t n, const float 
   LGPU_HIP(dbins.alloc(128ull * n));
   LGPU_HIP(dhdr.alloc(8ull * n));
   GuideQuerySrc src{};
+  src.list = nullptr;
   src.scalars = din.as<float>();
-  const uint32_t blocks = std::min<uint32_t>((n + kInferBlock - 1) / kInferBlock, 2048);
+  constexpr uint32_t kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
+  const uint32_t blocks = std::min<uint32_t>((n + kPerBlock - 1) / kPerBlock, 2048);
   hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, n,
                      nullptr, dbins.as<uint32_t>(), dhdr.as<float2>());
   const hipError_t le = hipGetLastError();

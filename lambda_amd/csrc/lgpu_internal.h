@@ -62,6 +62,7 @@ struct BvhTri {
 // that a wave runs one material family's code. 0 = ray left the scene, 1 = pass-through
 // surface, 2 + (root lobe kind - kLambertian) otherwise.
 constexpr uint32_t kShadeKeys = 16;
+constexpr uint32_t kKeyGuidable = 0x80000000u;  // key bit: the material may be guided (no delta lobe)
 // Shading attributes per triangle in BVH order (7 x float4 = 112 B), present
 // for meshes with smooth normals or uvs:
 //   {n0, uv0.x} {n1, uv0.y} {n2, uv1.x} {t0} {t1} {t2} {uv1.y, uv2.x, uv2.y, 0}

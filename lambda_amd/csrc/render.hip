@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   uint32_t key = 0, rank = 0;
   if (threadIdx.x < m) {
     const uint32_t hb = __float_as_uint(ps.hit[base + threadIdx.x].w);
-    key = hb == kNone ? 0u : sc.tris[hb].key;
+    key = hb == kNone ? 0u : (sc.tris[hb].key & (kShadeKeys - 1));
     rank = atomicAdd(&keyStart[key], 1u);
   }
   __syncthreads();
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   vm.woLy = ev.woL.y;
   shd.vm = vm;
   const V3 shadeP = h.point, shadeN = h.normalS;
-  const bool guided = kGuided && (mat.pad[0] & 1u);
+  const bool guided = kGuided && (mat.pad[0] & 1u) && depth < gr.max_depth;
   float gc = 0.f, gtot = 1.f;
   const uint8_t *gtab = nullptr;
   if (guided) {
@@ -605,8 +605,8 @@ static void LaunchShade(bool corrected, bool programs, const GuideRefs *gr, uint
 // Guiding hooks (guide.hip).
 lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expectedVertices, GuideRefs *refs,
                              const uint4 *bins, const float2 *hdr, hipStream_t s);
-lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps, uint32_t ub, const uint32_t *np,
-                       uint4 *bins, float2 *hdr, hipStream_t s);
+lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const DevScene &sc, const PathState &ps, uint32_t ub,
+                       const uint32_t *np, uint32_t *list, uint32_t *count, uint4 *bins, float2 *hdr, hipStream_t s);
 lgpu_status GuideFinishWave(lgpu_guide g, const SampleState &ss, int mode, uint32_t trainSteps, uint64_t vertices,
                             hipStream_t s, uint32_t *records);
 uint64_t GuideExpectedVertices(lgpu_guide g);
@@ -616,6 +616,7 @@ struct Wave {
   // two path-state buffers (ping-pong), per-sample radiance, shadow queue
   DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene, ring;
   DevBuf gbins, ghdr;  // guided renders: per path slot, the guide's bin table and {c, sum}
+  DevBuf glist, gcount; // guided renders: the bounce's queue slots that shade a guided vertex
   uint32_t cap = 0;
   PathState ps[2]{};
   SampleState ss{};
@@ -638,10 +639,10 @@ struct Wave {
   // Device bytes one path in flight holds (both state buffers, sample state, shadow slot, spill;
   // guided: the bin table and header).
   static uint64_t BytesPerPath(uint32_t spillWords, bool guided = false) {
-    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords + (guided ? 136ull : 0ull);
+    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords + (guided ? 140ull : 0ull);
   }
   uint64_t bytes() const {
-    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes + gbins.bytes + ghdr.bytes;
+    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes + gbins.bytes + ghdr.bytes + glist.bytes;
     for (const auto &row : st)
       for (const DevBuf &x : row) b += x.bytes;
     return b;
@@ -651,6 +652,8 @@ struct Wave {
     if (guided) {
       LGPU_HIP(gbins.alloc(128ull * n));
       LGPU_HIP(ghdr.alloc(8ull * n));
+      LGPU_HIP(glist.alloc(4ull * n));
+      LGPU_HIP(gcount.alloc(256));
     }
     if (!hring) {
       LGPU_HIP(hipHostMalloc((void **)&hring, 16 * kSlots, hipHostMallocDefault));
@@ -848,8 +851,9 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
                                W.spill_p, W.tcount_p);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[1], stream));
-          if (G) {  // the guide's distribution for every path of the queue (before shading reads it)
-            lgpu_status gs = GuideInfer(G, gref, W.ps[cur], ub, cin, W.gbins.as<uint4>(), W.ghdr.as<float2>(), stream);
+          if (G) {  // the guide's distribution for the queue's guided vertices (before shading reads it)
+            lgpu_status gs = GuideInfer(G, gref, dsc, W.ps[cur], ub, cin, W.glist.as<uint32_t>(), W.gcount.as<uint32_t>(),
+                                        W.gbins.as<uint4>(), W.ghdr.as<float2>(), stream);
             if (gs != LGPU_OK) return gs;
           }
           LGPU_HIP(hipEventRecord(e[5], stream));

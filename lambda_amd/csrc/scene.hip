@@ -854,6 +854,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
     const uint32_t root = mats[meshes[t.mesh].material].lobe;
     t.key = root == kNone ? 1u : 2u + (lobes[root].kind - (uint32_t)kLambertian);
     if (t.key >= kShadeKeys) t.key = kShadeKeys - 1;
+    if (mats[meshes[t.mesh].material].pad[0] & 1u) t.key |= kKeyGuidable;
   }
   // scene bounds (guiding inputs are positions normalised to them)
   {

@@ -351,8 +351,8 @@ class Progressive:
 class Guide:
     """Learned path guiding network (lgpu_guide_*): created on a device, passed to render_desc."""
 
-    def __init__(self, dev, seed=0, record_capacity=0, batch=0, learning_rate=0.0):
-        d = abi.GuideDesc(seed, record_capacity, batch, learning_rate)
+    def __init__(self, dev, seed=0, record_capacity=0, batch=0, learning_rate=0.0, max_depth=0):
+        d = abi.GuideDesc(seed, record_capacity, batch, learning_rate, max_depth)
         h = C.c_void_p()
         _check(lib().lgpu_guide_create(dev.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -140,20 +140,22 @@ def test_guided_matches_plain_estimator(dev):
 
 def test_guiding_reduces_error_on_indirect_light(dev):
     """Indirectly lit room: after online training, the guided estimator's relMSE at 64 spp (its
-    last 64 samples, rendered with the trained guide) is lower than the plain estimator's."""
+    last 64 samples, rendered with the trained guide) is lower than the plain estimator's (0.65 to
+    0.71 of it over two seeds and two learning rates; training is not bit-reproducible, as the
+    gradient is summed with atomics)."""
     sd = scene.cornell_indirect()
     W = H = 64
     cam = scene.cornell_camera(W, H)
-    ref, _ = _render(dev, sd, cam, W, H, 8192, seed=11)
+    ref, _ = _render(dev, sd, cam, W, H, 16384, seed=11)
 
     def relmse(x):
         return float(np.mean((x - ref) ** 2 / (ref * ref + 0.01)))
 
     plain, _ = _render(dev, sd, cam, W, H, 64, seed=3)
-    g = gpu.Guide(dev, seed=4, batch=16384)
+    g = gpu.Guide(dev, seed=4, batch=16384, learning_rate=5e-3)
     try:
-        # train online on 8 passes of 32 spp, then render 64 spp with the trained guide
-        _render(dev, sd, cam, W, H, 256, guide=g, steps=16, passes=8, seed=5)
+        # train online on 16 passes of 32 spp, then render 64 spp with the trained guide
+        _render(dev, sd, cam, W, H, 512, guide=g, steps=32, passes=16, seed=5)
         guided, _ = _render(dev, sd, cam, W, H, 64, guide=g, steps=0, seed=3)
         rp, rg = relmse(plain), relmse(guided)
         print("indirect room relMSE: plain %.4e guided %.4e (ratio %.3f), mean plain %s guided %s ref %s"
