@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 6u
+#define LGPU_ABI_VERSION 7u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -316,6 +316,13 @@ lgpu_status lgpu_scene_commit(lgpu_scene scene, uint32_t light_strategy);
 /* Introspection for tests: counts after commit. */
 lgpu_status lgpu_scene_info(lgpu_scene scene, uint32_t *num_triangles, uint32_t *num_bvh_nodes,
                             uint32_t *num_lights, uint32_t *num_tree_nodes);
+/* In-place parameter updates of a committed scene (no reference counterpart: the inverse-
+ * rendering loop of the adjoint render changes textures and density grids between iterations
+ * without a re-commit). texels: the texture's w * h * channels floats; density: the grid's
+ * nx * ny * nz floats (host memory). A texture that an emission socket reads cannot be updated in
+ * place (LGPU_ERR_UNSUPPORTED: it feeds the light sampler; re-commit instead). */
+lgpu_status lgpu_scene_update_texture(lgpu_scene scene, uint32_t texture_id, const float *texels);
+lgpu_status lgpu_scene_update_density(lgpu_scene scene, uint32_t medium_id, const float *density);
 /* Light-tree export for tests: per node {center xyz, diag, axis xyz, thetaO,
  * thetaE, power, child0, child1 (as float bits), first, count, parent} = 15 floats. */
 lgpu_status lgpu_scene_light_tree(lgpu_scene scene, float *nodes, uint32_t max_nodes);
@@ -349,6 +356,35 @@ lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
  * for this scene) beyond that is rejected with LGPU_ERR_INVALID_ARGUMENT. Every call on one GPU
  * shares its array: calls that overlap on the same GPU (from different device objects) race. */
 lgpu_status lgpu_render_noalloc(const lgpu_render_desc *desc, lgpu_render_stats *stats);
+
+/* ---------------- path-replay adjoint (no reference counterpart: the differentiation extension
+ * of VolumetricPathIntegrator::Li, VolumetricPathIntegrator.cpp:20-162, and through it of
+ * PathIntegrator::Li, PathIntegrator.cpp:15-88). The render is traced twice per wave with the same
+ * random numbers: the primal pass accumulates the film as lgpu_render does, the replay adds
+ * d/dtheta sum_px,c d_image[px][c] * mean_px,c to the gradients, where mean is the film mean of
+ * this call's samples (spp_end - spp_begin per pixel). Sampling (directions, distances, light
+ * picks, MIS weights, Russian roulette) is detached, as path replay backpropagation takes it.
+ * Requires LGPU_INTEGRATOR_VOLPATH, LGPU_ESTIMATOR_CORRECTED, LGPU_SPECTRAL_RGB and no guide.
+ * Parameters:
+ *   textures: the albedo of Lambertian and microfacet lobes and the roughness of isotropic
+ *     microfacet lobes, when the socket is one image lookup (IMAGE_COLOUR / IMAGE_SCALAR /
+ *     IMAGE_CHANNEL) through at most one conversion; other reads of a texture (emission, mix
+ *     ratios, procedural inputs) are constants, and a texture read both ways is refused.
+ *     Layout: every texture's texels in texture id order (w * h * channels floats each).
+ *   density: grid media, in medium id order (nx * ny * nz floats each; homogeneous media none).
+ *   emission: per material, the gradient of its emission_intensity (mesh lights).
+ * Gradients are added to the arrays (not cleared). */
+#define LGPU_ADJOINT_DEVICE_POINTERS 1u /* pointers are device memory (this library's HIP runtime) */
+typedef struct lgpu_adjoint_desc {
+  const float *d_image;   /* width * height * 3, the loss gradient per film mean value */
+  float *grad_textures;   /* or NULL */
+  float *grad_density;    /* or NULL */
+  float *grad_emission;   /* one per material, or NULL */
+  uint32_t flags;         /* LGPU_ADJOINT_* */
+  uint32_t reserved;
+} lgpu_adjoint_desc;
+lgpu_status lgpu_render_adjoint(const lgpu_render_desc *desc, const lgpu_adjoint_desc *adjoint,
+                                lgpu_render_stats *stats);
 
 /* ---------------- learned path guiding (no reference counterpart; the north-star extension of
  * the BSDF sampling step, PathIntegrator.cpp:46-67). A 6-layer, width-64 MLP (BF16 MFMA, FP32

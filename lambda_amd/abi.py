@@ -5,7 +5,7 @@ binding, so both are driven from the same scene description.
 """
 import ctypes as C
 
-ABI_VERSION = 6  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
+ABI_VERSION = 7  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
 NONE = 0xFFFFFFFF
 
 # lgpu_status
@@ -94,6 +94,13 @@ class GuideDesc(C.Structure):
                 ("learning_rate", C.c_float), ("max_depth", C.c_uint32)]
 
 RENDER_COUNT_TRAVERSAL = 1
+
+ADJOINT_DEVICE_POINTERS = 1
+
+
+class AdjointDesc(C.Structure):
+    _fields_ = [("d_image", C.c_void_p), ("grad_textures", C.c_void_p), ("grad_density", C.c_void_p),
+                ("grad_emission", C.c_void_p), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 TONEMAP_NONE, TONEMAP_ACES_FITTED = 0, 1

@@ -1,5 +1,6 @@
 // C ABI entry points: device, scene assembly, film, camera helpers, errors.
 // The render entry points live in render.hip.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -113,6 +114,35 @@ lgpu_status lgpu_scene_add_medium(lgpu_scene s, const lgpu_medium_desc *d, uint3
   c.density = nullptr;
   s->media.push_back(c);
   if (id) *id = (uint32_t)s->media.size() - 1;
+  return LGPU_OK;
+}
+
+// In-place parameter updates of a committed scene (inverse rendering loops): the device copy and
+// the host copy change together; nothing derived from the texture or grid needs rebuilding.
+lgpu_status lgpu_scene_update_texture(lgpu_scene s, uint32_t id, const float *texels) {
+  CHECK_ARG(s && texels, "update_texture: null argument");
+  if (!s->committed) return Fail(LGPU_ERR_STATE, "update_texture: scene not committed");
+  CHECK_ARG(id < s->textures.size(), "update_texture: unknown texture id");
+  if (s->tex_emissive[id])
+    return Fail(LGPU_ERR_UNSUPPORTED, "update_texture: an emission socket reads this texture (re-commit the scene)");
+  std::vector<float> &t = s->textures[id];
+  std::copy(texels, texels + t.size(), t.begin());
+  LGPU_HIP(hipSetDevice(s->dev->device));
+  LGPU_HIP(hipMemcpy(const_cast<float *>(s->dsc.texels) + s->texel_off[id], t.data(), t.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
+  return LGPU_OK;
+}
+
+lgpu_status lgpu_scene_update_density(lgpu_scene s, uint32_t id, const float *density) {
+  CHECK_ARG(s && density, "update_density: null argument");
+  if (!s->committed) return Fail(LGPU_ERR_STATE, "update_density: scene not committed");
+  CHECK_ARG(id < s->media.size(), "update_density: unknown medium id");
+  CHECK_ARG(s->media[id].type == LGPU_MEDIUM_GRID, "update_density: the medium has no density grid");
+  std::vector<float> &g = s->media_density[id];
+  std::copy(density, density + g.size(), g.begin());
+  LGPU_HIP(hipSetDevice(s->dev->device));
+  LGPU_HIP(hipMemcpy(const_cast<float *>(s->dsc.density) + s->density_off[id], g.data(), g.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
   return LGPU_OK;
 }
 

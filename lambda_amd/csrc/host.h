@@ -115,6 +115,11 @@ struct lgpu_scene_s {
   uint32_t num_tlights = 0;
 
   float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene box (min xyz, max xyz) of all triangles
+  // adjoint render and in-place parameter updates
+  bool adj_texture_shared = false;       // a texture-bound socket's texture is also read elsewhere
+  std::vector<uint8_t> tex_emissive;     // per texture: read by an emission socket
+  std::vector<uint32_t> texel_off;       // per texture: offset of its texels in DevScene::texels
+  std::vector<uint32_t> density_off;     // per medium: offset of its grid in DevScene::density
   lgpu::DevScene dsc{};
   std::deque<lgpu::DevBuf> bufs;  // owns every array referenced by dsc (stable addresses)
 };

@@ -207,6 +207,20 @@ struct Lobe {
   float p[10];          // constant parameters (microfacet p[0..9], ior, etaT)
 };
 
+// Differentiable texture bindings of one lobe (the adjoint render). A socket bound to a texture
+// reads one bilinear lookup of it through at most one conversion, so its value is linear in the
+// lookup's four components (TexBilinear's out[0..3]):
+//   albedo (spectrum socket, RGB mode): albedo_c = sum_k albedo_w[c][k] out[k]
+//   roughness (scalar socket):          r        = sum_k rough_w[k] out[k]
+// kNone: the socket is not texture-bound (its gradient is not taken).
+struct AdjLobe {
+  uint32_t albedo_tex;   // DevScene::textures index
+  uint32_t rough_tex;
+  uint32_t pad[2];
+  float albedo_w[3][4];
+  float rough_w[4];
+};
+
 struct MaterialRec {
   uint32_t lobe;        // root lobe index, kNone = pass-through
   uint32_t emission;    // socket index, kNone = not emissive
@@ -311,6 +325,7 @@ struct DevScene {
   uint32_t spill_words;      // per-thread traversal-stack words beyond the LDS part this tree can need
   uint32_t has_programs;     // some socket runs VM code (shading kernels specialise on it)
   unsigned long long *voxel_count;  // non-null in counting renders: voxels visited by grid walks
+  const AdjLobe *adj_lobes;  // per lobe: texture bindings for the adjoint render
 };
 
 struct CameraRec {
@@ -344,9 +359,11 @@ struct RenderParams {
   uint32_t pass_normal_g;
 };
 
-// Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator.
+// Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator;
+// with `adj`, the adjoint replay (adjoint.h).
+struct AdjRefs;
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
-                    SampleState ss, uint32_t n, Queues q, uint32_t *spill);
+                    SampleState ss, uint32_t n, Queues q, uint32_t *spill, const AdjRefs *adj = nullptr);
 // Utility-pass shade stage (utility.hip): DepthPass, NormalPass, AOVPass.
 void LaunchShadeUtility(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                         SampleState ss, uint32_t n);

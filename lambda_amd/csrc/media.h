@@ -49,9 +49,10 @@ __device__ inline void SlabClip(float o, float d, float bmin, float bmax, float 
   if (ta > tn) tn = ta;
   if (tb < tf) tf = tb;
 }
-// Visits the voxels of [t0, t1] along o + t d in order (3-D DDA); fn(rho, ta, tb) returns true to stop.
+// Visits the voxels of [t0, t1] along o + t d in order (3-D DDA); fn(rho, ta, tb, voxel) returns
+// true to stop (voxel: index into the medium's grid).
 template <class F>
-__device__ inline void GridWalk(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float t0, float t1, F &&fn) {
+__device__ inline void GridWalkV(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float t0, float t1, F &&fn) {
   float tn = t0, tf = t1;
   SlabClip(o.x, d.x, m.bmin[0], m.bmax[0], tn, tf);
   SlabClip(o.y, d.y, m.bmin[1], m.bmax[1], tn, tf);
@@ -73,8 +74,9 @@ __device__ inline void GridWalk(const DevScene &sc, const MediumRec &m, V3 o, V3
     const int ax = (tNx <= tNy && tNx <= tNz) ? 0 : (tNy <= tNz ? 1 : 2);
     const float tnext = ax == 0 ? tNx : (ax == 1 ? tNy : tNz);
     const float tc = StdMax(t, StdMin(tnext, tf));
-    const float rho = dens[((size_t)cz * m.ny + cy) * m.nx + cx];
-    if (fn(rho, t, tc)) break;
+    const uint32_t vox = ((uint32_t)cz * m.ny + (uint32_t)cy) * m.nx + (uint32_t)cx;
+    const float rho = dens[vox];
+    if (fn(rho, t, tc, vox)) break;
     t = tc;
     if (t >= tf) break;
     if (ax == 0) { cx += sx; if (cx < 0 || cx >= nx) break; tNx += dX; }
@@ -82,6 +84,11 @@ __device__ inline void GridWalk(const DevScene &sc, const MediumRec &m, V3 o, V3
     else { cz += sz; if (cz < 0 || cz >= nz) break; tNz += dZ; }
   }
   if (sc.voxel_count) atomicAdd(sc.voxel_count, (unsigned long long)it);
+}
+// GridWalkV without the voxel index: fn(rho, ta, tb).
+template <class F>
+__device__ inline void GridWalk(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float t0, float t1, F &&fn) {
+  GridWalkV(sc, m, o, d, t0, t1, [&](float rho, float ta, float tb, uint32_t) { return fn(rho, ta, tb); });
 }
 struct DensAcc {
   float D, rho;
@@ -295,6 +302,14 @@ __device__ inline bool IntersectTr(const DevScene &sc, V3 o, V3 d, uint32_t med,
 // How an NEE ray is resolved inside the shading kernels: plain occlusion (Scene::MutualVisibility /
 // RayEscapes) or, with volumes, an IntersectTr walk carrying the transmittance (Light.cpp:10-49,
 // hasVolumes branch; a ray that leaves every boundary is unoccluded).
+// The last transmittance query a VisCtx resolved (the adjoint render walks it again to scatter
+// the gradient of its transmittance).
+struct VisQuery {
+  V3 o, d;
+  float maxT;      // IntersectTr's limit (< 0: none)
+  uint32_t med;    // start medium
+  uint32_t valid;  // 1: a query through media was made
+};
 struct VisCtx {
   TraceCtx tc;
   bool volumes;
@@ -302,6 +317,7 @@ struct VisCtx {
   uint32_t pathMed;
   const MaterialRec *surf;    // surface vertex: GetMedium(surf, dir, surfNg) gives the start medium
   V3 surfNg;
+  VisQuery *rec = nullptr;    // when set, records the query
 };
 __device__ inline uint32_t NeeMedium(const VisCtx &vc, V3 dir) {
   return vc.atMedium ? vc.pathMed : GetMedium(*vc.surf, dir, vc.surfNg);
@@ -321,6 +337,7 @@ __device__ inline bool Visible(const DevScene &sc, const VisCtx &vc, V3 p1, V3 d
   h.tFar = INFINITY;
   uint32_t bi;
   bool left;
+  if (vc.rec) *vc.rec = VisQuery{p1, dir, point ? mag : -1.f, NeeMedium(vc, dir), 1u};
   IntersectTr(sc, p1, dir, NeeMedium(vc, dir), point ? mag : -1.f, Tr, vm, true, vc.tc, h, bi, &left, false);
   if (left) return true;
   return h.tFar > minT;
@@ -333,6 +350,7 @@ __device__ inline bool Escapes(const DevScene &sc, const VisCtx &vc, V3 o, V3 d,
   }
   Hit h;
   uint32_t bi;
+  if (vc.rec) *vc.rec = VisQuery{o, d, -1.f, NeeMedium(vc, d), 1u};
   return !IntersectTr(sc, o, d, NeeMedium(vc, d), -1.f, Tr, vm, true, vc.tc, h, bi, nullptr, false);
 }
 

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstring>
 
+#include "adjoint.h"
 #include "guide.h"
 #include "host.h"
 #include "lights.h"
@@ -616,6 +617,7 @@ struct Wave {
   // two path-state buffers (ping-pong), per-sample radiance, shadow queue
   DevBuf st[2][9], L, pix, lam, shadow, counters, spill, tcount, scene, ring;
   DevBuf gbins, ghdr;  // guided renders: per path slot, the guide's bin table and {c, sum}
+  DevBuf ltot;         // adjoint renders: per path slot, the primal pass's radiance
   DevBuf glist, gcount; // guided renders: the bounce's queue slots that shade a guided vertex
   uint32_t cap = 0;
   PathState ps[2]{};
@@ -638,11 +640,13 @@ struct Wave {
   }
   // Device bytes one path in flight holds (both state buffers, sample state, shadow slot, spill;
   // guided: the bin table and header).
-  static uint64_t BytesPerPath(uint32_t spillWords, bool guided = false) {
-    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords + (guided ? 140ull : 0ull);
+  static uint64_t BytesPerPath(uint32_t spillWords, bool guided = false, bool adjoint = false) {
+    return 2ull * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4ull * spillWords + (guided ? 140ull : 0ull) +
+           (adjoint ? 16ull : 0ull);
   }
   uint64_t bytes() const {
-    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes + gbins.bytes + ghdr.bytes + glist.bytes;
+    uint64_t b = L.bytes + pix.bytes + lam.bytes + shadow.bytes + spill.bytes + gbins.bytes + ghdr.bytes + glist.bytes +
+                 ltot.bytes;
     for (const auto &row : st)
       for (const DevBuf &x : row) b += x.bytes;
     return b;
@@ -753,9 +757,12 @@ static RenderParams MakeParams(const lgpu_render_desc *d, uint32_t w, uint32_t h
 // Renders `total` paths; list mode when listPix != nullptr (trace_paths). `bound`: a wave already
 // laid out in caller memory (no-allocation entry point): no device or pinned allocation happens and
 // the bounce loop synchronises per bounce instead of running ahead through pinned counter copies.
+// `adj`: adjoint render (image-order waves, volumetric integrator): every wave is traced twice with
+// the same random numbers, the primal pass (film accumulation, per-path radiance kept) and the
+// adjoint replay that scatters the gradients.
 static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, uint64_t total, const uint32_t *listPix,
                             const uint32_t *listSmp, double *film, uint32_t *counts, float *out, lgpu_render_stats *stats,
-                            Wave *bound = nullptr) {
+                            Wave *bound = nullptr, const AdjRefs *adj = nullptr) {
   lgpu_scene s = d->scene;
   lgpu_device dev = s->dev;
   LGPU_HIP(hipSetDevice(dev->device));
@@ -779,7 +786,8 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       cap = kDefaultPaths;
       size_t freeB = 0, totalB = 0;
       if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
-        const uint64_t fit = (freeB + W0.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words, d->guide != nullptr);
+        const uint64_t fit =
+            (freeB + W0.bytes()) / 4 / Wave::BytesPerPath(s->dsc.spill_words, d->guide != nullptr, adj != nullptr);
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(fit, 1ull << 16));
       }
     }
@@ -787,8 +795,15 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
     if (cap == 0) cap = 1;
     lgpu_status st = W0.alloc((uint32_t)cap, s->dsc.spill_words, d->guide != nullptr);
     if (st != LGPU_OK) return st;
+    if (adj && W0.ltot.bytes < 16ull * cap) LGPU_HIP(W0.ltot.alloc(16ull * cap));
   }
   Wave &W = bound ? *bound : *static_cast<Wave *>(dev->wave);
+  AdjRefs adjW{};
+  if (adj) {
+    if (bound) return Fail(LGPU_ERR_UNSUPPORTED, "render: no adjoint in caller memory");
+    adjW = *adj;
+    adjW.ltot = W.ltot.as<float4>();
+  }
   const RenderParams rp = MakeParams(d, w, h);
   const bool countTrav = (d->flags & LGPU_RENDER_COUNT_TRAVERSAL) != 0;
   DevScene dsc = s->dsc;
@@ -815,6 +830,9 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       lgpu_status gs = GuidePrepareWave(G, s->bounds, expect, &gref, W.gbins.as<uint4>(), W.ghdr.as<float2>(), stream);
       if (gs != LGPU_OK) return gs;
     }
+    for (int rep = 0; rep < (adj ? 2 : 1); ++rep) {  // adjoint: primal pass, then the replay
+    const AdjRefs *ap = rep == 1 ? &adjW : nullptr;
+    cur = 0;
     LGPU_HIP(hipEventRecord(ev[0], stream));
     hipLaunchKernelGGL(k_camera, dim3(Blocks(n)), dim3(kBlock), 0, stream, rp, W.ps[0], W.ss, n, base,
                        listPix ? listPix + base : nullptr, listSmp ? listSmp + base : nullptr);
@@ -931,7 +949,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         else if (rp.integrator >= LGPU_INTEGRATOR_DEPTH)
           LaunchShadeUtility(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count);
         else
-          LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill_p);
+          LaunchShadeVol(Blocks(count), stream, dsc, rp, W.ps[cur], W.ss, count, q, W.spill_p, ap);
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[4], stream));
         uint32_t cnt[2];
@@ -962,6 +980,8 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       }
       if (count > 0) return Fail(LGPU_ERR_STATE, "render: path wave did not terminate");
     }
+    if (rep == 1) break;  // the replay's radiance equals the primal pass's: accumulated once
+    if (adj) LGPU_HIP(hipMemcpyAsync(W.ltot.as<float4>(), W.ss.L, 16ull * n, hipMemcpyDeviceToDevice, stream));
     const uint32_t npx = w * h;
     if (!listPix && !out && npx > 0)
       hipLaunchKernelGGL(k_accumulate_px, dim3(Blocks(std::min(n, npx))), dim3(kBlock), 0, stream, rp, W.ss, n, npx, film,
@@ -970,6 +990,7 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
       hipLaunchKernelGGL(k_accumulate, dim3(Blocks(n)), dim3(kBlock), 0, stream, rp, W.ss, n, film, counts,
                          out ? out + 5 * base : nullptr);
     LGPU_HIP(hipGetLastError());
+    }  // rep
     S.waves++;
   }
   LGPU_HIP(hipStreamSynchronize(stream));
@@ -1002,6 +1023,75 @@ extern "C" lgpu_status lgpu_render(const lgpu_render_desc *d, lgpu_render_stats 
   lgpu_film f = d->film;
   const uint64_t total = (uint64_t)f->width * f->height * (d->spp_end - d->spp_begin);
   return RunWaves(d, f->width, f->height, total, nullptr, nullptr, f->sum.as<double>(), f->count.as<uint32_t>(), nullptr, stats);
+}
+
+// Adjoint render: the primal samples go to the film as in lgpu_render, and the gradient of
+// sum(d_image * film mean of this call's samples) is added to the parameter gradients.
+extern "C" lgpu_status lgpu_render_adjoint(const lgpu_render_desc *d, const lgpu_adjoint_desc *a,
+                                           lgpu_render_stats *stats) {
+  lgpu_status st = ValidateRender(d);
+  if (st != LGPU_OK) return st;
+  if (!a || !a->d_image) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_adjoint: null adjoint descriptor or d_image");
+  if (!d->film) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_adjoint: null film");
+  if (d->spp_end <= d->spp_begin) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_adjoint: empty spp range");
+  if (d->film->dev != d->scene->dev) return Fail(LGPU_ERR_INVALID_ARGUMENT, "render_adjoint: film and scene on different devices");
+  if (d->integrator != LGPU_INTEGRATOR_VOLPATH || d->estimator_mode != LGPU_ESTIMATOR_CORRECTED ||
+      d->spectral_mode != LGPU_SPECTRAL_RGB || d->guide)
+    return Fail(LGPU_ERR_UNSUPPORTED,
+                "render_adjoint: needs the volumetric path integrator, the corrected estimator, RGB spectra and no guide");
+  lgpu_scene s = d->scene;
+  if (a->grad_textures && s->adj_texture_shared)
+    return Fail(LGPU_ERR_UNSUPPORTED, "render_adjoint: a texture-bound albedo or roughness texture is also read elsewhere");
+  std::lock_guard<std::mutex> lk(s->dev->mu);
+  lgpu_film f = d->film;
+  const uint64_t npx = (uint64_t)f->width * f->height;
+  const uint64_t total = npx * (d->spp_end - d->spp_begin);
+  size_t nTex = 0, nDens = 0;
+  for (const auto &t : s->textures) nTex += t.size();
+  for (const auto &g : s->media_density) nDens += g.size();
+  const size_t nMat = s->materials.size();
+  LGPU_HIP(hipSetDevice(s->dev->device));
+  AdjRefs ar{};
+  ar.inv_spp = 1.f / (float)(d->spp_end - d->spp_begin);
+  const bool onDevice = (a->flags & LGPU_ADJOINT_DEVICE_POINTERS) != 0;
+  DevBuf bImg, bTex, bDens, bEmit;
+  if (onDevice) {
+    ar.dimage = a->d_image;
+    ar.gtex = nTex ? a->grad_textures : nullptr;
+    ar.gdens = nDens ? a->grad_density : nullptr;
+    ar.gemit = nMat ? a->grad_emission : nullptr;
+  } else {
+    LGPU_HIP(bImg.upload(a->d_image, 12ull * npx));
+    ar.dimage = bImg.as<float>();
+    auto zeroed = [&](DevBuf &b, size_t count, float *host) -> float * {
+      if (!host || !count) return nullptr;
+      if (b.alloc(4ull * count) != hipSuccess || hipMemset(b.ptr, 0, 4ull * count) != hipSuccess) return nullptr;
+      return b.as<float>();
+    };
+    ar.gtex = zeroed(bTex, nTex, a->grad_textures);
+    ar.gdens = zeroed(bDens, nDens, a->grad_density);
+    ar.gemit = zeroed(bEmit, nMat, a->grad_emission);
+    if ((a->grad_textures && nTex && !ar.gtex) || (a->grad_density && nDens && !ar.gdens) ||
+        (a->grad_emission && nMat && !ar.gemit))
+      return Fail(LGPU_ERR_OUT_OF_MEMORY, "render_adjoint: gradient buffers");
+    LGPU_HIP(hipDeviceSynchronize());  // the clears land before the render stream reads them
+  }
+  st = RunWaves(d, f->width, f->height, total, nullptr, nullptr, f->sum.as<double>(), f->count.as<uint32_t>(), nullptr,
+                stats, nullptr, &ar);
+  if (st != LGPU_OK || onDevice) return st;
+  // host pointers: add the device sums into the caller's arrays
+  auto addBack = [&](const DevBuf &b, size_t count, float *host) -> hipError_t {
+    if (!host || !count) return hipSuccess;
+    std::vector<float> t(count);
+    const hipError_t e = hipMemcpy(t.data(), b.ptr, 4ull * count, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (size_t k = 0; k < count; ++k) host[k] += t[k];
+    return hipSuccess;
+  };
+  LGPU_HIP(addBack(bTex, nTex, a->grad_textures));
+  LGPU_HIP(addBack(bDens, nDens, a->grad_density));
+  LGPU_HIP(addBack(bEmit, nMat, a->grad_emission));
+  return LGPU_OK;
 }
 
 namespace lgpu {

@@ -850,6 +850,103 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
     for (MaterialRec &r : mats)
       if (r.lobe != kNone && guidable(r.lobe)) r.pad[0] |= 1u;
   }
+  // Adjoint texture bindings (AdjLobe): the albedo socket of Lambertian and microfacet lobes and
+  // the roughness socket of isotropic microfacet lobes, when the socket is one bilinear lookup of
+  // an image texture through at most one conversion. A texture that is also read anywhere else is
+  // flagged: the adjoint render refuses texture gradients for such scenes (that other read would
+  // miss its gradient).
+  std::vector<AdjLobe> adjLobes(lobes.size());
+  {
+    std::vector<uint32_t> refs(texRecs.size(), 0), bound(texRecs.size(), 0);
+    for (const Instr &in : code)
+      if ((in.op == kOpTex || in.op == kOpTexChannel || in.op == kOpSpecTex) && in.imm < refs.size()) refs[in.imm]++;
+    // weights of the lookup's components for a socket, or false if it is not a bound lookup
+    auto decode = [&](uint32_t si, bool spectrum, uint32_t *tex, float w[3][4]) -> bool {
+      if (si == kNone) return false;
+      const Socket &sk = sockets[si];
+      if (sk.kind != kSockProgram || sk.prog_len == 0 || sk.prog_len > 2) return false;
+      const Instr &a = code[sk.prog_off];
+      if ((a.op != kOpTex && a.op != kOpTexChannel) || a.imm >= texRecs.size()) return false;
+      float comp[3][4] = {};  // value channel (3 for a spectrum, 1 otherwise) from lookup component
+      uint32_t t = kScalar;   // type of the lookup's register: a colour (kOpTex) or one channel
+      float base[4][4] = {};  // register lanes x..w as weights of the components
+      if (a.op == kOpTex) {
+        for (int k = 0; k < 4; ++k) base[k][k] = 1.f;
+        t = kColour;
+      } else {
+        base[0][std::min<uint32_t>(a.c, 3)] = 1.f;
+        t = kScalar;
+      }
+      uint32_t reg = a.dst;
+      if (sk.prog_len == 2) {
+        const Instr &b = code[sk.prog_off + 1];
+        if (b.a != reg) return false;
+        if (b.op == kOpToSpectrum && spectrum) {
+          if (b.c == kColour) {  // RGB mode: the colour's lanes x, y, z
+            for (int c = 0; c < 3; ++c) for (int k = 0; k < 4; ++k) comp[c][k] = base[c][k];
+          } else if (b.c == kScalar) {
+            for (int c = 0; c < 3; ++c) for (int k = 0; k < 4; ++k) comp[c][k] = base[0][k];
+          } else {
+            return false;
+          }
+        } else if (b.op == kOpToReal && !spectrum) {
+          if (b.c == kColour) {
+            const float cw[3] = {.3f, .59f, .11f};
+            for (int k = 0; k < 4; ++k) comp[0][k] = cw[0] * base[0][k] + cw[1] * base[1][k] + cw[2] * base[2][k];
+          } else if (b.c == kScalar) {
+            for (int k = 0; k < 4; ++k) comp[0][k] = base[0][k];
+          } else {
+            return false;
+          }
+        } else {
+          return false;
+        }
+        reg = b.dst;
+      } else {
+        if (spectrum || !(sk.res_type == kScalar)) return false;
+        for (int k = 0; k < 4; ++k) comp[0][k] = base[0][k];
+      }
+      if (reg != sk.res) return false;
+      (void)t;
+      *tex = a.imm;
+      for (int c = 0; c < 3; ++c) for (int k = 0; k < 4; ++k) w[c][k] = comp[c][k];
+      return true;
+    };
+    for (size_t li = 0; li < lobes.size(); ++li) {
+      AdjLobe &A = adjLobes[li];
+      std::memset(&A, 0, sizeof(A));
+      A.albedo_tex = A.rough_tex = kNone;
+      const Lobe &L = lobes[li];
+      if (L.kind != kLambertian && L.kind != kMicrofacet) continue;
+      uint32_t t;
+      float w[3][4];
+      if (decode(L.sock[0], true, &t, w)) {
+        A.albedo_tex = t;
+        std::memcpy(A.albedo_w, w, sizeof(w));
+        bound[t]++;
+      }
+      if (L.kind == kMicrofacet && decode(L.sock[1], false, &t, w)) {
+        A.rough_tex = t;
+        std::memcpy(A.rough_w, w[0], sizeof(A.rough_w));
+        bound[t]++;
+      }
+    }
+    s->adj_texture_shared = false;
+    for (size_t t = 0; t < refs.size(); ++t)
+      if (bound[t] > 0 && refs[t] > bound[t]) s->adj_texture_shared = true;
+    // textures an emission socket reads feed the light sampler's power estimates: updating them
+    // in place would leave those stale
+    s->tex_emissive.assign(s->textures.size(), 0);
+    for (const MaterialRec &r : mats) {
+      if (r.emission == kNone || sockets[r.emission].kind != kSockProgram) continue;
+      const Socket &sk = sockets[r.emission];
+      for (uint32_t pc = sk.prog_off; pc < sk.prog_off + sk.prog_len; ++pc) {
+        const Instr &in = code[pc];
+        if ((in.op == kOpTex || in.op == kOpTexChannel || in.op == kOpSpecTex) && in.imm < s->tex_emissive.size())
+          s->tex_emissive[in.imm] = 1;
+      }
+    }
+  }
   for (BvhTri &t : btris) {
     const uint32_t root = mats[meshes[t.mesh].material].lobe;
     t.key = root == kNone ? 1u : 2u + (lobes[root].kind - (uint32_t)kLambertian);
@@ -1210,6 +1307,11 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   UP(pool.f, dist);
   UP(media, media);
   UP(density, density);
+  UP(adjLobes, adj_lobes);
+  s->density_off.assign(s->media.size(), 0);
+  for (size_t k = 0; k < media.size(); ++k) s->density_off[k] = media[k].dens_off;
+  s->texel_off.assign(s->textures.size(), 0);
+  for (size_t t = 0; t < s->textures.size(); ++t) s->texel_off[t] = texRecs[texIds[t]].w;
 #undef UP
   s->committed = true;
   return LGPU_OK;

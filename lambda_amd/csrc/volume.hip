@@ -13,6 +13,7 @@
 //   4. otherwise the surface vertex (NEE + BSDF sample) or a pass-through boundary;
 //   5. Russian roulette, then the continuation is appended to the next queue.
 // NEE visibility and the medium points' phase-sampled light lookups are traced inline.
+#include "adjoint.h"
 #include "host.h"
 #include "lights.h"
 #include "media.h"
@@ -91,14 +92,17 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
 
 // kPrograms: the scene has program sockets (DevScene::has_programs); without them the kernel is
 // compiled without the shader VM.
-template <bool kPrograms>
+// kAdjoint: the replay of the adjoint render (corrected estimator, RGB, VOLPATH): the same path
+// decisions and radiance as the primal launch, plus the gradient pieces of adjoint.h.
+template <bool kPrograms, bool kAdjoint>
 __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
-                                                      uint32_t n, Queues q, uint32_t *spillBase) {
+                                                      uint32_t n, Queues q, uint32_t *spillBase, AdjRefs ar) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
   __shared__ TreeImp treeLds[kTreeLds];
   LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  EmitAcc eacc{{kNone, kNone}, {0.f, 0.f}};
+  if (i < n) {  // the adjoint's emission flush below needs every lane of the wave
   const TraceCtx tc{stk, spillBase + (size_t)i * sc.spill_words};
   const bool corrected = rp.corrected != 0;
   const bool volumes = corrected && sc.num_media > 0;  // Scene::hasVolumes: never set as written
@@ -118,6 +122,12 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
   const uint32_t med = aux.w;
   Spec beta = S4(ps.beta[i]);
   Spec Lacc = S(0.f);
+  Spec adjd = S(0.f), rest = S(0.f);  // adjoint: loss gradient of the sample, radiance still to come
+  VisQuery vq{};
+  if constexpr (kAdjoint) {
+    adjd = AdjDelta(ar, ss.pixel[sid]);
+    rest = S4(ar.ltot[sid]) - S4(ss.L[sid]);
+  }
   Rng rng;
   rng.init(ids.x, ids.y, ids.z, rp.seed);
   VmCtx vm;
@@ -155,7 +165,13 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
     vmm.medium = pn.w != 0.f;  // looked up for a medium interaction
     const Spec C = MisLight<kPrograms>(sc, aux.y, mis.y, ToV3(ps.prev_p[i]), ToV3(pn), si, th, tbi, rd, mis.x,
                             (flags & kFlagSpecular) != 0, false, vmm, corrected);
-    Lacc = Lacc + beta * (C * Tr * mis.w);
+    const Spec C1 = beta * (C * Tr * mis.w);
+    Lacc = Lacc + C1;
+    if constexpr (kAdjoint) {  // the light's intensity; the transmittance of the lookup's walk
+      const Spec X = adjd * C1;
+      if (si && sc.meshes[th.mesh].mesh_light != kNone) EmitAdd(sc, eacc, 0, sc.meshes[th.mesh].material, X);
+      if ((flags & kFlagTrMIS) && hit) TrGradWalk(sc, ar, ro, rd, med, -1.f, X, vm, tc);
+    }
     if ((flags & kFlagEmitOnly) || !hit) alive = false;
   } else if (!(flags & kFlagCamera) && !hit) {
     alive = false;
@@ -173,6 +189,11 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         Spec Le = EmissionOf<kPrograms>(sc, h.mesh, vm) * sc.lights[m.mesh_light].intensity * kInvPi;
         if (med != kNone) Le = Le * MediumTr(sc, sc.media[med], ro, rd, h.tFar, vm);
         Lacc = Lacc + (corrected ? beta * Le : Le);  // beta taken as 1 as written
+        if constexpr (kAdjoint) {
+          const Spec X = adjd * (beta * Le);
+          EmitAdd(sc, eacc, 0, m.material, X);
+          if (med != kNone) TrGradSegment(sc, ar, sc.media[med], ro, rd, h.tFar, X, vm);
+        }
       }
     }
   }
@@ -192,10 +213,12 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
       ev.woL = ev.wo;
       ev.wiL = ev.wi;
       ev.sidedness = sideFlag ? -1 : 1;
-      const VisCtx vc{tc, volumes, true, med, nullptr, v3(0, 0, 0)};
+      const VisCtx vc{tc, volumes, true, med, nullptr, v3(0, 0, 0), kAdjoint ? &vq : nullptr};
       if (!misVol) {
         const float u0 = rng.Get1D(), u1 = rng.Get1D();
         const MediumSample ds = SampleDistance(sc, M, ro, rd, h.tFar, u0, u1, vm);
+        // every contribution from here on carries the distance sample's weight
+        if constexpr (kAdjoint) DistanceGrad(sc, ar, M, ro, rd, h.tFar, u0, u1, ds.inter, adjd * (rest - Lacc), vm);
         beta = beta * ds.w;
         if (ds.inter) {  // VolumetricPathIntegrator.cpp:99-148
           medInter = true;
@@ -223,6 +246,11 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
               const float w = (corrected && delta) ? 1.f : PowerHeuristic(lightPDF, ph);
               Ld = Ld + Li * ph * w / lightPDF;
             }
+          }
+          if constexpr (kAdjoint) {  // the NEE contribution's light intensity and transmittance
+            const Spec X = adjd * (beta * Ld);
+            EmitAdd(sc, eacc, 1, EmitMaterial(sc, pick), X);
+            if (vq.valid) TrGradWalk(sc, ar, vq.o, vq.d, vq.med, vq.maxT, X, vm, tc);
           }
           const float hu0 = rng.Get1D(), hu1 = rng.Get1D();
           V3 wi;
@@ -349,7 +377,9 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         vm.woLy = ev.woL.y;
         shd.vm = vm;
         const V3 shadeP = h.point, shadeN = h.normalS;
-        const VisCtx vc{tc, volumes, false, med, &mat, h.normalG};
+        const VisCtx vc{tc, volumes, false, med, &mat, h.normalG, kAdjoint ? &vq : nullptr};
+        VertexGrad vg;
+        if constexpr (kAdjoint) VertexGradInit(shd, mat.lobe, vg);
         float lightDistPdf = 1.f;
         const uint32_t pick = SampleLight(sc, h.point, h.normalS, rng, &lightDistPdf, treeLds, corrected);
         float lightPDF = 0.f;
@@ -364,22 +394,34 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         lightPDF *= lightDistPdf;
         Spec cNee = S(0.f);
         if (lightPDF > 0 && !IsBlack(Li, mode)) {
-          const Spec f = shd.F(mat.lobe, ev) * fabsf(ev.wiL.y);
+          const Spec f0 = shd.F(mat.lobe, ev);
+          const Spec f = f0 * fabsf(ev.wiL.y);
           const float sp = shd.Pdf(mat.lobe, ev.woL, ev.wiL);
           if (!IsBlack(f, mode) && sp > 0) {
             const float w = (corrected && delta) ? 1.f : PowerHeuristic(lightPDF, sp);
             cNee = Li * f * w / lightPDF;
+            if constexpr (kAdjoint) {  // NEE: the BSDF at the light direction, intensity, transmittance
+              const Spec X = adjd * (beta * cNee);
+              VertexGradAdd(shd, ev, X, f0, vg);
+              EmitAdd(sc, eacc, 1, EmitMaterial(sc, pick), X);
+              if (vq.valid) TrGradWalk(sc, ar, vq.o, vq.d, vq.med, vq.maxT, X, vm, tc);
+            }
           }
         }
         float scatPdf = 0.f;
         bool spec = false;
-        Spec f = shd.Sample(mat.lobe, ev, h, rng, scatPdf, spec);
-        f = f * fabsf(ev.wiL.y);
+        const Spec fS = shd.Sample(mat.lobe, ev, h, rng, scatPdf, spec);
+        const Spec f = fS * fabsf(ev.wiL.y);
         const uint32_t side = ev.sidedness < 0 ? kFlagSideNeg : 0u;
         if (!(scatPdf > 0 && !IsBlack(f, mode))) {
           if (corrected) Lacc = Lacc + beta * cNee;  // NEE is dropped by `break` as written
+          if constexpr (kAdjoint) VertexGradFlush(sc, ar, vm, vg);
         } else {
           Lacc = Lacc + beta * cNee;
+          if constexpr (kAdjoint) {  // continuation: everything later launches gather
+            if (!spec) VertexGradAdd(shd, ev, adjd * (rest - Lacc), fS, vg);
+            VertexGradFlush(sc, ar, vm, vg);
+          }
           beta = beta * (f / scatPdf);
           out.o = h.point; out.d = ev.wi; out.eta = ev.eta;
           out.flags = (spec ? kFlagSpecular : 0u) | (volumes ? kFlagTrMIS : 0u) | side;
@@ -419,16 +461,25 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
     L.x += Lacc.c[0]; L.y += Lacc.c[1]; L.z += Lacc.c[2]; L.w += Lacc.c[3];
     ss.L[sid] = L;
   }
+  }  // i < n
+  if constexpr (kAdjoint) EmitFlush(ar, eacc);
 }
 
 }  // namespace
 
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
-                    SampleState ss, uint32_t n, Queues q, uint32_t *spill) {
-  if (sc.has_programs)
-    hipLaunchKernelGGL(k_shade_vol<true>, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
-  else
-    hipLaunchKernelGGL(k_shade_vol<false>, dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill);
+                    SampleState ss, uint32_t n, Queues q, uint32_t *spill, const AdjRefs *adj) {
+  const AdjRefs ar = adj ? *adj : AdjRefs{};
+  if (adj) {
+    if (sc.has_programs)
+      hipLaunchKernelGGL((k_shade_vol<true, true>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+    else
+      hipLaunchKernelGGL((k_shade_vol<false, true>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+  } else if (sc.has_programs) {
+    hipLaunchKernelGGL((k_shade_vol<true, false>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+  } else {
+    hipLaunchKernelGGL((k_shade_vol<false, false>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+  }
 }
 
 }  // namespace lgpu

@@ -30,7 +30,8 @@ EXPORTS = [
     "lgpu_obj_material_info", "lgpu_obj_destroy", "lgpu_film_post", "lgpu_write_png", "lgpu_film_write", "lgpu_film_denoise",
     "lgpu_render_noalloc", "lgpu_guide_create", "lgpu_guide_destroy", "lgpu_guide_num_params",
     "lgpu_guide_get_params", "lgpu_guide_set_params", "lgpu_guide_forward", "lgpu_guide_gradient",
-    "lgpu_guide_train", "lgpu_guide_stats",
+    "lgpu_guide_train", "lgpu_guide_stats", "lgpu_render_adjoint", "lgpu_scene_update_texture",
+    "lgpu_scene_update_density",
 ]
 
 
@@ -55,7 +56,8 @@ def lib():
             "lgpu_scene_info": [vp, vp, vp, vp, vp], "lgpu_scene_light_tree": [vp, vp, u32],
             "lgpu_film_create": [vp, u32, u32, vp], "lgpu_film_destroy": [vp], "lgpu_film_clear": [vp],
             "lgpu_film_read": [vp, vp, vp], "lgpu_film_read_rgba": [vp, vp], "lgpu_render": [vp, vp],
-            "lgpu_render_noalloc": [vp, vp],
+            "lgpu_render_noalloc": [vp, vp], "lgpu_render_adjoint": [vp, vp, vp],
+            "lgpu_scene_update_texture": [vp, u32, vp], "lgpu_scene_update_density": [vp, u32, vp],
             "lgpu_guide_create": [vp, vp, vp], "lgpu_guide_destroy": [vp], "lgpu_guide_get_params": [vp, vp],
             "lgpu_guide_set_params": [vp, vp], "lgpu_guide_forward": [vp, u32, vp, vp],
             "lgpu_guide_gradient": [vp, u32, vp, vp, vp], "lgpu_guide_train": [vp, u32, vp],
@@ -179,6 +181,16 @@ class Scene:
 
     def commit(self, strategy=abi.LIGHTS_TREE):
         _check(lib().lgpu_scene_commit(self.h, strategy))
+
+    def update_texture(self, tex_id, texels):
+        """Replaces a committed texture's texels (w * h * channels floats) in place."""
+        t = np.ascontiguousarray(texels, dtype=np.float32)
+        _check(lib().lgpu_scene_update_texture(self.h, tex_id, _ptr(t)))
+
+    def update_density(self, medium_id, density):
+        """Replaces a committed grid medium's densities (nx * ny * nz floats) in place."""
+        g = np.ascontiguousarray(density, dtype=np.float32)
+        _check(lib().lgpu_scene_update_density(self.h, medium_id, _ptr(g)))
 
     def info(self):
         v = [C.c_uint32() for _ in range(4)]
@@ -434,6 +446,25 @@ def render(desc):
     st = abi.RenderStats()
     _check(lib().lgpu_render(C.byref(desc), C.byref(st)))
     return st
+
+
+def render_adjoint(desc, d_image, n_texels=0, n_density=0, n_materials=0):
+    """Path-replay adjoint render: accumulates the primal samples into desc's film and returns the
+    gradients of sum(d_image * film mean of this call's samples) as float32 arrays
+    (textures in id order, grid densities in medium id order, per-material emission intensity;
+    None where the count is 0), and the render stats."""
+    d_image = np.ascontiguousarray(d_image, dtype=np.float32)
+    gt = np.zeros(n_texels, dtype=np.float32) if n_texels else None
+    gd = np.zeros(n_density, dtype=np.float32) if n_density else None
+    ge = np.zeros(n_materials, dtype=np.float32) if n_materials else None
+    a = abi.AdjointDesc()
+    a.d_image = d_image.ctypes.data
+    a.grad_textures = gt.ctypes.data if gt is not None else None
+    a.grad_density = gd.ctypes.data if gd is not None else None
+    a.grad_emission = ge.ctypes.data if ge is not None else None
+    st = abi.RenderStats()
+    _check(lib().lgpu_render_adjoint(C.byref(desc), C.byref(a), C.byref(st)))
+    return gt, gd, ge, st
 
 
 def trace_paths(desc, pixels, samples):

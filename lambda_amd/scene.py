@@ -539,6 +539,108 @@ def lobe_zoo(level=2):
     return sd
 
 
+def _planar_uv(v):
+    """uv of box vertices: (x, y) and z folded into [0, 1] so every face varies over the texture."""
+    v = np.asarray(v, dtype=np.float32)
+    return np.stack([(v[:, 0] + 1.0) * 0.4 + (v[:, 2] + 1.0) * 0.1, (v[:, 1] + 1.0) * 0.45], axis=1).astype(np.float32)
+
+
+def inverse_room(tex_res=16, medium=True, point_light=False, light_intensity=10.0, seed=3):
+    """Differentiable-rendering workload (config-4 style): the Cornell room whose floor, back wall
+    and tall box read their albedo from one RGB texture and their roughness from two scalar
+    textures; optionally a grid medium (6^3 random densities) in a pass-through box
+    [-0.8,0.8] x [0.3,0.75] x [-0.8,0.8] under the ceiling light, or a point light instead of the
+    emissive quad.
+      texture 0: albedo, tex_res^2 RGB in [0.2, 0.8]
+      texture 1: roughness, tex_res^2 in [0.25, 0.7] (GGX conductor, through RoughnessToAlpha)
+      texture 2: alpha, tex_res^2 in [0.1, 0.4] (Beckmann conductor, alpha given directly)
+      floor:     Lambertian(albedo)
+      back wall: Mix(Lambertian(albedo), Microfacet GGX conductor(roughness), 0.5)
+      tall box:  Microfacet Beckmann conductor(R = albedo, alpha)
+    Returns the SceneDesc and a dict of ids (textures, medium, light material)."""
+    rng = np.random.default_rng(seed)
+    R = tex_res
+    sd = SceneDesc()
+    N = abi
+    t_alb = sd.add_texture(rng.uniform(0.2, 0.8, size=(R, R, 3)).astype(np.float32))
+    t_rough = sd.add_texture(rng.uniform(0.25, 0.7, size=(R, R)).astype(np.float32))
+    t_alpha = sd.add_texture(rng.uniform(0.1, 0.4, size=(R, R)).astype(np.float32))
+    med = abi.NONE
+    if medium:
+        dens = rng.uniform(0.0, 1.5, size=(6, 6, 6)).astype(np.float32)
+        med = grid_medium(sd, dens, (0.15, 0.15, 0.15), (0.9, 0.75, 0.6), 0.3, (-0.8, 0.3, -0.8), (0.8, 0.75, 0.8))
+    white = sd.add_material(lambertian((0.73, 0.73, 0.73)))
+    red = sd.add_material(lambertian((0.65, 0.05, 0.05)))
+    green = sd.add_material(lambertian((0.12, 0.45, 0.15)))
+    floor = sd.add_material(Material([node(N.NODE_IMAGE_COLOUR, p=[t_alb]), node(N.NODE_LAMBERTIAN, [0])], bxdf=1))
+    ggx = [1.0, 1.0, 1.45, *COPPER_ETA, *COPPER_K, 0.0]
+    back = sd.add_material(Material([
+        node(N.NODE_IMAGE_COLOUR, p=[t_alb]), node(N.NODE_LAMBERTIAN, [0]), node(N.NODE_RGB, p=(0.9, 0.9, 0.9)),
+        node(N.NODE_IMAGE_SCALAR, p=[t_rough]), node(N.NODE_MICROFACET_BRDF, [2, 3], p=ggx),
+        node(N.NODE_SCALAR, p=[0.5]), node(N.NODE_MIX_BSDF, [1, 4, 5])], bxdf=6))
+    beck = [0.0, 1.0, 1.45, *COPPER_ETA, *COPPER_K, 1.0]
+    tall = sd.add_material(Material([
+        node(N.NODE_IMAGE_COLOUR, p=[t_alb]), node(N.NODE_IMAGE_CHANNEL, p=[t_alpha, 0]),
+        node(N.NODE_MICROFACET_BRDF, [0, 1], p=beck)], bxdf=2))
+    light = abi.NONE
+    if not point_light:
+        light = sd.add_material(emitter_blackbody(6500.0, light_intensity))
+    v, i, uv = _quad_uv((-1, -1, -1), (-1, -1, 1), (1, -1, 1), (1, -1, -1))
+    sd.add_mesh(v, i, floor, uvs=uv)
+    v, i = _quad((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1))
+    sd.add_mesh(v, i, white)
+    v, i, uv = _quad_uv((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1))
+    sd.add_mesh(v, i, back, uvs=uv)
+    v, i = _quad((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1))
+    sd.add_mesh(v, i, red)
+    v, i = _quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1))
+    sd.add_mesh(v, i, green)
+    v, i = _box((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 15.0)
+    sd.add_mesh(v, i, tall, uvs=_planar_uv(v))
+    v, i = _box((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -18.0)
+    sd.add_mesh(v, i, white)
+    if point_light:
+        sd.add_point_light((0.1, 0.85, 0.2), 1.5)
+    else:
+        y = 0.999
+        v, i = _quad((-0.25, y, -0.25), (0.25, y, -0.25), (0.25, y, 0.25), (-0.25, y, 0.25))
+        sd.add_mesh(v, i, light)
+    if medium:
+        fog = sd.add_material(passthrough(interior=med))
+        v, i = aabb((-0.8, 0.3, -0.8), (0.8, 0.75, 0.8))
+        sd.add_mesh(v, i, fog)
+    return sd, dict(albedo=t_alb, roughness=t_rough, alpha=t_alpha, medium=med, light=light)
+
+
+def slab_scene(tex_res=8, grid=(8, 4, 8), seed=5):
+    """A textured floor lit by an emissive quad through a grid-medium slab
+    [-0.8,0.8] x [0.1,0.5] x [-0.8,0.8], seen by slab_camera from below the slab: camera rays never
+    enter the medium, every light path to the floor crosses it (transmittance only)."""
+    rng = np.random.default_rng(seed)
+    sd = SceneDesc()
+    N = abi
+    t_alb = sd.add_texture(rng.uniform(0.3, 0.9, size=(tex_res, tex_res, 3)).astype(np.float32))
+    nx, ny, nz = grid
+    dens = rng.uniform(0.2, 1.8, size=(nz, ny, nx)).astype(np.float32)
+    med = grid_medium(sd, dens, (0.6, 0.8, 1.0), (0.4, 0.3, 0.2), 0.0, (-0.8, 0.1, -0.8), (0.8, 0.5, 0.8))
+    floor = sd.add_material(Material([node(N.NODE_IMAGE_COLOUR, p=[t_alb]), node(N.NODE_LAMBERTIAN, [0])], bxdf=1))
+    light = sd.add_material(emitter_blackbody(6500.0, 12.0))
+    v, i, uv = _quad_uv((-1, -1, -1), (-1, -1, 1), (1, -1, 1), (1, -1, -1))
+    sd.add_mesh(v, i, floor, uvs=uv)
+    y = 0.95
+    v, i = _quad((-0.4, y, -0.4), (0.4, y, -0.4), (0.4, y, 0.4), (-0.4, y, 0.4))
+    sd.add_mesh(v, i, light)
+    fog = sd.add_material(passthrough(interior=med))
+    v, i = aabb((-0.8, 0.1, -0.8), (0.8, 0.5, 0.8))
+    sd.add_mesh(v, i, fog)
+    return sd, dict(albedo=t_alb, medium=med, light=light)
+
+
+def slab_camera(width=64, height=64):
+    """Pinhole at (0, -0.2, 2.2) pitched 0.45 rad down, SetFov(0.6): every ray descends."""
+    return camera_pinhole((0.0, -0.2, 2.2), width, height, math.pi, -0.45, 0.6)
+
+
 def camera_spherical(origin, width, height, phi=math.pi, theta=0.0, offset_phi=0.0, offset_theta=0.0):
     """SphericalCamera (Camera.cpp:66-71): u -> phi = 2 pi u + offset, v -> theta = pi v + offset."""
     cam = _camera(abi.CAMERA_SPHERICAL, origin, width, height, phi, theta, 1.0)

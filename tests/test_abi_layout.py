@@ -21,7 +21,7 @@ STRUCTS = {
     "LightDesc": "lgpu_light_desc", "MediumDesc": "lgpu_medium_desc", "TextureDesc": "lgpu_texture_desc",
     "Camera": "lgpu_camera", "RenderDesc": "lgpu_render_desc", "RenderStats": "lgpu_render_stats",
     "PostDesc": "lgpu_post_desc", "DenoiseDesc": "lgpu_denoise_desc", "ObjMaterial": "lgpu_obj_material",
-    "GuideDesc": "lgpu_guide_desc",
+    "GuideDesc": "lgpu_guide_desc", "AdjointDesc": "lgpu_adjoint_desc",
 }
 C_FIELD = {"in_": "in"}  # Python keyword clashes
 
