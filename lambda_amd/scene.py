@@ -887,6 +887,87 @@ def many_light_scene(n_spheres=200, sphere_level=4, ground_res=297, n_light_quad
     return sd
 
 
+def _smooth_field(rng, res, coarse, lo, hi, channels=1):
+    """A res x res texture: a coarse random grid upsampled bilinearly, in [lo, hi]."""
+    g = rng.uniform(lo, hi, size=(coarse + 1, coarse + 1, channels))
+    t = (np.arange(res) + 0.5) / res * coarse
+    i = np.minimum(t.astype(int), coarse - 1)
+    f = (t - i)[:, None]
+    rows = g[i] * (1 - f)[:, :, None] + g[i + 1] * f[:, :, None]          # (res, coarse + 1, ch)
+    out = rows[:, i] * (1 - f.T)[:, :, None] + rows[:, i + 1] * f.T[:, :, None]
+    out = out.astype(np.float32)
+    return out[..., 0] if channels == 1 else out
+
+
+def inverse_scene(tex_res=512, grid_res=64, n_spheres=9, ground_res=48, n_light_quads=64, seed=0, albedo=None,
+                  roughness=None, density=None):
+    """The differentiable inverse-rendering workload: a 12 x 12 patch of the many-light field
+    (ground_res^2-quad ground and n_spheres level-4 displaced icospheres: 50,688 triangles),
+    tex_res^2 FP32 albedo (RGB) and roughness textures on every surface (ground: Lambertian with
+    the albedo; spheres: Mix(Lambertian(albedo), GGX conductor(roughness through
+    RoughnessToAlpha)) with spherical uvs), a grid_res^3 fBm density grid (sigma_t 2 d, albedo
+    0.8, HG g 0.3) in a pass-through box [-4,4] x [1.2,2.7] x [-4,4] (a cloud layer between the
+    spheres and the lights: no surface lies inside it), n_light_quads emissive quads at heights
+    2.8 to 4 through the light tree, and the HDR sky.
+    albedo / roughness / density: parameter arrays replacing the generated ones (same shapes).
+    Returns the SceneDesc and the ids {albedo, roughness (texture ids), medium}."""
+    rng = np.random.default_rng(seed)
+    sd = SceneDesc()
+    N = abi
+    alb = _smooth_field(rng, tex_res, 16, 0.25, 0.85, 3) if albedo is None else albedo
+    rough = _smooth_field(rng, tex_res, 16, 0.2, 0.7) if roughness is None else roughness
+    t_alb = sd.add_texture(alb)
+    t_rough = sd.add_texture(rough)
+    dens = fbm_density(grid_res, octaves=4, seed=seed + 11) * 1.5 if density is None else density
+    med = grid_medium(sd, dens, (0.4, 0.4, 0.4), (1.6, 1.6, 1.6), 0.3, (-4.0, 1.2, -4.0), (4.0, 2.7, 4.0))
+    ground_m = sd.add_material(Material([node(N.NODE_IMAGE_COLOUR, p=[t_alb]), node(N.NODE_LAMBERTIAN, [0])], bxdf=1))
+    ggx = [1.0, 1.0, 1.45, *COPPER_ETA, *COPPER_K, 0.0]
+    sphere_m = sd.add_material(Material([
+        node(N.NODE_IMAGE_COLOUR, p=[t_alb]), node(N.NODE_LAMBERTIAN, [0]), node(N.NODE_RGB, p=(0.95, 0.95, 0.95)),
+        node(N.NODE_IMAGE_SCALAR, p=[t_rough]), node(N.NODE_MICROFACET_BRDF, [2, 3], p=ggx),
+        node(N.NODE_SCALAR, p=[0.5]), node(N.NODE_MIX_BSDF, [1, 4, 5])], bxdf=6))
+    g = ground_res
+    xs = np.linspace(-6, 6, g + 1)
+    gx, gz = np.meshgrid(xs, xs, indexing="ij")
+    gv = np.stack([gx.ravel(), np.zeros(gx.size), gz.ravel()], axis=1)
+    guv = np.stack([(gx.ravel() + 6) / 12, (gz.ravel() + 6) / 12], axis=1)
+    idx = np.arange((g + 1) * (g + 1)).reshape(g + 1, g + 1)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
+    gf = np.concatenate([np.stack([a, d, c], 1), np.stack([a, c, b], 1)])
+    sd.add_mesh(gv, gf, ground_m, uvs=guv)
+    base_v, base_f = icosphere(4)
+    cells = rng.choice(144, size=n_spheres, replace=False)
+    suv = np.stack([np.arctan2(base_v[:, 2], base_v[:, 0]) / (2 * np.pi) + 0.5,
+                    np.arccos(np.clip(base_v[:, 1], -1, 1)) / np.pi], axis=1)
+    for cell in cells:
+        cx, cz = -6 + (cell % 12) + 0.5, -6 + (cell // 12) + 0.5
+        r = rng.uniform(0.3, 0.5)
+        centre = np.array([cx, r, cz])
+        p = base_v * r + centre
+        p = p + base_v * (perlin3(p * 4.0, seed=seed + 1) * 0.05)[:, None]
+        n = _vertex_normals(p, base_f)
+        sd.add_mesh(p, base_f, sphere_m, normals=n, tangents=_tangents(n), uvs=suv)
+    for q in range(n_light_quads):
+        T = float(rng.uniform(3000, 9000))
+        inten = float(10 ** rng.uniform(1, 2.5))
+        c = np.array([rng.uniform(-6, 6), rng.uniform(2.8, 4), rng.uniform(-6, 6)])
+        s_ = 0.05
+        v = np.array([[c[0] - s_, c[1], c[2] - s_], [c[0] + s_, c[1], c[2] - s_], [c[0] + s_, c[1], c[2] + s_],
+                      [c[0] - s_, c[1], c[2] + s_]])
+        m = sd.add_material(emitter_blackbody(T, inten))
+        sd.add_mesh(v, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32), m)
+    fog = sd.add_material(passthrough(interior=med))
+    v, i = aabb((-4.0, 1.2, -4.0), (4.0, 2.7, 4.0))
+    sd.add_mesh(v, i, fog)
+    sd.add_env_light(sky(256, 128), intensity=0.5)
+    return sd, dict(albedo=t_alb, roughness=t_rough, medium=med)
+
+
+def inverse_camera(width=512, height=512):
+    """Pinhole at (0, 5, 11) looking at the patch centre, SetFov(0.9)."""
+    return camera_pinhole((0.0, 5.0, 11.0), width, height, math.pi, -math.atan2(5.0, 11.0), 0.9)
+
+
 def many_light_camera(width=1920, height=1080):
     """Thin lens at (0, 6, 24) looking at the field centre, aperture 0.05, focused on the centre."""
     origin = (0.0, 6.0, 24.0)
