@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define LGPU_NONE 0xFFFFFFFFu
-#define LGPU_ABI_VERSION 7u
+#define LGPU_ABI_VERSION 8u
 
 typedef enum lgpu_status {
   LGPU_OK = 0,
@@ -356,6 +356,33 @@ lgpu_status lgpu_render(const lgpu_render_desc *desc, lgpu_render_stats *stats);
  * for this scene) beyond that is rejected with LGPU_ERR_INVALID_ARGUMENT. Every call on one GPU
  * shares its array: calls that overlap on the same GPU (from different device objects) race. */
 lgpu_status lgpu_render_noalloc(const lgpu_render_desc *desc, lgpu_render_stats *stats);
+
+/* ---------------- batched renders (MosaicRenderer::Render's tile dispatch, MosaicRenderer.cpp:22-80
+ * and Render.cpp:11-39, across many scenes): every job (scene, film, camera, spp range) renders
+ * from one shared path pool whose segments are sized in proportion to the jobs' paths. Each chunk
+ * is one hipGraph holding every job's whole wave (camera, bounce iterations with device-side
+ * queue counts, film update) on several independent branches; chunks are captured while the
+ * previous ones run. Jobs take the path or volumetric path integrators with the corrected
+ * estimator and no guide; films must be distinct. Each film equals lgpu_render's for the same job
+ * (same per-path radiance; the FP64 sums group samples per chunk, so they agree to rounding). */
+#define LGPU_BATCH_AUTOTUNE 1u /* time pool capacity 2^22..2^25 x branches 1..16 on one-sample trials; cached */
+typedef struct lgpu_batch_options {
+  uint32_t path_capacity;  /* paths in flight over all jobs (0: 2^24, or the autotuner's choice) */
+  uint32_t flags;          /* LGPU_BATCH_* */
+  uint32_t max_iterations; /* bounce iterations captured per chunk (0: 2 max_bounces + 8); paths still
+                              alive after them fail the call with LGPU_ERR_STATE */
+  uint32_t branches;       /* independent graph branches (capture streams), 1..16 (0: 8) */
+} lgpu_batch_options;
+typedef struct lgpu_batch_stats {
+  uint64_t samples;
+  double ms_total;         /* the call, autotuning excluded */
+  double ms_render;        /* first capture to completion */
+  double ms_capture;       /* host time capturing and instantiating the chunk graphs */
+  double ms_autotune;      /* trials of this call (0 when the choice was cached) */
+  uint32_t chunks, graph_nodes, path_capacity, branches, iterations, jobs;
+} lgpu_batch_stats;
+lgpu_status lgpu_render_batch(const lgpu_render_desc *descs, uint32_t count, const lgpu_batch_options *options,
+                              lgpu_batch_stats *stats);
 
 /* ---------------- path-replay adjoint (no reference counterpart: the differentiation extension
  * of VolumetricPathIntegrator::Li, VolumetricPathIntegrator.cpp:20-162, and through it of

@@ -5,7 +5,7 @@ binding, so both are driven from the same scene description.
 """
 import ctypes as C
 
-ABI_VERSION = 7  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
+ABI_VERSION = 8  # LGPU_ABI_VERSION of include/lambda_gpu.h (struct layouts below)
 NONE = 0xFFFFFFFF
 
 # lgpu_status
@@ -96,6 +96,20 @@ class GuideDesc(C.Structure):
 RENDER_COUNT_TRAVERSAL = 1
 
 ADJOINT_DEVICE_POINTERS = 1
+
+BATCH_AUTOTUNE = 1
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("path_capacity", C.c_uint32), ("flags", C.c_uint32), ("max_iterations", C.c_uint32),
+                ("branches", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("ms_total", C.c_double), ("ms_render", C.c_double),
+                ("ms_capture", C.c_double), ("ms_autotune", C.c_double), ("chunks", C.c_uint32),
+                ("graph_nodes", C.c_uint32), ("path_capacity", C.c_uint32), ("branches", C.c_uint32),
+                ("iterations", C.c_uint32), ("jobs", C.c_uint32)]
 
 
 class AdjointDesc(C.Structure):

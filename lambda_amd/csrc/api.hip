@@ -63,6 +63,7 @@ lgpu_status lgpu_device_destroy(lgpu_device d) {
   (void)hipSetDevice(d->device);
   (void)hipStreamSynchronize(d->stream);
   if (d->wave && d->wave_free) d->wave_free(d->wave);
+  if (d->batch && d->batch_free) d->batch_free(d->batch);
   // guides still alive lose their device memory here; their handles stay valid for destroy
   if (d->guide_orphan)
     for (void *g : d->guides) d->guide_orphan(g);

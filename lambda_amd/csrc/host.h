@@ -83,6 +83,8 @@ struct lgpu_device_s {
   hipEvent_t ev[8] = {};
   void *wave = nullptr;               // path-state pool of the render loop
   void (*wave_free)(void *) = nullptr;
+  void *batch = nullptr;              // capture streams and events of batched renders
+  void (*batch_free)(void *) = nullptr;
   void *scratch = nullptr;            // device address of this GPU's no-allocation scratch array
   std::vector<void *> guides;         // live guiding networks (released with the device)
   void (*guide_orphan)(void *) = nullptr;
@@ -127,4 +129,5 @@ struct lgpu_scene_s {
 namespace lgpu {
 lgpu_status CommitScene(lgpu_scene s, uint32_t strategy);
 lgpu_status ResolveScratch(lgpu_device dev);  // render.hip: the no-allocation scratch array's address
+RenderParams MakeParams(const lgpu_render_desc *d, uint32_t w, uint32_t h);  // render.hip
 }

@@ -359,11 +359,26 @@ struct RenderParams {
   uint32_t pass_normal_g;
 };
 
+// Batched renders (batch.hip) record the single-render kernels through these (render.hip).
+void BatchLaunchInit(hipStream_t s, uint32_t *ctr, uint32_t words, uint32_t n);
+void BatchLaunchTrunc(hipStream_t s, const uint32_t *cnt, unsigned long long *acc);
+void BatchLaunchCamera(hipStream_t s, const RenderParams &rp, const PathState &ps, const SampleState &ss, uint32_t n,
+                       uint64_t base);
+void BatchLaunchExtend(hipStream_t s, const DevScene &dsc, const PathState &ps, uint32_t ub, const uint32_t *np,
+                       uint32_t *spill);
+void BatchLaunchShade(hipStream_t s, const DevScene *scp, const RenderParams &rp, const PathState &ps,
+                      const SampleState &ss, uint32_t ub, const uint32_t *np, const Queues &q, bool programs);
+void BatchLaunchShadow(hipStream_t s, const DevScene &dsc, const SampleState &ss, const ShadowRay *rays, uint32_t ub,
+                       const uint32_t *np, uint32_t *spill);
+void BatchLaunchAccumulate(hipStream_t s, const RenderParams &rp, const SampleState &ss, uint32_t n, uint32_t npx,
+                           double *film, uint32_t *counts);
+
 // Volumetric shade stage (volume.hip): VolumetricPathIntegrator / MISVolumetricPathIntegrator;
 // with `adj`, the adjoint replay (adjoint.h).
 struct AdjRefs;
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
-                    SampleState ss, uint32_t n, Queues q, uint32_t *spill, const AdjRefs *adj = nullptr);
+                    SampleState ss, uint32_t n, Queues q, uint32_t *spill, const AdjRefs *adj = nullptr,
+                    const uint32_t *np = nullptr);
 // Utility-pass shade stage (utility.hip): DepthPass, NormalPass, AOVPass.
 void LaunchShadeUtility(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
                         SampleState ss, uint32_t n);

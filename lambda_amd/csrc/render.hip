@@ -603,6 +603,44 @@ static void LaunchShade(bool corrected, bool programs, const GuideRefs *gr, uint
   else hipLaunchKernelGGL((k_shade<false, false, false>), g, b, 0, stream, scp, rp, ps, ss, n, np, q, none);
 }
 
+// ---------------------------------------------------------------- batched renders (batch.hip)
+// Counter slots of one job's chunk: slot 0 = the camera's path count, the rest cleared (inside a
+// captured graph, where memset nodes are avoided).
+__global__ void k_batch_init(uint32_t *ctr, uint32_t words, uint32_t n) {
+  for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) ctr[k] = k == 0 ? n : 0u;
+}
+// Paths still queued after a chunk's last captured iteration (must stay 0).
+__global__ void k_batch_trunc(const uint32_t *cnt, unsigned long long *acc) {
+  if (threadIdx.x == 0 && *cnt) atomicAdd(acc, (unsigned long long)*cnt);
+}
+void BatchLaunchInit(hipStream_t s, uint32_t *ctr, uint32_t words, uint32_t n) {
+  hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, ctr, words, n);
+}
+void BatchLaunchTrunc(hipStream_t s, const uint32_t *cnt, unsigned long long *acc) {
+  hipLaunchKernelGGL(k_batch_trunc, dim3(1), dim3(64), 0, s, cnt, acc);
+}
+void BatchLaunchCamera(hipStream_t s, const RenderParams &rp, const PathState &ps, const SampleState &ss, uint32_t n,
+                       uint64_t base) {
+  hipLaunchKernelGGL(k_camera, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, ps, ss, n, base, nullptr, nullptr);
+}
+void BatchLaunchExtend(hipStream_t s, const DevScene &dsc, const PathState &ps, uint32_t ub, const uint32_t *np,
+                       uint32_t *spill) {
+  hipLaunchKernelGGL(k_extend<false>, dim3((ub + kBlock - 1) / kBlock), dim3(kBlock), 0, s, dsc, ps, ub, np, spill, nullptr);
+}
+void BatchLaunchShade(hipStream_t s, const DevScene *scp, const RenderParams &rp, const PathState &ps,
+                      const SampleState &ss, uint32_t ub, const uint32_t *np, const Queues &q, bool programs) {
+  LaunchShade(true, programs, nullptr, (ub + kBlock - 1) / kBlock, s, scp, rp, ps, ss, ub, np, q);
+}
+void BatchLaunchShadow(hipStream_t s, const DevScene &dsc, const SampleState &ss, const ShadowRay *rays, uint32_t ub,
+                       const uint32_t *np, uint32_t *spill) {
+  hipLaunchKernelGGL(k_shadow, dim3((ub + kBlock - 1) / kBlock), dim3(kBlock), 0, s, dsc, ss, rays, ub, np, spill);
+}
+void BatchLaunchAccumulate(hipStream_t s, const RenderParams &rp, const SampleState &ss, uint32_t n, uint32_t npx,
+                           double *film, uint32_t *counts) {
+  const uint32_t m = n < npx ? n : npx;
+  hipLaunchKernelGGL(k_accumulate_px, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, ss, n, npx, film, counts);
+}
+
 // Guiding hooks (guide.hip).
 lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expectedVertices, GuideRefs *refs,
                              const uint4 *bins, const float2 *hdr, hipStream_t s);
@@ -732,7 +770,7 @@ static lgpu_status ValidateRender(const lgpu_render_desc *d) {
   return LGPU_OK;
 }
 
-static RenderParams MakeParams(const lgpu_render_desc *d, uint32_t w, uint32_t h) {
+RenderParams MakeParams(const lgpu_render_desc *d, uint32_t w, uint32_t h) {
   RenderParams rp;
   std::memset(&rp, 0, sizeof(rp));
   const lgpu_camera &c = d->camera;

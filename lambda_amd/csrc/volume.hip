@@ -94,11 +94,14 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
 // compiled without the shader VM.
 // kAdjoint: the replay of the adjoint render (corrected estimator, RGB, VOLPATH): the same path
 // decisions and radiance as the primal launch, plus the gradient pieces of adjoint.h.
+// np: when set, the item count is read from this device counter (the grid covers an upper bound).
 template <bool kPrograms, bool kAdjoint>
 __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams rp, PathState ps, SampleState ss,
-                                                      uint32_t n, Queues q, uint32_t *spillBase, AdjRefs ar) {
+                                                      uint32_t n, const uint32_t *np, Queues q, uint32_t *spillBase,
+                                                      AdjRefs ar) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
   __shared__ TreeImp treeLds[kTreeLds];
+  if (np) n = *np;
   LoadTreeLds(sc, treeLds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   EmitAcc eacc{{kNone, kNone}, {0.f, 0.f}};
@@ -468,17 +471,18 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
 }  // namespace
 
 void LaunchShadeVol(uint32_t blocks, hipStream_t stream, const DevScene &sc, const RenderParams &rp, PathState ps,
-                    SampleState ss, uint32_t n, Queues q, uint32_t *spill, const AdjRefs *adj) {
+                    SampleState ss, uint32_t n, Queues q, uint32_t *spill, const AdjRefs *adj, const uint32_t *np) {
   const AdjRefs ar = adj ? *adj : AdjRefs{};
+  const dim3 g(blocks), b(kBlock);
   if (adj) {
     if (sc.has_programs)
-      hipLaunchKernelGGL((k_shade_vol<true, true>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+      hipLaunchKernelGGL((k_shade_vol<true, true>), g, b, 0, stream, sc, rp, ps, ss, n, np, q, spill, ar);
     else
-      hipLaunchKernelGGL((k_shade_vol<false, true>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+      hipLaunchKernelGGL((k_shade_vol<false, true>), g, b, 0, stream, sc, rp, ps, ss, n, np, q, spill, ar);
   } else if (sc.has_programs) {
-    hipLaunchKernelGGL((k_shade_vol<true, false>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+    hipLaunchKernelGGL((k_shade_vol<true, false>), g, b, 0, stream, sc, rp, ps, ss, n, np, q, spill, ar);
   } else {
-    hipLaunchKernelGGL((k_shade_vol<false, false>), dim3(blocks), dim3(kBlock), 0, stream, sc, rp, ps, ss, n, q, spill, ar);
+    hipLaunchKernelGGL((k_shade_vol<false, false>), g, b, 0, stream, sc, rp, ps, ss, n, np, q, spill, ar);
   }
 }
 

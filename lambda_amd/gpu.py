@@ -31,7 +31,7 @@ EXPORTS = [
     "lgpu_render_noalloc", "lgpu_guide_create", "lgpu_guide_destroy", "lgpu_guide_num_params",
     "lgpu_guide_get_params", "lgpu_guide_set_params", "lgpu_guide_forward", "lgpu_guide_gradient",
     "lgpu_guide_train", "lgpu_guide_stats", "lgpu_render_adjoint", "lgpu_scene_update_texture",
-    "lgpu_scene_update_density",
+    "lgpu_scene_update_density", "lgpu_render_batch",
 ]
 
 
@@ -56,7 +56,7 @@ def lib():
             "lgpu_scene_info": [vp, vp, vp, vp, vp], "lgpu_scene_light_tree": [vp, vp, u32],
             "lgpu_film_create": [vp, u32, u32, vp], "lgpu_film_destroy": [vp], "lgpu_film_clear": [vp],
             "lgpu_film_read": [vp, vp, vp], "lgpu_film_read_rgba": [vp, vp], "lgpu_render": [vp, vp],
-            "lgpu_render_noalloc": [vp, vp], "lgpu_render_adjoint": [vp, vp, vp],
+            "lgpu_render_noalloc": [vp, vp], "lgpu_render_adjoint": [vp, vp, vp], "lgpu_render_batch": [vp, u32, vp, vp],
             "lgpu_scene_update_texture": [vp, u32, vp], "lgpu_scene_update_density": [vp, u32, vp],
             "lgpu_guide_create": [vp, vp, vp], "lgpu_guide_destroy": [vp], "lgpu_guide_get_params": [vp, vp],
             "lgpu_guide_set_params": [vp, vp], "lgpu_guide_forward": [vp, u32, vp, vp],
@@ -445,6 +445,19 @@ def render(desc):
     """The render call (MosaicRenderer::Render): accumulates spp [begin, end) into the film."""
     st = abi.RenderStats()
     _check(lib().lgpu_render(C.byref(desc), C.byref(st)))
+    return st
+
+
+def render_batch(descs, path_capacity=0, autotune=False, branches=0, max_iterations=0):
+    """Renders every job (render_desc) together from one path pool, one hipGraph per chunk."""
+    arr = (abi.RenderDesc * len(descs))(*descs)
+    o = abi.BatchOptions()
+    o.path_capacity = path_capacity
+    o.flags = abi.BATCH_AUTOTUNE if autotune else 0
+    o.max_iterations = max_iterations
+    o.branches = branches
+    st = abi.BatchStats()
+    _check(lib().lgpu_render_batch(arr, len(descs), C.byref(o), C.byref(st)))
     return st
 
 

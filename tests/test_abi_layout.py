@@ -22,6 +22,7 @@ STRUCTS = {
     "Camera": "lgpu_camera", "RenderDesc": "lgpu_render_desc", "RenderStats": "lgpu_render_stats",
     "PostDesc": "lgpu_post_desc", "DenoiseDesc": "lgpu_denoise_desc", "ObjMaterial": "lgpu_obj_material",
     "GuideDesc": "lgpu_guide_desc", "AdjointDesc": "lgpu_adjoint_desc",
+    "BatchOptions": "lgpu_batch_options", "BatchStats": "lgpu_batch_stats",
 }
 C_FIELD = {"in_": "in"}  # Python keyword clashes
 
