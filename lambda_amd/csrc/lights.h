@@ -84,7 +84,7 @@ __device__ LGPU_LOBE_INLINE inline float LightPdf(const DevScene &sc, uint32_t p
 }
 
 __device__ inline Spec EnvLookup(const DevScene &sc, float u, float v, const VmCtx &vm) {
-  float c[3];
+  float c[4];  // TexBilinear also writes out[3] (alpha)
   TexBilinear(sc.texels + sc.env.tex_off, sc.env.w, sc.env.h, 3, Fract(u), Fract(v), c);
   return S4(UpliftV(c[0], c[1], c[2], vm));
 }

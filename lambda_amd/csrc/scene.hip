@@ -1065,7 +1065,7 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
         const float sinT = std::sin(kPi * (float)(y + .5) / (float)h);
         for (uint32_t x = 0; x < w; ++x) {
           const float up = (float)x / (float)w;
-          float c[3];
+          float c[4];  // TexBilinear also writes out[3] (alpha)
           TexBilinear(hl.env.data(), w, h, 3, Fract(up), Fract(vp), c);
           img[x + y * w] = std::fabs(0.212671f * c[0] + 0.715160f * c[1] + 0.072169f * c[2]);
           img[x + y * w] *= sinT;

@@ -968,6 +968,37 @@ def inverse_camera(width=512, height=512):
     return camera_pinhole((0.0, 5.0, 11.0), width, height, math.pi, -math.atan2(5.0, 11.0), 0.9)
 
 
+def batch_job(k, seed=0):
+    """Job k of the 64-scene batch workload: a patch of the many-light field of log-uniform size
+    (about 10k to 2M triangles: icosphere level 3 or 4 spheres plus a ground grid), 8 to 2048
+    emissive quads (16 to 4096 emissive triangles) through the light tree, a square resolution
+    log-uniform in 64..1024; one job in four adds a grid medium cloud (32^3 or 64^3 fBm) over the
+    field and renders with the volumetric path integrator, the others with the path integrator and
+    the HDR sky. Returns (SceneDesc, camera, width, height, integrator, max_bounces)."""
+    rng = np.random.default_rng(1000 + 7919 * seed + k)
+    tris = 10000.0 * 200.0 ** rng.uniform()
+    level = 4 if tris > 60000 else 3
+    per = 5120 if level == 4 else 1280
+    ground = int(max(8, min(160, math.sqrt(tris * 0.1 / 2))))
+    n_spheres = int(max(1, min(1600, (tris - 2 * ground * ground) / per)))
+    n_quads = int(8 * 256 ** rng.uniform())
+    media = k % 4 == 3
+    sd = many_light_scene(n_spheres=n_spheres, sphere_level=level, ground_res=ground, n_light_quads=n_quads,
+                          env=not media, seed=1000 + k)
+    if media:
+        res = 64 if rng.uniform() < 0.5 else 32
+        dens = fbm_density(res, octaves=4, seed=k) * 1.2
+        med = grid_medium(sd, dens, (0.3, 0.3, 0.3), (1.2, 1.1, 1.0), 0.4, (-20.0, 1.5, -20.0), (20.0, 4.0, 20.0))
+        fog = sd.add_material(passthrough(interior=med))
+        v, i = aabb((-20.0, 1.5, -20.0), (20.0, 4.0, 20.0))
+        sd.add_mesh(v, i, fog)
+    W = int(64 * 16 ** rng.uniform())
+    W = max(64, min(1024, W - W % 8))
+    cam = many_light_camera(W, W)
+    integ = abi.INTEGRATOR_VOLPATH if media else abi.INTEGRATOR_PATH
+    return sd, cam, W, W, integ, 8
+
+
 def many_light_camera(width=1920, height=1080):
     """Thin lens at (0, 6, 24) looking at the field centre, aperture 0.05, focused on the centre."""
     origin = (0.0, 6.0, 24.0)
