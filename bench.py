@@ -64,6 +64,30 @@ def build_workload(name):
     return sd, cam, cfg
 
 
+def measured_traffic(workload):
+    """HBM bytes per unit of work measured with FETCH_SIZE / WRITE_SIZE (copy-kernel calibrated)
+    for this workload: the `traffic` block of its committed counter summary, or {}."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        {"many-light": "hbm_traffic_many_light_1080p_pmc.md",
+                         "media": "hbm_traffic_media_1024_pmc.md"}.get(workload, ""))
+    out = {}
+    if not os.path.isfile(path):
+        return out
+    block = False
+    for line in open(path):
+        if line.startswith("```traffic"):
+            block = True
+            continue
+        if block and line.startswith("```"):
+            break
+        if block:
+            f = line.split()
+            if len(f) == 6 and f[2] == "read" and f[4] == "write":
+                out[f[0]] = {"read": float(f[3]), "write": float(f[5]), "unit": f[1],
+                             "source": "profiles/" + os.path.basename(path)}
+    return out
+
+
 def integrator_id(cfg):
     from lambda_amd import abi
     return {"path": abi.INTEGRATOR_PATH, "volpath": abi.INTEGRATOR_VOLPATH,
@@ -98,7 +122,22 @@ def cpu_baseline(sd, cam, cfg, spectral, estimator, budget_s):
     del keep
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": "%dx%d centre crop at %d spp (%d samples, %.1f s) of the same workload and estimator"
-                      % (cw, ch, spp, samples, dt)}
+                      % (cw, ch, spp, samples, dt), "host": host_cpu()}
+
+
+def host_cpu():
+    """The host the CPU baseline ran on: logical CPUs, this process's affinity, the CPU model.
+    The baseline uses at most 16 threads: the CPU share one GPU's job gets on these machines."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"os_cpu_count": os.cpu_count(), "affinity_cpus": aff, "model": model, "threads_cap": 16}
 
 
 def main():
@@ -116,11 +155,15 @@ def main():
     ap.add_argument("--capacity", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true",
+                    help="skip the traversal-counting pass (counter runs: every dispatch is then the timed render's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
     ap.add_argument("--save-image", default="")
     ap.add_argument("--relmse", action="store_true", help="measure time-to-relMSE <= 1e-3 instead")
     ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--cpu-reference", action="store_true",
+                    help="relMSE against the CPU checker's render at --ref-spp (FP64 film sums, 16 host threads)")
     ap.add_argument("--ref-guided", action="store_true",
                     help="with --relmse: render the reference with the guided estimator (its own guide, trained "
                          "online in the reference's first 256 spp): lower variance than a plain reference")
@@ -190,14 +233,18 @@ def main():
         dev.close()
         return
 
-    # traversal work per extension ray (untimed counting pass, 1 spp)
-    cst = gpu.render(desc(10000, 10001, abi.RENDER_COUNT_TRAVERSAL))
-    fetch_per_ray = cst.nodes_visited / max(1, cst.extension_rays)
-    tris_per_ray = cst.tris_tested / max(1, cst.extension_rays)
-    film.clear()
-    voxels_per_vertex = cst.voxel_steps / max(1, cst.extension_rays)
-    log("counting pass: %.2f node fetches, %.2f triangles, %.1f voxel steps per extension ray"
-        % (fetch_per_ray, tris_per_ray, voxels_per_vertex))
+    # traversal work per extension ray (untimed counting pass, 1 spp): 4-wide node and triangle
+    # records the render's walk fetches, density voxels per path vertex
+    if args.no_count:
+        nodes4_per_ray = tris4_per_ray = voxels_per_vertex = None
+    else:
+        cst = gpu.render(desc(10000, 10001, abi.RENDER_COUNT_TRAVERSAL))
+        nodes4_per_ray = cst.nodes4_fetched / max(1, cst.extension_rays)
+        tris4_per_ray = cst.tris4_fetched / max(1, cst.extension_rays)
+        voxels_per_vertex = cst.voxel_steps / max(1, cst.extension_rays)
+        film.clear()
+        log("counting pass: %.2f 4-wide node records, %.2f triangle records, %.1f voxel steps per extension ray"
+            % (nodes4_per_ray, tris4_per_ray, voxels_per_vertex))
 
     if args.warmup:
         gpu.render(desc(0, args.warmup))
@@ -214,27 +261,36 @@ def main():
                                    st.extension_rays, st.shadow_rays))
 
     # Rooflines (HIP-event kernel times on the render stream).
-    # extend: algorithmic bytes per ray = 32 B per binary-BVH node examined (two per
-    #   BVH2 fetch) + 36 B per triangle tested + 32 B ray read + 20 B hit write.
-    # shade: algorithmic bytes per path vertex = 2 x 96 B path state (read + write),
-    #   plus 4 B per density-grid voxel visited by the vertex's medium walks.
-    bytes_per_ray = 32.0 * 2.0 * fetch_per_ray + 36.0 * tris_per_ray + 32.0 + 20.0
-    bytes_per_vertex = 192.0 + 4.0 * voxels_per_vertex
+    # extend: algorithmic bytes per ray = 64 B per 4-wide node record + 48 B per triangle record
+    #   the walk fetches (counting pass) + 32 B ray read + 16 B hit write.
+    # shade: algorithmic bytes per path vertex = 2 x 96 B path state (read + write), plus 4 B per
+    #   density-grid voxel visited by the vertex's medium walks.
+    # traffic: the HBM bytes per unit that FETCH_SIZE / WRITE_SIZE measured for this kernel and
+    #   workload (calibrated against a copy kernel), read from the committed counter summary.
+    bytes_per_ray = None if nodes4_per_ray is None else 64.0 * nodes4_per_ray + 48.0 * tris4_per_ray + 48.0
+    bytes_per_vertex = None if voxels_per_vertex is None else 192.0 + 4.0 * voxels_per_vertex
     shade_name = {"path": "k_shade", "direct": "k_shade_direct"}.get(cfg["integrator"], "k_shade_vol")
+    measured = measured_traffic(args.workload)
     ext_s = st.ms_extend / 1000.0
     sh_s = st.ms_shade / 1000.0
-    ext = {"bound": "hbm", "achieved": round(bytes_per_ray * st.extension_rays / ext_s / 1e9, 1),
-           "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": "k_extend",
-           "launches": st.launches_extend, "avg_launch_ms": round(st.ms_extend / max(1, st.launches_extend), 4),
-           "alg_bytes_per_unit": round(bytes_per_ray, 1), "unit_of_work": "extension ray",
-           "units_per_s": round(st.extension_rays / ext_s, 1)}
-    shd = {"bound": "hbm", "achieved": round(bytes_per_vertex * st.extension_rays / sh_s / 1e9, 1),
-           "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": shade_name,
-           "launches": st.launches_shade, "avg_launch_ms": round(st.ms_shade / max(1, st.launches_shade), 4),
-           "alg_bytes_per_unit": round(bytes_per_vertex, 1), "unit_of_work": "path vertex",
-           "units_per_s": round(st.extension_rays / sh_s, 1)}
-    for r in (ext, shd):
-        r["frac"] = round(r["achieved"] / HBM_PEAK_GBS, 4)
+
+    def roof(kernel, alg, units, secs, launches, unit):
+        r = {"bound": "hbm", "achieved": None if alg is None else round(alg * units / secs / 1e9, 1),
+             "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": kernel, "launches": launches,
+             "avg_launch_ms": round(secs * 1000.0 / max(1, launches), 4),
+             "alg_bytes_per_unit": None if alg is None else round(alg, 1), "unit_of_work": unit,
+             "units_per_launch": round(units / max(1, launches), 1), "units_per_s": round(units / secs, 1)}
+        r["frac"] = None if alg is None else round(r["achieved"] / HBM_PEAK_GBS, 4)
+        m = measured.get(kernel)
+        if m:
+            per = m["read"] + m["write"]
+            r["traffic"] = round(per * units / max(1, launches), 1)
+            r["traffic_per_unit"] = {"read": m["read"], "write": m["write"], "source": m["source"]}
+            r["hbm_achieved_GBps"] = round(per * units / secs / 1e9, 1)
+        return r
+
+    ext = roof("k_extend", bytes_per_ray, st.extension_rays, ext_s, st.launches_extend, "extension ray")
+    shd = roof(shade_name, bytes_per_vertex, st.extension_rays, sh_s, st.launches_shade, "path vertex")
     roofline = dict(shd if st.ms_shade >= st.ms_extend else ext)
     roofline["other_kernels"] = [shd if roofline["kernel"] == "k_extend" else ext]
     if args.workload != "many-light":
@@ -435,6 +491,29 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     t0 = time.perf_counter()
     rguide = gpu.Guide(dev, seed=101) if args.ref_guided else None
     b = 0
+    cpu_ref = None
+    if args.cpu_reference:
+        import oracle
+        from lambda_amd import abi
+        o = oracle.OracleScene()
+        okeep = sd.push(o, abi.LIGHTS_TREE)
+        d = abi.RenderDesc()
+        d.camera = cam
+        d.integrator, d.max_bounces, d.min_bounces = integrator_id(cfg), cfg["max_bounces"], 3
+        d.spectral_mode, d.estimator_mode, d.seed = spectral, estimator, 1
+        acc, cnt = np.zeros((H, W, 3)), np.zeros((H, W))
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        per = 256
+        while b < args.ref_spp:
+            d.spp_begin, d.spp_end = b, min(args.ref_spp, b + per)
+            crgb, ccnt = o.render(d, W, H, threads=threads)
+            acc += crgb
+            cnt += ccnt
+            b = d.spp_end
+            log("CPU reference %d / %d spp (%.0f s)" % (b, args.ref_spp, time.perf_counter() - t0))
+        cpu_ref = acc / cnt[..., None]
+        del okeep
+        b = args.ref_spp  # no GPU reference
     while b < args.ref_spp:
         if rguide is not None:
             per = 8 if b < 256 else 64
@@ -449,8 +528,11 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     if rguide is not None:
         rguide.close()
     ref_s = time.perf_counter() - t0
-    rsum, rcnt = ref.read()
-    r = rsum / rcnt[..., None]
+    if cpu_ref is not None:
+        r = cpu_ref
+    else:
+        rsum, rcnt = ref.read()
+        r = rsum / rcnt[..., None]
     def progressive(guide=None):
         """2^k-spp checkpoints of one progressive render (cumulative wall time of the render calls,
         training included for the guided estimator); stops at relMSE <= 1e-3."""
@@ -508,7 +590,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             g.close()
     # CPU validation of the reference on a 64x64 centre crop
     valid = None
-    if not args.no_cpu:
+    if not args.no_cpu and cpu_ref is None:
         import oracle
         from lambda_amd import abi
         o = oracle.OracleScene()
@@ -535,7 +617,10 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             "spp_at_target": hit["spp"] if hit else None, "dtype": "f32", "data": "synthetic (procedural scene)",
             "config": dict(cfg, spectral=args.spectral, estimator=args.estimator, reference_spp=args.ref_spp,
                            reference_seed=1, reference_s=round(ref_s, 1),
-                           reference_estimator="guided" if args.ref_guided else "plain"),
+                           reference_estimator="guided" if args.ref_guided else "plain",
+                           reference_renderer="CPU checker (oracle/, FP64 film sums, %d host threads)"
+                                              % max(1, min(os.cpu_count() or 1, 16)) if cpu_ref is not None
+                                              else "this GPU renderer"),
             "checkpoints": checkpoints, "reference_check": valid, "guided": guided}
     print(json.dumps(line), flush=True)
     ref.close()
