@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblambda_gpu.so")
-SOURCES = ["api.hip", "scene.hip", "render.hip", "volume.hip", "direct.hip", "utility.hip", "progressive.hip", "obj.hip", "post.hip", "denoise.hip", "guide.hip", "batch.hip"]
+SOURCES = ["api.hip", "scene.hip", "render.hip", "volume.hip", "direct.hip", "utility.hip", "progressive.hip", "obj.hip", "post.hip", "denoise.hip", "guide.hip", "batch.hip", "compat.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",
