@@ -55,22 +55,9 @@ LGPU_HD uint16_t ToBf16(float f) {
 }
 LGPU_HD float FromBf16(uint16_t h) { return AsF32((uint32_t)h << 16); }
 
-// Feature w (< 16) of one input x: octave w / 2, sin for even w, cos for odd: sin(2 pi t) with
-// t = fract(2^(octave - 1) x), the argument reduced exactly. Feature f (< 80) of the encoding is
-// feature f % 16 of input f / 16.
-LGPU_HD float GuideFeatureOf(float x, int w) {
-  const int oct = w >> 1;
-  const float t = Fract(x * (float)(1 << oct) * .5f);
-#if defined(__HIP_DEVICE_COMPILE__)
-  // the hardware sine and cosine take revolutions (sin(2 pi t)), one instruction each; their
-  // error (about 1e-6) is far below the BF16 rounding the features go through
-  return (w & 1) ? __builtin_amdgcn_cosf(t) : __builtin_amdgcn_sinf(t);
-#else
-  const float a = t * kPi2;
-  return (w & 1) ? cosf(a) : sinf(a);
-#endif
-}
-LGPU_HD float GuideFeature(const float s[kGuideScalars], int f) { return GuideFeatureOf(s[f >> 4], f & 15); }
+// The encoding: feature w (< 16) of input x is sin (even w) or cos (odd w) of 2 pi t, octave
+// w / 2, t = fract(2^(octave - 1) x) (the argument reduced exactly); feature f (< 80) of a query
+// is feature f % 16 of input f / 16. The kernels evaluate it in guide.hip (EncodeB).
 
 // Network inputs of a shading point: position in the scene bounds, wo = -ray direction.
 LGPU_HD void GuideInputs(V3 p, V3 wo, const float bmin[3], const float binv[3], float s[kGuideScalars]) {

@@ -29,31 +29,34 @@ __device__ inline F4v Mfma(Bf4 a, Bf4 b, F4v c) { return __builtin_amdgcn_mfma_f
 // layer as they are.
 typedef __bf16 Bf8v __attribute__((ext_vector_type(8)));
 typedef short S8 __attribute__((ext_vector_type(8)));
-__device__ inline F4v Mfma32(Bf4 a0, Bf4 a1, Bf4 b0, Bf4 b1, F4v c) {
-  const S8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
-  const S8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+__device__ inline F4v Mfma32(S8 a, S8 b, F4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(Bf8v, a), __builtin_bit_cast(Bf8v, b), c, 0, 0, 0);
 }
+__device__ inline S8 Cat8(Bf4 lo, Bf4 hi) { return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7); }
+__device__ inline F4v Mfma32(Bf4 a0, Bf4 a1, Bf4 b0, Bf4 b1, F4v c) { return Mfma32(Cat8(a0, a1), Cat8(b0, b1), c); }
+// FP32 -> BF16 pairs on gfx950's v_cvt_pk_bf16_f32 (round to nearest even, as ToBf16).
+typedef __bf16 Bf2v __attribute__((ext_vector_type(2)));
+typedef float F2v __attribute__((ext_vector_type(2)));
+typedef short S2 __attribute__((ext_vector_type(2)));
+__device__ inline S2 CvtBf2(float a, float b) { return __builtin_bit_cast(S2, __builtin_convertvector((F2v){a, b}, Bf2v)); }
 __device__ inline Bf4 Pack4(float a, float b, float c, float d) {
-  Bf4 r;
-  r[0] = (short)ToBf16(a); r[1] = (short)ToBf16(b); r[2] = (short)ToBf16(c); r[3] = (short)ToBf16(d);
-  return r;
+  return __builtin_shufflevector(CvtBf2(a, b), CvtBf2(c, d), 0, 1, 2, 3);
 }
-__device__ inline Bf4 LoadBf4(const uint16_t *p) {  // 8-byte aligned
-  const uint2 u = *reinterpret_cast<const uint2 *>(p);
-  Bf4 r;
-  r[0] = (short)(u.x & 0xFFFFu); r[1] = (short)(u.x >> 16); r[2] = (short)(u.y & 0xFFFFu); r[3] = (short)(u.y >> 16);
-  return r;
+// max(x, 0) of four accumulators, as BF16. A BF16 pattern read as a 16-bit integer orders like
+// its value around zero (negative values have the sign bit, so they are negative integers), so
+// the ReLU is one packed integer max per pair after the conversion.
+__device__ inline Bf4 PackRelu4(F4v v) {
+  const S2 z = {0, 0};
+  return __builtin_shufflevector(__builtin_elementwise_max(CvtBf2(v[0], v[1]), z),
+                                 __builtin_elementwise_max(CvtBf2(v[2], v[3]), z), 0, 1, 2, 3);
 }
+__device__ inline Bf4 LoadBf4(const uint16_t *p) { return *reinterpret_cast<const Bf4 *>(p); }  // 8-byte aligned
+typedef uint32_t U2v __attribute__((ext_vector_type(2)));
 __device__ inline uint2 AsU2(Bf4 v) {
-  return make_uint2(((uint32_t)(uint16_t)v[0]) | ((uint32_t)(uint16_t)v[1] << 16),
-                    ((uint32_t)(uint16_t)v[2]) | ((uint32_t)(uint16_t)v[3] << 16));
+  const U2v u = __builtin_bit_cast(U2v, v);
+  return make_uint2(u.x, u.y);
 }
-__device__ inline Bf4 FromU2(uint2 u) {
-  Bf4 r;
-  r[0] = (short)(u.x & 0xFFFFu); r[1] = (short)(u.x >> 16); r[2] = (short)(u.y & 0xFFFFu); r[3] = (short)(u.y >> 16);
-  return r;
-}
+__device__ inline Bf4 FromU2(uint2 u) { return __builtin_bit_cast(Bf4, (U2v){u.x, u.y}); }
 // Register-resident access to tile pt of a 4-tile row with a run-time pt (selects, no indexing).
 __device__ inline Bf4 Get4(const Bf4 v[4], int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3])); }
 __device__ inline void Put4(Bf4 v[4], int i, Bf4 a) {
@@ -63,12 +66,18 @@ __device__ inline float Lane4Sum(float v) { v += __shfl_xor(v, 16); v += __shfl_
 __device__ inline float Lane4Max(float v) { v = fmaxf(v, __shfl_xor(v, 16)); v = fmaxf(v, __shfl_xor(v, 32)); return v; }
 
 // Encoded inputs of one query as this lane's B operand of layer 0: features 16 kt + 4 g + r.
-// (K tile kt holds the 16 features of input kt: octaves 2 g and 2 g + 1, sin and cos.)
+// (K tile kt holds the 16 features of input kt: octaves 2 g and 2 g + 1, sin and cos.) The angle
+// of octave 2 g + 1 is twice that of octave 2 g, so its sine and cosine come from the double-angle
+// identities (sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a) instead of two more transcendentals.
 __device__ inline void EncodeB(const float s[kGuideScalars], int g, Bf4 x[5]) {
+  const float scale = (float)(1 << (2 * g)) * .5f;  // 2^(2 g - 1): octave 2 g
 #pragma unroll
-  for (int kt = 0; kt < 5; ++kt)
-    x[kt] = Pack4(GuideFeatureOf(s[kt], 4 * g), GuideFeatureOf(s[kt], 4 * g + 1), GuideFeatureOf(s[kt], 4 * g + 2),
-                  GuideFeatureOf(s[kt], 4 * g + 3));
+  for (int kt = 0; kt < 5; ++kt) {
+    const float t = Fract(s[kt] * scale);
+    const float s1 = __builtin_amdgcn_sinf(t), c1 = __builtin_amdgcn_cosf(t);  // revolutions
+    const float m2 = -2.f * s1;
+    x[kt] = Pack4(s1, c1, -m2 * c1, fmaf(m2, s1, 1.f));
+  }
 }
 
 // Hidden layer L of one wave's 64 queries (4 column tiles): x[kt][pt] in, result in x (the next
@@ -133,70 +142,95 @@ __device__ inline void HiddenLayers(const uint16_t *W, const float *B, Bf4 x[5][
   HiddenLayer<4, kStore>(W, B, x, store);
 }
 
-// Inference: a wave handles kInferPT column tiles (16 kInferPT queries), a block 4 waves.
-constexpr int kInferPT = 4;
+// Inference: a wave handles kInferPT column tiles (16 kInferPT queries); the kInferBlock threads
+// of a block share one LDS copy of the network.
+constexpr int kInferPT = 4;  // = 64 / 16: lane l derives the inputs of query l of the wave
+constexpr int kInferBlock = 512;
 
-// Inference form of a hidden layer: K tiles in pairs through the K = 32 MFMA (layer 0's fifth
-// tile through the K = 16 one); same result layout as HiddenLayer.
+// The inference kernel keeps the weights in LDS in MFMA fragment order: for each layer, each
+// 16-row tile m and each K pair j (32 columns), the 64 lanes' A operands of one K = 32 MFMA are
+// 64 consecutive 16-byte fragments, lane (c16, g) holding row 16 m + c16, columns 32 j + 4 g ..
+// + 3 and 32 j + 16 + 4 g .. + 3; a layer whose width is an odd number of 16-column tiles (layer
+// 0) ends with its last tile as 8-byte K = 16 fragments. One ds_read_b128 per lane and K pair,
+// consecutive lanes at consecutive addresses, instead of rows 128 bytes apart in the same banks.
+// Same size as the row-major weights: fragments of layer l start at byte 2 GuideWOff(l).
+LGPU_HD constexpr int GuidePairs(int l) { return GuideCols(l) / 32; }
+__device__ inline void FillWeightFragments(const uint16_t *__restrict__ Wg, uint16_t *W) {
+#pragma unroll 1
+  for (int l = 0; l < kGuideLayers; ++l) {
+    const int R = GuideRows(l), C = GuideCols(l), np = GuidePairs(l);
+    const uint16_t *src = Wg + GuideWOff(l);
+    uint16_t *dst = W + GuideWOff(l);
+    const int n32 = (R / 16) * np * 64;  // 16-byte fragments
+    for (int f = threadIdx.x; f < n32; f += blockDim.x) {
+      const int lane = f & 63, t = f >> 6, m = t / np, j = t % np;
+      const uint16_t *row = src + (16 * m + (lane & 15)) * C + 32 * j + 4 * (lane >> 4);
+      const uint2 lo = *reinterpret_cast<const uint2 *>(row), hi = *reinterpret_cast<const uint2 *>(row + 16);
+      reinterpret_cast<uint4 *>(dst)[f] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+    if (C & 16) {  // the last 16 columns as K = 16 fragments, after the pairs
+      const int n16 = (R / 16) * 64;
+      for (int f = threadIdx.x; f < n16; f += blockDim.x) {
+        const int lane = f & 63, m = f >> 6;
+        reinterpret_cast<uint2 *>(dst + 8 * n32)[f] =
+            *reinterpret_cast<const uint2 *>(src + (16 * m + (lane & 15)) * C + 32 * np + 4 * (lane >> 4));
+      }
+    }
+  }
+}
+__device__ inline S8 LoadFrag32(const uint16_t *W, int l, int m, int j, int lane) {
+  return reinterpret_cast<const S8 *>(W + GuideWOff(l))[(m * GuidePairs(l) + j) * 64 + lane];
+}
+
+// Activations of the inference kernel: the B operands of a 64-wide layer as two K pairs per
+// column tile (xp[j][pt]: K tiles 2 j and 2 j + 1, one K = 32 MFMA operand each); layer 0's
+// fifth K tile (features 64 .. 79) apart, as a K = 16 operand.
+// Inference form of a hidden layer on fragment-ordered weights: the biases as the accumulators'
+// start, ReLU and BF16 conversion on the packed results; same arithmetic as HiddenLayer.
 template <int L>
-__device__ inline void HiddenLayer32(const uint16_t *W, const float *B, Bf4 x[5][kInferPT]) {
-  constexpr int C = GuideCols(L), nk = C / 16;
-  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
-  const uint16_t *Wl = W + GuideWOff(L);
+__device__ inline void HiddenLayer32(const uint16_t *W, const float *B, S8 xp[2][kInferPT], const Bf4 *x4) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
   const float *Bl = B + GuideBOff(L);
   F4v acc[4][kInferPT];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = F4v{0.f, 0.f, 0.f, 0.f};
-  // K pairs in a rolled loop (operands picked by selects): one pair's weights in flight at a time
-#pragma unroll 1
-  for (int j = 0; j < nk / 2; ++j) {
-    Bf4 x0[kInferPT], x1[kInferPT];
-#pragma unroll
-    for (int pt = 0; pt < kInferPT; ++pt) {
-      x0[pt] = j == 0 ? x[0][pt] : x[2][pt];
-      x1[pt] = j == 0 ? x[1][pt] : x[3][pt];
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const uint16_t *row = Wl + (16 * m + c16) * C + 32 * j + 4 * g;
-      const Bf4 a0 = LoadBf4(row), a1 = LoadBf4(row + 16);
-#pragma unroll
-      for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = Mfma32(a0, a1, x0[pt], x1[pt], acc[m][pt]);
-    }
-  }
-  if constexpr (nk & 1) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const Bf4 a = LoadBf4(Wl + (16 * m + c16) * C + 16 * (nk - 1) + 4 * g);
-#pragma unroll
-      for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = Mfma(a, x[nk - 1][pt], acc[m][pt]);
-    }
-  }
-#pragma unroll
   for (int m = 0; m < 4; ++m) {
-    const float b0 = Bl[16 * m + 4 * g], b1 = Bl[16 * m + 4 * g + 1], b2 = Bl[16 * m + 4 * g + 2],
-                b3 = Bl[16 * m + 4 * g + 3];
+    const F4v b = *reinterpret_cast<const F4v *>(Bl + 16 * m + 4 * g);
 #pragma unroll
-    for (int pt = 0; pt < kInferPT; ++pt)
-      x[m][pt] = Pack4(fmaxf(acc[m][pt][0] + b0, 0.f), fmaxf(acc[m][pt][1] + b1, 0.f), fmaxf(acc[m][pt][2] + b2, 0.f),
-                       fmaxf(acc[m][pt][3] + b3, 0.f));
+    for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = b;
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const S8 a = LoadFrag32(W, L, m, j, lane);
+#pragma unroll
+      for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = Mfma32(a, xp[j][pt], acc[m][pt]);
+    }
+  if constexpr (L == 0) {  // K = 16 fragments of the last 16 columns, after the pairs
+    const Bf4 *tail = reinterpret_cast<const Bf4 *>(W + GuideWOff(0) + 8 * 4 * GuidePairs(0) * 64);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const Bf4 a = tail[m * 64 + lane];
+#pragma unroll
+      for (int pt = 0; pt < kInferPT; ++pt) acc[m][pt] = Mfma(a, x4[pt], acc[m][pt]);
+    }
+  }
+#pragma unroll
+  for (int pt = 0; pt < kInferPT; ++pt) {
+    xp[0][pt] = Cat8(PackRelu4(acc[0][pt]), PackRelu4(acc[1][pt]));
+    xp[1][pt] = Cat8(PackRelu4(acc[2][pt]), PackRelu4(acc[3][pt]));
   }
 }
-__device__ inline void OutputLayer32(const uint16_t *W, const float *B, const Bf4 b0, const Bf4 b1, const Bf4 b2,
-                                     const Bf4 b3, F4v z[9]) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
-  const uint16_t *Wl = W + GuideWOff(kGuideLayers - 1);
-  const float *Bl = B + GuideBOff(kGuideLayers - 1);
+// Output layer of one column tile on fragment-ordered weights: z[m] (m < 9) as OutputLayer.
+__device__ inline void OutputLayer32(const uint16_t *W, const float *B, S8 x0, S8 x1, F4v z[9]) {
+  constexpr int L = kGuideLayers - 1;
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const float *Bl = B + GuideBOff(L);
 #pragma unroll
   for (int m = 0; m < 9; ++m) {
-    const uint16_t *row = Wl + (16 * m + c16) * kGuideHidden + 4 * g;
-    F4v acc = F4v{0.f, 0.f, 0.f, 0.f};
-    acc = Mfma32(LoadBf4(row), LoadBf4(row + 16), b0, b1, acc);
-    acc = Mfma32(LoadBf4(row + 32), LoadBf4(row + 48), b2, b3, acc);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] += Bl[16 * m + 4 * g + r];
+    F4v acc = *reinterpret_cast<const F4v *>(Bl + 16 * m + 4 * g);
+    acc = Mfma32(LoadFrag32(W, L, m, 0, lane), x0, acc);
+    acc = Mfma32(LoadFrag32(W, L, m, 1, lane), x1, acc);
     z[m] = acc;
   }
 }
@@ -225,27 +259,43 @@ struct GuideQuerySrc {
   const float *scalars;               // or 5 scalars per query
   float bmin[3], binv[3];
 };
-__device__ inline void QueryScalars(const GuideQuerySrc &src, uint32_t q, float s[kGuideScalars]) {
+// A query's raw inputs (loads only), so the next query's loads can be in flight while this one
+// is evaluated: hit, ray origin and direction of its queue slot, or its 5 scalars (in h and o.x).
+struct QueryRaw {
+  float4 h, o, d;
+};
+__device__ inline QueryRaw QueryLoad(const GuideQuerySrc &src, uint32_t q) {
+  QueryRaw r;
   if (src.scalars) {
-#pragma unroll
-    for (int k = 0; k < kGuideScalars; ++k) s[k] = src.scalars[kGuideScalars * (size_t)q + k];
-    return;
+    const float *p = src.scalars + kGuideScalars * (size_t)q;
+    r.h = make_float4(p[0], p[1], p[2], p[3]);
+    r.o = make_float4(p[4], 0.f, 0.f, 0.f);
+    r.d = make_float4(0.f, 0.f, 0.f, 0.f);
+    return r;
   }
   q = src.list[q];
-  const float4 h = src.hit[q];
-  if (__float_as_uint(h.w) == kNone) {
+  r.h = src.hit[q];
+  r.o = src.ray_o[q];
+  r.d = src.ray_d[q];
+  return r;
+}
+__device__ inline void QueryScalars(const GuideQuerySrc &src, const QueryRaw &r, float s[kGuideScalars]) {
+  if (src.scalars) {
+    s[0] = r.h.x; s[1] = r.h.y; s[2] = r.h.z; s[3] = r.h.w; s[4] = r.o.x;
+    return;
+  }
+  if (__float_as_uint(r.h.w) == kNone) {
 #pragma unroll
     for (int k = 0; k < kGuideScalars; ++k) s[k] = 0.f;
     return;
   }
-  const float4 o = src.ray_o[q], d = src.ray_d[q];
-  const V3 p = v3(o.x + d.x * h.x, o.y + d.y * h.x, o.z + d.z * h.x);
+  const float4 o = r.o, d = r.d;
+  const float t = r.h.x;
+  const V3 p = v3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
   // local copies: the kernel argument's address would otherwise reach the callee (scratch copy)
   const float bmin[3] = {src.bmin[0], src.bmin[1], src.bmin[2]}, binv[3] = {src.binv[0], src.binv[1], src.binv[2]};
   GuideInputs(p, v3(-d.x, -d.y, -d.z), bmin, binv, s);
 }
-
-constexpr int kInferBlock = 256;  // 4 waves of 16 kInferPT queries
 
 // The queue slots whose next shading is a guided scattering event: a hit on a guidable material
 // by a path that is not terminating (kFlagEmitOnly gathers emission only). One atomic per wave.
@@ -270,80 +320,89 @@ __global__ void k_guide_list(const BvhTri *__restrict__ tris, const float4 *__re
 }
 
 // Inference: per query, the quantised bin table (bytes max(1, round(255 exp(z - z_max))), the
-// distribution is byte / sum) and the mixing probability. Persistent blocks; weights in LDS.
+// distribution is byte / sum) and the mixing probability. Persistent blocks; weights in LDS in
+// fragment order.
 __global__ void __launch_bounds__(kInferBlock) k_guide_infer(const uint16_t *__restrict__ Wg, const float *__restrict__ Bg,
                                                                GuideQuerySrc src, uint32_t n, const uint32_t *np,
                                                                uint32_t *__restrict__ bins, float2 *__restrict__ hdr) {
-  __shared__ __align__(16) uint16_t W[kGuideWeights];
-  __shared__ float B[kGuideBiases];
+  __shared__ __align__(16) uint16_t Wlds[kGuideWeights];
+  __shared__ __align__(16) float Blds[kGuideBiases];
   constexpr int kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
   if (np) n = *np;
   if ((uint64_t)blockIdx.x * kPerBlock >= n) return;
-  for (int i = threadIdx.x; i < kGuideWeights / 8; i += kInferBlock)
-    reinterpret_cast<uint4 *>(W)[i] = reinterpret_cast<const uint4 *>(Wg)[i];
-  for (int i = threadIdx.x; i < kGuideBiases; i += kInferBlock) B[i] = Bg[i];
+  FillWeightFragments(Wg, Wlds);
+  for (int i = threadIdx.x; i < kGuideBiases; i += kInferBlock) Blds[i] = Bg[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15, wave = threadIdx.x >> 6;
-  for (uint64_t base = (uint64_t)blockIdx.x * kPerBlock; base < n; base += (uint64_t)gridDim.x * kPerBlock) {
+  const float kLog2e = 1.44269504088896341f;
+  const uint64_t stride = (uint64_t)gridDim.x * kPerBlock;
+  // each lane derives the inputs of one query (column tile g, column c16) of its wave; the loads
+  // of the next iteration's query are issued before this iteration's network
+  uint64_t base = (uint64_t)blockIdx.x * kPerBlock;
+  QueryRaw raw = QueryLoad(src, (uint32_t)std::min<uint64_t>(base + 16 * kInferPT * wave + lane, n - 1));
+  for (; base < n; base += stride) {
     const uint64_t wbase = base + 16 * kInferPT * wave;
-    // gather every tile's inputs first, so the dependent loads of the tiles overlap
-    float s[kInferPT][kGuideScalars];
+    const QueryRaw cur = raw;
+    if (base + stride < n) raw = QueryLoad(src, (uint32_t)std::min<uint64_t>(wbase + stride + lane, n - 1));
+    // an offset the compiler cannot see through: the LDS weights are loop-invariant, and hoisting
+    // their loads out of this loop would hold the whole network in (spilled) registers
+    int lo = 0;
+    asm volatile("" : "+s"(lo));
+    const uint16_t *W = Wlds + lo;
+    const float *B = Blds + lo;
+    // the four lanes that encode a query's features gather its inputs
+    float sq[kGuideScalars];
+    QueryScalars(src, cur, sq);
+    S8 xp[2][kInferPT];
+    Bf4 x4[kInferPT];
 #pragma unroll
-    for (int pt = 0; pt < kInferPT; ++pt) {
-      const uint64_t q = wbase + 16 * pt + c16;
-      QueryScalars(src, (uint32_t)(q < n ? q : n - 1), s[pt]);
-    }
-    Bf4 x[5][kInferPT];
-#pragma unroll 1
     for (int pt = 0; pt < kInferPT; ++pt) {
       float sp[kGuideScalars];
 #pragma unroll
-      for (int k = 0; k < kGuideScalars; ++k)
-        sp[k] = pt == 0 ? s[0][k] : (pt == 1 ? s[1][k] : (pt == 2 ? s[2][k] : s[3][k]));
+      for (int k = 0; k < kGuideScalars; ++k) sp[k] = __shfl(sq[k], 16 * pt + c16);
       Bf4 e[5];
       EncodeB(sp, g, e);
-#pragma unroll
-      for (int kt = 0; kt < 5; ++kt) Put4(x[kt], pt, e[kt]);
+      xp[0][pt] = Cat8(e[0], e[1]);
+      xp[1][pt] = Cat8(e[2], e[3]);
+      x4[pt] = e[4];
     }
-    HiddenLayer32<0>(W, B, x);
-    HiddenLayer32<1>(W, B, x);
-    HiddenLayer32<2>(W, B, x);
-    HiddenLayer32<3>(W, B, x);
-    HiddenLayer32<4>(W, B, x);
-#pragma unroll 1
+    HiddenLayer32<0>(W, B, xp, x4);
+    HiddenLayer32<1>(W, B, xp, nullptr);
+    HiddenLayer32<2>(W, B, xp, nullptr);
+    HiddenLayer32<3>(W, B, xp, nullptr);
+    HiddenLayer32<4>(W, B, xp, nullptr);
+#pragma unroll
     for (int pt = 0; pt < kInferPT; ++pt) {
       F4v z[9];
-      OutputLayer32(W, B, Get4(x[0], pt), Get4(x[1], pt), Get4(x[2], pt), Get4(x[3], pt), z);
+      OutputLayer32(W, B, xp[0][pt], xp[1][pt], z);
       float zmax = -INFINITY;
 #pragma unroll
-      for (int m = 0; m < 8; ++m)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) zmax = fmaxf(zmax, z[m][r]);
+      for (int m = 0; m < 8; ++m) zmax = fmaxf(zmax, fmaxf(fmaxf(z[m][0], z[m][1]), fmaxf(z[m][2], z[m][3])));
       zmax = Lane4Max(zmax);
-      float tot = 0.f;
-      uint32_t words[8];
+      const float off = -zmax * kLog2e;
+      uint32_t words[8], tot = 0u;
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        uint32_t w = 0;
+        uint32_t w = 0u;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const float e = __expf(z[m][r] - zmax);
-          uint32_t b = (uint32_t)(255.f * e + .5f);
-          b = b < 1u ? 1u : (b > 255u ? 255u : b);
-          tot += (float)b;
-          w |= b << (8 * r);
+          // exp(z - zmax) = 2^(z log2 e - zmax log2 e) on the hardware exp2 (inputs <= 0)
+          const float e = __builtin_amdgcn_exp2f(fmaf(z[m][r], kLog2e, off));
+          const uint32_t b = (uint32_t)fmaf(e, 255.f, .5f);
+          w |= (b < 1u ? 1u : b) << (8 * r);
         }
         words[m] = w;
+        tot = __builtin_amdgcn_udot4(w, 0x01010101u, tot, false);
       }
-      tot = Lane4Sum(tot);
-      const float zc = Lane4Sum(g == 0 ? z[8][0] : 0.f);
+      const float sum = Lane4Sum((float)tot);
+      const float zc = __shfl(z[8][0], c16);  // the mixing logit, row 128: lane group 0
       const uint64_t q = wbase + 16 * pt + c16;
       if (q < n) {
         const uint64_t slot = src.list ? src.list[q] : q;
         // lane group g holds bytes 16 m + 4 g .. + 3: words 4 m + g
 #pragma unroll
         for (int m = 0; m < 8; ++m) bins[slot * 32 + 4 * m + g] = words[m];
-        if (g == 0) hdr[slot] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), tot);
+        if (g == 0) hdr[slot] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), sum);
       }
     }
   }
@@ -586,6 +645,7 @@ struct lgpu_guide_s {
   uint32_t last_records = 0;
   float last_loss = 0.f;
   uint64_t expected_vertices = 0;  // guided vertices of the last wave (recording rate)
+  uint32_t infer_grid = 0;         // persistent inference grid: resident blocks per CU x CUs
 };
 
 namespace lgpu {
@@ -674,7 +734,7 @@ lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const DevScene &sc, 
   src.ray_o = ps.ray_o; src.ray_d = ps.ray_d; src.hit = ps.hit; src.list = list; src.scalars = nullptr;
   for (int k = 0; k < 3; ++k) { src.bmin[k] = refs.bmin[k]; src.binv[k] = refs.binv[k]; }
   constexpr uint32_t kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
-  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((ub + kPerBlock - 1) / kPerBlock, 2048));
+  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((ub + kPerBlock - 1) / kPerBlock, g->infer_grid));
   hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, ub,
                      count, reinterpret_cast<uint32_t *>(bins), hdr);
   LGPU_HIP(hipGetLastError());
@@ -734,6 +794,12 @@ extern "C" lgpu_status lgpu_guide_create(lgpu_device dev, const lgpu_guide_desc 
   if ((e = g->sumw.alloc(256)) != hipSuccess) return fail(e);
   if ((e = g->rec.alloc(sizeof(GuideRecord) * (size_t)d.record_capacity)) != hipSuccess) return fail(e);
   if ((e = g->rec_count.alloc(256)) != hipSuccess) return fail(e);
+  {  // the persistent inference grid: as many blocks as are resident at once
+    int perCu = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_guide_infer, kInferBlock, 0)) != hipSuccess) return fail(e);
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->device)) != hipSuccess) return fail(e);
+    g->infer_grid = (uint32_t)std::max(1, perCu * cus);
+  }
   // He-uniform initial weights from the seed; zero biases except the mixing logit (c = 0.3 at start)
   std::vector<float> p(kGuideParams, 0.f);
   std::mt19937 rng(d.seed ^ 0x9E3779B9u);
@@ -824,7 +890,7 @@ extern "C" lgpu_status lgpu_guide_forward(lgpu_guide g, uint32_t n, const float 
   src.list = nullptr;
   src.scalars = din.as<float>();
   constexpr uint32_t kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
-  const uint32_t blocks = std::min<uint32_t>((n + kPerBlock - 1) / kPerBlock, 2048);
+  const uint32_t blocks = std::min<uint32_t>((n + kPerBlock - 1) / kPerBlock, g->infer_grid);
   hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, n,
                      nullptr, dbins.as<uint32_t>(), dhdr.as<float2>());
   const hipError_t le = hipGetLastError();
