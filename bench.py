@@ -534,19 +534,21 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
         rsum, rcnt = ref.read()
         r = rsum / rcnt[..., None]
     def progressive(guide=None):
-        """2^k-spp checkpoints of one progressive render (cumulative wall time of the render calls,
+        """Checkpoints of one progressive render (cumulative wall time of the render calls,
         training included for the guided estimator); stops at relMSE <= 1e-3."""
         film.clear()
         checkpoints, hit = [], None
-        spent, done, k = 0.0, 0, 0
+        spent, done = 0.0, 0
         kern = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_guide_infer": 0.0, "ms_guide_train": 0.0}
 
         def tally(st):
             for key in kern:
                 kern[key] += getattr(st, key)
 
-        while (1 << k) <= args.ref_spp // 2:
-            target = 1 << k
+        # checkpoints at 2^k and 3 * 2^(k-1) spp, up to half the reference's
+        targets = sorted({1 << j for j in range(31) if (1 << j) <= args.ref_spp // 2} |
+                         {3 << j for j in range(30) if (3 << j) <= args.ref_spp // 2})
+        for target in targets:
             t = time.perf_counter()
             if guide is None:
                 tally(gpu.render(desc(done, target)))
@@ -572,7 +574,6 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             if rel <= 1e-3:
                 hit = checkpoints[-1]
                 break
-            k += 1
         return checkpoints, hit
 
     checkpoints, hit = progressive()

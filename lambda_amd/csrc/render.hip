@@ -251,19 +251,39 @@ __device__ inline void BlockPush(const Queues &q, bool pushE, const ExtPush &e, 
 __device__ inline float GuideBinPdf(const uint8_t *tab, float tot, uint32_t bin) {
   return (float)tab[bin] / (tot * kGuideBinSr);
 }
-__device__ inline uint32_t GuideSampleBin(const uint32_t *tab, float target) {  // target in [0, tot)
-  float acc = 0.f;
-  for (uint32_t k = 0; k < kGuideBins / 4; ++k) {
-    const uint32_t w = tab[k];
-    const float b0 = (float)(w & 255u), b1 = (float)((w >> 8) & 255u), b2 = (float)((w >> 16) & 255u), b3 = (float)(w >> 24);
-    const float s = b0 + b1 + b2 + b3;
-    if (acc + s > target) {
-      if (acc + b0 > target) return 4 * k;
-      if (acc + b0 + b1 > target) return 4 * k + 1;
-      if (acc + b0 + b1 + b2 > target) return 4 * k + 2;
-      return 4 * k + 3;
+// The bin whose cumulative weight first exceeds target (in [0, tot)): 16 bins per 16-byte load,
+// byte sums on v_dot4_u32_u8; the sums are exact integers, so the comparisons are those of a
+// running float sum.
+__device__ inline uint32_t GuideSampleBin(const uint32_t *tab, float target) {
+  const uint4 *t4 = reinterpret_cast<const uint4 *>(tab);  // 128-byte tables
+  uint32_t acc = 0;
+  for (uint32_t k = 0; k < kGuideBins / 16; ++k) {
+    const uint4 q = t4[k];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    uint32_t s[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j] = __builtin_amdgcn_udot4(w[j], 0x01010101u, 0u, false);
+    if ((float)(acc + s[0] + s[1] + s[2] + s[3]) > target) {
+      uint32_t a = acc;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        if ((float)(a + s[j]) > target) {
+          const uint32_t b0 = w[j] & 255u, b1 = (w[j] >> 8) & 255u, b2 = (w[j] >> 16) & 255u;
+          const uint32_t bin = 16 * k + 4 * j;
+          if ((float)(a + b0) > target) return bin;
+          if ((float)(a + b0 + b1) > target) return bin + 1;
+          if ((float)(a + b0 + b1 + b2) > target) return bin + 2;
+          return bin + 3;
+        }
+        a += s[j];
+      }
+      const uint32_t b0 = w[3] & 255u, b1 = (w[3] >> 8) & 255u, b2 = (w[3] >> 16) & 255u;
+      if ((float)(a + b0) > target) return 16 * k + 12;
+      if ((float)(a + b0 + b1) > target) return 16 * k + 13;
+      if ((float)(a + b0 + b1 + b2) > target) return 16 * k + 14;
+      return 16 * k + 15;
     }
-    acc += s;
+    acc += s[0] + s[1] + s[2] + s[3];
   }
   return kGuideBins - 1;  // target rounded up to tot
 }
