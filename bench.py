@@ -568,6 +568,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             x = xs / xc[..., None]
             rel = float(np.mean((x - r) ** 2 / (r * r + 0.01)))
             checkpoints.append({"spp": target, "seconds": round(spent, 4), "relmse": rel,
+                                "msamples_per_s": round(W * H * target / spent / 1e6, 1),
                                 "kernel_ms": {key: round(v, 1) for key, v in kern.items()}})
             log("%s spp %d: %.3f s, relMSE %.3e; ms %s" % ("guided" if guide else "plain", target, spent, rel,
                                                             {key: round(v, 1) for key, v in kern.items()}))
