@@ -88,6 +88,20 @@ def measured_traffic(workload):
     return out
 
 
+def measured_valu_busy(workload):
+    """VALU busy share per kernel (SQ_ACTIVE_INST_VALU over SIMD cycles) from the committed counter
+    summary, for this workload: {kernel base name: fraction}, or {}."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "valu_lds_utilisation.md")
+    out = {}
+    if not os.path.isfile(path):
+        return out
+    for line in open(path):
+        f = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(f) == 10 and f[0] == workload and f[1].startswith("k_") and f[6].endswith("%"):
+            out.setdefault(f[1].split("<")[0], float(f[6].rstrip(" %")) / 100.0)
+    return out
+
+
 def integrator_id(cfg):
     from lambda_amd import abi
     return {"path": abi.INTEGRATOR_PATH, "volpath": abi.INTEGRATOR_VOLPATH,
@@ -271,6 +285,7 @@ def main():
     bytes_per_vertex = None if voxels_per_vertex is None else 192.0 + 4.0 * voxels_per_vertex
     shade_name = {"path": "k_shade", "direct": "k_shade_direct"}.get(cfg["integrator"], "k_shade_vol")
     measured = measured_traffic(args.workload)
+    valu = measured_valu_busy(args.workload)
     ext_s = st.ms_extend / 1000.0
     sh_s = st.ms_shade / 1000.0
 
@@ -287,6 +302,15 @@ def main():
             r["traffic"] = round(per * units / max(1, launches), 1)
             r["traffic_per_unit"] = {"read": m["read"], "write": m["write"], "source": m["source"]}
             r["hbm_achieved_GBps"] = round(per * units / secs / 1e9, 1)
+        if r["frac"] is not None and r["frac"] > 1.0:
+            # the algorithmic bytes are mostly served by L2 and MALL, so HBM does not bound this
+            # kernel: report it as VALU-bound with the measured VALU busy share, and its HBM share
+            # from the measured traffic
+            r["bound"] = "valu"
+            r["frac"] = valu.get(kernel)
+            r["valu_busy_source"] = "profiles/valu_lds_utilisation.md"
+            if "hbm_achieved_GBps" in r:
+                r["hbm_frac"] = round(r["hbm_achieved_GBps"] / HBM_PEAK_GBS, 4)
         return r
 
     ext = roof("k_extend", bytes_per_ray, st.extension_rays, ext_s, st.launches_extend, "extension ray")

@@ -209,6 +209,67 @@ def test_film_render_matches_oracle(dev):
     np.testing.assert_allclose(rgba[..., :3], rgb / spp, rtol=1e-6, atol=1e-7)
 
 
+# CIE 1931 analytic fit (Wyman, Sloan & Shirley 2013): (weight, mu, sigma below, sigma above) per lobe
+_CMF_LOBES = (((1.056, 599.8, 37.9, 31.0), (0.362, 442.0, 16.0, 26.7), (-0.065, 501.1, 20.4, 26.2)),
+              ((0.821, 568.8, 46.9, 40.5), (0.286, 530.9, 16.3, 31.1)),
+              ((1.217, 437.0, 11.8, 36.0), (0.681, 459.0, 26.0, 13.8)))
+_XYZ_TO_RGB = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556],
+                        [0.055648, -0.204043, 1.057311]])
+
+
+def _cmf_f32(lam):
+    """The fit as the CPU checker evaluates it: float32 division and expf per lobe."""
+    lam = lam.astype(np.float32)
+    out = []
+    for lobes in _CMF_LOBES:
+        acc = np.zeros_like(lam)
+        for w, mu, s1, s2 in lobes:
+            t = (lam - np.float32(mu)) / np.where(lam < np.float32(mu), np.float32(s1), np.float32(s2))
+            acc = acc + np.float32(w) * np.exp(np.float32(-0.5) * t * t)
+        out.append(acc)
+    return np.stack(out, axis=-1)
+
+
+def test_hero_film_update_matches_restated_colour_matching(dev):
+    """The device film update evaluates the fit with the hardware exp2 and the widths' reciprocals.
+    Per lobe its value differs from expf(-t^2/2) with a division by at most about
+    (t^2 + 1) e^(-t^2/2) 2^-23 <= 1.21 x 2^-23 times the lobe weight. With the restatement's own
+    expf rounding (1 ulp) and the lobe sums' roundings on both sides (2 ulps), each colour-matching
+    value is within 4.21 x 2^-23 x (sum of |weights|) of the restatement; the XYZ and RGB bounds
+    follow from the sample's |L| and |XYZ -> RGB| (FP64 from there on). Bar: every pixel of a 1-spp
+    hero film (one sample per pixel: the film value is that sample's update) within twice that
+    bound, over 360-830 nm. A 1 % error in one lobe's width exceeds it by orders of magnitude."""
+    W = H = 64
+    sd = scene.cornell_box()
+    s, k1 = gpu_scene(dev, sd)
+    cam = scene.cornell_camera(W, H)
+    film = gpu.Film(dev, W, H)
+    d = gpu.render_desc(s, film, cam, 0, 1, max_bounces=8, min_bounces=3, spectral=abi.SPECTRAL_HERO4,
+                        estimator=abi.ESTIMATOR_CORRECTED, seed=11)
+    gpu.render(d)
+    rgb, cnt = film.read()
+    assert (cnt == 1).all()
+    pix = np.arange(W * H, dtype=np.uint32)
+    g = gpu.trace_paths(d, pix, np.zeros(W * H, dtype=np.uint32))
+    L, l0 = g[:, :4].astype(np.float64), g[:, 4]
+    lams = np.stack([l0 + np.float32(117.5) * np.float32(k) for k in range(4)], axis=1).astype(np.float32)
+    lams = np.where(lams >= 830.0, lams - np.float32(470.0), lams).astype(np.float32)
+    assert lams.min() >= 360.0 and lams.max() < 830.0
+    cm = _cmf_f32(lams).astype(np.float64)                      # (n, 4, 3)
+    scale = 470.0 / 4.0 / 106.856895
+    xyz = (L[:, :, None] * cm).sum(axis=1) * scale
+    ref = xyz @ _XYZ_TO_RGB.T
+    wsum = np.array([sum(abs(l[0]) for l in lobes) for lobes in _CMF_LOBES])
+    dxyz = np.abs(L).sum(axis=1)[:, None] * wsum[None, :] * 4.21 * 2.0 ** -23 * scale
+    tol = 2.0 * dxyz @ np.abs(_XYZ_TO_RGB).T + 1e-12
+    got = rgb.reshape(-1, 3)
+    err = np.abs(got - ref)
+    print("max err / bound %.3f, max rel err %.2e, lambda %.1f..%.1f" % (
+        (err / tol).max(), (err / np.maximum(np.abs(ref), 1e-30)).max(), lams.min(), lams.max()))
+    assert (err <= tol).all()
+    film.close()
+
+
 def test_render_is_resumable_by_sample_range(dev):
     # spp [0,4) then [4,8) into one film equals [0,8) in one call (Philox keyed on the sample index)
     W = H = 32
