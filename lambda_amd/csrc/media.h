@@ -69,19 +69,26 @@ __device__ inline void GridWalkV(const DevScene &sc, const MediumRec &m, V3 o, V
   float t = tn;
   const int iters = nx + ny + nz + 3;
   int it = 0;
-  while (it < iters) {
+  // One exit test per step (the stepping below is selects, not branches), so a wave whose lanes
+  // leave at different steps carries one divergent branch per voxel instead of one per exit.
+  bool go = true;
+  while (go && it < iters) {
     ++it;
-    const int ax = (tNx <= tNy && tNx <= tNz) ? 0 : (tNy <= tNz ? 1 : 2);
-    const float tnext = ax == 0 ? tNx : (ax == 1 ? tNy : tNz);
+    const bool bx = tNx <= tNy && tNx <= tNz, by = !bx && tNy <= tNz, bz = !bx && !by;
+    const float tnext = bx ? tNx : (by ? tNy : tNz);
     const float tc = StdMax(t, StdMin(tnext, tf));
     const uint32_t vox = ((uint32_t)cz * m.ny + (uint32_t)cy) * m.nx + (uint32_t)cx;
     const float rho = dens[vox];
-    if (fn(rho, t, tc, vox)) break;
+    const bool stop = fn(rho, t, tc, vox);
     t = tc;
-    if (t >= tf) break;
-    if (ax == 0) { cx += sx; if (cx < 0 || cx >= nx) break; tNx += dX; }
-    else if (ax == 1) { cy += sy; if (cy < 0 || cy >= ny) break; tNy += dY; }
-    else { cz += sz; if (cz < 0 || cz >= nz) break; tNz += dZ; }
+    cx += bx ? sx : 0;
+    cy += by ? sy : 0;
+    cz += bz ? sz : 0;
+    const bool inside = cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz;
+    tNx = bx ? tNx + dX : tNx;
+    tNy = by ? tNy + dY : tNy;
+    tNz = bz ? tNz + dZ : tNz;
+    go = !stop && t < tf && inside;
   }
   if (sc.voxel_count) atomicAdd(sc.voxel_count, (unsigned long long)it);
 }
@@ -114,6 +121,7 @@ struct MediumSample {
   Spec w;
   float t, pdf;
   bool inter;
+  float D, rho;  // equiangular samples: density integral over [0, t] and the density at t
 };
 // HomogeneousMedium::SampleDistance (HomogeneousMedium.cpp:17-38); grids invert the chosen
 // channel's optical depth voxel by voxel.
@@ -164,16 +172,10 @@ __device__ inline MediumSample SampleDistance(const DevScene &sc, const MediumRe
   return r;
 }
 // HomogeneousMedium::PDFDistance (:40-47): 0 as written (integer 1/nSamples), channel average corrected.
-__device__ inline float PdfDistance(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float t, const VmCtx &vm,
-                                   bool corrected) {
+// From the density integral D over [0, t] and the density rho at t (homogeneous: D = t, rho = 1).
+__device__ inline float PdfDistanceAt(const MediumRec &m, float D, float rho, const VmCtx &vm, bool corrected) {
   if (!corrected) return 0.f;
   const Spec st = SigmaT(m, vm);
-  float rho = 1.f, D = t;
-  if (m.type != LGPU_MEDIUM_HOMOGENEOUS) {
-    const DensAcc a = DensityUpTo(sc, m, o, d, t);
-    D = a.D;
-    rho = a.rho;
-  }
   const Spec Tr = ExpS(st * -1.f * D);
   const int ns = NumLanes(vm);
   float pdf = 0.f;
@@ -194,6 +196,8 @@ __device__ inline MediumSample SampleEquiangular(const DevScene &sc, const Mediu
     r.t = tFarHit;
     r.pdf = 0.f;
     r.w = S(0.f);
+    r.D = 0.f;
+    r.rho = 1.f;
     return r;
   }
   const float thetaA = -atan2f(lp, D);
@@ -214,6 +218,8 @@ __device__ inline MediumSample SampleEquiangular(const DevScene &sc, const Mediu
     Dn = a.D;
     rho = a.rho;
   }
+  r.D = Dn;  // PdfDistance at this t reuses the walk
+  r.rho = rho;
   const Spec Tr = ExpS(st * -1.f * StdMin(Dn, 3.40282347e38f));
   r.w = Tr * (ss * rho) / pdf;
   return r;

@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
             beta = beta * sD.w;
           } else {
             medInter = true;
-            const float pdfD2 = PdfDistance(sc, M, ro, rd, sE.t, vm, false);
+            const float pdfD2 = PdfDistanceAt(M, sE.D, sE.rho, vm, false);
             const float pdfE2 = PdfEquiangular(ro, rd, ls.point, h.tFar, sD.t, false);
             const float wD = PowerHeuristic(sD.pdf, pdfE2), wE = PowerHeuristic(sE.pdf, pdfD2);
             V3 wi0, wi1;
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
           if (sE.inter) {
             PartialLS l1 = ls;
             Hit eh = h;
-            const float wE = PowerHeuristic(sE.pdf, PdfDistance(sc, M, ro, rd, sE.t, vm, true));
+            const float wE = PowerHeuristic(sE.pdf, PdfDistanceAt(M, sE.D, sE.rho, vm, true));
             Ld = Ld + LdMediumPoint<kPrograms>(sc, M, med, ev, eh, mp1, l1, refP, refN, &wi1, rng, vm, true, vc) * sE.w * wE;
           }
           Lacc = Lacc + beta * Ld;
