@@ -124,39 +124,40 @@ struct MediumSample {
   float D, rho;  // equiangular samples: density integral over [0, t] and the density at t
 };
 // HomogeneousMedium::SampleDistance (HomogeneousMedium.cpp:17-38); grids invert the chosen
-// channel's optical depth voxel by voxel.
-__device__ inline MediumSample SampleDistance(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float tFarHit, float u0,
-                                              float u1, const VmCtx &vm) {
-  const Spec st = SigmaT(m, vm), ss = SigmaS(m, vm);
+// channel's optical depth voxel by voxel. DistWalk is that inversion as a per-voxel step, so a
+// walk along the same ray can serve other integrals too (SampleDistanceEquiangular).
+struct DistWalk {
+  float stc, target, acc = 0.f, t, D = 0.f, rhoAt = 1.f;
+  bool found = false;
+  // one voxel segment [ta, tb] of density rho; true once the sample is placed
+  __device__ __attribute__((always_inline)) bool Step(float rho, float ta, float tb) {
+    const float k = stc * rho;
+    const float seg = k * (tb - ta);
+    if (k > 0 && acc + seg >= target) {
+      t = ta + (target - acc) / k;
+      D += rho * (t - ta);
+      rhoAt = rho;
+      found = true;
+      return true;
+    }
+    acc += seg;
+    D += rho * (tb - ta);
+    return false;
+  }
+};
+__device__ inline float DistanceChannelSigma(const MediumRec &m, float u0, const VmCtx &vm) {
+  const Spec st = SigmaT(m, vm);
   const int ns = NumLanes(vm);
   const uint32_t c0 = (uint32_t)(u0 * (float)ns);
   const uint32_t ch = c0 < (uint32_t)ns - 1 ? c0 : (uint32_t)ns - 1;
-  const float stc = ch == 0 ? st.c[0] : (ch == 1 ? st.c[1] : (ch == 2 ? st.c[2] : st.c[3]));
-  float t, rhoAt = 1.f, D = 0.f;
-  if (m.type == LGPU_MEDIUM_HOMOGENEOUS) {
-    const float dist = -logf(1 - u1) / stc;
-    t = StdMin(dist, tFarHit);
-  } else {
-    const float target = -logf(1 - u1);
-    t = tFarHit;
-    float acc = 0.f;
-    bool found = false;
-    GridWalk(sc, m, o, d, 0.f, tFarHit, [&](float rho, float ta, float tb) {
-      const float k = stc * rho;
-      const float seg = k * (tb - ta);
-      if (k > 0 && acc + seg >= target) {
-        t = ta + (target - acc) / k;
-        D += rho * (t - ta);
-        rhoAt = rho;
-        found = true;
-        return true;
-      }
-      acc += seg;
-      D += rho * (tb - ta);
-      return false;
-    });
-    if (!found) t = tFarHit;
-  }
+  return ch == 0 ? st.c[0] : (ch == 1 ? st.c[1] : (ch == 2 ? st.c[2] : st.c[3]));
+}
+// The sample's weight and pdf from its distance t, the density integral D over [0, t] and the
+// density rhoAt at t (grids; homogeneous media use t).
+__device__ inline MediumSample DistanceFinish(const MediumRec &m, float t, float D, float rhoAt, float tFarHit,
+                                              const VmCtx &vm) {
+  const Spec st = SigmaT(m, vm), ss = SigmaS(m, vm);
+  const int ns = NumLanes(vm);
   MediumSample r;
   r.inter = t < tFarHit;
   const Spec Tr = m.type == LGPU_MEDIUM_HOMOGENEOUS ? ExpS(st * -1.f * StdMin(t, 3.40282347e38f)) : ExpS(st * -1.f * D);
@@ -169,7 +170,23 @@ __device__ inline MediumSample SampleDistance(const DevScene &sc, const MediumRe
   r.t = t;
   r.pdf = pdf;
   r.w = r.inter ? Tr * ssl / pdf : Tr / pdf;
+  r.D = D;
+  r.rho = rhoAt;
   return r;
+}
+__device__ inline MediumSample SampleDistance(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float tFarHit, float u0,
+                                              float u1, const VmCtx &vm) {
+  const float stc = DistanceChannelSigma(m, u0, vm);
+  if (m.type == LGPU_MEDIUM_HOMOGENEOUS) {
+    const float dist = -logf(1 - u1) / stc;
+    return DistanceFinish(m, StdMin(dist, tFarHit), 0.f, 1.f, tFarHit, vm);
+  }
+  DistWalk w;
+  w.stc = stc;
+  w.target = -logf(1 - u1);
+  w.t = tFarHit;
+  GridWalk(sc, m, o, d, 0.f, tFarHit, [&](float rho, float ta, float tb) { return w.Step(rho, ta, tb); });
+  return DistanceFinish(m, w.found ? w.t : tFarHit, w.D, w.rhoAt, tFarHit, vm);
 }
 // HomogeneousMedium::PDFDistance (:40-47): 0 as written (integer 1/nSamples), channel average corrected.
 // From the density integral D over [0, t] and the density rho at t (homogeneous: D = t, rho = 1).
@@ -183,10 +200,11 @@ __device__ inline float PdfDistanceAt(const MediumRec &m, float D, float rho, co
   return pdf * (1 / (float)ns);
 }
 // HomogeneousMedium::SampleEquiangular (:49-78). faithful: origin at the surface hit, t = D tan(theta)
-// taken as the ray distance; corrected: origin ray.o, t = l + D tan(theta).
-__device__ inline MediumSample SampleEquiangular(const DevScene &sc, const MediumRec &m, V3 o, V3 d, V3 hitPoint,
-                                                 float tFarHit, V3 lightPoint, float eps, const VmCtx &vm, bool corrected) {
-  MediumSample r;
+// taken as the ray distance; corrected: origin ray.o, t = l + D tan(theta). EquiangularSetup places
+// the sample (no density needed; false: no sample), EquiangularFinish weights it from the density
+// integral over [0, t] and the density at t (DensityUpTo for grids; t and 1 for homogeneous media).
+__device__ inline bool EquiangularSetup(V3 o, V3 d, V3 hitPoint, float tFarHit, V3 lightPoint, float eps, bool corrected,
+                                        MediumSample &r) {
   const V3 delta = lightPoint - (corrected ? o : hitPoint);
   const float lp = Dot(d, delta);
   const float q = Dot(delta, delta) - lp * lp;
@@ -198,7 +216,7 @@ __device__ inline MediumSample SampleEquiangular(const DevScene &sc, const Mediu
     r.w = S(0.f);
     r.D = 0.f;
     r.rho = 1.f;
-    return r;
+    return false;
   }
   const float thetaA = -atan2f(lp, D);
   const float thetaB = atan2f(tFarHit - lp, D);
@@ -208,21 +226,51 @@ __device__ inline MediumSample SampleEquiangular(const DevScene &sc, const Mediu
   const float t = StdMin(tc, tFarHit);
   r.inter = t < tFarHit;
   const float x = corrected ? t - lp : t;
-  const float pdf = D / ((thetaB - thetaA) * (D * D + x * x));
   r.t = t;
-  r.pdf = pdf;
-  const Spec st = SigmaT(m, vm), ss = SigmaS(m, vm);
-  float Dn = t, rho = 1.f;
-  if (m.type != LGPU_MEDIUM_HOMOGENEOUS) {
-    const DensAcc a = DensityUpTo(sc, m, o, d, t);
-    Dn = a.D;
-    rho = a.rho;
-  }
-  r.D = Dn;  // PdfDistance at this t reuses the walk
+  r.pdf = D / ((thetaB - thetaA) * (D * D + x * x));
+  return true;
+}
+__device__ inline void EquiangularFinish(const MediumRec &m, MediumSample &r, float Dn, float rho, const VmCtx &vm) {
+  const Spec ss = SigmaS(m, vm);
+  r.D = Dn;  // the distance pdf at this t reuses these (PdfDistanceAt)
   r.rho = rho;
-  const Spec Tr = ExpS(st * -1.f * StdMin(Dn, 3.40282347e38f));
-  r.w = Tr * (ss * rho) / pdf;
-  return r;
+  const Spec Tr = ExpS(SigmaT(m, vm) * -1.f * StdMin(Dn, 3.40282347e38f));
+  r.w = Tr * (ss * rho) / r.pdf;
+}
+// Both samples of the MIS volumetric integrator along one ray, with one grid walk: the distance
+// sample's inversion over [0, tFarHit] and the equiangular sample's density integral over
+// [0, t_e] (t_e <= tFarHit) visit the same voxels in the same order with the same segment bounds
+// (a segment reaching past t_e counts up to t_e, as DensityUpTo's walk clipped at t_e does), so both
+// give the values of separate walks. The walk ends when the distance sample is placed and t_e passed.
+__device__ inline void SampleDistanceEquiangular(const DevScene &sc, const MediumRec &m, V3 o, V3 d, float tFarHit,
+                                                 float u0, float u1, V3 hitPoint, V3 lightPoint, float eps,
+                                                 const VmCtx &vm, bool corrected, MediumSample &sD, MediumSample &sE) {
+  const bool eq = EquiangularSetup(o, d, hitPoint, tFarHit, lightPoint, eps, corrected, sE);
+  if (m.type == LGPU_MEDIUM_HOMOGENEOUS) {
+    sD = SampleDistance(sc, m, o, d, tFarHit, u0, u1, vm);
+    if (eq) EquiangularFinish(m, sE, sE.t, 1.f, vm);
+    return;
+  }
+  DistWalk w;
+  w.stc = DistanceChannelSigma(m, u0, vm);
+  w.target = -logf(1 - u1);
+  w.t = tFarHit;
+  const float te = sE.t;
+  float De = 0.f, rhoE = 0.f;  // DensityUpTo's start: no voxel before t_e leaves rho 0
+  bool eDone = !eq;
+  GridWalk(sc, m, o, d, 0.f, tFarHit, [&](float rho, float ta, float tb) {
+    if (!eDone) {
+      if (ta < te) {
+        De += rho * (StdMin(tb, te) - ta);
+        rhoE = rho;
+      }
+      eDone = !(tb < te);
+    }
+    const bool dDone = w.found || w.Step(rho, ta, tb);
+    return dDone && eDone;
+  });
+  sD = DistanceFinish(m, w.found ? w.t : tFarHit, w.D, w.rhoAt, tFarHit, vm);
+  if (eq) EquiangularFinish(m, sE, De, rhoE, vm);
 }
 // HomogeneousMedium::PDFEquiangular (:80-92)
 __device__ inline float PdfEquiangular(V3 o, V3 d, V3 lightPoint, float tFar, float t, bool corrected) {

@@ -285,11 +285,11 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
         else { ls.pdf = 0.f; ls.Le = S(0.f); }
         const int choice = rng.Get1D() > .5f ? 0 : 1;  // 0 distance, 1 equiangular
         const float du0 = rng.Get1D(), du1 = rng.Get1D();
-        const MediumSample sD = SampleDistance(sc, M, ro, rd, h.tFar, du0, du1, vm);
         const float eps = rng.Get1D();
         rng.Get1D();  // channel draw, unused (HomogeneousMedium.cpp:70)
         rng.Get1D();
-        const MediumSample sE = SampleEquiangular(sc, M, ro, rd, h.point, h.tFar, ls.point, eps, vm, corrected);
+        MediumSample sD, sE;  // one grid walk for both samples
+        SampleDistanceEquiangular(sc, M, ro, rd, h.tFar, du0, du1, h.point, ls.point, eps, vm, corrected, sD, sE);
         const V3 mp0 = ro + rd * sD.t, mp1 = ro + rd * sE.t;
         if (!corrected) {
           if (!(sD.inter && sE.inter)) {
