@@ -187,8 +187,11 @@ def main():
                     help="Adam steps after each guided wave of the warm-up (waves of 8 spp)")
     ap.add_argument("--guide-warm-spp", type=int, default=256,
                     help="guided samples per pixel rendered in 8-spp waves with --guide-steps of training each")
-    ap.add_argument("--guide-late-steps", type=int, default=4,
-                    help="Adam steps after each later guided wave (waves of --guide-late-wave spp)")
+    ap.add_argument("--guide-late-steps", default="4",
+                    help="Adam steps after each later guided wave (waves of --guide-late-wave spp); a "
+                         "comma-separated list measures each")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="with --relmse --guided: skip the plain estimator's run")
     ap.add_argument("--guide-late-wave", type=int, default=32)
     ap.add_argument("--guide-depth", default="0",
                     help="guide vertices below this path depth only (0: all); a comma-separated list measures each")
@@ -557,7 +560,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     else:
         rsum, rcnt = ref.read()
         r = rsum / rcnt[..., None]
-    def progressive(guide=None):
+    def progressive(guide=None, late_steps=4):
         """Checkpoints of one progressive render (cumulative wall time of the render calls,
         training included for the guided estimator); stops at relMSE <= 1e-3."""
         film.clear()
@@ -583,7 +586,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
                 while b < target:
                     warm = b < args.guide_warm_spp
                     per = min(target - b, 8 if warm else args.guide_late_wave)
-                    steps = args.guide_steps if warm else args.guide_late_steps
+                    steps = args.guide_steps if warm else late_steps
                     tally(gpu.render(desc(b, b + per, guide=guide, steps=steps, capacity=W * H * per)))
                     b += per
             spent += time.perf_counter() - t
@@ -601,19 +604,20 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
                 break
         return checkpoints, hit
 
-    checkpoints, hit = progressive()
+    checkpoints, hit = ([], None) if (args.guided and args.no_plain) else progressive()
     guided = None
     if args.guided:
         guided = []
         for depth in [int(x) for x in args.guide_depth.split(",")]:
-            g = gpu.Guide(dev, seed=7, max_depth=depth)
-            gck, ghit = progressive(g)
-            guided.append({"value": ghit["seconds"] if ghit else None, "spp_at_target": ghit["spp"] if ghit else None,
-                           "checkpoints": gck, "guide": g.stats(),
-                           "schedule": {"warm_spp": args.guide_warm_spp, "warm_wave_spp": 8,
-                                        "warm_steps": args.guide_steps, "late_wave_spp": args.guide_late_wave,
-                                        "late_steps": args.guide_late_steps, "guided_depth": depth or None}})
-            g.close()
+            for late in [int(x) for x in args.guide_late_steps.split(",")]:
+                g = gpu.Guide(dev, seed=7, max_depth=depth)
+                gck, ghit = progressive(g, late)
+                guided.append({"value": ghit["seconds"] if ghit else None,
+                               "spp_at_target": ghit["spp"] if ghit else None, "checkpoints": gck, "guide": g.stats(),
+                               "schedule": {"warm_spp": args.guide_warm_spp, "warm_wave_spp": 8,
+                                            "warm_steps": args.guide_steps, "late_wave_spp": args.guide_late_wave,
+                                            "late_steps": late, "guided_depth": depth or None}})
+                g.close()
     # CPU validation of the reference on a 64x64 centre crop
     valid = None
     if not args.no_cpu and cpu_ref is None:
