@@ -365,7 +365,7 @@ lgpu_status lgpu_render_noalloc(const lgpu_render_desc *desc, lgpu_render_stats 
  * previous ones run. Jobs take the path or volumetric path integrators with the corrected
  * estimator and no guide; films must be distinct. Each film equals lgpu_render's for the same job
  * (same per-path radiance; the FP64 sums group samples per chunk, so they agree to rounding). */
-#define LGPU_BATCH_AUTOTUNE 1u /* time pool capacity 2^22..2^25 x branches 1..16 on one-sample trials; cached */
+#define LGPU_BATCH_AUTOTUNE 1u /* time pool capacities from 2^23 up to one chunk x 4, 8, 16 branches on whole-batch trials; cached */
 typedef struct lgpu_batch_options {
   uint32_t path_capacity;  /* paths in flight over all jobs (0: 2^24, or the autotuner's choice) */
   uint32_t flags;          /* LGPU_BATCH_* */

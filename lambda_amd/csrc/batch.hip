@@ -312,14 +312,28 @@ extern "C" lgpu_status lgpu_render_batch(const lgpu_render_desc *descs, uint32_t
       const auto t0 = std::chrono::steady_clock::now();
       double best = -1.0;
       TuneChoice pick{capacity, branches};
-      for (uint64_t c : {1ull << 22, 1ull << 23, 1ull << 24, 1ull << 25})
-        for (uint32_t b : {1u, 4u, 8u, 16u}) {
+      // Candidate pools from 2^23 paths up to the first that holds the whole batch (one chunk),
+      // within a quarter of the free device memory; each trial renders the whole batch, since the
+      // chunk count (and with it the graph's node count) depends on the real sample ranges.
+      uint64_t paths = 0, spillWords = 0;
+      for (uint32_t j = 0; j < count; ++j) {
+        paths += (uint64_t)descs[j].film->width * descs[j].film->height * (descs[j].spp_end - descs[j].spp_begin);
+        spillWords = std::max<uint64_t>(spillWords, descs[j].scene->dsc.spill_words);
+      }
+      const uint64_t bytesPerPath = 2 * 9 * 16 + 16 + 4 + 4 + sizeof(ShadowRay) + 4 * spillWords;
+      size_t freeB = 0, totalB = 0;
+      LGPU_HIP(hipMemGetInfo(&freeB, &totalB));
+      for (uint64_t c = 1ull << 23; c <= (1ull << 27); c <<= 1) {
+        if (c > (1ull << 23) && c * bytesPerPath > freeB / 4) break;
+        for (uint32_t b : {4u, 8u, 16u}) {
           lgpu_batch_stats t;
-          st = RunBatch(descs, count, c, b, 1, false, o.max_iterations, &t);
+          st = RunBatch(descs, count, c, b, 0, false, o.max_iterations, &t);
           if (st != LGPU_OK) return st;
           const double rate = (double)t.samples / std::max(1e-6, t.ms_render);
           if (rate > best) { best = rate; pick = TuneChoice{c, b}; }
         }
+        if (c >= paths) break;  // one chunk already
+      }
       tuneMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       it = TuneCache().emplace(key, pick).first;
     }
