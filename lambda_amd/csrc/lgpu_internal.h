@@ -311,6 +311,7 @@ struct DevScene {
   const float *dist;         // Piecewise1D pool
   EnvRec env;
   uint32_t has_env;
+  uint32_t has_mesh_lights;  // some mesh is emissive (a BSDF- or phase-sampled ray can reach a light)
   uint32_t env_light;        // scene light index of the environment light
   uint32_t strategy;         // 0 power, 1 tree
   uint32_t num_lights;

@@ -79,6 +79,9 @@ __device__ inline Spec LdMediumPoint(const DevScene &sc, const MediumRec &M, uin
   const float scatPdf = HGSample(M.g, ev.wo, &wi, u0, u1, corrected);
   ev.wi = wi;
   *wiOut = wi;
+  // corrected (the caller's hit record is a copy): without emissive meshes and an environment the
+  // phase-sampled ray reaches no light, its term adds zero, and its transmittance walk is skipped
+  if (corrected && !(sc.has_mesh_lights || sc.has_env)) return Ld;
   Spec Tr = S(1.f);
   uint32_t tbi = kNone;
   const bool si = IntersectTr(sc, p, wi, med, -1.f, &Tr, vm, corrected, vc.tc, eh, tbi, nullptr, false);
@@ -155,8 +158,12 @@ __global__ void __launch_bounds__(kBlock) k_shade_vol(DevScene sc, RenderParams 
   else VmAtHit(vm, 0.f, 0.f, v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f));
   bool alive = true;
 
-  // ---- 1. light reached by the previous vertex's sampled direction
-  if (!(flags & kFlagCamera) && !(flags & kFlagNoEmit)) {
+  // ---- 1. light reached by the previous vertex's sampled direction. Without emissive meshes and
+  // an environment no light can be reached (MisLight finds none), so the term and its transmittance
+  // walk are skipped: they add zero.
+  if (!(flags & kFlagCamera) && !(flags & kFlagNoEmit) && !(sc.has_mesh_lights || sc.has_env)) {
+    if ((flags & kFlagEmitOnly) || !hit) alive = false;
+  } else if (!(flags & kFlagCamera) && !(flags & kFlagNoEmit)) {
     const float4 mis = ps.mis[i];
     bool si = hit;
     Hit th = h;
