@@ -1103,9 +1103,9 @@ lgpu_status CommitScene(lgpu_scene s, uint32_t strategy) {
   for (const Socket &k : sockets)
     if (k.kind != kSockScalar && k.kind != kSockColour && k.kind != kSockBlackbody && k.kind != kSockVec2) d.has_programs = 1;
   d.has_env = hasEnv;
-  d.has_mesh_lights = 0;
-  for (const LightRec &L : lights)
-    if (L.kind == kLightMesh) d.has_mesh_lights = 1;
+  d.has_mesh_lights = 0;  // as MisLight reads it: some mesh record names a light
+  for (const MeshRec &m : meshes)
+    if (m.mesh_light != kNone) d.has_mesh_lights = 1;
   d.env_light = envLight;
   d.env = env;
   d.num_tris = nt;
