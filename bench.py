@@ -20,6 +20,7 @@ rendering against an FP64 GPU reference at --ref-spp with another seed.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -572,9 +573,9 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             for key in kern:
                 kern[key] += getattr(st, key)
 
-        # checkpoints at 2^k and 3 * 2^(k-1) spp, up to half the reference's
-        targets = sorted({1 << j for j in range(31) if (1 << j) <= args.ref_spp // 2} |
-                         {3 << j for j in range(30) if (3 << j) <= args.ref_spp // 2})
+        # checkpoints at 2^k, 1.25, 1.5 and 1.75 x 2^k spp (integers), up to half the reference's
+        targets = sorted({(m << j) // 4 for j in range(31) for m in (4, 5, 6, 7)
+                          if (m << j) % 4 == 0 and 0 < (m << j) // 4 <= args.ref_spp // 2})
         for target in targets:
             t = time.perf_counter()
             if guide is None:
@@ -600,7 +601,17 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
             log("%s spp %d: %.3f s, relMSE %.3e; ms %s" % ("guided" if guide else "plain", target, spent, rel,
                                                             {key: round(v, 1) for key, v in kern.items()}))
             if rel <= 1e-3:
-                hit = checkpoints[-1]
+                hit = dict(checkpoints[-1])
+                if len(checkpoints) > 1:
+                    # where the curve crosses 1e-3 between the last two checkpoints (log-log
+                    # interpolation of relMSE against spp, time linear in spp between them)
+                    a, b = checkpoints[-2], checkpoints[-1]
+                    if a["relmse"] > 1e-3 > 0 and b["relmse"] > 0:
+                        f = math.log(a["relmse"] / 1e-3) / math.log(a["relmse"] / b["relmse"])
+                        spp_x = a["spp"] * (b["spp"] / a["spp"]) ** f
+                        hit["interpolated_spp"] = round(spp_x, 1)
+                        hit["interpolated_s"] = round(a["seconds"] + (b["seconds"] - a["seconds"]) *
+                                                      (spp_x - a["spp"]) / (b["spp"] - a["spp"]), 3)
                 break
         return checkpoints, hit
 
@@ -613,7 +624,9 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
                 g = gpu.Guide(dev, seed=7, max_depth=depth)
                 gck, ghit = progressive(g, late)
                 guided.append({"value": ghit["seconds"] if ghit else None,
-                               "spp_at_target": ghit["spp"] if ghit else None, "checkpoints": gck, "guide": g.stats(),
+                               "spp_at_target": ghit["spp"] if ghit else None,
+                               "interpolated_s": ghit.get("interpolated_s") if ghit else None,
+                               "checkpoints": gck, "guide": g.stats(),
                                "schedule": {"warm_spp": args.guide_warm_spp, "warm_wave_spp": 8,
                                             "warm_steps": args.guide_steps, "late_wave_spp": args.guide_late_wave,
                                             "late_steps": late, "guided_depth": depth or None}})
@@ -644,7 +657,8 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
         del keep
     line = {"metric": "time-to-relMSE<=1e-3 (plain wavefront, 1 MI355X)",
             "value": hit["seconds"] if hit else None, "unit": "s", "n_gpus": 1, "higher_is_better": False,
-            "spp_at_target": hit["spp"] if hit else None, "dtype": "f32", "data": "synthetic (procedural scene)",
+            "spp_at_target": hit["spp"] if hit else None,
+            "interpolated_s": hit.get("interpolated_s") if hit else None, "dtype": "f32", "data": "synthetic (procedural scene)",
             "config": dict(cfg, spectral=args.spectral, estimator=args.estimator, reference_spp=args.ref_spp,
                            reference_seed=1, reference_s=round(ref_s, 1),
                            reference_estimator="guided" if args.ref_guided else "plain",
