@@ -264,12 +264,15 @@ def main():
         log("counting pass: %.2f 4-wide node records, %.2f triangle records, %.1f voxel steps per extension ray"
             % (nodes4_per_ray, tris4_per_ray, voxels_per_vertex))
 
+    # the path pool of the timed render (as the library would size it by default), allocated by the
+    # warm-up so that the timed region measures the steady state, not the pool's growth
+    pool = args.capacity or min(W * H * args.steps, 1 << 27)
     if args.warmup:
-        gpu.render(desc(0, args.warmup))
+        gpu.render(desc(0, args.warmup, capacity=pool))
         log("warmup %d spp done" % args.warmup)
     film.clear()
     t0 = time.perf_counter()
-    st = gpu.render(desc(args.warmup, args.warmup + args.steps))
+    st = gpu.render(desc(args.warmup, args.warmup + args.steps, capacity=pool))
     dt = time.perf_counter() - t0
     samples = W * H * args.steps
     assert st.samples == samples

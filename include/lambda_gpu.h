@@ -244,8 +244,10 @@ typedef struct lgpu_render_desc {
   uint32_t spectral_mode;
   uint32_t estimator_mode;
   uint32_t seed;            /* Philox key word 0 */
-  uint32_t path_capacity;   /* paths in flight; 0 = up to 134217728, held to a quarter of free device memory
-                               (about 400-460 B per path; the pool stays allocated on the device handle) */
+  uint32_t path_capacity;   /* paths in flight (the pool, which stays allocated on the device handle, about
+                               400-460 B per path): 0 = up to 134217728, no more than this render's samples,
+                               held to a quarter of free device memory; a value sizes the pool as given even
+                               for a smaller render (so an untimed render can prepare it) */
   uint32_t flags;           /* LGPU_RENDER_COUNT_TRAVERSAL: count BVH nodes / triangles visited */
   /* utility passes (UtilityIntegrators.cpp:26-53); the reference's defaults are power 1,
    * background (1, 0, 1), shading normals */

@@ -923,7 +923,9 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(fit, 1ull << 16));
       }
     }
-    if (cap > total) cap = total;
+    // the default pool holds no more than this render's samples; an explicit capacity sizes the
+    // pool as asked, so a small render can prepare it for a larger one
+    if (!d->path_capacity && cap > total) cap = total;
     if (cap == 0) cap = 1;
     lgpu_status st = W0.alloc((uint32_t)cap, s->dsc.spill_words, d->guide != nullptr);
     if (st != LGPU_OK) return st;
