@@ -564,6 +564,9 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     else:
         rsum, rcnt = ref.read()
         r = rsum / rcnt[..., None]
+    # untimed: the path pool the plain checkpoints' largest call needs (as the timed bench's warm-up)
+    gpu.render(desc(0, 1, capacity=min(W * H * max(1, args.ref_spp // 8), 1 << 27)))
+
     def progressive(guide=None, late_steps=4):
         """Checkpoints of one progressive render (cumulative wall time of the render calls,
         training included for the guided estimator); stops at relMSE <= 1e-3."""
