@@ -415,8 +415,8 @@ typedef struct lgpu_adjoint_desc {
 lgpu_status lgpu_render_adjoint(const lgpu_render_desc *desc, const lgpu_adjoint_desc *adjoint,
                                 lgpu_render_stats *stats);
 
-/* ---------------- learned path guiding (no reference counterpart; the north-star extension of
- * the BSDF sampling step, PathIntegrator.cpp:46-67). A 6-layer, width-64 MLP (BF16 MFMA, FP32
+/* ---------------- learned path guiding (no reference counterpart; an extension of the BSDF
+ * sampling step, PathIntegrator.cpp:46-67). A 6-layer, width-64 MLP (BF16 MFMA, FP32
  * accumulation) maps a shading point and its outgoing direction (5 inputs, 8-octave frequency
  * encoding) to a distribution over 128 world-space direction bins and the probability c of
  * sampling it instead of the BSDF; trained online with Adam on the renderer's own radiance
