@@ -193,9 +193,11 @@ def main():
                          "comma-separated list measures each")
     ap.add_argument("--no-plain", action="store_true",
                     help="with --relmse --guided: skip the plain estimator's run")
-    ap.add_argument("--guide-late-wave", type=int, default=32)
-    ap.add_argument("--guide-depth", default="0",
-                    help="guide vertices below this path depth only (0: all); a comma-separated list measures each")
+    ap.add_argument("--guide-late-wave", default="64",
+                    help="spp per later guided wave; a comma list runs one guided measurement per value")
+    ap.add_argument("--guide-depth", default="2",
+                    help="guide vertices below this path depth only (0: all; 2: the first two scattering "
+                         "vertices); a comma-separated list measures each")
     args = ap.parse_args()
     if not args.steps:
         args.steps = DEFAULT_STEPS[args.workload]
@@ -567,7 +569,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     # untimed: the path pool the plain checkpoints' largest call needs (as the timed bench's warm-up)
     gpu.render(desc(0, 1, capacity=min(W * H * max(1, args.ref_spp // 8), 1 << 27)))
 
-    def progressive(guide=None, late_steps=4):
+    def progressive(guide=None, late_steps=4, late_wave=64):
         """Checkpoints of one progressive render (cumulative wall time of the render calls,
         training included for the guided estimator); stops at relMSE <= 1e-3."""
         film.clear()
@@ -592,7 +594,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
                 b = done
                 while b < target:
                     warm = b < args.guide_warm_spp
-                    per = min(target - b, 8 if warm else args.guide_late_wave)
+                    per = min(target - b, 8 if warm else late_wave)
                     steps = args.guide_steps if warm else late_steps
                     tally(gpu.render(desc(b, b + per, guide=guide, steps=steps, capacity=W * H * per)))
                     b += per
@@ -626,15 +628,16 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     if args.guided:
         guided = []
         for depth in [int(x) for x in args.guide_depth.split(",")]:
-            for late in [int(x) for x in args.guide_late_steps.split(",")]:
+            for late, wave in [(int(x), int(y)) for x in args.guide_late_steps.split(",")
+                               for y in args.guide_late_wave.split(",")]:
                 g = gpu.Guide(dev, seed=7, max_depth=depth)
-                gck, ghit = progressive(g, late)
+                gck, ghit = progressive(g, late, wave)
                 guided.append({"value": ghit["seconds"] if ghit else None,
                                "spp_at_target": ghit["spp"] if ghit else None,
                                "interpolated_s": ghit.get("interpolated_s") if ghit else None,
                                "checkpoints": gck, "guide": g.stats(),
                                "schedule": {"warm_spp": args.guide_warm_spp, "warm_wave_spp": 8,
-                                            "warm_steps": args.guide_steps, "late_wave_spp": args.guide_late_wave,
+                                            "warm_steps": args.guide_steps, "late_wave_spp": wave,
                                             "late_steps": late, "guided_depth": depth or None}})
                 g.close()
     # CPU validation of the reference on a 64x64 centre crop

@@ -297,52 +297,6 @@ __device__ inline void QueryScalars(const GuideQuerySrc &src, const QueryRaw &r,
   GuideInputs(p, v3(-d.x, -d.y, -d.z), bmin, binv, s);
 }
 
-// The queue slots whose next shading is a guided scattering event: a hit on a guidable material
-// by a path that is not terminating (kFlagEmitOnly gathers emission only). A block covers
-// kListPer x kListBlock consecutive slots and takes its list range with one atomic (device-scope
-// atomics on one counter serialise across the GPU, so they are kept few).
-constexpr int kListBlock = 256, kListPer = 8;
-__global__ void __launch_bounds__(kListBlock) k_guide_list(const BvhTri *__restrict__ tris, const float4 *__restrict__ hit,
-                                                          const uint4 *__restrict__ ids, uint32_t n, const uint32_t *np,
-                                                          uint32_t maxDepth, uint32_t *list, uint32_t *count) {
-  __shared__ uint32_t wcnt[kListPer][kListBlock / 64];
-  __shared__ uint32_t blockBase;
-  if (np) n = *np;
-  const uint32_t base = blockIdx.x * (uint32_t)(kListBlock * kListPer);
-  if (base >= n) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint64_t masks[kListPer];
-#pragma unroll
-  for (int k = 0; k < kListPer; ++k) {
-    const uint32_t i = base + k * kListBlock + threadIdx.x;
-    bool take = false;
-    if (i < n) {
-      const uint32_t bi = __float_as_uint(hit[i].w);
-      const uint32_t w = ids[i].w;
-      take = bi != kNone && (tris[bi].key & kKeyGuidable) && !(w & kFlagEmitOnly) && (w >> 8) < maxDepth;
-    }
-    masks[k] = __ballot(take);
-    if (lane == 0) wcnt[k][wv] = (uint32_t)__popcll(masks[k]);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int k = 0; k < kListPer; ++k)
-      for (int w = 0; w < kListBlock / 64; ++w) {
-        const uint32_t c = wcnt[k][w];
-        wcnt[k][w] = acc;
-        acc += c;
-      }
-    blockBase = acc ? atomicAdd(count, acc) : 0u;
-  }
-  __syncthreads();
-  const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int k = 0; k < kListPer; ++k)
-    if ((masks[k] >> lane) & 1ull)
-      list[blockBase + wcnt[k][wv] + (uint32_t)__popcll(masks[k] & below)] = base + k * kListBlock + threadIdx.x;
-}
-
 // Inference: per query, the quantised bin table (bytes max(1, round(255 exp(z - z_max))), the
 // distribution is byte / sum) and the mixing probability. Persistent blocks; weights in LDS in
 // fragment order.
@@ -748,13 +702,10 @@ lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expec
   return LGPU_OK;
 }
 
-lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const DevScene &sc, const PathState &ps, uint32_t ub,
-                       const uint32_t *np, uint32_t *list, uint32_t *count, uint4 *bins, float2 *hdr, hipStream_t s) {
-  LGPU_HIP(hipMemsetAsync(count, 0, 4, s));
-  constexpr uint32_t kListSpan = kListBlock * kListPer;
-  hipLaunchKernelGGL(k_guide_list, dim3((ub + kListSpan - 1) / kListSpan), dim3(kListBlock), 0, s, sc.tris, ps.hit, ps.ids,
-                     ub, np, refs.max_depth, list, count);
-  LGPU_HIP(hipGetLastError());
+// `list` holds the bounce's guided vertices (`count` of them, at most `ub`), written by the
+// extension stage.
+lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps, uint32_t ub, const uint32_t *list,
+                       const uint32_t *count, uint4 *bins, float2 *hdr, hipStream_t s) {
   GuideQuerySrc src{};
   src.ray_o = ps.ray_o; src.ray_d = ps.ray_d; src.hit = ps.hit; src.list = list; src.scalars = nullptr;
   for (int k = 0; k < 3; ++k) { src.bmin[k] = refs.bmin[k]; src.binv[k] = refs.binv[k]; }

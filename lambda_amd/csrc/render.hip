@@ -126,33 +126,92 @@ __global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, PathState ps
   ss.lambda0[i] = lambda0;
 }
 
-template <bool kCount>
+// Guided renders: the extension stage also lists the queue slots whose hit is a guided vertex
+// (guidable material, path still scattering, depth below the guide's limit), for the guide's
+// inference before shading. One device atomic per block allocates the block's entries, taken by
+// the block's last wave to finish its rays, so no wave waits at a barrier for the others.
+struct GuideList {
+  uint32_t *list;
+  uint32_t *count;  // zeroed before the extension launch
+  uint32_t max_depth;
+};
+
+struct GuideListLds {
+  uint64_t mask[kBlock / 64];  // per wave: its lanes that take an entry
+  uint32_t arrived;            // waves done (zeroed, then a barrier, before any wave arrives)
+};
+
+__device__ inline void GuideListAppend(bool take, uint32_t blockFirst, const GuideList &gl, GuideListLds &lds) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t m = __ballot(take);
+  uint32_t last = 0;
+  if (lane == 0) {
+    lds.mask[wv] = m;
+    __threadfence_block();  // the mask is in LDS before the arrival is counted
+    last = atomicAdd(&lds.arrived, 1u) == (uint32_t)(kBlock / 64 - 1);
+  }
+  if (!__shfl(last, 0)) return;
+  __threadfence_block();
+  uint64_t mk[kBlock / 64];
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    mk[w] = lds.mask[w];
+    total += (uint32_t)__popcll(mk[w]);
+  }
+  if (!total) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(gl.count, total);
+  base = __shfl(base, 0);
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if ((mk[w] >> lane) & 1ull) gl.list[base + (uint32_t)__popcll(mk[w] & below)] = blockFirst + w * 64 + lane;
+    base += (uint32_t)__popcll(mk[w]);
+  }
+}
+
 // Kernels of the path loop take their item count either as a value or, when `np` is set, from a
 // device counter written by the previous stage (the grid then covers an upper bound of it).
+template <bool kCount, bool kList = false>
 __global__ void __launch_bounds__(kBlock) k_extend(DevScene sc, PathState ps, uint32_t n, const uint32_t *np,
-                                                   uint32_t *spill, unsigned long long *counts) {
+                                                   uint32_t *spill, unsigned long long *counts, GuideList gl) {
   __shared__ uint32_t stk[kLdsStack][kBlock];
+  __shared__ GuideListLds listLds;
   if (np) n = *np;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const V3 o = ToV3(ps.ray_o[i]), d = ToV3(ps.ray_d[i]);
-  float t, u, v;
-  uint32_t prim, bi;
-  TraceStats ts{0, 0};
-  const bool hit = Trace<false, kCount>(sc, o, d, 0.f, INFINITY, &t, &u, &v, &prim, &bi, stk,
-                                        spill + (size_t)i * sc.spill_words, &ts);
-  ps.hit[i] = hit ? make_float4(t, u, v, __uint_as_float(bi)) : make_float4(INFINITY, 0.f, 0.f, __uint_as_float(kNone));
-  if (kCount) {
-    atomicAdd(&counts[0], (unsigned long long)ts.nodes);
-    atomicAdd(&counts[1], (unsigned long long)ts.tris);
-    // the same ray through the 4-wide tree the render walks: node and triangle records fetched
-    TraceStats t4{0, 0};
-    float t2, u2, v2;
-    uint32_t p2, b2;
-    Trace4Closest<true>(sc, o, d, 0.f, INFINITY, &t2, &u2, &v2, &p2, &b2, stk, spill + (size_t)i * sc.spill_words, &t4);
-    atomicAdd(&counts[3], (unsigned long long)t4.nodes);
-    atomicAdd(&counts[4], (unsigned long long)t4.tris);
+  // with the list, every wave of a partial block arrives at it; whole idle blocks leave
+  if (kList ? blockIdx.x * kBlock >= n : i >= n) return;
+  if (kList) {
+    if (threadIdx.x == 0) listLds.arrived = 0u;
+    __syncthreads();
   }
+  bool take = false;
+  if (!kList || i < n) {
+    const V3 o = ToV3(ps.ray_o[i]), d = ToV3(ps.ray_d[i]);
+    float t, u, v;
+    uint32_t prim, bi;
+    TraceStats ts{0, 0};
+    const bool hit = Trace<false, kCount>(sc, o, d, 0.f, INFINITY, &t, &u, &v, &prim, &bi, stk,
+                                          spill + (size_t)i * sc.spill_words, &ts);
+    ps.hit[i] = hit ? make_float4(t, u, v, __uint_as_float(bi)) : make_float4(INFINITY, 0.f, 0.f, __uint_as_float(kNone));
+    if (kList && hit && (sc.tris[bi].key & kKeyGuidable)) {
+      const uint32_t w = ps.ids[i].w;
+      take = !(w & kFlagEmitOnly) && (w >> 8) < gl.max_depth;
+    }
+    if (kCount) {
+      atomicAdd(&counts[0], (unsigned long long)ts.nodes);
+      atomicAdd(&counts[1], (unsigned long long)ts.tris);
+      // the same ray through the 4-wide tree the render walks: node and triangle records fetched
+      TraceStats t4{0, 0};
+      float t2, u2, v2;
+      uint32_t p2, b2;
+      Trace4Closest<true>(sc, o, d, 0.f, INFINITY, &t2, &u2, &v2, &p2, &b2, stk, spill + (size_t)i * sc.spill_words, &t4);
+      atomicAdd(&counts[3], (unsigned long long)t4.nodes);
+      atomicAdd(&counts[4], (unsigned long long)t4.tris);
+    }
+  }
+  if (kList) GuideListAppend(take, blockIdx.x * kBlock, gl, listLds);
 }
 
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene sc, SampleState ss, const ShadowRay *rays, uint32_t n,
@@ -527,19 +586,21 @@ __global__ void __launch_bounds__(kBlock, kPrograms ? 6 : 4) k_shade(const DevSc
   if (guided) {
     // one-sample MIS: the guide with probability c, else the BSDF; pdf c q + (1 - c) p_bsdf.
     // Directions the BSDF cannot sample (p_bsdf = 0) are rejected, as BSDF sampling never
-    // reaches them either.
+    // reaches them either. Either strategy only draws the direction; the BSDF value and pdf are
+    // evaluated once after the two branches join.
+    bool drawn = true;
     if (rng.Get1D() < gc) {
       gbin = GuideSampleBin(reinterpret_cast<const uint32_t *>(gtab), rng.Get1D() * gtot);
       const float u0 = rng.Get1D(), u1 = rng.Get1D();
       ev.wi = GuideBinDirection(gbin, u0, u1);
       ev.wiL = ToLocal(h, ev.wi);
-      pbsdf = shd.Pdf(mat.lobe, ev.woL, ev.wiL);
-      f = pbsdf > 0 ? shd.F(mat.lobe, ev) : S(0.f);
-      h.point = h.point + h.normalG * (Dot(ev.wi, h.normalG) > 0 ? kSurfaceEpsilon : -kSurfaceEpsilon);
     } else {
-      f = shd.Sample(mat.lobe, ev, h, rng, pbsdf, spec);
+      drawn = shd.SampleDir(mat.lobe, ev, h, rng);
       gbin = GuideBin(ev.wi);
     }
+    pbsdf = drawn ? shd.Pdf(mat.lobe, ev.woL, ev.wiL) : 0.f;
+    f = pbsdf > 0 ? shd.F(mat.lobe, ev) : S(0.f);
+    h.point = h.point + h.normalG * (Dot(ev.wi, h.normalG) > 0 ? kSurfaceEpsilon : -kSurfaceEpsilon);
     scatPdf = pbsdf > 0 ? gc * GuideBinPdf(gtab, gtot, gbin) + (1.f - gc) * pbsdf : 0.f;
   } else {
     f = shd.Sample(mat.lobe, ev, h, rng, scatPdf, spec);
@@ -719,7 +780,8 @@ void BatchLaunchCamera(hipStream_t s, const RenderParams &rp, const PathState &p
 }
 void BatchLaunchExtend(hipStream_t s, const DevScene &dsc, const PathState &ps, uint32_t ub, const uint32_t *np,
                        uint32_t *spill) {
-  hipLaunchKernelGGL(k_extend<false>, dim3((ub + kBlock - 1) / kBlock), dim3(kBlock), 0, s, dsc, ps, ub, np, spill, nullptr);
+  hipLaunchKernelGGL(k_extend<false>, dim3((ub + kBlock - 1) / kBlock), dim3(kBlock), 0, s, dsc, ps, ub, np, spill, nullptr,
+                     GuideList{});
 }
 void BatchLaunchShade(hipStream_t s, const DevScene *scp, const RenderParams &rp, const PathState &ps,
                       const SampleState &ss, uint32_t ub, const uint32_t *np, const Queues &q, bool programs) {
@@ -738,8 +800,8 @@ void BatchLaunchAccumulate(hipStream_t s, const RenderParams &rp, const SampleSt
 // Guiding hooks (guide.hip).
 lgpu_status GuidePrepareWave(lgpu_guide g, const float bounds[6], uint64_t expectedVertices, GuideRefs *refs,
                              const uint4 *bins, const float2 *hdr, hipStream_t s);
-lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const DevScene &sc, const PathState &ps, uint32_t ub,
-                       const uint32_t *np, uint32_t *list, uint32_t *count, uint4 *bins, float2 *hdr, hipStream_t s);
+lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps, uint32_t ub, const uint32_t *list,
+                       const uint32_t *count, uint4 *bins, float2 *hdr, hipStream_t s);
 lgpu_status GuideFinishWave(lgpu_guide g, const SampleState &ss, int mode, uint32_t trainSteps, uint64_t vertices,
                             hipStream_t s, uint32_t *records);
 uint64_t GuideExpectedVertices(lgpu_guide g);
@@ -994,18 +1056,27 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
           uint32_t *cin = ring + 4 * prev, *cout = ring + 4 * sl;
           hipEvent_t *e = W.pev[sl];
           LGPU_HIP(hipMemsetAsync(cout, 0, 8, stream));
+          // guided: the extension stage lists the bounce's guided vertices for the inference
+          const GuideList gl{W.glist.as<uint32_t>(), W.gcount.as<uint32_t>(), G ? gref.max_depth : 0u};
+          if (G) LGPU_HIP(hipMemsetAsync(gl.count, 0, 4, stream));
           LGPU_HIP(hipEventRecord(e[0], stream));
-          if (countTrav)
+          if (countTrav && G)
+            hipLaunchKernelGGL((k_extend<true, true>), dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
+                               W.spill_p, W.tcount_p, gl);
+          else if (countTrav)
             hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
-                               W.spill_p, W.tcount_p);
+                               W.spill_p, W.tcount_p, gl);
+          else if (G)
+            hipLaunchKernelGGL((k_extend<false, true>), dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
+                               W.spill_p, W.tcount_p, gl);
           else
             hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(ub)), dim3(kBlock), 0, stream, dsc, W.ps[cur], ub, cin,
-                               W.spill_p, W.tcount_p);
+                               W.spill_p, W.tcount_p, gl);
           LGPU_HIP(hipGetLastError());
           LGPU_HIP(hipEventRecord(e[1], stream));
           if (G) {  // the guide's distribution for the queue's guided vertices (before shading reads it)
-            lgpu_status gs = GuideInfer(G, gref, dsc, W.ps[cur], ub, cin, W.glist.as<uint32_t>(), W.gcount.as<uint32_t>(),
-                                        W.gbins.as<uint4>(), W.ghdr.as<float2>(), stream);
+            lgpu_status gs = GuideInfer(G, gref, W.ps[cur], ub, gl.list, gl.count, W.gbins.as<uint4>(),
+                                        W.ghdr.as<float2>(), stream);
             if (gs != LGPU_OK) return gs;
           }
           LGPU_HIP(hipEventRecord(e[5], stream));
@@ -1069,10 +1140,10 @@ static lgpu_status RunWaves(const lgpu_render_desc *d, uint32_t w, uint32_t h, u
         LGPU_HIP(hipEventRecord(ev[2], stream));
         if (countTrav)
           hipLaunchKernelGGL(k_extend<true>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count, nullptr,
-                             W.spill_p, W.tcount_p);
+                             W.spill_p, W.tcount_p, GuideList{});
         else
           hipLaunchKernelGGL(k_extend<false>, dim3(Blocks(count)), dim3(kBlock), 0, stream, dsc, W.ps[cur], count, nullptr,
-                             W.spill_p, W.tcount_p);
+                             W.spill_p, W.tcount_p, GuideList{});
         LGPU_HIP(hipGetLastError());
         LGPU_HIP(hipEventRecord(ev[3], stream));
         Queues q{W.ps[cur ^ 1], W.shadow_p, W.counters_p};

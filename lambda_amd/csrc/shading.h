@@ -460,6 +460,64 @@ struct ShaderT {
     else if (corrected) pdf *= r;
     return bf * r + LeafF(A, ev) * (1.f - r);
   }
+
+  // The direction choice of LeafSample alone, for lobes without a delta component (Lambertian,
+  // Oren-Nayar BRDF, microfacet BRDF, rough dielectric): sets ev.wiL and ev.wi, false when the
+  // sample fails. The guided shade stage evaluates F and Pdf once for whichever strategy (guide
+  // or BSDF) drew the direction, so a wave whose lanes chose differently runs the evaluation once.
+  __device__ LGPU_LOBE_INLINE bool LeafSampleDir(const Lobe &L, Ev &ev, const Hit &h, Rng &rng) const {
+    switch (L.kind) {
+      case kLambertian:
+      case kOrenNayarR: {
+        const float u0 = rng.Get1D(), u1 = rng.Get1D();
+        ev.wiL = SampleCosineHemisphere(u0, u1);
+        if (ev.woL.y < 0) ev.wiL.y *= -1;
+        break;
+      }
+      case kMicrofacet: {
+        if (ev.woL.y == 0) return false;
+        float ax, ay;
+        Alpha(L, &ax, &ay);
+        const float u0 = rng.Get1D(), u1 = rng.Get1D();
+        const V3 wh = Norm(MfSampleWh(L.p[0] != 0.f, u0, u1, ev.woL, ax));
+        ev.wiL = Reflect(-ev.woL, wh);
+        if (!SameHemisphere(ev.woL, ev.wiL)) return false;
+        break;
+      }
+      case kRoughDielectric: {
+        const float alpha = SockR(L.sock[1]), ior = L.p[0];
+        const float ux = rng.Get1D(), uy = rng.Get1D();
+        const float uc = rng.Get1D();
+        const float tan2 = alpha * alpha * ux / (1 - ux);
+        const float cosT = 1 / sqrtf(1 + tan2);
+        const float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT));
+        V3 wh = SphericalDirection(sinT, cosT, kPi2 * uy);
+        const float Fr = FrDielectric(Dot(ev.woL, wh), 1.f, ior);
+        if (ev.woL.y < 0) wh = -wh;
+        if (uc < Fr) {
+          ev.wiL = Reflect(-ev.woL, wh);
+          if (!SameHemisphere(ev.woL, ev.wiL)) return false;
+        } else {
+          const float etaI = ev.woL.y > 0 ? 1.f : ior, etaT = ev.woL.y > 0 ? ior : 1.f;
+          if (!(Dot(ev.woL, wh) > 0)) return false;
+          if (!Refract(ev.woL, wh, etaI / etaT, &ev.wiL)) return false;
+          if (SameHemisphere(ev.woL, ev.wiL) || ev.wiL.y == 0) return false;
+        }
+        break;
+      }
+      default: return false;
+    }
+    ev.wi = ToWorld(h, ev.wiL);
+    return true;
+  }
+  // Sample's direction choice for a guidable lobe or a mix of two (the child picked as Sample
+  // picks it).
+  __device__ LGPU_LOBE_INLINE bool SampleDir(uint32_t li, Ev &ev, const Hit &h, Rng &rng) const {
+    const Lobe &L = sc->lobes[li];
+    if (L.kind != kMix) return LeafSampleDir(L, ev, h, rng);
+    const float r = SockR(L.sock[2]);
+    return LeafSampleDir(sc->lobes[L.child[rng.Get1D() > r ? 0 : 1]], ev, h, rng);
+  }
 };
 using Shader = ShaderT<true>;
 
