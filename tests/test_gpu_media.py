@@ -2,7 +2,7 @@
 MISVolumetricPathIntegrator.cpp:23-215) with homogeneous and grid media, through the
 C ABI, against the CPU checker on the same seeded inputs.
 
-Bar: per-path radiance |gpu - cpu| <= 1e-4 * max(|gpu|, |cpu|) + 1e-6 for at least 99%
+Bar: per-path radiance |gpu - cpu| <= 1e-4 * max(|gpu|, |cpu|) + 1e-6 for at least 99.9%
 of paths. Media paths take more discrete decisions per vertex (distance sample inside or
 beyond the segment, equiangular inside or not) on results of expf/logf/atan2f/tanf, which
 differ by an ulp between the device library and glibc, so slightly more paths diverge
