@@ -70,13 +70,18 @@ LGPU_HD void GuideInputs(V3 p, V3 wo, const float bmin[3], const float binv[3], 
   s[4] = Clampf(ph * (1.f / kPi2), 0.f, 1.f);
 }
 
-// Direction -> bin (row-major: cos rows, phi columns) and back.
+// Direction -> bin (row-major: cos rows, phi columns) and back. Column c holds phi = atan2(z, x)
+// + pi in [c, c + 1) x 2 pi / 16. It is found without an arctangent: the angle a = atan2(|z|, |x|)
+// lies in the k-th 22.5-degree step of its quadrant, k = [|z| >= tan(22.5) |x|] + [|z| >= |x|]
+// + [|z| >= tan(67.5) |x|], and the signs of x and z place that step among the 16 columns
+// (quadrant x >= 0, z >= 0: 8 + k; x < 0, z >= 0: 15 - k; x < 0, z < 0: k; x >= 0, z < 0: 7 - k).
 LGPU_HD uint32_t GuideBin(V3 w) {
+  static_assert(kGuideBinsPhi == 16, "the column test assumes 16 steps of phi");
   int r = (int)floorf((Clampf(w.y, -1.f, 1.f) + 1.f) * (.5f * kGuideBinsCos));
   r = r < 0 ? 0 : (r > kGuideBinsCos - 1 ? kGuideBinsCos - 1 : r);
-  float ph = atan2f(w.z, w.x) + 3.14159265358979323846f;  // [0, 2 pi]
-  int c = (int)floorf(ph * ((float)kGuideBinsPhi / kPi2));
-  c = c < 0 ? 0 : (c > kGuideBinsPhi - 1 ? kGuideBinsPhi - 1 : c);
+  const float ax = fabsf(w.x), az = fabsf(w.z);
+  const int k = (az >= 0.41421356237f * ax) + (az >= ax) + (az >= 2.41421356237f * ax);
+  const int c = w.z < 0.f ? (w.x < 0.f ? k : 7 - k) : (w.x < 0.f ? 15 - k : 8 + k);
   return (uint32_t)(r * kGuideBinsPhi + c);
 }
 LGPU_HD V3 GuideBinDirection(uint32_t bin, float u0, float u1) {

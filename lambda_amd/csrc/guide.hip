@@ -299,7 +299,8 @@ __device__ inline void QueryScalars(const GuideQuerySrc &src, const QueryRaw &r,
 
 // Inference: per query, the quantised bin table (bytes max(1, round(255 exp(z - z_max))), the
 // distribution is byte / sum) and the mixing probability. Persistent blocks; weights in LDS in
-// fragment order.
+// fragment order. kList: queries are the queue slots src.list[q] (render), else q (test hook).
+template <bool kList>
 __global__ void __launch_bounds__(kInferBlock) k_guide_infer(const uint16_t *__restrict__ Wg, const float *__restrict__ Bg,
                                                                GuideQuerySrc src, uint32_t n, const uint32_t *np,
                                                                uint32_t *__restrict__ bins, float2 *__restrict__ hdr) {
@@ -374,14 +375,16 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer(const uint16_t *__r
       }
       const float sum = Lane4Sum((float)tot);
       const float zc = __shfl(z[8][0], c16);  // the mixing logit, row 128: lane group 0
-      const uint64_t q = wbase + 16 * pt + c16;
-      if (q < n) {
-        const uint64_t slot = src.list ? src.list[q] : q;
-        // lane group g holds bytes 16 m + 4 g .. + 3: words 4 m + g
+      // Stores without a branch, so that the four column tiles' output layers and quantisation
+      // form one straight-line block the scheduler can interleave. Columns past the last query
+      // were given query n - 1's inputs (the clamped loads above), so they write its slot with
+      // the same values; every lane group holds the same c and sum.
+      const uint64_t q = std::min<uint64_t>(wbase + 16 * pt + c16, n - 1);
+      const uint64_t slot = kList ? src.list[q] : q;
+      // lane group g holds bytes 16 m + 4 g .. + 3: words 4 m + g
 #pragma unroll
-        for (int m = 0; m < 8; ++m) bins[slot * 32 + 4 * m + g] = words[m];
-        if (g == 0) hdr[slot] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), sum);
-      }
+      for (int m = 0; m < 8; ++m) bins[slot * 32 + 4 * m + g] = words[m];
+      hdr[slot] = make_float2(kGuideCMin + kGuideCSpan / (1.f + __expf(-zc)), sum);
     }
   }
 }
@@ -711,7 +714,7 @@ lgpu_status GuideInfer(lgpu_guide g, const GuideRefs &refs, const PathState &ps,
   for (int k = 0; k < 3; ++k) { src.bmin[k] = refs.bmin[k]; src.binv[k] = refs.binv[k]; }
   constexpr uint32_t kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
   const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((ub + kPerBlock - 1) / kPerBlock, g->infer_grid));
-  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, ub,
+  hipLaunchKernelGGL(k_guide_infer<true>, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, ub,
                      count, reinterpret_cast<uint32_t *>(bins), hdr);
   LGPU_HIP(hipGetLastError());
   return LGPU_OK;
@@ -772,7 +775,7 @@ extern "C" lgpu_status lgpu_guide_create(lgpu_device dev, const lgpu_guide_desc 
   if ((e = g->rec_count.alloc(256)) != hipSuccess) return fail(e);
   {  // the persistent inference grid: as many blocks as are resident at once
     int perCu = 0, cus = 0;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_guide_infer, kInferBlock, 0)) != hipSuccess) return fail(e);
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_guide_infer<true>, kInferBlock, 0)) != hipSuccess) return fail(e);
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->device)) != hipSuccess) return fail(e);
     g->infer_grid = (uint32_t)std::max(1, perCu * cus);
   }
@@ -867,7 +870,7 @@ extern "C" lgpu_status lgpu_guide_forward(lgpu_guide g, uint32_t n, const float 
   src.scalars = din.as<float>();
   constexpr uint32_t kPerBlock = (kInferBlock / 64) * 16 * kInferPT;
   const uint32_t blocks = std::min<uint32_t>((n + kPerBlock - 1) / kPerBlock, g->infer_grid);
-  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, n,
+  hipLaunchKernelGGL(k_guide_infer<false>, dim3(blocks), dim3(kInferBlock), 0, s, g->wbf.as<uint16_t>(), g->bias.as<float>(), src, n,
                      nullptr, dbins.as<uint32_t>(), dhdr.as<float2>());
   const hipError_t le = hipGetLastError();
   const hipError_t se = hipStreamSynchronize(s);
