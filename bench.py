@@ -103,6 +103,51 @@ def measured_valu_busy(workload):
     return out
 
 
+def measured_time_to_relmse():
+    """Plain and guided time-to-relMSE <= 1e-3 on the many-light workload from the committed
+    measurement (bench.py --relmse --guided), or None."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "relmse_many_light_1080p_plain_vs_guided.md")
+    if not os.path.isfile(path):
+        return None
+    m = re.search(r"\*\*plain ([0-9.]+) s\*\* \(([0-9]+) spp.*?\*\*guided ([0-9.]+) s\*\* \(([0-9]+) spp",
+                  open(path).read())
+    if not m:
+        return None
+    p, g = float(m.group(1)), float(m.group(3))
+    return {"plain_s": p, "plain_spp": int(m.group(2)), "guided_s": g, "guided_spp": int(m.group(4)),
+            "guided_over_plain": round(g / p, 3), "source": "profiles/" + os.path.basename(path)}
+
+
+def guided_throughput(desc, film, W, H, dev):
+    """The guided estimator on the same workload: a fresh guide trained online as in the
+    time-to-relMSE measurement (256 spp in 8-spp waves with 16 Adam steps each, untimed), then 256
+    timed spp in 64-spp waves with 4 steps each (wall time of the render calls, inference and
+    training included). Guided vertices: path depth 0 and 1."""
+    from lambda_amd import gpu
+    g = gpu.Guide(dev, seed=7, max_depth=2)
+    try:
+        gpu.render(desc(0, 1, guide=g, capacity=W * H * 64))  # sizes the guided path pool (untimed)
+        for b in range(0, 256, 8):
+            gpu.render(desc(b, b + 8, guide=g, steps=16, capacity=W * H * 8))
+        film.clear()
+        kern = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_guide_infer": 0.0, "ms_guide_train": 0.0}
+        t0 = time.perf_counter()
+        for b in range(256, 512, 64):
+            st = gpu.render(desc(b, b + 64, guide=g, steps=4, capacity=W * H * 64))
+            for k in kern:
+                kern[k] += getattr(st, k)
+        dt = time.perf_counter() - t0
+    finally:
+        g.close()
+    out = {"msamples_per_s": round(W * H * 256 / dt / 1e6, 1), "spp": 256, "seconds": round(dt, 4),
+           "schedule": "after 256 spp of online training in 8-spp waves: 64-spp waves, 4 Adam steps each, "
+                       "guided vertices at path depth 0 and 1"}
+    out.update({k: round(v, 1) for k, v in kern.items()})
+    return out
+
+
 def integrator_id(cfg):
     from lambda_amd import abi
     return {"path": abi.INTEGRATOR_PATH, "volpath": abi.INTEGRATOR_VOLPATH,
@@ -170,6 +215,8 @@ def main():
     ap.add_argument("--capacity", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-guided", action="store_true",
+                    help="many-light: skip the guided estimator's throughput beside the plain one")
     ap.add_argument("--no-count", action="store_true",
                     help="skip the traversal-counting pass (counter runs: every dispatch is then the timed render's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -333,6 +380,13 @@ def main():
         import numpy as np
         np.save(args.save_image, rgba)
 
+    guided = None
+    if args.workload == "many-light" and not args.no_guided and not args.integrator and args.estimator == "corrected":
+        log("guided estimator ...")
+        guided = guided_throughput(desc, film, W, H, dev)
+        guided["time_to_relmse"] = measured_time_to_relmse()
+        log("guided: %s" % guided)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -346,7 +400,7 @@ def main():
             "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(dt * 1000.0 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scene, no assets)",
-            "config": cfg, "roofline": roofline, "cpu_baseline": cpu,
+            "config": cfg, "roofline": roofline, "cpu_baseline": cpu, "guided": guided,
             "stats": {"extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
                       "ms_extend": round(st.ms_extend, 2), "ms_shade": round(st.ms_shade, 2),
                       "ms_shadow": round(st.ms_shadow, 2), "ms_camera": round(st.ms_camera, 2),

@@ -23,3 +23,15 @@ def test_help_lists_the_workloads():
     assert r.returncode == 0
     for w in ("many-light", "media", "cornell"):
         assert w in r.stdout
+
+
+def test_committed_measurements_parse():
+    """The bench line's traffic, VALU-busy and time-to-relMSE fields come from the committed
+    profile summaries; each parser finds its figures."""
+    sys.path.insert(0, ROOT)
+    import bench
+    t = bench.measured_traffic("many-light")
+    assert t["k_shade"]["read"] > 0 and t["k_extend"]["read"] > 0
+    assert 0 < bench.measured_valu_busy("many-light")["k_extend"] <= 1
+    r = bench.measured_time_to_relmse()
+    assert r["plain_s"] > 0 and r["guided_s"] > 0 and r["plain_spp"] > r["guided_spp"]
