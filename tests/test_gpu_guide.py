@@ -162,6 +162,36 @@ def test_guided_matches_plain_estimator(dev):
         g.close()
 
 
+def test_guided_traversal_count_keeps_the_film(dev):
+    """A guided render with traversal counting (the extension stage that counts node and triangle
+    fetches and lists the guided vertices) gives the same film, bit for bit, as the same guided
+    render without counting (same guide, no training in between), and counts its fetches."""
+    sd = scene.cornell_box()
+    W = H = 64
+    cam = scene.cornell_camera(W, H)
+    s = gpu.Scene(dev)
+    keep = sd.push(s, abi.LIGHTS_TREE)
+    g = gpu.Guide(dev, seed=8, batch=4096)
+    films = [gpu.Film(dev, W, H) for _ in range(2)]
+    try:
+        st = []
+        for f, flags in zip(films, (0, abi.RENDER_COUNT_TRAVERSAL)):
+            d = gpu.render_desc(s, f, cam, 0, 16, max_bounces=8, min_bounces=3, spectral=abi.SPECTRAL_HERO4,
+                                estimator=abi.ESTIMATOR_CORRECTED, seed=3, guide=g, guide_train_steps=0,
+                                capacity=W * H * 16, flags=flags)
+            st.append(gpu.render(d))
+        (a, ca), (b, cb) = films[0].read(), films[1].read()
+        print("guided counting pass: nodes4", st[1].nodes4_fetched, "rays", st[1].extension_rays)
+        assert st[1].nodes4_fetched > 0 and st[1].extension_rays == st[0].extension_rays
+        assert np.array_equal(a, b) and np.array_equal(ca, cb)
+    finally:
+        for f in films:
+            f.close()
+        g.close()
+        del keep
+        s.close()
+
+
 def test_guided_matches_plain_on_glossy_lobes(dev):
     """Room of microfacet BRDFs (Beckmann and GGX, dielectric and conductor Fresnel), a rough
     dielectric and Lambertian walls, beside delta lobes that stay unguided: guided and plain film
