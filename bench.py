@@ -233,8 +233,9 @@ def main():
                     help="with --relmse: also measure the guided estimator (online-trained guiding network)")
     ap.add_argument("--guide-steps", type=int, default=16,
                     help="Adam steps after each guided wave of the warm-up (waves of 8 spp)")
-    ap.add_argument("--guide-warm-spp", type=int, default=256,
-                    help="guided samples per pixel rendered in 8-spp waves with --guide-steps of training each")
+    ap.add_argument("--guide-warm-spp", default="256",
+                    help="guided samples per pixel rendered in 8-spp waves with --guide-steps of training each; "
+                         "a comma-separated list measures each")
     ap.add_argument("--guide-late-steps", default="4",
                     help="Adam steps after each later guided wave (waves of --guide-late-wave spp); a "
                          "comma-separated list measures each")
@@ -623,7 +624,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     # untimed: the path pool the plain checkpoints' largest call needs (as the timed bench's warm-up)
     gpu.render(desc(0, 1, capacity=min(W * H * max(1, args.ref_spp // 8), 1 << 27)))
 
-    def progressive(guide=None, late_steps=4, late_wave=64):
+    def progressive(guide=None, late_steps=4, late_wave=64, warm_spp=256):
         """Checkpoints of one progressive render (cumulative wall time of the render calls,
         training included for the guided estimator); stops at relMSE <= 1e-3."""
         film.clear()
@@ -647,7 +648,7 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
                 # warm-up, then larger waves with --guide-late-steps
                 b = done
                 while b < target:
-                    warm = b < args.guide_warm_spp
+                    warm = b < warm_spp
                     per = min(target - b, 8 if warm else late_wave)
                     steps = args.guide_steps if warm else late_steps
                     tally(gpu.render(desc(b, b + per, guide=guide, steps=steps, capacity=W * H * per)))
@@ -682,15 +683,15 @@ def relmse_run(args, sd, cam, cfg, spectral, estimator, desc, film, dev):
     if args.guided:
         guided = []
         for depth in [int(x) for x in args.guide_depth.split(",")]:
-            for late, wave in [(int(x), int(y)) for x in args.guide_late_steps.split(",")
-                               for y in args.guide_late_wave.split(",")]:
+            for late, wave, wspp in [(int(x), int(y), int(z)) for x in args.guide_late_steps.split(",")
+                                     for y in args.guide_late_wave.split(",") for z in args.guide_warm_spp.split(",")]:
                 g = gpu.Guide(dev, seed=7, max_depth=depth)
-                gck, ghit = progressive(g, late, wave)
+                gck, ghit = progressive(g, late, wave, wspp)
                 guided.append({"value": ghit["seconds"] if ghit else None,
                                "spp_at_target": ghit["spp"] if ghit else None,
                                "interpolated_s": ghit.get("interpolated_s") if ghit else None,
                                "checkpoints": gck, "guide": g.stats(),
-                               "schedule": {"warm_spp": args.guide_warm_spp, "warm_wave_spp": 8,
+                               "schedule": {"warm_spp": wspp, "warm_wave_spp": 8,
                                             "warm_steps": args.guide_steps, "late_wave_spp": wave,
                                             "late_steps": late, "guided_depth": depth or None}})
                 g.close()
